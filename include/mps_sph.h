@@ -1,0 +1,252 @@
+/*
+ * mps_sph.h — C ABI of the MI355X (gfx950) SPH time step.
+ *
+ * This is the drop-in boundary for the compute that MPS_CUDA runs in MPS_Computer and
+ * MPS_Object. The reference exposes that compute as mangled C++ DLL exports taking POD
+ * bundles of raw device pointers (src/MPS_Computer/HeaderPre.h:1-19,
+ * src/MPS_Computer/MPSSPHDFSPHUtil.h:9-87, src/MPS_Object/MPSSpatialHash.h:20-57).
+ * Here every entry point is extern "C", takes plain pointers and sizes, returns an int
+ * status (0 = OK) and never throws. A context owns all device memory and one HIP stream;
+ * there is no global state, and a context is used by one host thread at a time.
+ *
+ * Reference entry points replaced (file:line of the reference interface):
+ *   mps_create / mps_destroy         mps::System::System, SetDevice       MPSSystem.cpp:19,86-93
+ *   mps_set_physics                  PhysicsParam (GBArchiver)             MPSUniformDef.h:19-25, MPSSystem.cpp:30-34
+ *   mps_set_fluid_material           SPHMaterial::SetParam/SetRadius/...   MPSSPHMaterial.h:22-43, .cpp:4-44
+ *   mps_upload_fluid                 SPHObject::Resize + CopyFromHost      MPSSystem.cpp:630-635
+ *   mps_upload_boundary              BoundaryParticleObject::Resize        MPSBoundaryParticleObject.cu:14-24
+ *   mps_sample_boundary_mesh         MeshObject::LoadMesh + kernel::ParticleSampling::ParticleSampling
+ *                                    MPSMeshObject.cu:58-103, MPSParticleSamplingUtil.cu:42-133
+ *   mps_step                         System::OnUpdate                      MPSSystem.cpp:191-291
+ *   mps_run_phase                    the kernel::SPH / kernel::Advect / SpatialHash calls made by
+ *                                    OnUpdate, one at a time              MPSSystem.cpp:220-278
+ *   mps_array_info/get/set           SpatialHash::GetHashKeys/GetHashIDs/GetStartIdx/GetEndIdx/
+ *                                    GetNeighborhood and the SPHParam pointers
+ *                                    MPSSpatialHash.h:33-38, MPSSPHParam.h:7-20
+ *   mps_readback_async/fence_wait    CUDA-GL interop buffer map/unmap       MCUDAGLBuffer.inl:49-79,
+ *                                    consumed by SPHRenderer::Draw         MPSSPHRenderer.cpp:18-32
+ */
+#ifndef MPS_SPH_H
+#define MPS_SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define MPS_OK 0
+#define MPS_ERR_INVALID 1   /* bad argument or size */
+#define MPS_ERR_HIP 2       /* a HIP runtime call failed */
+#define MPS_ERR_NOMEM 3     /* device or host allocation failed */
+#define MPS_ERR_STATE 4     /* call not valid in the current state (e.g. no fluid) */
+#define MPS_ERR_NONFINITE 5 /* a NaN/inf was detected in the particle state */
+
+/* Longest per-iteration error log kept in mps_step_stats (cap of 100 iterations + 1). */
+#define MPS_MAX_ITER_LOG 128
+
+/* ---- enums ------------------------------------------------------------------------ */
+/* Smoothing kernel family. POLY6_SPIKY is the reference's active choice
+ * (USE_CUBIC_KERNEL 0, MPSSPHKernel.cuh:5); CUBIC is the spline behind that switch
+ * (MPSSPHKernel.cuh:34-40,52-59). */
+enum mps_kernel_type { MPS_KERNEL_POLY6_SPIKY = 0, MPS_KERNEL_CUBIC = 1 };
+
+/* Implicit viscosity / surface-tension solver. CG is the live reference path
+ * (MPSSPHNonPressureUtil.cu:127-188); JACOBI is the 3x3 block-Jacobi viscosity solve that
+ * exists only as commented-out code (SPHUtil.cu:344-453); NONE skips the phase. */
+enum mps_visc_solver { MPS_VISC_CG = 0, MPS_VISC_JACOBI = 1, MPS_VISC_NONE = 2 };
+
+/* Phases of one step, in OnUpdate order (MPSSystem.cpp:220-278). */
+enum mps_phase {
+    MPS_PHASE_ZSORT = 1,            /* SpatialHash::ZSort, boundary then fluid (:220-224) */
+    MPS_PHASE_HASH = 2,             /* SpatialHash::UpdateHash, boundary then fluid (:226-227) */
+    MPS_PHASE_NEIGHBORS = 3,        /* BuildNeighorhood bb, ff, fb (:229-231) */
+    MPS_PHASE_BOUNDARY_VOLUME = 4,  /* ComputeBoundaryParticleVolumeSub/Final (:237-238) */
+    MPS_PHASE_DENSITY = 5,          /* ResetREAL, ComputeDensitySub x2, Final, DensityColorTest (:240-245) */
+    MPS_PHASE_ALPHA = 6,            /* ComputeDFSPHFactorSub x2, Final (:247-251) */
+    MPS_PHASE_GRAVITY = 7,          /* ResetForce, ApplyGravity, UpdateVelocity (:253-255) */
+    MPS_PHASE_DIVERGENCE = 8,       /* ComputeDFSPHDivergenceFree (:257-264) */
+    MPS_PHASE_VISCOSITY = 9,        /* ApplyImplicitViscosityNSurfaceTension (:266) */
+    MPS_PHASE_CONSTANT_DENSITY = 10,/* ComputeDFSPHConstantDensity (:268-275) */
+    MPS_PHASE_ADVECT = 11           /* BoundaryCollision, UpdatePosition (:277-278) */
+};
+
+/* Arrays readable/writable for parity checks. Element layout in brackets; "real" is
+ * float in the HIP library and the oracle's working precision in the oracle. */
+enum mps_array_id {
+    MPS_ARR_FLUID_POS = 1,       /* real[N][3]  SPHParam::pPosition */
+    MPS_ARR_FLUID_VEL = 2,       /* real[N][3]  pVelocity */
+    MPS_ARR_FLUID_RADIUS = 3,    /* real[N]     pRadius (particle radius r, support h = 4r) */
+    MPS_ARR_FLUID_MASS = 4,      /* real[N]     pMass */
+    MPS_ARR_FLUID_COLOR = 5,     /* float[N][4] pColor (DensityColorTest output) */
+    MPS_ARR_FLUID_DENSITY = 6,   /* real[N]     pDensity */
+    MPS_ARR_FLUID_ALPHA = 7,     /* real[N]     pFactorDFSPH */
+    MPS_ARR_FLUID_KAPPA = 8,     /* real[N]     pPressure (stiffness of the last evaluation) */
+    MPS_ARR_FLUID_DELTA = 9,     /* real[N]     pTempReal (density change rate of the last evaluation) */
+    MPS_ARR_FLUID_FACTOR_ST = 10,/* real[N]     pFactorST */
+    MPS_ARR_FLUID_CG_B = 11,     /* real[N][3]  pTempVec3 after ComputeSurfaceTensionCGb */
+    MPS_ARR_BND_POS = 20,        /* real[B][3] */
+    MPS_ARR_BND_RADIUS = 21,     /* real[B] */
+    MPS_ARR_BND_VOLUME = 22,     /* real[B]     pVolume (Akinci volume) */
+    MPS_ARR_BND_MASS = 23,       /* real[B] */
+    MPS_ARR_BND_VEL = 24,        /* real[B][3] */
+    MPS_ARR_BND_FACE_ID = 25,    /* uint32[B] */
+    MPS_ARR_FHASH_KEY = 30,      /* uint32[N]   sorted cell keys of the fluid hash */
+    MPS_ARR_FHASH_ID = 31,       /* uint32[N]   particle ids in key order */
+    MPS_ARR_FHASH_START = 32,    /* uint32[C]   first sorted slot of each cell, 0xFFFFFFFF if empty */
+    MPS_ARR_FHASH_END = 33,      /* uint32[C]   one past the last slot, 0xFFFFFFFF if empty */
+    MPS_ARR_BHASH_KEY = 34,
+    MPS_ARR_BHASH_ID = 35,
+    MPS_ARR_BHASH_START = 36,
+    MPS_ARR_BHASH_END = 37,
+    MPS_ARR_NEI_FF_IDX = 40,     /* uint32[N+1] CSR offsets, fluid->fluid (self excluded) */
+    MPS_ARR_NEI_FF = 41,         /* uint32[nnz] neighbour ids in 27-cell z,y,x order */
+    MPS_ARR_NEI_FB_IDX = 42,     /* uint32[N+1] fluid->boundary */
+    MPS_ARR_NEI_FB = 43,
+    MPS_ARR_NEI_BB_IDX = 44,     /* uint32[B+1] boundary->boundary (self excluded) */
+    MPS_ARR_NEI_BB = 45
+};
+
+/* ---- parameter structs -------------------------------------------------------------- */
+/* PhysicsParam (MPSUniformDef.h:19-25): time step, gravity and collision box. */
+typedef struct mps_physics {
+    double dt;
+    double gravity[3];
+    double box_min[3];
+    double box_max[3];
+} mps_physics;
+
+/* SPHMaterialParam (MPSSPHMaterial.h:9-21). `radius` is the material radius R, equal to
+ * the support h = 4 r of the particles; volume = R^3/48*3.141592 and mass = density*volume
+ * are derived exactly as SPHMaterial::SetParam does (MPSSPHMaterial.cpp:13-19). */
+typedef struct mps_sph_material {
+    double radius;
+    double density;
+    double viscosity;
+    double xsph;
+    double surface_tension;
+} mps_sph_material;
+
+/* MeshMaterialParam (MPSMeshMaterial.h:9-20): sampling radius R_m; boundary particles get
+ * radius R_m/4 and spacing R_m/2 (MPSParticleSamplingUtil.cuh:32,92). */
+typedef struct mps_mesh_material {
+    double radius;
+    double density;
+} mps_mesh_material;
+
+/* Everything the reference hard-codes as defines or literals, in one struct.
+ * mps_config_init fills the reference's own values. */
+typedef struct mps_config {
+    int32_t kernel;              /* enum mps_kernel_type; reference: POLY6_SPIKY */
+    int32_t visc_solver;         /* enum mps_visc_solver; reference: CG */
+    int32_t faithful_quirks;     /* 1: boundary volume accumulates across steps and is not
+                                    permuted by the Z-sort (MPSSystem.cpp:237-238,
+                                    MPSSpatialHash.cu:132-139); 0: V_b = 1/S_b each step */
+    int32_t device;              /* HIP device ordinal (HIP library only) */
+    uint32_t hash_size[3];       /* cells per axis, powers of two; reference 64^3 */
+    double cell_size;            /* hash cell edge; <=0 means "fluid material radius" (= h) */
+    double df_tol;               /* divergence-free error tolerance; reference 1e-3 */
+    uint32_t df_min_iter;        /* break only when l > df_min_iter; reference 1 */
+    double cd_tol;               /* constant-density tolerance; reference 1e-4 */
+    uint32_t cd_min_iter;        /* reference 2 */
+    uint32_t max_iter;           /* DF/CD cap; reference 100 */
+    double cg_tol;               /* absolute r.r threshold; reference 1e-2 */
+    uint32_t cg_max_iter;        /* reference 100 */
+    double cg_factor;            /* diagonal shift f used after the first A.x; reference 0.01 */
+    uint32_t jacobi_max_iter;    /* Jacobi cap; reference 100 */
+    double jacobi_tol;           /* Jacobi stop on mean error*dt/N; reference 1e-4 */
+    uint32_t zsort_interval;     /* Z-sort when frame % interval == 1; reference 50; 0 = never */
+    uint32_t nei_capacity;       /* initial per-particle neighbour slots (grows on overflow) */
+    uint32_t host_sync_every;    /* HIP library: solver iterations enqueued between
+                                    convergence checks on the host (0 = adaptive) */
+    uint32_t reserved[7];
+} mps_config;
+
+/* Per-step statistics (the reference prints these to the debugger only:
+ * MPSSPHDFSPHUtil.cu:335-339,378-383,421-425,435-440, MPSSolver.cuh:44-48). */
+typedef struct mps_step_stats {
+    uint32_t frame;        /* frame counter before the step (System::m_frame) */
+    uint32_t zsorted;      /* 1 if the Z-sort ran in this step */
+    uint32_t l_df;         /* value of the loop variable l when the DF loop ended */
+    uint32_t l_cd;
+    uint32_t l_cg;         /* CG loop count (MPSSolver.cuh return value) or Jacobi sweeps */
+    uint32_t hit_cap_df;   /* 1 if the loop ran to its cap without converging */
+    uint32_t hit_cap_cd;
+    uint32_t hit_cap_cg;
+    uint32_t nonfinite;    /* 1 if a NaN/inf was found in positions/velocities after the step */
+    uint32_t n_fluid;
+    uint32_t n_boundary;
+    uint32_t nei_regrow;   /* number of neighbour-capacity grow-and-retry events */
+    uint64_t nnz_ff;
+    uint64_t nnz_fb;
+    uint64_t nnz_bb;
+    double err_df[MPS_MAX_ITER_LOG];  /* mean positive error of each DF evaluation */
+    double err_cd[MPS_MAX_ITER_LOG];
+    double rr_cg[MPS_MAX_ITER_LOG];   /* r.r of each CG iteration (or Jacobi error) */
+} mps_step_stats;
+
+typedef struct mps_ctx mps_ctx;
+
+/* Completion handle of an asynchronous readback. */
+typedef struct mps_fence {
+    void* event;       /* opaque */
+    uint64_t frame;
+} mps_fence;
+
+/* ---- lifetime --------------------------------------------------------------------- */
+int mps_config_init(mps_config* cfg);
+int mps_create(const mps_config* cfg, mps_ctx** out);
+void mps_destroy(mps_ctx* ctx);
+const char* mps_last_error(const mps_ctx* ctx);
+const char* mps_version(void);
+
+/* ---- scene setup ------------------------------------------------------------------- */
+int mps_set_physics(mps_ctx* ctx, const mps_physics* p);
+int mps_set_fluid_material(mps_ctx* ctx, const mps_sph_material* m);
+/* pos, vel: n*3 floats; radius, mass: n floats. vel may be NULL (zero); mass may be NULL
+ * (material mass, MPSSystem.cpp:583). Resets the frame counter and all derived state. */
+int mps_upload_fluid(mps_ctx* ctx, size_t n, const float* pos, const float* vel,
+                     const float* radius, const float* mass);
+/* Pre-sampled boundary particles: pos n*3, radius n. mass may be NULL (zero). */
+int mps_upload_boundary(mps_ctx* ctx, size_t n, const float* pos, const float* radius,
+                        const float* mass);
+/* Triangle mesh -> boundary particles. verts: nv*3 doubles, tris: nt*3 vertex indices.
+ * If center/size are non-NULL the mesh is first fitted into that box as
+ * MeshObject::Translate does (MPSMeshObject.cu:130-151). The vertex mass is
+ * total area * density / nv (MPSMeshObject.cu:169-174). */
+int mps_sample_boundary_mesh(mps_ctx* ctx, const double* verts, size_t nv,
+                             const uint32_t* tris, size_t nt, const double* center,
+                             const double* size, const mps_mesh_material* m);
+
+/* ---- stepping ---------------------------------------------------------------------- */
+/* One System::OnUpdate. `out` may be NULL. */
+int mps_step(mps_ctx* ctx, mps_step_stats* out);
+/* Run one phase in isolation on the current state (teacher-forced parity checks).
+ * Solver phases fill the matching fields of `out` (may be NULL). */
+int mps_run_phase(mps_ctx* ctx, int32_t phase, mps_step_stats* out);
+int mps_synchronize(mps_ctx* ctx);
+
+/* ---- state access -------------------------------------------------------------------- */
+int mps_array_info(mps_ctx* ctx, int32_t id, size_t* count, size_t* elem_bytes);
+/* Blocking copies of a whole array; `bytes` must equal count*elem_bytes. */
+int mps_get_array(mps_ctx* ctx, int32_t id, void* dst, size_t bytes);
+int mps_set_array(mps_ctx* ctx, int32_t id, const void* src, size_t bytes);
+int mps_get_frame(mps_ctx* ctx, uint32_t* frame);
+int mps_set_frame(mps_ctx* ctx, uint32_t frame);
+
+/* ---- render feed (replaces the CUDA-GL interop buffers) -------------------------------- */
+/* Enqueue device->host copies of positions (n*3 float), radii (n) and colours (n*4) on the
+ * context's stream; any pointer may be NULL. Host buffers should be pinned
+ * (mps_host_alloc) for the copy to be asynchronous. */
+int mps_readback_async(mps_ctx* ctx, float* host_pos, float* host_radius, float* host_rgba,
+                       mps_fence* fence);
+int mps_fence_wait(mps_ctx* ctx, mps_fence* fence);
+int mps_host_alloc(mps_ctx* ctx, size_t bytes, void** out);
+int mps_host_free(mps_ctx* ctx, void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS_SPH_H */

@@ -1,0 +1,1053 @@
+// capi.cpp — the C ABI of include/mps_sph.h on top of the gfx950 kernels.
+//
+// One context = one HIP stream + all device buffers of one scene. The step runs the
+// phases of System::OnUpdate (MPSSystem.cpp:191-291) in order. Solver loops are driven
+// from the host in chunks: a chunk of iterations is enqueued back to back, the kernels
+// keep the loop state on the device (SolverDev) and exit at once after convergence, and
+// the host reads the 8-byte loop header once per chunk. The chunk length follows the
+// previous step's iteration count, so a converging solve usually costs one host wait.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mps_sph.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+using mps::HashView;
+using mps::NbrCounters;
+using mps::SolverDev;
+using mps::StepParams;
+using mps::dev::HashGeom;
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+#define HIPCHK(x)                                         \
+    do {                                                  \
+        hipError_t e_ = (x);                              \
+        if (e_ != hipSuccess) throw HipError{e_, #x};     \
+    } while (0)
+
+struct ApiError {
+    int code;
+    std::string msg;
+};
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        if (p) HIPCHK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        if (n) HIPCHK(hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T)));
+        cap = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+struct HashSet {
+    DBuf<uint32_t> key, id, key2, id2, start, end, temp;
+    DBuf<float4> spos;
+    uint32_t n_valid = 0;  // length of the sorted key array the cell table was built from
+    bool cells_init = false;
+    void release() {
+        key.release(); id.release(); key2.release(); id2.release(); start.release(); end.release(); temp.release();
+        spos.release();
+    }
+};
+
+int bits_for(size_t count) {  // number of bits needed for values < count
+    int b = 0;
+    while ((size_t(1) << b) < count) b++;
+    return std::max(b, 1);
+}
+
+}  // namespace
+
+struct mps_ctx {
+    mps_config cfg{};
+    mps_physics phys{};
+    mps_sph_material mat{};
+    double mV = 0.0, mMass = 0.0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    uint32_t N = 0, B = 0, frame = 0;
+    // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
+    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
+    DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
+    // boundary: bpos = (x, y, z, r), bvel = (vx, vy, vz, m)
+    DBuf<float4> bpos, bvel, bcolor;
+    DBuf<float> bvol, bS;
+    DBuf<uint32_t> bface;
+    DBuf<float2> bbcc;
+    bool bnd_dirty = true;
+    HashSet fh, bh;
+    DBuf<uint32_t> zkey, zid, zkey2, zid2, ztemp, tmpu;
+    DBuf<float2> tmp2;
+    // neighbour lists (column-major ELL)
+    DBuf<uint32_t> nff, cff, nfb, cfb, nbb, cbb;
+    uint32_t Kff = 0, Kfb = 0, Kbb = 0;
+    uint32_t nei_regrow = 0;
+    // small device records + pinned host mirrors
+    NbrCounters* d_ctr = nullptr;
+    NbrCounters* h_ctr = nullptr;
+    SolverDev* d_solver = nullptr;  // [3]: DF, CD, CG
+    SolverDev* h_solver = nullptr;
+    uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch
+    uint32_t* h_flags = nullptr;
+    double* d_err3 = nullptr;
+    double* h_err3 = nullptr;
+    DBuf<double> partials;
+    DBuf<float> rb_xyz, rb_rad;
+    DBuf<float4> rb_rgba;
+    uint32_t pred[3] = {2, 3, 8};
+
+    // ------------------------------------------------------------------ helpers
+    HashGeom geom() const {
+        const double cell = cfg.cell_size > 0.0 ? double(float(cfg.cell_size)) : double(float(mat.radius));
+        HashGeom g;
+        g.hx = cfg.hash_size[0];
+        g.hy = cfg.hash_size[1];
+        g.hz = cfg.hash_size[2];
+        g.inv_cell = 1.0 / cell;
+        return g;
+    }
+    size_t cells() const { return size_t(cfg.hash_size[0]) * cfg.hash_size[1] * cfg.hash_size[2]; }
+    StepParams params() const {
+        StepParams P{};
+        P.N = N;
+        P.B = B;
+        P.V = float(mV);
+        P.rho0 = float(mat.density);
+        P.mass = float(mMass);
+        P.mu = float(mat.viscosity);
+        P.xsph = float(mat.xsph);
+        P.sigma = float(mat.surface_tension);
+        P.dt = float(phys.dt);
+        P.gx = float(phys.gravity[0]);
+        P.gy = float(phys.gravity[1]);
+        P.gz = float(phys.gravity[2]);
+        for (int k = 0; k < 3; k++) {
+            P.bmin[k] = float(phys.box_min[k]);
+            P.bmax[k] = float(phys.box_max[k]);
+        }
+        P.cg_factor = float(cfg.cg_factor);
+        P.faithful = cfg.faithful_quirks;
+        return P;
+    }
+    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p}; }
+    int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
+    void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+
+    void ensure_partials() {
+        const size_t nblk = (std::max<size_t>(std::max<size_t>(N, B), 1) + mps::dev::kBlock - 1) / mps::dev::kBlock;
+        partials.reserve(3 * nblk + 8);
+    }
+
+    // ------------------------------------------------------------------ phases
+    void zsort_set(uint32_t n, DBuf<float4>& p4, bool boundary) {
+        if (n == 0) return;
+        const HashGeom g = geom();
+        zkey.reserve(n); zid.reserve(n); zkey2.reserve(n); zid2.reserve(n);
+        ztemp.reserve(mps::launch::radix_temp_words(n));
+        mps::launch::cell_keys(p4.p, n, g, 1, zkey.p, zid.p, stream);
+        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
+        const int bits = 3 * std::min(bits_for(hmax), 10);
+        uint32_t *k = zkey.p, *v = zid.p, *k2 = zkey2.p, *v2 = zid2.p;
+        mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream);
+        const uint32_t* perm = v;
+        tmp4.reserve(n);
+        auto perm4 = [&](DBuf<float4>& a) {
+            mps::launch::gather_f4(perm, n, a.p, tmp4.p, stream);
+            std::swap(a.p, tmp4.p);
+            std::swap(a.cap, tmp4.cap);
+        };
+        perm4(p4);
+        if (boundary) {
+            perm4(bvel);
+            perm4(bcolor);
+            tmpu.reserve(n);
+            mps::launch::gather_u32(perm, n, bface.p, tmpu.p, stream);
+            std::swap(bface.p, tmpu.p); std::swap(bface.cap, tmpu.cap);
+            tmp2.reserve(n);
+            mps::launch::gather_f2(perm, n, bbcc.p, tmp2.p, stream);
+            std::swap(bbcc.p, tmp2.p); std::swap(bbcc.cap, tmp2.cap);
+            if (!cfg.faithful_quirks) {
+                tmp1.reserve(n);
+                mps::launch::gather_f1(perm, n, bvol.p, tmp1.p, stream);
+                std::swap(bvol.p, tmp1.p); std::swap(bvol.cap, tmp1.cap);
+            }
+            bnd_dirty = true;
+        } else {
+            perm4(vel);
+            perm4(color);
+        }
+    }
+    void zsort() {
+        zsort_set(B, bpos, true);
+        zsort_set(N, pos, false);
+    }
+    void hash_set(HashSet& h, uint32_t n, const float4* p4) {
+        const size_t C = cells();
+        h.start.reserve(C);
+        h.end.reserve(C);
+        if (!h.cells_init) {
+            mps::launch::fill_u32(h.start.p, C, mps::dev::kEmpty, stream);
+            mps::launch::fill_u32(h.end.p, C, mps::dev::kEmpty, stream);
+            h.cells_init = true;
+        } else {
+            mps::launch::clear_cells(h.key.p, h.n_valid, h.start.p, h.end.p, stream);
+        }
+        h.n_valid = 0;
+        if (n == 0) return;
+        h.key.reserve(n); h.id.reserve(n); h.key2.reserve(n); h.id2.reserve(n); h.spos.reserve(n);
+        h.temp.reserve(mps::launch::radix_temp_words(n));
+        mps::launch::cell_keys(p4, n, geom(), 0, h.key.p, h.id.p, stream);
+        mps::launch::radix_sort(h.key.p, h.id.p, h.key2.p, h.id2.p, h.temp.p, n, bits_for(C), stream);
+        // key/key2 (id/id2) are always reserved together, so their capacities are equal and
+        // the pointer swap inside radix_sort keeps each DBuf consistent.
+        mps::launch::cell_table(h.key.p, h.id.p, n, h.start.p, h.end.p, p4, h.spos.p, stream);
+        h.n_valid = n;
+    }
+    void hash() {
+        if (bnd_dirty || !bh.cells_init) hash_set(bh, B, bpos.p);
+        hash_set(fh, N, pos.p);
+    }
+    HashView view(const HashSet& h, uint32_t n) const { return HashView{h.start.p, h.end.p, h.id.p, h.spos.p, n}; }
+
+    void reset_counters() { mps::launch::zero_words(reinterpret_cast<uint32_t*>(d_ctr), sizeof(NbrCounters) / 4, stream); }
+
+    void neighbors() {
+        const HashGeom g = geom();
+        if (Kff == 0) Kff = std::max<uint32_t>(cfg.nei_capacity, 8);
+        if (Kfb == 0) Kfb = std::max<uint32_t>(cfg.nei_capacity / 2, 8);
+        if (Kbb == 0) Kbb = std::max<uint32_t>(cfg.nei_capacity / 2, 8);
+        for (int attempt = 0; attempt < 8; attempt++) {
+            reset_counters();
+            const bool do_b = bnd_dirty;
+            if (do_b && B) {
+                nbb.reserve(size_t(Kbb) * B);
+                cbb.reserve(B);
+                mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream);
+            }
+            nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
+            cff.reserve(std::max<uint32_t>(N, 1));
+            nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
+            cfb.reserve(std::max<uint32_t>(N, 1));
+            mps::launch::neighbors_fluid(pos.p, N, g, view(fh, N), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
+                                         d_ctr, stream);
+            HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
+            sync();
+            bool grow = false;
+            if (h_ctr->max_ff > Kff) { Kff = (h_ctr->max_ff + 15u) & ~15u; grow = true; }
+            if (h_ctr->max_fb > Kfb) { Kfb = (h_ctr->max_fb + 15u) & ~15u; grow = true; }
+            if (do_b && h_ctr->max_bb > Kbb) { Kbb = (h_ctr->max_bb + 15u) & ~15u; grow = true; }
+            if (!grow) {
+                if (do_b) {
+                    nnz_bb = h_ctr->nnz_bb;
+                    bS.reserve(std::max<uint32_t>(B, 1));
+                    mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, cbb.p, bS.p, stream);
+                    bnd_dirty = false;
+                }
+                nnz_ff = h_ctr->nnz_ff;
+                nnz_fb = h_ctr->nnz_fb;
+                return;
+            }
+            nei_regrow++;
+        }
+        throw ApiError{MPS_ERR_STATE, "neighbour capacity did not converge"};
+    }
+    unsigned long long nnz_ff = 0, nnz_fb = 0, nnz_bb = 0;
+
+    void boundary_volume() { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); }
+    void density_alpha(int flags) {
+        mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p, stream);
+    }
+
+    void fetch_solver(int idx) {
+        HIPCHK(hipMemcpyAsync(&h_solver[idx], &d_solver[idx], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+    }
+    // DF / CD loop (MPSSPHDFSPHUtil.cu:270-441)
+    uint32_t pressure_solve(bool df, mps_step_stats* st) {
+        const int idx = df ? 0 : 1;
+        if (N == 0) return 0;
+        SolverDev* sd = &d_solver[idx];
+        mps::launch::solver_reset(sd, stream);
+        ensure_partials();
+        const StepParams P = params();
+        const double tol = df ? cfg.df_tol : cfg.cd_tol;
+        const uint32_t lmin = df ? cfg.df_min_iter : cfg.cd_min_iter;
+        const uint32_t cap = cfg.max_iter;
+        uint32_t it = 1;
+        bool done = false;
+        while (it <= cap && !done) {
+            uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[idx], lmin + 1);
+            chunk = std::min(chunk, cap - it + 1);
+            for (uint32_t c = 0; c < chunk; c++, it++) {
+                mps::launch::pressure_eval(cubic(), df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
+                                           kappa.p, delta.p, sd, partials.p, it, tol, lmin, stream);
+                mps::launch::pressure_apply(cubic(), P, nbr(), pos.p, vel.p, bpos.p, bvol.p, kappa.p, sd, stream);
+            }
+            fetch_solver(idx);
+            done = h_solver[idx].done != 0;
+        }
+        const uint32_t l = done ? h_solver[idx].l : cap + 1;
+        pred[idx] = std::max<uint32_t>(l, 1);
+        HIPCHK(hipMemcpyAsync(&h_solver[idx], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
+        sync();
+        if (st) {
+            double* log = df ? st->err_df : st->err_cd;
+            const uint32_t nlog = std::min<uint32_t>(std::min(l, cap), MPS_MAX_ITER_LOG);
+            for (uint32_t k = 0; k < nlog; k++) log[k] = h_solver[idx].log[k];
+            if (df) { st->l_df = l; st->hit_cap_df = l > cap; }
+            else { st->l_cd = l; st->hit_cap_cd = l > cap; }
+        }
+        return l;
+    }
+    // ApplyImplicitViscosityNSurfaceTension + CGSolver (MPSSPHNonPressureUtil.cu:127-188,
+    // MPSSolver.cuh:12-73)
+    uint32_t viscosity_cg(mps_step_stats* st) {
+        if (N == 0) return 0;
+        SolverDev* sd = &d_solver[2];
+        ensure_partials();
+        const StepParams P = params();
+        mps::launch::st_factor(cubic(), P, nbr(), pos.p, vel.p, fst.p, cgx.p, stream);
+        mps::launch::st_rhs(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, cgb.p, stream);
+        mps::launch::solver_reset(sd, stream);
+        const double tol = cfg.cg_tol;
+        mps::launch::cg_apply(cubic(), 0, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgx.p, cgAp.p, cgb.p, cgr.p, cgp.p, sd,
+                              partials.p, 0, tol, stream);
+        const uint32_t cap = cfg.cg_max_iter;
+        uint32_t it = 1;
+        bool done = false;
+        while (it <= cap && !done) {
+            uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
+            chunk = std::min(chunk, cap - it + 1);
+            for (uint32_t c = 0; c < chunk; c++, it++) {
+                mps::launch::cg_apply(cubic(), 1, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgp.p, cgAp.p, cgb.p, cgr.p, cgp.p,
+                                      sd, partials.p, it, tol, stream);
+                mps::launch::cg_update(N, cgx.p, cgr.p, cgp.p, cgAp.p, sd, partials.p, it, tol, stream);
+                mps::launch::cg_direction(N, cgr.p, cgp.p, sd, stream);
+            }
+            fetch_solver(2);
+            done = h_solver[2].done != 0;
+        }
+        const uint32_t l = done ? h_solver[2].l : cap + 1;
+        pred[2] = std::max<uint32_t>(l, 1);
+        mps::launch::copy_vel_out(N, cgx.p, vel.p, stream);
+        HIPCHK(hipMemcpyAsync(&h_solver[2], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
+        sync();
+        if (st) {
+            const uint32_t nlog = std::min<uint32_t>(std::min(l, cap), MPS_MAX_ITER_LOG);
+            for (uint32_t k = 0; k < nlog; k++) st->rr_cg[k] = h_solver[2].log[k];
+            st->l_cg = l;
+            st->hit_cap_cg = l > cap;
+        }
+        return l;
+    }
+    // ApplyImplicitJacobiViscosity (SPHUtil.cu:344-453, commented out in the reference)
+    uint32_t viscosity_jacobi(mps_step_stats* st) {
+        if (N == 0) return 0;
+        ensure_partials();
+        const StepParams P = params();
+        jprev.reserve(N); jcand.reserve(N);
+        for (auto& t : jt) t.reserve(N);
+        HIPCHK(hipMemcpyAsync(cgx.p, vel.p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(jprev.p, vel.p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
+        const double underRelax = 0.9, delta_ = 0.005;
+        double omega = 1.0, rhoS = 0.995;
+        double om[3] = {1.0, 1.0, 1.0};
+        const uint32_t cap = cfg.jacobi_max_iter;
+        uint32_t l = 0;
+        bool conv = false;
+        while (l < cap) {
+            uint32_t minID = 0;
+            double errs[3] = {0, 0, 0};
+            if (l < 10u) {
+                mps::launch::jacobi_candidate(cubic(), P, nbr(), pos.p, vel.p, rho.p, cgx.p, jt[0].p, stream);
+                mps::launch::jacobi_error(cubic(), P, nbr(), pos.p, vel.p, rho.p, jt[0].p, jt[0].p, jt[0].p, 1, partials.p,
+                                          d_err3, stream);
+                HIPCHK(hipMemcpyAsync(h_err3, d_err3, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+                sync();
+                errs[0] = h_err3[0];
+            } else {
+                if (l == 10u) {
+                    om[0] = 2.0 / (2.0 - rhoS * rhoS);
+                    om[1] = 2.0 / (2.0 - (rhoS + delta_) * (rhoS + delta_));
+                    om[2] = 2.0 / (2.0 - (rhoS - delta_) * (rhoS - delta_));
+                } else {
+                    om[0] = 4.0 / (4.0 - rhoS * rhoS * omega);
+                    om[1] = 4.0 / (4.0 - (rhoS + delta_) * (rhoS + delta_) * omega);
+                    om[2] = 4.0 / (4.0 - (rhoS - delta_) * (rhoS - delta_) * omega);
+                }
+                const float w3[3] = {float(om[0]), float(om[1]), float(om[2])};
+                mps::launch::jacobi_candidate(cubic(), P, nbr(), pos.p, vel.p, rho.p, cgx.p, jcand.p, stream);
+                mps::launch::jacobi_relax(N, jcand.p, cgx.p, jprev.p, float(underRelax), w3, jt[0].p, jt[1].p, jt[2].p, stream);
+                mps::launch::jacobi_error(cubic(), P, nbr(), pos.p, vel.p, rho.p, jt[0].p, jt[1].p, jt[2].p, 3, partials.p,
+                                          d_err3, stream);
+                HIPCHK(hipMemcpyAsync(h_err3, d_err3, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+                sync();
+                for (int k = 0; k < 3; k++) errs[k] = h_err3[k];
+                if (errs[minID] > errs[1]) minID = 1;
+                if (errs[minID] > errs[2]) minID = 2;
+                if (minID == 1) rhoS += delta_;
+                else if (minID == 2) rhoS -= delta_;
+                rhoS = std::min(std::max(0.5 + delta_, rhoS), 1.0 - delta_);
+                omega = double(float(om[minID]));
+            }
+            // ApplyJacobiViscosity_kernel: previous <- current, current <- chosen candidate
+            std::swap(jprev.p, cgx.p); std::swap(jprev.cap, cgx.cap);
+            HIPCHK(hipMemcpyAsync(cgx.p, jt[minID].p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
+            const double e = errs[minID] * (double(float(phys.dt)) / double(N));
+            if (st && l < MPS_MAX_ITER_LOG) st->rr_cg[l] = e;
+            if (e < cfg.jacobi_tol) { conv = true; break; }
+            l++;
+        }
+        mps::launch::copy_vel_out(N, cgx.p, vel.p, stream);
+        if (st) {
+            st->l_cg = l;
+            st->hit_cap_cg = conv ? 0u : 1u;
+        }
+        return l;
+    }
+    void viscosity(mps_step_stats* st) {
+        if (cfg.visc_solver == MPS_VISC_CG) viscosity_cg(st);
+        else if (cfg.visc_solver == MPS_VISC_JACOBI) viscosity_jacobi(st);
+    }
+    void advect() { mps::launch::advect(params(), pos.p, vel.p, d_flags, stream); }
+
+    void run_phase(int32_t ph, mps_step_stats* st) {
+        switch (ph) {
+            case MPS_PHASE_ZSORT: zsort(); break;
+            case MPS_PHASE_HASH: hash(); break;
+            case MPS_PHASE_NEIGHBORS: neighbors(); break;
+            case MPS_PHASE_BOUNDARY_VOLUME: boundary_volume(); break;
+            case MPS_PHASE_DENSITY: density_alpha(mps::launch::kDoDensity); break;
+            case MPS_PHASE_ALPHA: density_alpha(mps::launch::kDoAlpha); break;
+            case MPS_PHASE_GRAVITY: density_alpha(mps::launch::kDoGravity); break;
+            case MPS_PHASE_DIVERGENCE: pressure_solve(true, st); break;
+            case MPS_PHASE_VISCOSITY: viscosity(st); break;
+            case MPS_PHASE_CONSTANT_DENSITY: pressure_solve(false, st); break;
+            case MPS_PHASE_ADVECT: advect(); break;
+            default: throw ApiError{MPS_ERR_INVALID, "unknown phase"};
+        }
+    }
+    void step(mps_step_stats* st) {
+        mps_step_stats local;
+        if (!st) st = &local;
+        std::memset(st, 0, sizeof(*st));
+        st->frame = frame;
+        st->n_fluid = N;
+        st->n_boundary = B;
+        const uint32_t regrow0 = nei_regrow;
+        mps::launch::zero_words(d_flags, 1, stream);
+        if (cfg.zsort_interval && (frame % cfg.zsort_interval) == 1u) {
+            zsort();
+            st->zsorted = 1;
+        }
+        hash();
+        neighbors();
+        boundary_volume();
+        density_alpha(mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoGravity);
+        pressure_solve(true, st);
+        viscosity(st);
+        pressure_solve(false, st);
+        advect();
+        HIPCHK(hipMemcpyAsync(h_flags, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        st->nonfinite = h_flags[0] ? 1u : 0u;
+        st->nnz_ff = nnz_ff;
+        st->nnz_fb = nnz_fb;
+        st->nnz_bb = nnz_bb;
+        st->nei_regrow = nei_regrow - regrow0;
+        frame++;
+    }
+
+    // ------------------------------------------------------------------ setup
+    void alloc_fluid(uint32_t n) {
+        const size_t m = std::max<uint32_t>(n, 1);
+        pos.reserve(m); vel.reserve(m); color.reserve(m); cgb.reserve(m); cgx.reserve(m); cgr.reserve(m); cgp.reserve(m);
+        cgAp.reserve(m); rho.reserve(m); alpha.reserve(m); kappa.reserve(m); delta.reserve(m); fst.reserve(m);
+        cff.reserve(m); cfb.reserve(m);
+    }
+    void alloc_boundary(uint32_t n) {
+        const size_t m = std::max<uint32_t>(n, 1);
+        bpos.reserve(m); bvel.reserve(m); bcolor.reserve(m); bvol.reserve(m); bS.reserve(m); bface.reserve(m); bbcc.reserve(m);
+        cbb.reserve(m);
+    }
+    void upload_fluid(size_t n, const float* p, const float* v, const float* r, const float* m) {
+        if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
+        alloc_fluid(uint32_t(n));
+        std::vector<float4> hp(n), hv(n), hc(n, make_float4(0.f, 0.f, 1.f, 1.f));
+        for (size_t i = 0; i < n; i++) {
+            hp[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], r[i]);
+            hv[i] = make_float4(v ? v[3 * i] : 0.f, v ? v[3 * i + 1] : 0.f, v ? v[3 * i + 2] : 0.f, m ? m[i] : float(mMass));
+        }
+        N = uint32_t(n);
+        if (n) {
+            HIPCHK(hipMemcpyAsync(pos.p, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(vel.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(color.p, hc.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(rho.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(alpha.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(kappa.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(delta.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(fst.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(cgb.p), 4 * n, stream);
+            mps::launch::zero_words(cff.p, n, stream);
+            mps::launch::zero_words(cfb.p, n, stream);
+        }
+        fh.cells_init = false;
+        fh.n_valid = 0;
+        frame = 0;
+        pred[0] = 2; pred[1] = 3; pred[2] = 8;
+        sync();
+    }
+    void upload_boundary(size_t n, const float* p, const float* r, const float* m) {
+        if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
+        alloc_boundary(uint32_t(n));
+        std::vector<float4> hp(n), hv(n);
+        for (size_t i = 0; i < n; i++) {
+            hp[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], r[i]);
+            hv[i] = make_float4(0.f, 0.f, 0.f, m ? m[i] : 0.f);
+        }
+        B = uint32_t(n);
+        if (n) {
+            HIPCHK(hipMemcpyAsync(bpos.p, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipMemcpyAsync(bvel.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bcolor.p), 4 * n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bvol.p), n, stream);
+            mps::launch::zero_words(bface.p, n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bbcc.p), 2 * n, stream);
+            mps::launch::zero_words(cbb.p, n, stream);
+        }
+        bh.cells_init = false;
+        bh.n_valid = 0;
+        bnd_dirty = true;
+        sync();
+    }
+    // Face ownership bits: RTriangleBuild_kernel over per-vertex face lists sorted by face id
+    // (MPSMeshObject.cu:178-237, MPSMeshObject.cuh:41-101). Integer setup work on the host.
+    static void rtri_build(size_t nv, const uint32_t* F, size_t nt, std::vector<uint32_t>& rtri) {
+        std::vector<uint32_t> idx(nv + 1, 0u);
+        for (size_t t = 0; t < 3 * nt; t++) idx[F[t] + 1]++;
+        for (size_t v = 0; v < nv; v++) idx[v + 1] += idx[v];
+        std::vector<uint32_t> nb(idx[nv]), fill(idx.begin(), idx.end() - 1);
+        for (size_t t = 0; t < nt; t++)
+            for (int k = 0; k < 3; k++) nb[fill[F[3 * t + k]]++] = uint32_t(t);
+        // faces were appended in increasing id, so each vertex list is already sorted
+        rtri.assign(nt, 0u);
+        for (size_t id = 0; id < nt; id++) {
+            const uint32_t* face = F + 3 * id;
+            uint32_t info = 0u;
+            uint32_t js = idx[face[0]], je = idx[face[0] + 1];
+            for (uint32_t i = 0, j = 1; i < 3u; i++, j = (j == 2u ? 0u : j + 1u)) {
+                uint32_t ino = js;
+                const uint32_t ie = je;
+                uint32_t itri = nb[ino++];
+                if (itri == id) info |= 1u << i;
+                js = idx[face[j]];
+                je = idx[face[j] + 1];
+                uint32_t of = mps::dev::kEmpty;
+                for (uint32_t q = js; q < je; q++)
+                    if (nb[q] == itri && id != itri) { of = itri; break; }
+                for (; ino < ie && of == mps::dev::kEmpty; ino++) {
+                    itri = nb[ino];
+                    for (uint32_t q = js; q < je; q++)
+                        if (nb[q] == itri && id != itri) { of = itri; break; }
+                }
+                if (id < of) info |= 1u << (i + 3u);
+            }
+            rtri[id] = info;
+        }
+    }
+    void sample_mesh(const double* verts, size_t nv, const uint32_t* tris, size_t nt, const double* center,
+                     const double* size, const mps_mesh_material* m) {
+        if (!verts || !tris || nv == 0 || nt == 0) throw ApiError{MPS_ERR_INVALID, "empty mesh"};
+        for (size_t i = 0; i < 3 * nt; i++)
+            if (tris[i] >= nv) throw ApiError{MPS_ERR_INVALID, "triangle index out of range"};
+        std::vector<double> V(verts, verts + 3 * nv);
+        if (center && size) {  // Translate with AABB::AddPoint's else-if (MPSAABB.h:20-28, MPSMeshObject.cu:130-151)
+            double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+            double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+            for (size_t i = 0; i < nv; i++)
+                for (int k = 0; k < 3; k++) {
+                    const double q = verts[3 * i + k];
+                    if (mn[k] > q) mn[k] = q;
+                    else if (mx[k] < q) mx[k] = q;
+                }
+            for (int k = 0; k < 3; k++) {
+                const double oc = (mn[k] + mx[k]) * 0.5, os = mx[k] - mn[k];
+                const double sc = os > 1.0e-10 ? size[k] / os : 0.0;
+                for (size_t i = 0; i < nv; i++) V[3 * i + k] = (V[3 * i + k] - oc) * sc + center[k];
+            }
+        }
+        double area = 0.0;  // InitFaceInfo (MPSMeshObject.cu:153-175)
+        for (size_t t = 0; t < nt; t++) {
+            const uint32_t* f = tris + 3 * t;
+            const double ax = V[3 * f[1]] - V[3 * f[0]], ay = V[3 * f[1] + 1] - V[3 * f[0] + 1], az = V[3 * f[1] + 2] - V[3 * f[0] + 2];
+            const double bx = V[3 * f[2]] - V[3 * f[0]], by = V[3 * f[2] + 1] - V[3 * f[0] + 1], bz = V[3 * f[2] + 2] - V[3 * f[0] + 2];
+            const double cx = ay * bz - by * az, cy = az * bx - bz * ax, cz = ax * by - bx * ay;
+            area += 0.5 * std::sqrt((cx * cx + cy * cy) + cz * cz);
+        }
+        std::vector<double> vmass(nv, area * m->density / double(nv));
+        std::vector<uint32_t> rtri;
+        rtri_build(nv, tris, nt, rtri);
+        DBuf<double> dV, dmass, dpos;
+        DBuf<uint32_t> dF, dR, dcnt, dface;
+        dV.reserve(3 * nv); dmass.reserve(nv); dF.reserve(3 * nt); dR.reserve(nt); dcnt.reserve(nt + 1);
+        HIPCHK(hipMemcpyAsync(dV.p, V.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dmass.p, vmass.data(), nv * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dF.p, tris, 3 * nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dR.p, rtri.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        mps::launch::sample_count(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, stream);
+        std::vector<uint32_t> cnt(nt + 1, 0u);
+        HIPCHK(hipMemcpyAsync(cnt.data(), dcnt.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<uint32_t> offs(nt + 1, 0u);
+        for (size_t t = 0; t < nt; t++) offs[t + 1] = offs[t] + cnt[t];
+        const uint32_t n = offs[nt];
+        HIPCHK(hipMemcpyAsync(dcnt.p, offs.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        dpos.reserve(3 * std::max<uint32_t>(n, 1));
+        dface.reserve(std::max<uint32_t>(n, 1));
+        mps::launch::zero_words(d_flags + 1, 1, stream);
+        mps::launch::fill_u32(dface.p, std::max<uint32_t>(n, 1), 0xFFFFFFFFu, stream);
+        mps::launch::sample_generate(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, dpos.p, dface.p, d_flags + 1, stream);
+        alloc_boundary(n);
+        B = n;
+        mps::launch::sample_finish(dV.p, dF.p, uint32_t(nt), dmass.p, n, dpos.p, dface.p, m->radius, bpos.p, bvel.p, bbcc.p, stream);
+        if (n) {
+            HIPCHK(hipMemcpyAsync(bface.p, dface.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bcolor.p), 4 * size_t(n), stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bvol.p), n, stream);
+            mps::launch::zero_words(cbb.p, n, stream);
+        }
+        HIPCHK(hipMemcpyAsync(h_flags + 1, d_flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        dV.release(); dmass.release(); dpos.release(); dF.release(); dR.release(); dcnt.release(); dface.release();
+        bh.cells_init = false;
+        bh.n_valid = 0;
+        bnd_dirty = true;
+        if (h_flags[1]) throw ApiError{MPS_ERR_STATE, "sample count mismatch between count and generate passes"};
+    }
+
+    // ------------------------------------------------------------------ array access
+    template <class T>
+    std::vector<T> down(const T* p, size_t n) {
+        std::vector<T> h(n);
+        if (n) {
+            HIPCHK(hipMemcpyAsync(h.data(), p, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+            sync();
+        }
+        return h;
+    }
+    template <class T>
+    void up(T* p, const std::vector<T>& h) {
+        if (!h.empty()) {
+            HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+            sync();
+        }
+    }
+    std::vector<uint32_t> csr_idx(const uint32_t* cnt, uint32_t n) {
+        std::vector<uint32_t> c = down(cnt, n), idx(size_t(n) + 1, 0u);
+        for (uint32_t i = 0; i < n; i++) idx[i + 1] = idx[i] + c[i];
+        return idx;
+    }
+    std::vector<uint32_t> csr_nei(const uint32_t* nbrs, const uint32_t* cnt, uint32_t n, uint32_t K) {
+        std::vector<uint32_t> c = down(cnt, n);
+        std::vector<uint32_t> ell = down(nbrs, size_t(K) * n);
+        std::vector<uint32_t> out;
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t k = 0; k < c[i]; k++) out.push_back(ell[size_t(k) * n + i]);
+        return out;
+    }
+    // returns (count, elem_bytes) and optionally the data as bytes
+    void array_io(int32_t id, size_t* count, size_t* eb, void* dst, const void* src, size_t bytes) {
+        auto f4xyz = [&](DBuf<float4>& a, uint32_t n) {
+            *count = n;
+            *eb = 12;
+            if (dst || src) {
+                if (bytes != size_t(n) * 12) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+                std::vector<float4> h = down(a.p, n);
+                if (dst) {
+                    float* d = static_cast<float*>(dst);
+                    for (uint32_t i = 0; i < n; i++) { d[3 * i] = h[i].x; d[3 * i + 1] = h[i].y; d[3 * i + 2] = h[i].z; }
+                } else {
+                    const float* s = static_cast<const float*>(src);
+                    for (uint32_t i = 0; i < n; i++) { h[i].x = s[3 * i]; h[i].y = s[3 * i + 1]; h[i].z = s[3 * i + 2]; }
+                    up(a.p, h);
+                }
+            }
+        };
+        auto f4w = [&](DBuf<float4>& a, uint32_t n) {
+            *count = n;
+            *eb = 4;
+            if (dst || src) {
+                if (bytes != size_t(n) * 4) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+                std::vector<float4> h = down(a.p, n);
+                if (dst) {
+                    float* d = static_cast<float*>(dst);
+                    for (uint32_t i = 0; i < n; i++) d[i] = h[i].w;
+                } else {
+                    const float* s = static_cast<const float*>(src);
+                    for (uint32_t i = 0; i < n; i++) h[i].w = s[i];
+                    up(a.p, h);
+                }
+            }
+        };
+        auto raw = [&](auto* p, size_t n, size_t e) {
+            *count = n;
+            *eb = e;
+            if (dst || src) {
+                if (bytes != n * e) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+                if (n == 0) return;
+                if (dst) HIPCHK(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToHost, stream));
+                else HIPCHK(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, stream));
+                sync();
+            }
+        };
+        auto ro = [&](const std::vector<uint32_t>& v) {
+            *count = v.size();
+            *eb = 4;
+            if (src) throw ApiError{MPS_ERR_INVALID, "array is read-only"};
+            if (dst) {
+                if (bytes != v.size() * 4) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+                std::memcpy(dst, v.data(), bytes);
+            }
+        };
+        auto ro_raw = [&](const uint32_t* p, size_t n) {
+            if (src) throw ApiError{MPS_ERR_INVALID, "array is read-only"};
+            raw(const_cast<uint32_t*>(p), n, 4);
+        };
+        switch (id) {
+            case MPS_ARR_FLUID_POS: f4xyz(pos, N); break;
+            case MPS_ARR_FLUID_VEL: f4xyz(vel, N); break;
+            case MPS_ARR_FLUID_RADIUS: f4w(pos, N); break;
+            case MPS_ARR_FLUID_MASS: f4w(vel, N); break;
+            case MPS_ARR_FLUID_COLOR: raw(color.p, N, 16); break;
+            case MPS_ARR_FLUID_DENSITY: raw(rho.p, N, 4); break;
+            case MPS_ARR_FLUID_ALPHA: raw(alpha.p, N, 4); break;
+            case MPS_ARR_FLUID_KAPPA: raw(kappa.p, N, 4); break;
+            case MPS_ARR_FLUID_DELTA: raw(delta.p, N, 4); break;
+            case MPS_ARR_FLUID_FACTOR_ST: raw(fst.p, N, 4); break;
+            case MPS_ARR_FLUID_CG_B: f4xyz(cgb, N); break;
+            case MPS_ARR_BND_POS:
+                f4xyz(bpos, B);
+                if (src) bnd_dirty = true;
+                break;
+            case MPS_ARR_BND_RADIUS:
+                f4w(bpos, B);
+                if (src) bnd_dirty = true;
+                break;
+            case MPS_ARR_BND_VOLUME: raw(bvol.p, B, 4); break;
+            case MPS_ARR_BND_MASS: f4w(bvel, B); break;
+            case MPS_ARR_BND_VEL: f4xyz(bvel, B); break;
+            case MPS_ARR_BND_FACE_ID: raw(bface.p, B, 4); break;
+            case MPS_ARR_FHASH_KEY: ro_raw(fh.key.p, fh.n_valid); break;
+            case MPS_ARR_FHASH_ID: ro_raw(fh.id.p, fh.n_valid); break;
+            case MPS_ARR_FHASH_START: ro_raw(fh.start.p, fh.cells_init ? cells() : 0); break;
+            case MPS_ARR_FHASH_END: ro_raw(fh.end.p, fh.cells_init ? cells() : 0); break;
+            case MPS_ARR_BHASH_KEY: ro_raw(bh.key.p, bh.n_valid); break;
+            case MPS_ARR_BHASH_ID: ro_raw(bh.id.p, bh.n_valid); break;
+            case MPS_ARR_BHASH_START: ro_raw(bh.start.p, bh.cells_init ? cells() : 0); break;
+            case MPS_ARR_BHASH_END: ro_raw(bh.end.p, bh.cells_init ? cells() : 0); break;
+            case MPS_ARR_NEI_FF_IDX: ro(csr_idx(cff.p, N)); break;
+            case MPS_ARR_NEI_FF: ro(csr_nei(nff.p, cff.p, N, Kff)); break;
+            case MPS_ARR_NEI_FB_IDX: ro(csr_idx(cfb.p, N)); break;
+            case MPS_ARR_NEI_FB: ro(csr_nei(nfb.p, cfb.p, N, Kfb)); break;
+            case MPS_ARR_NEI_BB_IDX: ro(csr_idx(cbb.p, B)); break;
+            case MPS_ARR_NEI_BB: ro(csr_nei(nbb.p, cbb.p, B, Kbb)); break;
+            default: throw ApiError{MPS_ERR_INVALID, "unknown array id"};
+        }
+    }
+
+    ~mps_ctx() {
+        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release();
+        cgAp.release(); jprev.release(); for (auto& t : jt) t.release(); jcand.release(); tmp4.release();
+        rho.release(); alpha.release(); kappa.release(); delta.release(); fst.release(); tmp1.release();
+        bpos.release(); bvel.release(); bcolor.release(); bvol.release(); bS.release(); bface.release(); bbcc.release();
+        fh.release(); bh.release();
+        zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
+        nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
+        rb_xyz.release(); rb_rad.release(); rb_rgba.release();
+        if (d_ctr) (void)hipFree(d_ctr);
+        if (d_solver) (void)hipFree(d_solver);
+        if (d_flags) (void)hipFree(d_flags);
+        if (d_err3) (void)hipFree(d_err3);
+        if (h_ctr) (void)hipHostFree(h_ctr);
+        if (h_solver) (void)hipHostFree(h_solver);
+        if (h_flags) (void)hipHostFree(h_flags);
+        if (h_err3) (void)hipHostFree(h_err3);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+template <class F>
+int guarded(mps_ctx* c, F&& f) {
+    if (!c) return MPS_ERR_INVALID;
+    try {
+        HIPCHK(hipSetDevice(c->device));
+        f();
+        c->err.clear();
+        return MPS_OK;
+    } catch (const HipError& e) {
+        c->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return MPS_ERR_HIP;
+    } catch (const ApiError& e) {
+        c->err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        c->err = "host allocation failed";
+        return MPS_ERR_NOMEM;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char* mps_version(void) { return "mps_sph 0.1 (gfx950)"; }
+
+int mps_config_init(mps_config* cfg) {
+    if (!cfg) return MPS_ERR_INVALID;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->kernel = MPS_KERNEL_POLY6_SPIKY;       // USE_CUBIC_KERNEL 0 (MPSSPHKernel.cuh:5)
+    cfg->visc_solver = MPS_VISC_CG;             // MPSSystem.cpp:266
+    cfg->faithful_quirks = 1;
+    cfg->device = 0;
+    cfg->hash_size[0] = cfg->hash_size[1] = cfg->hash_size[2] = 64;  // MPSSystem.cpp:72
+    cfg->cell_size = 0.0;                       // = material radius (MPSSystem.cpp:71)
+    cfg->df_tol = 1.0e-3;                       // MPSSPHDFSPHUtil.cu:419
+    cfg->df_min_iter = 1;
+    cfg->cd_tol = 1.0e-4;                       // MPSSPHDFSPHUtil.cu:333
+    cfg->cd_min_iter = 2;
+    cfg->max_iter = 100;                        // MPSSPHDFSPHUtil.cu:299,405
+    cfg->cg_tol = 1.0e-2;                       // MPSSPHNonPressureUtil.cu:159
+    cfg->cg_max_iter = 100;                     // MPSSolver.cuh:31
+    cfg->cg_factor = 0.01;                      // MPSSPHNonPressureUtil.cu:166-169
+    cfg->jacobi_max_iter = 100;                 // SPHUtil.cu:373
+    cfg->jacobi_tol = 1.0e-4;                   // SPHUtil.cu:436-437
+    cfg->zsort_interval = 50;                   // MPSSystem.cpp:220
+    cfg->nei_capacity = 64;
+    cfg->host_sync_every = 0;
+    return MPS_OK;
+}
+
+int mps_create(const mps_config* cfg, mps_ctx** out) {
+    if (!cfg || !out) return MPS_ERR_INVALID;
+    for (int k = 0; k < 3; k++)
+        if (cfg->hash_size[k] == 0 || (cfg->hash_size[k] & (cfg->hash_size[k] - 1))) return MPS_ERR_INVALID;
+    if (size_t(cfg->hash_size[0]) * cfg->hash_size[1] * cfg->hash_size[2] > (size_t(1) << 31)) return MPS_ERR_INVALID;
+    *out = nullptr;
+    mps_ctx* c = new (std::nothrow) mps_ctx;
+    if (!c) return MPS_ERR_NOMEM;
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    c->mat = mps_sph_material{0.01, 1.0, 0.025, 0.1, 100.0};  // SPHMaterial() defaults (MPSSPHMaterial.cpp:4-11)
+    c->mV = c->mat.radius * c->mat.radius * c->mat.radius / 48.0 * 3.141592;
+    c->mMass = c->mat.density * c->mV;
+    const int rc = guarded(c, [&] {
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_ctr), sizeof(NbrCounters)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_solver), 3 * sizeof(SolverDev)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_flags), 4 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_err3), 4 * sizeof(double)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ctr), sizeof(NbrCounters)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_solver), 3 * sizeof(SolverDev)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 4 * sizeof(uint32_t)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_err3), 4 * sizeof(double)));
+        std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
+        mps::launch::zero_words(c->d_flags, 4, c->stream);
+        for (int k = 0; k < 3; k++) mps::launch::solver_reset(&c->d_solver[k], c->stream);
+        c->sync();
+    });
+    if (rc != MPS_OK) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MPS_OK;
+}
+
+void mps_destroy(mps_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+const char* mps_last_error(const mps_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mps_set_physics(mps_ctx* c, const mps_physics* p) {
+    if (!c || !p) return MPS_ERR_INVALID;
+    if (!(p->dt > 0.0)) {
+        c->err = "dt must be positive";
+        return MPS_ERR_INVALID;
+    }
+    c->phys = *p;
+    return MPS_OK;
+}
+
+// SPHMaterial::SetParam (MPSSPHMaterial.cpp:13-19): volume = R^3/48*3.141592, mass = rho0*volume
+int mps_set_fluid_material(mps_ctx* c, const mps_sph_material* m) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    if (!(m->radius > 0.0) || !(m->density > 0.0)) {
+        c->err = "material radius and density must be positive";
+        return MPS_ERR_INVALID;
+    }
+    c->mat = *m;
+    c->mV = m->radius * m->radius * m->radius / 48.0 * 3.141592;
+    c->mMass = m->density * c->mV;
+    return MPS_OK;
+}
+
+int mps_upload_fluid(mps_ctx* c, size_t n, const float* pos, const float* vel, const float* radius, const float* mass) {
+    if (!c || (n && (!pos || !radius))) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->upload_fluid(n, pos, vel, radius, mass); });
+}
+
+int mps_upload_boundary(mps_ctx* c, size_t n, const float* pos, const float* radius, const float* mass) {
+    if (!c || (n && (!pos || !radius))) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->upload_boundary(n, pos, radius, mass); });
+}
+
+int mps_sample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const uint32_t* tris, size_t nt,
+                             const double* center, const double* size, const mps_mesh_material* m) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->sample_mesh(verts, nv, tris, nt, center, size, m); });
+}
+
+int mps_step(mps_ctx* c, mps_step_stats* out) {
+    if (!c) return MPS_ERR_INVALID;
+    int rc = guarded(c, [&] { c->step(out); });
+    if (rc == MPS_OK && out && out->nonfinite) {
+        c->err = "non-finite position or velocity after the step";
+        rc = MPS_ERR_NONFINITE;
+    }
+    return rc;
+}
+
+int mps_run_phase(mps_ctx* c, int32_t phase, mps_step_stats* out) {
+    if (!c) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        c->run_phase(phase, out);
+        c->sync();
+    });
+}
+
+int mps_synchronize(mps_ctx* c) {
+    return guarded(c, [&] { c->sync(); });
+}
+
+int mps_array_info(mps_ctx* c, int32_t id, size_t* count, size_t* elem_bytes) {
+    size_t n = 0, e = 0;
+    const int rc = guarded(c, [&] { c->array_io(id, &n, &e, nullptr, nullptr, 0); });
+    if (count) *count = n;
+    if (elem_bytes) *elem_bytes = e;
+    return rc;
+}
+
+int mps_get_array(mps_ctx* c, int32_t id, void* dst, size_t bytes) {
+    if (!c || (!dst && bytes)) return MPS_ERR_INVALID;
+    size_t n, e;
+    return guarded(c, [&] {
+        if (bytes == 0) {
+            c->array_io(id, &n, &e, nullptr, nullptr, 0);
+            if (n * e != 0) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            return;
+        }
+        c->array_io(id, &n, &e, dst, nullptr, bytes);
+    });
+}
+
+int mps_set_array(mps_ctx* c, int32_t id, const void* src, size_t bytes) {
+    if (!c || (!src && bytes)) return MPS_ERR_INVALID;
+    size_t n, e;
+    return guarded(c, [&] {
+        if (bytes == 0) {
+            c->array_io(id, &n, &e, nullptr, nullptr, 0);
+            if (n * e != 0) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            return;
+        }
+        c->array_io(id, &n, &e, nullptr, src, bytes);
+    });
+}
+
+int mps_get_frame(mps_ctx* c, uint32_t* f) {
+    if (!c || !f) return MPS_ERR_INVALID;
+    *f = c->frame;
+    return MPS_OK;
+}
+
+int mps_set_frame(mps_ctx* c, uint32_t f) {
+    if (!c) return MPS_ERR_INVALID;
+    c->frame = f;
+    return MPS_OK;
+}
+
+int mps_readback_async(mps_ctx* c, float* host_pos, float* host_radius, float* host_rgba, mps_fence* fence) {
+    if (!c || !fence) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        const uint32_t n = c->N;
+        if (host_pos) c->rb_xyz.reserve(3 * size_t(std::max<uint32_t>(n, 1)));
+        if (host_radius) c->rb_rad.reserve(std::max<uint32_t>(n, 1));
+        mps::launch::pack_readback(n, c->pos.p, c->color.p, host_pos ? c->rb_xyz.p : nullptr,
+                                   host_radius ? c->rb_rad.p : nullptr, nullptr, c->stream);
+        if (n) {
+            if (host_pos)
+                HIPCHK(hipMemcpyAsync(host_pos, c->rb_xyz.p, 3 * sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+            if (host_radius)
+                HIPCHK(hipMemcpyAsync(host_radius, c->rb_rad.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+            if (host_rgba)
+                HIPCHK(hipMemcpyAsync(host_rgba, c->color.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+        }
+        hipEvent_t ev;
+        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(ev, c->stream));
+        fence->event = ev;
+        fence->frame = c->frame;
+    });
+}
+
+int mps_fence_wait(mps_ctx* c, mps_fence* fence) {
+    if (!c || !fence) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        if (!fence->event) return;
+        hipEvent_t ev = static_cast<hipEvent_t>(fence->event);
+        HIPCHK(hipEventSynchronize(ev));
+        HIPCHK(hipEventDestroy(ev));
+        fence->event = nullptr;
+    });
+}
+
+int mps_host_alloc(mps_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return MPS_ERR_INVALID;
+    return guarded(c, [&] { HIPCHK(hipHostMalloc(out, bytes)); });
+}
+
+int mps_host_free(mps_ctx* c, void* p) {
+    return guarded(c, [&] {
+        if (p) HIPCHK(hipHostFree(p));
+    });
+}
+
+}  // extern "C"

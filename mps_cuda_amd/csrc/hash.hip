@@ -1,0 +1,315 @@
+// hash.hip — spatial hash and neighbour search for gfx950.
+//
+// Replaces SpatialHash::UpdateHash / ZSort / BuildNeighorhood (MPSSpatialHash.cu:53-218,
+// MPSSpatialHash.cuh:83-271). Design:
+//   * cell keys computed in double exactly like GetGridPos (trunc(x*(1/cell)) + H/2);
+//   * a stable LSD radix sort written here (8-bit digits, wave64 ballot ranking), so ties
+//     keep ascending particle id — the order the reference's neighbour lists depend on;
+//   * cell start/end tables cleared incrementally (only cells occupied last step), plus a
+//     copy of the positions in key order so the 27-cell scan reads contiguous memory;
+//   * one search pass per particle (no count/scan/fill double scan): neighbour ids go to
+//     column-major ELL storage nbr[k*N + i] (a wave reads 256 contiguous bytes per k), the
+//     per-particle count to cnt[i]; overflow past the capacity is counted and reported so
+//     the host can grow the capacity and search again.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mps {
+namespace dev {
+
+// ===================================================================== radix sort
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+
+__global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                           uint32_t* __restrict__ hist, uint32_t nblk) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortRounds; r++) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of a u32 array of length L in one 1024-thread block (L is small:
+// 256 * number of sort tiles).
+__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t* __restrict__ a, uint32_t L) {
+    __shared__ uint32_t s[1024];
+    const uint32_t per = (L + 1023u) / 1024u;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = min(b + per, L);
+    uint32_t sum = 0u;
+    for (uint32_t k = b; k < e; k++) sum += a[k];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t k = b; k < e; k++) {
+        const uint32_t v = a[k];
+        a[k] = run;
+        run += v;
+    }
+}
+
+// Stable scatter: items are ranked round by round (256 per round, wave-major, lane-minor),
+// which preserves input order among equal digits.
+__global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                              uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                              int shift, const uint32_t* __restrict__ offs, uint32_t nblk) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t wcnt[4][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    base[threadIdx.x] = offs[threadIdx.x * nblk + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kSortRounds; r++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) wcnt[k][threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t i = blockIdx.x * kSortTile + r * kSortThreads + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(m & lt);
+        if (valid && rank == 0u) wcnt[w][d] = __popcll(m);
+        __syncthreads();
+        {
+            const uint32_t t = threadIdx.x;
+            const uint32_t c0 = wcnt[0][t], c1 = wcnt[1][t], c2 = wcnt[2][t], c3 = wcnt[3][t];
+            const uint32_t b0 = base[t];
+            wcnt[0][t] = b0;
+            wcnt[1][t] = b0 + c0;
+            wcnt[2][t] = b0 + c0 + c1;
+            wcnt[3][t] = b0 + c0 + c1 + c2;
+            base[t] = b0 + c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = wcnt[w][d] + rank;
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ===================================================================== hashing
+__global__ void __launch_bounds__(kBlock) k_cell_keys(const float4* __restrict__ pos, uint32_t n, HashGeom g, int morton,
+                                                    uint32_t* __restrict__ key, uint32_t* __restrict__ id) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    const i3 c = grid_pos(p.x, p.y, p.z, g);
+    key[i] = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
+    id[i] = i;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Reset only the cells that the previous sorted keys occupied.
+__global__ void __launch_bounds__(kBlock) k_clear_cells(const uint32_t* __restrict__ key, uint32_t n, uint32_t* __restrict__ start,
+                                                      uint32_t* __restrict__ end) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t k = key[s];
+    if (s == 0 || key[s - 1] != k) {
+        start[k] = kEmpty;
+        end[k] = kEmpty;
+    }
+}
+
+// ReorderHash_kernel (MPSSpatialHash.cuh:115-145) + positions in key order.
+__global__ void __launch_bounds__(kBlock) k_cell_table(const uint32_t* __restrict__ key, const uint32_t* __restrict__ id, uint32_t n,
+                                                     uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                     const float4* __restrict__ pos, float4* __restrict__ spos) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t k = key[s];
+    if (s == 0 || key[s - 1] != k) {
+        start[k] = s;
+        if (s > 0) end[key[s - 1]] = s;
+    }
+    if (s == n - 1) end[k] = n;
+    spos[s] = pos[id[s]];
+}
+
+// 27-cell scan (Research, MPSSpatialHash.cuh:47-80): z outer, y, x inner; slots in key order.
+// Neighbour when |x_i - x_j| < (4 r_i + 4 r_j) * 0.5, self excluded when SELF.
+template <bool SELF>
+__device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const HashGeom& g, const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ end, const uint32_t* __restrict__ ids,
+                                                 const float4* __restrict__ spos, uint32_t* __restrict__ out, uint32_t K,
+                                                 uint32_t stride) {
+    const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
+    const float ri = xi.w * 4.0f;
+    uint32_t k = 0u;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const uint32_t c = grid_index(p.x + dx, p.y + dy, p.z + dz, g);
+                uint32_t s = start[c];
+                const uint32_t e = end[c];
+                for (; s < e; s++) {
+                    const float4 q = spos[s];
+                    const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
+                    const float dist = __builtin_sqrtf((ddx * ddx + ddy * ddy) + ddz * ddz);
+                    const float rj = q.w * 4.0f;
+                    if (dist < (ri + rj) * 0.5f) {
+                        const uint32_t j = ids[s];
+                        if (SELF && j == i) continue;
+                        if (k < K) out[size_t(k) * stride + i] = j;
+                        k++;
+                    }
+                }
+            }
+    return k;
+}
+
+__device__ __forceinline__ void report_counts(uint32_t c, uint32_t* maxc, unsigned long long* nnz) {
+    // wave-level max/sum then one atomic per wave
+    uint32_t mx = c;
+    unsigned long long sm = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        sm += __shfl_xor(sm, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(maxc, mx);
+        atomicAdd(nnz, sm);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __restrict__ pos, uint32_t N, HashGeom g,
+                                                          HashView fl, HashView bd, uint32_t* __restrict__ nff, uint32_t Kff,
+                                                          uint32_t* __restrict__ cff, uint32_t* __restrict__ nfb, uint32_t Kfb,
+                                                          uint32_t* __restrict__ cfb, NbrCounters* __restrict__ ctr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t a = 0u, b = 0u;
+    if (i < N) {
+        const float4 xi = pos[i];
+        a = search_cells<true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff, N);
+        if (bd.n) b = search_cells<false>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb, N);
+        cff[i] = a;
+        cfb[i] = b;
+    }
+    report_counts(a, &ctr->max_ff, &ctr->nnz_ff);
+    report_counts(b, &ctr->max_fb, &ctr->nnz_fb);
+}
+
+__global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __restrict__ pos, uint32_t B, HashGeom g, HashView bd,
+                                                             uint32_t* __restrict__ nbb, uint32_t Kbb, uint32_t* __restrict__ cbb,
+                                                             NbrCounters* __restrict__ ctr) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t a = 0u;
+    if (i < B) {
+        a = search_cells<true>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb, B);
+        cbb[i] = a;
+    }
+    report_counts(a, &ctr->max_bb, &ctr->nnz_bb);
+}
+
+// Z-sort payload permutation (MPSSpatialHash.cu:101-140): out[s] = in[perm[s]].
+__global__ void __launch_bounds__(kBlock) k_gather_f4(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ in,
+                                                    float4* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = in[perm[s]];
+}
+__global__ void __launch_bounds__(kBlock) k_gather_u32(const uint32_t* __restrict__ perm, uint32_t n, const uint32_t* __restrict__ in,
+                                                     uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = in[perm[s]];
+}
+__global__ void __launch_bounds__(kBlock) k_gather_f2(const uint32_t* __restrict__ perm, uint32_t n, const float2* __restrict__ in,
+                                                    float2* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = in[perm[s]];
+}
+__global__ void __launch_bounds__(kBlock) k_gather_f1(const uint32_t* __restrict__ perm, uint32_t n, const float* __restrict__ in,
+                                                    float* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = in[perm[s]];
+}
+
+}  // namespace dev
+
+// ===================================================================== host launchers
+namespace launch {
+using namespace dev;
+
+static inline unsigned nb(size_t n, unsigned b = kBlock) { return unsigned((n + b - 1) / b); }
+
+size_t radix_temp_words(uint32_t n) { return size_t(256) * ((n + kSortTile - 1) / kSortTile) + 1; }
+
+int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,
+               int key_bits, hipStream_t s) {
+    if (n == 0) return 0;
+    const uint32_t nblk = (n + kSortTile - 1) / kSortTile;
+    const uint32_t L = 256u * nblk;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        k_radix_hist<<<nblk, kSortThreads, 0, s>>>(keys, n, shift, temp, nblk);
+        k_scan_exclusive<<<1, 1024, 0, s>>>(temp, L);
+        k_radix_scatter<<<nblk, kSortThreads, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, temp, nblk);
+        std::swap(keys, keys_alt);
+        std::swap(vals, vals_alt);
+    }
+    return 0;
+}
+
+void cell_keys(const float4* pos, uint32_t n, const HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s) {
+    if (n) k_cell_keys<<<nb(n), kBlock, 0, s>>>(pos, n, g, morton, key, id);
+}
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (n) k_fill_u32<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n, v);
+}
+void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s) {
+    if (n) k_clear_cells<<<nb(n), kBlock, 0, s>>>(key, n, start, end);
+}
+void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
+                float4* spos, hipStream_t s) {
+    if (n) k_cell_table<<<nb(n), kBlock, 0, s>>>(key, id, n, start, end, pos, spos);
+}
+void neighbors_fluid(const float4* pos, uint32_t N, const HashGeom& g, const HashView& fl, const HashView& bd, uint32_t* nff,
+                     uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr, hipStream_t s) {
+    if (N) k_neighbors_fluid<<<nb(N), kBlock, 0, s>>>(pos, N, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, ctr);
+}
+void neighbors_boundary(const float4* pos, uint32_t B, const HashGeom& g, const HashView& bd, uint32_t* nbb, uint32_t Kbb,
+                        uint32_t* cbb, NbrCounters* ctr, hipStream_t s) {
+    if (B) k_neighbors_boundary<<<nb(B), kBlock, 0, s>>>(pos, B, g, bd, nbb, Kbb, cbb, ctr);
+}
+void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s) {
+    if (n) k_gather_f4<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+}
+void gather_u32(const uint32_t* perm, uint32_t n, const uint32_t* in, uint32_t* out, hipStream_t s) {
+    if (n) k_gather_u32<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+}
+void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, hipStream_t s) {
+    if (n) k_gather_f2<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+}
+void gather_f1(const uint32_t* perm, uint32_t n, const float* in, float* out, hipStream_t s) {
+    if (n) k_gather_f1<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+}
+
+}  // namespace launch
+}  // namespace mps

@@ -1,0 +1,128 @@
+// kernels.hpp — host-side launchers for the gfx950 kernels and the small POD structs
+// they share with the context. No torch types, no CUDA types: plain HIP.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/mps_sph.h"
+
+namespace mps {
+namespace dev {
+struct HashGeom;
+}
+
+// Read-only view of one set's hash (cell table + ids and positions in key order).
+struct HashView {
+    const uint32_t* start;
+    const uint32_t* end;
+    const uint32_t* id;
+    const float4* spos;
+    uint32_t n;
+};
+
+// Neighbour statistics written by the search kernels.
+struct NbrCounters {
+    uint32_t max_ff, max_fb, max_bb, pad;
+    unsigned long long nnz_ff, nnz_fb, nnz_bb;
+};
+
+// Device-resident state of one iterative solve. The last workgroup of each reduction
+// kernel finalises the sums here, so the host only needs to look at `done`/`l`.
+struct SolverDev {
+    uint32_t done;     // loop finished (converged); later kernels of the chunk exit at once
+    uint32_t l;        // reference loop variable when the loop ended (or next iteration)
+    uint32_t counter;  // last-workgroup ticket
+    uint32_t pad;
+    double rr;         // CG: r.r of the current residual
+    double pAp;        // CG: p.Ap
+    float alpha, beta; // CG step scalars (rounded to float like the FP32 reference)
+    double log[MPS_MAX_ITER_LOG];
+};
+
+// Everything a particle kernel needs by value.
+struct StepParams {
+    uint32_t N, B;
+    float V, rho0, mass, mu, xsph, sigma;  // SPH material (MPSSPHMaterial.h:9-21)
+    float dt, gx, gy, gz;
+    float bmin[3], bmax[3];
+    float cg_factor;
+    int faithful;
+};
+
+// Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i].
+struct Nbr {
+    const uint32_t* ff;
+    const uint32_t* cff;
+    const uint32_t* fb;
+    const uint32_t* cfb;
+};
+
+namespace launch {
+// ---- hash.hip
+size_t radix_temp_words(uint32_t n);
+int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,
+               int key_bits, hipStream_t s);
+void cell_keys(const float4* pos, uint32_t n, const dev::HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s);
+void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
+void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
+                float4* spos, hipStream_t s);
+void neighbors_fluid(const float4* pos, uint32_t N, const dev::HashGeom& g, const HashView& fl, const HashView& bd,
+                     uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr,
+                     hipStream_t s);
+void neighbors_boundary(const float4* pos, uint32_t B, const dev::HashGeom& g, const HashView& bd, uint32_t* nbb,
+                        uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s);
+void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s);
+void gather_u32(const uint32_t* perm, uint32_t n, const uint32_t* in, uint32_t* out, hipStream_t s);
+void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, hipStream_t s);
+void gather_f1(const uint32_t* perm, uint32_t n, const float* in, float* out, hipStream_t s);
+
+// ---- sph.hip
+enum DensityFlags { kDoDensity = 1, kDoAlpha = 2, kDoGravity = 4 };
+void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, const uint32_t* cbb, float* S,
+                  hipStream_t s);
+void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
+void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
+                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, hipStream_t s);
+void pressure_eval(int cubic, int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
+                   float* kappa, float* delta, SolverDev* st, double* partials, uint32_t it, double tol,
+                   uint32_t lmin, hipStream_t s);
+void pressure_apply(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+                    const float* bvol, const float* kappa, const SolverDev* st, hipStream_t s);
+void st_factor(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, float* fst,
+               float4* cgx, hipStream_t s);
+void st_rhs(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+            const float* fst, float4* cgb, hipStream_t s);
+// mode 0: A x0 with f = 0, then r = b - A x0, p = r, r.r; mode 1: A p with f, p.Ap, alpha
+void cg_apply(int cubic, int mode, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+              const float* rho, const float* fst, const float4* in, float4* Ap, const float4* b, float4* r, float4* p,
+              SolverDev* st, double* partials, uint32_t it, double tol, hipStream_t s);
+void cg_update(uint32_t N, float4* x, float4* r, const float4* p, const float4* Ap, SolverDev* st, double* partials,
+               uint32_t it, double tol, hipStream_t s);
+void cg_direction(uint32_t N, const float4* r, float4* p, const SolverDev* st, hipStream_t s);
+void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s);
+void jacobi_candidate(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                      const float* rho, const float4* u, float4* out, hipStream_t s);
+void jacobi_relax(uint32_t N, const float4* cand, const float4* cur, const float4* prev, float ur, const float* w3,
+                  float4* o0, float4* o1, float4* o2, hipStream_t s);
+// sums |u - a sum(...) - v0| for up to three candidates into err3 (double, deterministic)
+void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                  const float* rho, const float4* u0, const float4* u1, const float4* u2, int ncand, double* partials,
+                  double* err3, hipStream_t s);
+void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s);
+void solver_reset(SolverDev* st, hipStream_t s);
+void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
+void pack_readback(uint32_t n, const float4* pos, const float4* color, float* xyz, float* rad, float* rgba, hipStream_t s);
+
+// ---- sampling.hip
+void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
+                  hipStream_t s);
+void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+                     const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad, hipStream_t s);
+void sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n, const double* pos,
+                   const uint32_t* face, double radius, float4* bpos, float4* bvel, float2* bbcc, hipStream_t s);
+}  // namespace launch
+}  // namespace mps

@@ -1,0 +1,178 @@
+// sampling.hip — triangle mesh -> Akinci boundary particles on gfx950 (setup path, FP64).
+//
+// Restates kernel::ParticleSampling (MPSParticleSamplingUtil.cu:14-133, .cuh:10-294):
+// per face, owned vertices, owned edges (interior points at spacing l/(n+1) with
+// n = ceil((l - 1.01 d)/d), d = R_m/2) and scan lines parallel to the shortest edge. One
+// thread per face counts, the host scans the counts, one thread per face writes. Face
+// ownership bits come from the host (integer adjacency, see context.cpp).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mps {
+namespace dev {
+
+struct d3 {
+    double x, y, z;
+};
+__device__ __forceinline__ d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ d3 operator*(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ double dotd(d3 a, d3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ double lend(d3 a) { return sqrt(dotd(a, a)); }
+__device__ __forceinline__ d3 crossd(d3 x, d3 y) {
+    return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+__device__ __forceinline__ d3 normalized(d3 v) { return v * (1.0 / sqrt(dotd(v, v))); }
+
+// GetLineSamplingNum (MPSParticleSamplingUtil.cuh:12-18): ceil taken in float.
+__device__ __forceinline__ uint32_t line_num(double l, double d) {
+    const double x = (l - d * 1.01) / d;
+    if (x <= 0.0) return 0u;
+    return static_cast<uint32_t>(ceilf(static_cast<float>(x)));
+}
+
+struct Emit {
+    double* pos;  // 3 per sample
+    uint32_t* face;
+    uint32_t cur, end;
+    __device__ void put(d3 x, uint32_t f) {
+        if (cur < end) {
+            pos[3 * size_t(cur)] = x.x;
+            pos[3 * size_t(cur) + 1] = x.y;
+            pos[3 * size_t(cur) + 2] = x.z;
+            face[cur] = f;
+        }
+        cur++;
+    }
+};
+
+// GenerateLineParticle (:39-67); `d` is a float parameter that receives the new spacing.
+__device__ void gen_line(d3 a, d3 b, float d, uint32_t f, Emit* e, uint32_t* count) {
+    b = b - a;
+    const double l = lend(b);
+    const uint32_t num = line_num(l, double(d));
+    if (count) *count += num;
+    if (!e || num == 0u) return;
+    d = static_cast<float>(l / static_cast<double>(num + 1u));
+    b = b * (double(d) / l);
+    for (uint32_t i = 0; i < num; i++) {
+        a = a + b;
+        e->put(a, f);
+    }
+}
+
+// ComputeSamplingNum_kernel (:69-144) when e == nullptr, GenerateBoundaryParticle_kernel
+// (:146-246) otherwise. The count pass uses the double spacing, the generate pass the
+// float one, as in the reference.
+__device__ void sample_face(const double* V, const uint32_t* F, const uint32_t* R, uint32_t id, double radius, Emit* e,
+                            uint32_t* count) {
+    const uint32_t f0 = F[3 * id], f1 = F[3 * id + 1], f2 = F[3 * id + 2];
+    d3 ns[3] = {{V[3 * f0], V[3 * f0 + 1], V[3 * f0 + 2]}, {V[3 * f1], V[3 * f1 + 1], V[3 * f1 + 2]},
+                {V[3 * f2], V[3 * f2 + 1], V[3 * f2 + 2]}};
+    d3 es[3] = {ns[1] - ns[0], ns[2] - ns[1], ns[0] - ns[2]};
+    const double ls[3] = {lend(es[0]), lend(es[1]), lend(es[2])};
+    uint32_t iShort = 0u;
+    if (ls[iShort] > ls[1]) iShort = 1u;
+    if (ls[iShort] > ls[2]) iShort = 2u;
+    const double d = (radius + radius) * 0.25;
+    const uint32_t rTri = R[id];
+    uint32_t num = 0u;
+    for (uint32_t i = 0; i < 3u; i++) {
+        if ((rTri >> i) & 1u) {
+            num++;
+            if (e) e->put(ns[i], id);
+        }
+        if ((rTri >> (i + 3u)) & 1u) {
+            if (e) gen_line(ns[i], ns[(i + 1u) % 3u], float(d), id, e, nullptr);
+            else num += line_num(ls[i], d);
+        }
+    }
+    const uint32_t ino0 = (iShort + 1u) % 3u, ino1 = (iShort + 2u) % 3u;
+    ns[ino1] = ns[iShort];
+    const d3 s = normalized(crossd(es[iShort], crossd(es[iShort], es[ino1])));
+    es[ino1] = d3{-es[ino1].x, -es[ino1].y, -es[ino1].z};
+    const double l = dotd(s, es[ino0]);
+    const uint32_t nt = line_num(l, d);
+    if (nt > 0u) {
+        const double stride = (l / static_cast<double>(nt + 1u)) / l;
+        for (uint32_t i = 0; i < nt; i++) {
+            ns[ino0] = ns[ino0] + stride * es[ino0];
+            ns[ino1] = ns[ino1] + stride * es[ino1];
+            if (e) gen_line(ns[ino0], ns[ino1], float(d), id, e, nullptr);
+            else num += line_num(lend(ns[ino0] - ns[ino1]), d);
+        }
+    }
+    if (count) *count = num;
+}
+
+__global__ void k_sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+                               uint32_t* counts) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nt) return;
+    uint32_t c = 0;
+    sample_face(V, F, R, id, radius, nullptr, &c);
+    counts[id] = c;
+}
+
+__global__ void k_sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+                                  const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nt) return;
+    Emit e{pos, face, offs[id], offs[id + 1]};
+    sample_face(V, F, R, id, radius, &e, nullptr);
+    if (e.cur != e.end) atomicOr(bad, 1u);
+}
+
+// SetBarycentric_kernel (:248-294) and the float4 boundary record (x, y, z, r = R_m/4).
+__global__ void k_sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n,
+                                const double* pos, const uint32_t* face, double radius, float4* bpos, float4* bvel,
+                                float2* bbcc) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    const uint32_t fi = face[id];
+    if (fi >= nt) return;  // only after a count/generate mismatch, which the host reports
+    const uint32_t f0 = F[3 * fi], f1 = F[3 * fi + 1], f2 = F[3 * fi + 2];
+    const d3 n0{V[3 * f0], V[3 * f0 + 1], V[3 * f0 + 2]}, n1{V[3 * f1], V[3 * f1 + 1], V[3 * f1 + 2]},
+        n2{V[3 * f2], V[3 * f2 + 1], V[3 * f2 + 2]};
+    const d3 x{pos[3 * size_t(id)], pos[3 * size_t(id) + 1], pos[3 * size_t(id) + 2]};
+    const d3 n20 = n0 - n2, n21 = n1 - n2;
+    const double t0 = dotd(n20, n20), t1 = dotd(n21, n21), t2 = dotd(n20, n21);
+    const double t3 = dotd(n20, x - n2), t4 = dotd(n21, x - n2);
+    const double det = t0 * t1 - t2 * t2;
+    double w0 = 0.0, w1 = 0.0;
+    if (fabs(det) > 1.0e-20) {
+        const double invDet = 1.0 / det;
+        w0 = (+t1 * t3 - t2 * t4) * invDet;
+        w1 = (-t2 * t3 + t0 * t4) * invDet;
+    }
+    w0 = fmin(fmax(w0, 0.0), 1.0);
+    w1 = fmin(fmax(w1, 0.0), 1.0);
+    const double m = vmass[f0] * w0 + vmass[f1] * w1 + vmass[f2] * (1.0 - w0 - w1);
+    bpos[id] = make_float4(float(x.x), float(x.y), float(x.z), float(radius * 0.25));
+    bvel[id] = make_float4(0.f, 0.f, 0.f, float(m));
+    bbcc[id] = make_float2(float(w0), float(w1));
+}
+
+}  // namespace dev
+
+namespace launch {
+using namespace dev;
+void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
+                  hipStream_t s) {
+    if (nt) k_sample_count<<<(nt + 63) / 64, 64, 0, s>>>(V, F, R, nt, radius, counts);
+}
+void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+                     const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad, hipStream_t s) {
+    if (nt) k_sample_generate<<<(nt + 63) / 64, 64, 0, s>>>(V, F, R, nt, radius, offs, pos, face, bad);
+}
+void sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n, const double* pos,
+                   const uint32_t* face, double radius, float4* bpos, float4* bvel, float2* bbcc, hipStream_t s) {
+    if (n) k_sample_finish<<<(n + 255) / 256, 256, 0, s>>>(V, F, nt, vmass, n, pos, face, radius, bpos, bvel, bbcc);
+}
+}  // namespace launch
+}  // namespace mps

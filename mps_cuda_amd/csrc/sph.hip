@@ -1,0 +1,777 @@
+// sph.hip — DFSPH / implicit viscosity / advection kernels for gfx950.
+//
+// One thread per particle walks its column-major ELL neighbour lists (fluid then
+// boundary), accumulating in registers in the reference's order, so a per-particle
+// result equals (0 + fluid sum) + boundary sum exactly as the reference's two kernels and
+// AtomicAdd produce it. Passes that the reference splits into several kernels over the
+// same neighbour list are fused (density + alpha + gravity; delta + stiffness + error
+// reduction; pressure force + velocity update; ST + XSPH + viscosity + final A.p).
+//
+// Convergence sums: each workgroup reduces its T-rounded terms in double, writes one
+// partial, and the last workgroup to finish (ticket counter) adds the partials in a fixed
+// order — deterministic, no FP64 atomics — and updates a SolverDev record on the device.
+// Kernels of later iterations read SolverDev::done first and exit at once, so the host can
+// enqueue several iterations before it looks at the result.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mps {
+namespace dev {
+
+__device__ __forceinline__ uint32_t nbr_ff(const Nbr& nb, uint32_t k, uint32_t i, uint32_t N) {
+    return nb.ff[size_t(k) * N + i];
+}
+__device__ __forceinline__ uint32_t nbr_fb(const Nbr& nb, uint32_t k, uint32_t i, uint32_t N) {
+    return nb.fb[size_t(k) * N + i];
+}
+
+// Block-reduce `v` (double), publish the partial, and let the last workgroup fold all
+// partials in index order and call fin(total) in thread 0.
+template <class Fin>
+__device__ __forceinline__ void reduce_last(double v, double* __restrict__ partials, uint32_t* counter, Fin fin) {
+    __shared__ double lds[kBlock / 64];
+    __shared__ int am_last;
+    const double bs = block_sum_d(v, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = bs;
+        __threadfence();
+        const uint32_t t = atomicAdd(counter, 1u);
+        am_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (am_last) {
+        __threadfence();
+        double s = 0.0;
+        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
+            s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = block_sum_d(s, lds);
+        if (threadIdx.x == 0) {
+            fin(s);
+            __threadfence();
+            *counter = 0u;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- boundary volume
+// ComputeBoundaryParticleVolume_kernel (MPSSPHBaseUtil.cuh:10-41): S_b = W(0) + sum W_bk.
+// S_b depends only on boundary positions, so it is computed when they change.
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_boundary_sum(const float4* __restrict__ bpos, uint32_t B, const uint32_t* __restrict__ nbb,
+                                                       const uint32_t* __restrict__ cbb, float* __restrict__ S) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const float4 xi4 = bpos[i];
+    const f3 xi = mk(xi4);
+    const float hi = xi4.w * 4.0f;
+    const float invHi = 1.0f / hi;
+    float vol = Wk<CUBIC>(0.0f, invHi);
+    const uint32_t c = cbb[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbb[size_t(k) * B + i];
+        vol += Wk<CUBIC>(len3(xi - mk(bpos[j])), invHi);
+    }
+    S[i] = vol;
+}
+// ComputeBoundaryParticleVolumeFinal_kernel (:78-86) on top of the AtomicAdd into the
+// never-cleared volume (Q1): V <- 1/(V + S).
+__global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restrict__ S, float* __restrict__ vol, uint32_t B, int faithful) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    float v = faithful ? vol[i] : 0.0f;
+    v += S[i];
+    vol[i] = 1.0f / v;
+}
+
+// ----------------------------------------------------------------- density, alpha, gravity
+// ComputeDensity_kernel ff/fb + Final + DensityColorTest (MPSSPHBaseUtil.cuh:88-221),
+// ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
+// ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams P, Nbr nb, const float4* __restrict__ pos,
+                                                        float4* __restrict__ vel, const float4* __restrict__ bpos,
+                                                        const float* __restrict__ bvol, float* __restrict__ rho,
+                                                        float* __restrict__ alpha, float4* __restrict__ color) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    if (i >= N) return;
+    const bool dens = flags & launch::kDoDensity, alph = flags & launch::kDoAlpha;
+    const float4 xi4 = pos[i];
+    const f3 xi = mk(xi4);
+    const float hi = xi4.w * 4.0f;
+    const float invHi = 1.0f / hi;
+    float sf = dens ? P.V * Wk<CUBIC>(0.0f, invHi) : 0.0f;
+    f3 gf{0.f, 0.f, 0.f};
+    float s = 0.0f;
+    if (dens || alph) {
+        const uint32_t c = nb.cff[i];
+        for (uint32_t k = 0; k < c; k++) {
+            const uint32_t j = nbr_ff(nb, k, i, N);
+            const f3 xij = xi - mk(pos[j]);
+            const float dist = len3(xij);
+            if (dens) sf += P.V * Wk<CUBIC>(dist, invHi);
+            if (alph) {
+                const f3 grad = (P.V * Gk<CUBIC>(dist, invHi)) * xij;
+                gf = gf + grad;
+                s += dot3(grad, grad);
+            }
+        }
+        float sb = 0.0f;
+        f3 gb{0.f, 0.f, 0.f};
+        const uint32_t cb = nb.cfb[i];
+        for (uint32_t k = 0; k < cb; k++) {
+            const uint32_t j = nbr_fb(nb, k, i, N);
+            const f3 xij = xi - mk(bpos[j]);
+            const float dist = len3(xij);
+            const float vb = bvol[j];
+            if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
+            if (alph) gb = gb + (vb * Gk<CUBIC>(dist, invHi)) * xij;
+        }
+        if (dens) {
+            float d = 0.0f;
+            d += sf;
+            d += sb;
+            const float r = d * P.rho0;
+            rho[i] = r;
+            const float ratio = r / P.rho0;
+            const float blue = fminf(fmaxf(1.0f - ratio, 0.0f), 1.0f);
+            const float green = fminf(fmaxf(ratio < 1.0f ? ratio : 1.0f - (ratio - 1.0f), 0.0f), 1.0f);
+            const float red = fminf(fmaxf(ratio - 1.0f, 0.0f), 1.0f);
+            color[i] = make_float4(red, green, blue, 1.0f);
+        }
+        if (alph) {
+            f3 g{0.f, 0.f, 0.f};
+            g = g + gf;
+            g = g + gb;
+            float a = 0.0f;
+            a += s;
+            a += dot3(g, g);
+            alpha[i] = a > kEps ? 1.0f / a : 0.0f;
+        }
+    }
+    if (flags & launch::kDoGravity) {
+        float4 v4 = vel[i];
+        const float m = v4.w;
+        const f3 F = f3{0.f, 0.f, 0.f} + f3{P.gx, P.gy, P.gz} * m;
+        if (m > 1.0e-10f) {
+            const float c = P.dt / m;
+            v4.x = v4.x + c * F.x;
+            v4.y = v4.y + c * F.y;
+            v4.z = v4.z + c * F.z;
+            vel[i] = v4;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- pressure solves
+// ComputeDensityDelta_kernel ff/fb (MPSSPHDFSPHUtil.cuh:139-257) fused with
+// ComputeDFStiffness_kernel / ComputeCDStiffness_kernel (:259-340) and the mean-error
+// readback of the drivers (MPSSPHDFSPHUtil.cu:331-333,417-419).
+template <bool CUBIC, bool DF>
+__global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel, const float4* __restrict__ bpos,
+                                                        const float* __restrict__ bvol, const float4* __restrict__ bvel,
+                                                        const float* __restrict__ rho, const float* __restrict__ alpha,
+                                                        float* __restrict__ kappa, float* __restrict__ delta, SolverDev* st,
+                                                        double* __restrict__ partials, uint32_t it, double tol, uint32_t lmin) {
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    double epos = 0.0;
+    if (i < N) {
+        const float4 xi4 = pos[i];
+        const f3 xi = mk(xi4), vi = mk(vel[i]);
+        const float invHi = 1.0f / (xi4.w * 4.0f);
+        float df = 0.0f;
+        const uint32_t c = nb.cff[i];
+        for (uint32_t k = 0; k < c; k++) {
+            const uint32_t j = nbr_ff(nb, k, i, N);
+            const f3 xij = xi - mk(pos[j]);
+            const f3 vij = vi - mk(vel[j]);
+            df += P.V * Gk<CUBIC>(len3(xij), invHi) * dot3(xij, vij);
+        }
+        float db = 0.0f;
+        const uint32_t cb = nb.cfb[i];
+        for (uint32_t k = 0; k < cb; k++) {
+            const uint32_t j = nbr_fb(nb, k, i, N);
+            const f3 xij = xi - mk(bpos[j]);
+            const f3 vij = vi - mk(bvel[j]);
+            db += bvol[j] * Gk<CUBIC>(len3(xij), invHi) * dot3(xij, vij);
+        }
+        float d = 0.0f;
+        d += df;
+        d += db;
+        delta[i] = d;
+        float e;
+        if (DF) {
+            const float a = d * P.dt;
+            const float b = rho[i] / P.rho0 + P.dt * d - 0.8f;
+            e = a < b ? a : b;
+        } else {
+            e = rho[i] / P.rho0 + P.dt * d - 1.0f;
+        }
+        if (e > 0.0f) {
+            epos = double(e);
+            e *= alpha[i] / (P.dt * P.dt);
+        } else {
+            e = 0.0f;
+        }
+        kappa[i] = e;
+    }
+    reduce_last(epos, partials, &st->counter, [&](double s) {
+        const double E = s / double(N);
+        if (it - 1 < MPS_MAX_ITER_LOG) st->log[it - 1] = E;
+        if (E < tol && it > lmin) {
+            st->done = 1u;
+            st->l = it;
+        } else {
+            st->l = it + 1;
+        }
+    });
+}
+
+// ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_pressure_apply(StepParams P, Nbr nb, const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                         const float4* __restrict__ bpos, const float* __restrict__ bvol,
+                                                         const float* __restrict__ kappa, const SolverDev* st) {
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    if (i >= N) return;
+    const float4 xi4 = pos[i];
+    const f3 xi = mk(xi4);
+    const float invHi = 1.0f / (xi4.w * 4.0f);
+    const float pi = kappa[i];
+    f3 ff{0.f, 0.f, 0.f};
+    const uint32_t c = nb.cff[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbr_ff(nb, k, i, N);
+        const f3 xij = xi - mk(pos[j]);
+        const f3 fij = 0.5f * P.V * (pi + kappa[j]) * Gk<CUBIC>(len3(xij), invHi) * xij;
+        ff = ff - fij;
+    }
+    f3 fb{0.f, 0.f, 0.f};
+    const uint32_t cb = nb.cfb[i];
+    for (uint32_t k = 0; k < cb; k++) {
+        const uint32_t j = nbr_fb(nb, k, i, N);
+        const f3 xij = xi - mk(bpos[j]);
+        const f3 fij = 0.5f * bvol[j] * pi * Gk<CUBIC>(len3(xij), invHi) * xij;
+        fb = fb - fij;
+    }
+    f3 F{0.f, 0.f, 0.f};
+    F = F + ff;
+    F = F + fb;
+    float4 v4 = vel[i];
+    const float m = v4.w;
+    F = m * F;
+    if (m > 1.0e-10f) {
+        const float cdt = P.dt / m;
+        v4.x = v4.x + cdt * F.x;
+        v4.y = v4.y + cdt * F.y;
+        v4.z = v4.z + cdt * F.z;
+        vel[i] = v4;
+    }
+}
+
+// ----------------------------------------------------------------- viscosity + surface tension CG
+// ComputeSurfaceTensionFactor_kernel (MPSSPHNonPressureUtil.cuh:425-465); also x0 = v
+// (the copy into pPredictVel, MPSSPHNonPressureUtil.cu:144-147).
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_st_factor(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                    float* __restrict__ fst, float4* __restrict__ cgx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    if (i >= N) return;
+    const float4 xi4 = pos[i];
+    const float4 vi4 = vel[i];
+    const f3 xi = mk(xi4), vi = mk(vi4);
+    const float invHi = 1.0f / (xi4.w * 4.0f);
+    float d = 0.0f;
+    const uint32_t c = nb.cff[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbr_ff(nb, k, i, N);
+        const f3 xij = xi - mk(pos[j]);
+        const float4 vj4 = vel[j];
+        const f3 vij = vi - mk(vj4);
+        const f3 grad = Gk<CUBIC>(len3(xij), invHi) * xij;
+        d += vj4.w * dot3(vij, grad);
+    }
+    fst[i] = d;
+    cgx[i] = make_float4(vi4.x, vi4.y, vi4.z, 0.0f);
+}
+
+// g_ij of the surface-tension operator (MPSSPHNonPressureUtil.cuh:503-512,563-572)
+__device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float dist, float invHi, float mi, float mj,
+                                      float di, float dj, float fi, float fj, float& mij) {
+    mij = 0.5f * (mi + mj);
+    const float invDij = 2.0f / (di + dj);
+    const float stW = STWk(dist, invHi);
+    const f3 stG = STGk(dist, invHi) * xij;
+    return (stW + P.dt * (dot3(vij, stG) - (fi + fj) * invDij * 0.5f * stW)) * invDij;
+}
+
+// ComputeSurfaceTensionCGb_kernel (MPSSPHNonPressureUtil.cuh:467-521)
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_st_rhs(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                 const float* __restrict__ rho, const float* __restrict__ fst, float4* __restrict__ cgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    if (i >= N) return;
+    const float4 xi4 = pos[i];
+    const float4 vi4 = vel[i];
+    const f3 xi = mk(xi4), vi = mk(vi4);
+    const float mi = vi4.w, di = rho[i], fi = fst[i];
+    const float invHi = 1.0f / (xi4.w * 4.0f);
+    f3 bi{0.f, 0.f, 0.f};
+    const uint32_t c = nb.cff[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbr_ff(nb, k, i, N);
+        const f3 xij = xi - mk(pos[j]);
+        const float4 vj4 = vel[j];
+        const float dist = len3(xij);
+        float mij;
+        const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, vj4.w, di, rho[j], fi, fst[j], mij);
+        bi = bi - (mij * gij) * xij;
+    }
+    const f3 b = (P.dt * P.sigma / mi) * bi + vi;
+    cgb[i] = make_float4(b.x, b.y, b.z, 0.0f);
+}
+
+// A.p = (1+f) p + dt [ ST + XSPH + viscosity ]
+// (ComputeSurfaceTensionCGAp_kernel :523-585, ComputeCGViscosityAp_kernel :587-635,
+//  ComputeFinalCGAp_kernel :637-648). MODE 0 = initial residual (x0, f = 0) fused with
+// InitCGResidual_kernel (MPSCGKernel.cuh:5-19); MODE 1 = iteration, fused with the p.Ap
+// half of ComputeCGAlphaParam_kernel (MPSCGKernel.cuh:21-67).
+template <bool CUBIC, int MODE>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                   const float* __restrict__ fst, const float4* __restrict__ in,
+                                                   float4* __restrict__ Ap, const float4* __restrict__ b,
+                                                   float4* __restrict__ r, float4* __restrict__ p, SolverDev* st,
+                                                   double* __restrict__ partials, uint32_t it, double tol) {
+    if (MODE == 1 && st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    double part = 0.0;
+    if (i < N) {
+        const float4 xi4 = pos[i];
+        const float4 vi4 = vel[i];
+        const f3 xi = mk(xi4), vi = mk(vi4);
+        const float mi = vi4.w, di = rho[i], fi = fst[i];
+        const float hi = xi4.w * 4.0f;
+        const float invHi = 1.0f / hi;
+        const float4 pi4 = in[i];
+        const f3 pi = mk(pi4);
+        f3 st_{0.f, 0.f, 0.f}, xs{0.f, 0.f, 0.f}, vis{0.f, 0.f, 0.f};
+        const float onePercentHSq = 0.01f * hi * hi;
+        const uint32_t c = nb.cff[i];
+        for (uint32_t k = 0; k < c; k++) {
+            const uint32_t j = nbr_ff(nb, k, i, N);
+            const f3 xij = xi - mk(pos[j]);
+            const float4 vj4 = vel[j];
+            const float mj = vj4.w, dj = rho[j];
+            const float dist = len3(xij);
+            float mij;
+            const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, mj, di, dj, fi, fst[j], mij);
+            const f3 pij = pi - mk(in[j]);
+            st_ = st_ + (mij * gij) * pij;
+            xs = xs + (mj / dj * pij) * Wk<CUBIC>(dist, invHi);
+            const f3 dpij = 0.5f * (mi + mj) / dj * dot3(pij, xij) / (dist * dist + onePercentHSq) * Gk<CUBIC>(dist, invHi) /
+                            (dist + kFltEps) * xij;
+            vis = vis - dpij;
+        }
+        st_ = st_ * (P.dt * P.sigma / mi);
+        xs = xs * (P.xsph / P.dt);
+        vis = vis * (10.0f * P.rho0 * P.mu / di);
+        f3 a{0.f, 0.f, 0.f};
+        a = a + (st_ + xs);
+        a = a + vis;
+        const float f = MODE == 0 ? 0.0f : P.cg_factor;
+        const f3 out = pi * (1.0f + f) + P.dt * a;
+        if (MODE == 0) {
+            const float4 bi = b[i];
+            const f3 ri{bi.x - out.x, bi.y - out.y, bi.z - out.z};
+            r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
+            p[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
+            part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
+        } else {
+            Ap[i] = make_float4(out.x, out.y, out.z, 0.0f);
+            part = double(pi.x * out.x) + double(pi.y * out.y) + double(pi.z * out.z);
+        }
+    }
+    reduce_last(part, partials, &st->counter, [&](double s) {
+        if (MODE == 0) {
+            st->rr = s;
+            st->log[0] = s;
+            if (s < tol) {
+                st->done = 1u;
+                st->l = 1u;
+            } else {
+                st->l = 1u;
+            }
+        } else {
+            st->pAp = s;
+            st->alpha = float(st->rr / s);
+        }
+    });
+}
+
+// UpdateCGResidual_kernel + ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121)
+__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t N, float4* __restrict__ x, float4* __restrict__ r,
+                                                    const float4* __restrict__ p, const float4* __restrict__ Ap, SolverDev* st,
+                                                    double* __restrict__ partials, uint32_t it, double tol) {
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float a = st->alpha;
+    double part = 0.0;
+    if (i < N) {
+        const float4 pi = p[i], api = Ap[i];
+        float4 xi = x[i], ri = r[i];
+        xi.x = xi.x + a * pi.x;
+        xi.y = xi.y + a * pi.y;
+        xi.z = xi.z + a * pi.z;
+        ri.x = ri.x - a * api.x;
+        ri.y = ri.y - a * api.y;
+        ri.z = ri.z - a * api.z;
+        x[i] = xi;
+        r[i] = ri;
+        part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
+    }
+    reduce_last(part, partials, &st->counter, [&](double s) {
+        st->beta = float(s / st->rr);
+        st->rr = s;
+        if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
+        st->l = it + 1;
+        if (s < tol) st->done = 1u;
+    });
+}
+
+// UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133)
+__global__ void __launch_bounds__(kBlock) k_cg_direction(uint32_t N, const float4* __restrict__ r, float4* __restrict__ p,
+                                                       const SolverDev* st) {
+    if (st->done) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float bta = st->beta;
+    const float4 ri = r[i];
+    float4 pi = p[i];
+    pi.x = ri.x + bta * pi.x;
+    pi.y = ri.y + bta * pi.y;
+    pi.z = ri.z + bta * pi.z;
+    p[i] = pi;
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_vel_out(uint32_t N, const float4* __restrict__ x, float4* __restrict__ vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 xi = x[i];
+    float4 v = vel[i];
+    v.x = xi.x;
+    v.y = xi.y;
+    v.z = xi.z;
+    vel[i] = v;
+}
+
+// ----------------------------------------------------------------- Jacobi viscosity
+// ComputeJacobiViscosity kernels (MPSSPHNonPressureUtil.cuh:90-194, commented out in the
+// reference): per particle a 3x3 block with A_ij = V G/((d^2+0.01h^2)(d+eps)) x_ij x_ij^T.
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_jacobi_candidate(StepParams P, Nbr nb, const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                           const float4* __restrict__ u, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    if (i >= N) return;
+    const float4 xi4 = pos[i];
+    const f3 xi = mk(xi4);
+    const float h = xi4.w * 4.0f;
+    const float invH = 1.0f / h;
+    const float onePercentHSq = 0.01f * h * h;
+    float A[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    f3 Avj{0.f, 0.f, 0.f};
+    const uint32_t c = nb.cff[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbr_ff(nb, k, i, N);
+        const f3 xij = xi - mk(pos[j]);
+        const float dist = len3(xij);
+        if (dist < h) {
+            const f3 uj = mk(u[j]);
+            const float cc = P.V * Gk<CUBIC>(dist, invH) / ((dist * dist + onePercentHSq) * (dist + kFltEps));
+            const float M[3][3] = {{cc * (xij.x * xij.x), cc * (xij.y * xij.x), cc * (xij.z * xij.x)},
+                                   {cc * (xij.x * xij.y), cc * (xij.y * xij.y), cc * (xij.z * xij.y)},
+                                   {cc * (xij.x * xij.z), cc * (xij.y * xij.z), cc * (xij.z * xij.z)}};
+            const f3 uM{dot3(uj, f3{M[0][0], M[0][1], M[0][2]}), dot3(uj, f3{M[1][0], M[1][1], M[1][2]}),
+                        dot3(uj, f3{M[2][0], M[2][1], M[2][2]})};
+            Avj = Avj - uM;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int bq = 0; bq < 3; bq++) A[a][bq] = A[a][bq] - M[a][bq];
+        }
+    }
+    const float al = P.dt * 10.0f * P.mu / rho[i];
+    Avj = Avj * al;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int bq = 0; bq < 3; bq++) A[a][bq] = A[a][bq] * al + (a == bq ? 1.0f : 0.0f);
+    const float oneOverDet = 1.0f / (+A[0][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2]) -
+                                     A[1][0] * (A[0][1] * A[2][2] - A[2][1] * A[0][2]) +
+                                     A[2][0] * (A[0][1] * A[1][2] - A[1][1] * A[0][2]));
+    float I[3][3];
+    I[0][0] = +(A[1][1] * A[2][2] - A[2][1] * A[1][2]) * oneOverDet;
+    I[1][0] = -(A[1][0] * A[2][2] - A[2][0] * A[1][2]) * oneOverDet;
+    I[2][0] = +(A[1][0] * A[2][1] - A[2][0] * A[1][1]) * oneOverDet;
+    I[0][1] = -(A[0][1] * A[2][2] - A[2][1] * A[0][2]) * oneOverDet;
+    I[1][1] = +(A[0][0] * A[2][2] - A[2][0] * A[0][2]) * oneOverDet;
+    I[2][1] = -(A[0][0] * A[2][1] - A[2][0] * A[0][1]) * oneOverDet;
+    I[0][2] = +(A[0][1] * A[1][2] - A[1][1] * A[0][2]) * oneOverDet;
+    I[1][2] = -(A[0][0] * A[1][2] - A[1][0] * A[0][2]) * oneOverDet;
+    I[2][2] = +(A[0][0] * A[1][1] - A[1][0] * A[0][1]) * oneOverDet;
+    if (isinf(I[0][0]) || isnan(I[0][0])) {
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int bq = 0; bq < 3; bq++) I[a][bq] = 0.0f;
+    }
+    const f3 rhs = mk(vel[i]) + Avj;
+    out[i] = make_float4(dot3(rhs, f3{I[0][0], I[0][1], I[0][2]}), dot3(rhs, f3{I[1][0], I[1][1], I[1][2]}),
+                         dot3(rhs, f3{I[2][0], I[2][1], I[2][2]}), 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock) k_jacobi_relax(uint32_t N, const float4* __restrict__ cand, const float4* __restrict__ cur,
+                                                       const float4* __restrict__ prev, float ur, float w0, float w1, float w2,
+                                                       float4* __restrict__ o0, float4* __restrict__ o1, float4* __restrict__ o2) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const f3 nw = mk(cand[i]), c = mk(cur[i]), pv = mk(prev[i]);
+    const f3 t = ur * (nw - c) + c - pv;
+    const f3 a = w0 * t + pv, b = w1 * t + pv, d = w2 * t + pv;
+    o0[i] = make_float4(a.x, a.y, a.z, 0.f);
+    o1[i] = make_float4(b.x, b.y, b.z, 0.f);
+    o2[i] = make_float4(d.x, d.y, d.z, 0.f);
+}
+
+// ComputeJacobiError_0/1_kernel (MPSSPHNonPressureUtil.cuh:195-333): per-block partial sums
+// of |u_i - a_i sum_j (x_ij.(u_i-u_j)) grad_ij - v0_i| for up to three candidates.
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_jacobi_error(StepParams P, Nbr nb, const float4* __restrict__ pos,
+                                                       const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                       const float4* __restrict__ u0, const float4* __restrict__ u1,
+                                                       const float4* __restrict__ u2, int ncand, double* __restrict__ partials) {
+    __shared__ double lds[kBlock / 64];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    const float4* U[3] = {u0, u1, u2};
+    for (int q = 0; q < ncand; q++) {
+        double e = 0.0;
+        if (i < N) {
+            const float4 xi4 = pos[i];
+            const f3 xi = mk(xi4);
+            const float h = xi4.w * 4.0f;
+            const float invH = 1.0f / h;
+            const float onePercentHSq = 0.01f * h * h;
+            const f3 ui = mk(U[q][i]);
+            f3 Av{0.f, 0.f, 0.f};
+            const uint32_t c = nb.cff[i];
+            for (uint32_t k = 0; k < c; k++) {
+                const uint32_t j = nbr_ff(nb, k, i, N);
+                const f3 xij = xi - mk(pos[j]);
+                const float dist = len3(xij);
+                if (dist < h) {
+                    const f3 grad = P.V * Gk<CUBIC>(dist, invH) / ((dist * dist + onePercentHSq) * (dist + kFltEps)) * xij;
+                    Av = Av + dot3(xij, ui - mk(U[q][j])) * grad;
+                }
+            }
+            Av = Av * (P.dt * 10.0f * P.mu / rho[i]);
+            e = double(len3(ui - Av - mk(vel[i])));
+        }
+        const double bs = block_sum_d(e, lds);
+        if (threadIdx.x == 0) partials[size_t(q) * gridDim.x + blockIdx.x] = bs;
+    }
+}
+__global__ void __launch_bounds__(kBlock) k_fold_partials(const double* __restrict__ partials, uint32_t nblk, int ncand,
+                                                        double* __restrict__ out) {
+    __shared__ double lds[kBlock / 64];
+    for (int q = 0; q < ncand; q++) {
+        double s = 0.0;
+        for (uint32_t k = threadIdx.x; k < nblk; k += blockDim.x) s += partials[size_t(q) * nblk + k];
+        s = block_sum_d(s, lds);
+        if (threadIdx.x == 0) out[q] = s;
+    }
+}
+
+// ----------------------------------------------------------------- advection
+// BoundaryCollision_kernel + UpdatePosition_kernel (MPSAdvectUtil.cuh:36-77)
+__global__ void __launch_bounds__(kBlock) k_advect(StepParams P, float4* __restrict__ pos, float4* __restrict__ vel,
+                                                 uint32_t* __restrict__ nonfinite) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.N) return;
+    float4 x4 = pos[i];
+    float4 v4 = vel[i];
+    float v[3] = {v4.x, v4.y, v4.z};
+    const float x[3] = {x4.x, x4.y, x4.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float pk = x[k] + v[k] * P.dt;
+        if (pk < P.bmin[k] && v[k] < 0.0f) v[k] = (fabsf(v[k]) < 1.0e-4f) ? 0.0f : -v[k] * 0.4f;
+        else if (pk > P.bmax[k] && v[k] > 0.0f) v[k] = (fabsf(v[k]) < 1.0e-4f) ? 0.0f : -v[k] * 0.4f;
+    }
+    v4.x = v[0];
+    v4.y = v[1];
+    v4.z = v[2];
+    x4.x = x[0] + P.dt * v[0];
+    x4.y = x[1] + P.dt * v[1];
+    x4.z = x[2] + P.dt * v[2];
+    vel[i] = v4;
+    pos[i] = x4;
+    const bool bad = !isfinite(x4.x) || !isfinite(x4.y) || !isfinite(x4.z) || !isfinite(v4.x) || !isfinite(v4.y) ||
+                     !isfinite(v4.z);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(nonfinite, 1u);
+}
+
+__global__ void k_solver_reset(SolverDev* st) {
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        st->done = 0u;
+        st->l = 1u;
+        st->counter = 0u;
+        st->rr = 0.0;
+        st->pAp = 0.0;
+        st->alpha = 0.0f;
+        st->beta = 0.0f;
+    }
+    for (uint32_t k = t; k < MPS_MAX_ITER_LOG; k += blockDim.x) st->log[k] = 0.0;
+}
+__global__ void k_zero_words(uint32_t* __restrict__ p, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+// Render feed: positions and radii de-interleaved for the host (replaces the GL buffers
+// SPHRenderer binds as SSBOs, MPSSPHRenderer.cpp:22-29).
+__global__ void __launch_bounds__(kBlock) k_pack_readback(uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ color,
+                                                        float* __restrict__ xyz, float* __restrict__ rad, float* __restrict__ rgba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    if (xyz) {
+        xyz[3 * size_t(i)] = p.x;
+        xyz[3 * size_t(i) + 1] = p.y;
+        xyz[3 * size_t(i) + 2] = p.z;
+    }
+    if (rad) rad[i] = p.w;
+    if (rgba) reinterpret_cast<float4*>(rgba)[i] = color[i];
+}
+
+}  // namespace dev
+
+// ===================================================================== host launchers
+namespace launch {
+using namespace dev;
+static inline unsigned nb_(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+
+void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, const uint32_t* cbb, float* S,
+                  hipStream_t s) {
+    if (!B) return;
+    if (cubic) k_boundary_sum<true><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, cbb, S);
+    else k_boundary_sum<false><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, cbb, S);
+}
+void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
+    if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
+}
+void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
+                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, hipStream_t s) {
+    if (!P.N) return;
+    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color);
+    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color);
+}
+void pressure_eval(int cubic, int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
+                   float* kappa, float* delta, SolverDev* st, double* partials, uint32_t it, double tol, uint32_t lmin,
+                   hipStream_t s) {
+    if (!P.N) return;
+    const unsigned g = nb_(P.N);
+    if (cubic) {
+        if (df) k_pressure_eval<true, true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
+        else k_pressure_eval<true, false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
+    } else {
+        if (df) k_pressure_eval<false, true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
+        else k_pressure_eval<false, false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
+    }
+}
+void pressure_apply(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+                    const float* bvol, const float* kappa, const SolverDev* st, hipStream_t s) {
+    if (!P.N) return;
+    if (cubic) k_pressure_apply<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, kappa, st);
+    else k_pressure_apply<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, kappa, st);
+}
+void st_factor(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, float* fst,
+               float4* cgx, hipStream_t s) {
+    if (!P.N) return;
+    if (cubic) k_st_factor<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, fst, cgx);
+    else k_st_factor<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, fst, cgx);
+}
+void st_rhs(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+            const float* fst, float4* cgb, hipStream_t s) {
+    if (!P.N) return;
+    if (cubic) k_st_rhs<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, cgb);
+    else k_st_rhs<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, cgb);
+}
+void cg_apply(int cubic, int mode, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+              const float* rho, const float* fst, const float4* in, float4* Ap, const float4* b, float4* r, float4* p,
+              SolverDev* st, double* partials, uint32_t it, double tol, hipStream_t s) {
+    if (!P.N) return;
+    const unsigned g = nb_(P.N);
+    if (cubic) {
+        if (mode == 0) k_cg_apply<true, 0><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
+        else k_cg_apply<true, 1><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
+    } else {
+        if (mode == 0) k_cg_apply<false, 0><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
+        else k_cg_apply<false, 1><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
+    }
+}
+void cg_update(uint32_t N, float4* x, float4* r, const float4* p, const float4* Ap, SolverDev* st, double* partials,
+               uint32_t it, double tol, hipStream_t s) {
+    if (N) k_cg_update<<<nb_(N), kBlock, 0, s>>>(N, x, r, p, Ap, st, partials, it, tol);
+}
+void cg_direction(uint32_t N, const float4* r, float4* p, const SolverDev* st, hipStream_t s) {
+    if (N) k_cg_direction<<<nb_(N), kBlock, 0, s>>>(N, r, p, st);
+}
+void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s) {
+    if (N) k_copy_vel_out<<<nb_(N), kBlock, 0, s>>>(N, x, vel);
+}
+void jacobi_candidate(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                      const float* rho, const float4* u, float4* out, hipStream_t s) {
+    if (!P.N) return;
+    if (cubic) k_jacobi_candidate<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, u, out);
+    else k_jacobi_candidate<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, u, out);
+}
+void jacobi_relax(uint32_t N, const float4* cand, const float4* cur, const float4* prev, float ur, const float* w3,
+                  float4* o0, float4* o1, float4* o2, hipStream_t s) {
+    if (N) k_jacobi_relax<<<nb_(N), kBlock, 0, s>>>(N, cand, cur, prev, ur, w3[0], w3[1], w3[2], o0, o1, o2);
+}
+void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+                  const float* rho, const float4* u0, const float4* u1, const float4* u2, int ncand, double* partials,
+                  double* err3, hipStream_t s) {
+    if (!P.N) return;
+    const unsigned g = nb_(P.N);
+    if (cubic) k_jacobi_error<true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    else k_jacobi_error<false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    k_fold_partials<<<1, kBlock, 0, s>>>(partials, g, ncand, err3);
+}
+void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s) {
+    if (P.N) k_advect<<<nb_(P.N), kBlock, 0, s>>>(P, pos, vel, nonfinite);
+}
+void solver_reset(SolverDev* st, hipStream_t s) { k_solver_reset<<<1, kBlock, 0, s>>>(st); }
+void zero_words(uint32_t* p, size_t n, hipStream_t s) {
+    if (n) k_zero_words<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n);
+}
+void pack_readback(uint32_t n, const float4* pos, const float4* color, float* xyz, float* rad, float* rgba, hipStream_t s) {
+    if (n) k_pack_readback<<<nb_(n), kBlock, 0, s>>>(n, pos, color, xyz, rad, rgba);
+}
+
+}  // namespace launch
+}  // namespace mps

@@ -1,0 +1,198 @@
+"""Scene definitions: the reference's own test scenes and the dam-break / droplet workloads.
+
+A Scene is plain data (numpy arrays + parameter dicts) that both the gfx950 library and
+the CPU oracle can be loaded from, so parity runs start from identical inputs.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as capi
+
+# 12-triangle cube standing in for the missing obj/cube.obj (MPSSystem.cpp:63). Vertex 0 is
+# the minimum corner so AABB::AddPoint's else-if (MPSAABB.h:20-28) sees the full extent.
+CUBE_VERTS = np.array(
+    [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float64)
+CUBE_TRIS = np.array(
+    [[0, 2, 1], [0, 3, 2],   # z = 0
+     [4, 5, 6], [4, 6, 7],   # z = 1
+     [0, 1, 5], [0, 5, 4],   # y = 0
+     [3, 7, 6], [3, 6, 2],   # y = 1
+     [0, 4, 7], [0, 7, 3],   # x = 0
+     [1, 2, 6], [1, 6, 5]],  # x = 1
+    np.uint32)
+
+
+@dataclass
+class Scene:
+    name: str
+    pos: np.ndarray                 # (N, 3) float64 (exact float32 values)
+    radius: np.ndarray              # (N,)
+    vel: np.ndarray | None
+    physics: dict
+    material: dict
+    config: dict
+    boundary_mesh: dict | None = None   # verts, tris, center, size, radius, density
+    boundary_pos: np.ndarray | None = None
+    boundary_radius: np.ndarray | None = None
+    steps: int = 1
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+
+def make_config(scene: Scene) -> capi.Config:
+    cfg = capi.default_config()
+    for k, v in scene.config.items():
+        if k == "hash_size":
+            for i in range(3):
+                cfg.hash_size[i] = int(v[i])
+        else:
+            setattr(cfg, k, v)
+    return cfg
+
+
+def load_into(sim, scene: Scene):
+    """Push a scene into a HipSim (or an oracle handle with the same methods)."""
+    p = scene.physics
+    sim.set_physics(p["dt"], p["gravity"], p["box_min"], p["box_max"])
+    m = scene.material
+    sim.set_fluid_material(m["radius"], m["density"], m["viscosity"], m["xsph"], m["surface_tension"])
+    if scene.boundary_mesh is not None:
+        b = scene.boundary_mesh
+        sim.sample_boundary_mesh(b["verts"], b["tris"], b["radius"], b.get("density", 1.0), b.get("center"), b.get("size"))
+    elif scene.boundary_pos is not None:
+        sim.upload_boundary(scene.boundary_pos, scene.boundary_radius)
+    sim.upload_fluid(scene.pos, scene.radius, scene.vel)
+
+
+def _f32(a) -> np.ndarray:
+    return np.asarray(a, np.float32).astype(np.float64)
+
+
+# ---------------------------------------------------------------------------- reference scenes
+def surface_tension_test_scene(size: int = 5) -> Scene:
+    """System::SurfaceTensionTestScene (MPSSystem.cpp:552-651): two size^3 blocks, r = 0.1."""
+    radius = 0.1
+    gap = radius * float(np.float32(0.9))
+    offset = float(np.float32(0.001))
+    stride = gap + gap + offset
+    L = stride * (size - 1) + gap + gap
+    start = -L
+    pos = []
+    for z in range(size):
+        for y in range(size):
+            for x in range(size):
+                pos.append([start - L * 0.025 + x * stride + gap, start + y * stride + gap, start + z * stride + gap])
+                pos.append([L * 0.025 + x * stride + gap, start + y * stride + gap, start + z * stride + gap])
+    pos = _f32(pos)
+    bmin = [start - L * 0.025 + gap - 3.5, start + gap - 3.5, start + gap - 3.5]
+    bmax = [L * 0.025 + (size - 1) * stride + gap + 3.5, start + (size - 1) * stride + gap + 3.5,
+            start + (size - 1) * stride + gap + 3.5]
+    center = [(a + b) * 0.5 for a, b in zip(bmin, bmax)]
+    ext = [b - a for a, b in zip(bmin, bmax)]
+    return Scene(
+        name=f"surface_tension_test_{size}",
+        pos=pos, radius=np.full(pos.shape[0], radius), vel=None,
+        physics=dict(dt=0.005, gravity=(0.0, -9.8, 0.0), box_min=bmin, box_max=bmax),
+        material=dict(radius=radius * 4, density=1.0, viscosity=0.025, xsph=0.1, surface_tension=100.0),
+        config=dict(kernel=capi.KERNEL_POLY6_SPIKY, hash_size=(64, 64, 64), cell_size=radius * 4),
+        boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=center, size=ext, radius=radius * 4, density=1.0),
+    )
+
+
+def viscosity_test_scene(size: int = 5, height: int = 20) -> Scene:
+    """System::ViscosityTestScene (MPSSystem.cpp:455-550): a size x height x size column."""
+    radius = 0.1
+    gap = radius * float(np.float32(0.9))
+    offset = float(np.float32(0.001))
+    stride = gap + gap + offset
+    lw = stride * (size - 1) + gap + gap
+    start = -lw * float(np.float32(0.5))
+    start_h = 0.0
+    pos = []
+    for y in range(height):
+        for z in range(size):
+            for x in range(size):
+                pos.append([start + x * stride + gap, start + y * stride + gap, start + z * stride + gap])
+    pos = _f32(pos)
+    bmin = [start + gap - 5.5, start_h - 1.0, start + gap - 5.5]
+    bmax = [start + (size - 1) * stride + gap + 5.5, start_h + (height - 1) * stride + gap + 5.5,
+            start + (size - 1) * stride + gap + 5.5]
+    center = [(a + b) * 0.5 for a, b in zip(bmin, bmax)]
+    ext = [b - a for a, b in zip(bmin, bmax)]
+    return Scene(
+        name=f"viscosity_test_{size}x{height}",
+        pos=pos, radius=np.full(pos.shape[0], radius), vel=None,
+        physics=dict(dt=0.005, gravity=(0.0, -9.8, 0.0), box_min=bmin, box_max=bmax),
+        material=dict(radius=radius * 4, density=1.0, viscosity=0.025, xsph=0.1, surface_tension=100.0),
+        config=dict(kernel=capi.KERNEL_POLY6_SPIKY, hash_size=(64, 64, 64), cell_size=radius * 4),
+        boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=center, size=ext, radius=radius * 4, density=1.0),
+    )
+
+
+# ---------------------------------------------------------------------------- dam breaks
+def dam_break(nx: int, ny: int, nz: int, spacing: float, jitter: float, seed: int, box_min, box_max,
+              dt: float, steps: int, hash_size, viscosity=0.01, surface_tension=0.05, name="dam_break") -> Scene:
+    """Fluid block x = spacing*(i+1) + U(-jitter, jitter) in the lower corner of a walled tank.
+
+    Particle radius = spacing/2 (support h = 2*spacing); walls are the tank's 12-triangle
+    mesh sampled at spacing (R_m = 2*spacing, boundary radius spacing/2).
+    """
+    rng = np.random.default_rng(seed)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    base = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+    base = np.asarray(box_min, np.float64)[None, :] + spacing * (base + 1.0)
+    pos = _f32(base + rng.uniform(-jitter, jitter, size=base.shape))
+    h = 2.0 * spacing
+    center = [(a + b) * 0.5 for a, b in zip(box_min, box_max)]
+    ext = [b - a for a, b in zip(box_min, box_max)]
+    return Scene(
+        name=name, pos=pos, radius=np.full(pos.shape[0], spacing * 0.5), vel=None,
+        physics=dict(dt=dt, gravity=(0.0, -9.81, 0.0), box_min=list(box_min), box_max=list(box_max)),
+        material=dict(radius=h, density=1000.0, viscosity=viscosity, xsph=0.1, surface_tension=surface_tension),
+        config=dict(kernel=capi.KERNEL_CUBIC, hash_size=tuple(hash_size), cell_size=h, df_tol=1.0e-2, cd_tol=1.0e-3),
+        boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=center, size=ext, radius=h, density=1.0),
+        steps=steps,
+    )
+
+
+def dam_break_cpu(n_side: int = 32, steps: int = 500) -> Scene:
+    """The CPU reference workload: 32^3 fluid on a 0.02 m lattice (+-0.002 m jitter, seed 0)
+    in a 1 m box, h = 0.04 m, dt = 1 ms, 500 steps."""
+    return dam_break(n_side, n_side, n_side, 0.02, 0.002, 0, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1.0e-3, steps,
+                     (64, 64, 64), name=f"dam_break_{n_side}^3_h0.04")
+
+
+def dam_break_1m(n_side: int = 100, steps: int = 1000) -> Scene:
+    """The single-GPU workload: 100^3 = 1,000,000 fluid on a 0.01 m lattice (+-0.001 m jitter,
+    seed 0) in a 3.0 x 1.5 x 1.5 m tank, h = 0.02 m, dt = 0.5 ms, hash 256 x 128 x 128."""
+    return dam_break(n_side, n_side, n_side, 0.01, 0.001, 0, (0.0, 0.0, 0.0), (3.0, 1.5, 1.5), 5.0e-4, steps,
+                     (256, 128, 128), name=f"dam_break_{n_side}^3_h0.02")
+
+
+def droplet(n: int = 65536, radius: float = 0.25, seed: int = 0, steps: int = 100) -> Scene:
+    """A jittered sphere of n particles (spacing ~0.01 m, h = 0.02 m) above the floor of a 1 m box."""
+    rng = np.random.default_rng(seed)
+    sp = (4.0 / 3.0 * np.pi * radius ** 3 / n) ** (1.0 / 3.0)
+    m = int(np.ceil(2 * radius / sp)) + 2
+    g = (np.arange(m) - (m - 1) / 2.0) * sp
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    d = np.linalg.norm(pts, axis=1)
+    pts = pts[np.argsort(d, kind="stable")[:n]]
+    center = np.array([0.5, 0.45, 0.5])
+    pos = _f32(pts + center + rng.uniform(-0.1 * sp, 0.1 * sp, size=pts.shape))
+    h = 0.02
+    return Scene(
+        name=f"droplet_{n}", pos=pos, radius=np.full(n, h / 4), vel=None,
+        physics=dict(dt=1.0e-3, gravity=(0.0, -9.81, 0.0), box_min=[0.0, 0.0, 0.0], box_max=[1.0, 1.0, 1.0]),
+        material=dict(radius=h, density=1000.0, viscosity=0.02, xsph=0.1, surface_tension=0.3),
+        config=dict(kernel=capi.KERNEL_CUBIC, hash_size=(64, 64, 64), cell_size=h, df_tol=1.0e-2, cd_tol=1.0e-3),
+        boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=[0.5, 0.5, 0.5], size=[1.0, 1.0, 1.0], radius=h, density=1.0),
+        steps=steps,
+    )

@@ -1,0 +1,70 @@
+/*
+ * sph_oracle.h — CPU restatement of the MPS_CUDA SPH step. TEST INFRASTRUCTURE ONLY.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library, and only as the checker (or as the timed CPU baseline). The product path
+ * (mps_cuda_amd) never links or calls it.
+ *
+ * The oracle mirrors the HIP library's C ABI (include/mps_sph.h) with an `orc_` prefix so
+ * that parity tests are ABI-to-ABI. It is templated on the working precision:
+ *   precision 64 — the reference's own arithmetic (REAL = double, src/MPS_Object/MPSDef.h:5-13);
+ *   precision 32 — the FP32 semantics the HIP library implements: every REAL value and
+ *                  literal is float, evaluated in the reference's expression order with no
+ *                  FMA contraction; cell coordinates use trunc(double(x) * (1.0/double(cell)))
+ *                  (MPSSpatialHash.cuh:9-19); convergence sums accumulate T-rounded terms in
+ *                  double; solver scalars (CG alpha/beta, Jacobi omega) are computed in double
+ *                  and rounded to T. Boundary sampling always runs in double and is stored as T.
+ * Arrays exchanged through orc_get_array/orc_set_array are in the working precision
+ * (float or double); colours are always float; ids/keys are uint32.
+ */
+#ifndef MPS_SPH_ORACLE_H
+#define MPS_SPH_ORACLE_H
+
+#include "../include/mps_sph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+int orc_create(const mps_config* cfg, int precision, orc_ctx** out);
+void orc_destroy(orc_ctx* c);
+const char* orc_last_error(const orc_ctx* c);
+int orc_precision(const orc_ctx* c);
+int orc_set_threads(int n);
+
+int orc_set_physics(orc_ctx* c, const mps_physics* p);
+int orc_set_fluid_material(orc_ctx* c, const mps_sph_material* m);
+/* Inputs in double; stored in the working precision. vel/mass may be NULL. */
+int orc_upload_fluid(orc_ctx* c, size_t n, const double* pos, const double* vel,
+                     const double* radius, const double* mass);
+int orc_upload_boundary(orc_ctx* c, size_t n, const double* pos, const double* radius,
+                        const double* mass);
+int orc_sample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const uint32_t* tris,
+                             size_t nt, const double* center, const double* size,
+                             const mps_mesh_material* m);
+
+int orc_step(orc_ctx* c, mps_step_stats* out);
+int orc_run_phase(orc_ctx* c, int32_t phase, mps_step_stats* out);
+
+int orc_array_info(orc_ctx* c, int32_t id, size_t* count, size_t* elem_bytes);
+int orc_get_array(orc_ctx* c, int32_t id, void* dst, size_t bytes);
+int orc_set_array(orc_ctx* c, int32_t id, const void* src, size_t bytes);
+int orc_get_frame(orc_ctx* c, uint32_t* frame);
+int orc_set_frame(orc_ctx* c, uint32_t frame);
+
+/* Scalar helpers for known-answer tests (double in/out, evaluated in `precision`). */
+int orc_kernel_eval(int precision, int kernel, size_t n, const double* dist, double invh,
+                    double* w, double* g, double* stw, double* stg);
+int orc_cell_keys(int precision, size_t n, const double* pos, double cell,
+                  const uint32_t* hash_size, int morton, uint32_t* keys);
+/* Mesh helpers: ownership bits (RTriangleBuild_kernel semantics) and per-face sample counts. */
+int orc_mesh_rtri(size_t nv, const uint32_t* tris, size_t nt, uint32_t* rtri);
+int orc_mesh_sample_counts(const double* verts, size_t nv, const uint32_t* tris, size_t nt,
+                           double radius, uint32_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
