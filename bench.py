@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH time step on one MI355X.
+
+Workload (default): plain DFSPH dam break, 100^3 = 1,000,000 fluid particles on a 0.01 m
+lattice (+-0.001 m jitter, seed 0) in a 3.0 x 1.5 x 1.5 m tank with wall particles at
+0.01 m, cubic kernel, h = 0.02 m, dt = 0.5 ms, density tol 0.1 %, divergence tol 1 %,
+viscosity 0.01, surface tension 0.05, FP32.
+
+A "step" is one System::OnUpdate: hash, neighbour lists, boundary volume, density/alpha,
+divergence-free solve, viscosity + surface-tension CG, constant-density solve, advection.
+`value` = fluid particles x timed steps / wall time of the timed steps (inputs resident in
+HBM). The timed steps are steps 0..K-1 of the workload: the W warm-up steps run first and
+the scene is then reloaded into the same context (no device allocation inside the region).
+
+Prints ONE JSON line. Extra objects: `roofline` (dominant kernel, algorithmic bytes from
+the measured neighbour counts and iteration counts / its live HIP-event time), `cpu_baseline`
+(the FP32 CPU oracle on a bounded sample of the same workload), per-kernel table.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 0.1% density err: batched vs plain; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch, per fluid particle N and per neighbour entry E (FP32 SoA,
+# 4-byte indices, compulsory traffic only): the per-pass figures of the measurement plan.
+#   A.p (ST + XSPH + viscosity + final):   68 N + 4 E_ff
+#   DF/CD evaluate (delta, stiffness, sum): 44 N + 4 E_f
+#   DF/CD apply (force + velocity):         48 N + 4 E_f
+#   density + alpha (two passes fused):     2 x (28 N + 4 E_f)
+#   ST factor / CG right-hand side:         36 N + 4 E_ff / 56 N + 4 E_ff
+#   CG vector update + direction:           132 N per iteration (x, r, p, Ap)
+#   neighbour search ff + fb:               96 N + 4 E_f
+def kernel_bytes(name: str, N: int, Eff: float, Efb: float) -> float | None:
+    Ef = Eff + Efb
+    table = {
+        "k_cg_apply<1>": 68 * N + 4 * Eff,
+        "k_cg_apply<0>": 68 * N + 4 * Eff,
+        "k_pressure_eval<DF>": 44 * N + 4 * Ef,
+        "k_pressure_eval<CD>": 44 * N + 4 * Ef,
+        "k_pressure_apply": 48 * N + 4 * Ef,
+        "k_density_alpha": 2 * (28 * N + 4 * Ef) + 24 * N,
+        "k_st_factor": 36 * N + 4 * Eff,
+        "k_st_rhs": 56 * N + 4 * Eff,
+        "k_neighbors_fluid": 96 * N + 4 * Ef,
+        "k_advect": 48 * N,
+    }
+    return table.get(name)
+
+
+def step_bytes(N, B, Eff, Efb, Ebb, l_df, l_cd, l_cg, cap, C):
+    """Whole-step algorithmic bytes (measurement plan table), from measured counts."""
+    Ef = Eff + Efb
+    J_df, J_cd = min(l_df, cap), min(l_cd, cap)
+    A_df, A_cd = l_df - 1, l_cd - 1
+    I_cg = max(min(l_cg - 1, cap), 0)
+    b = 48 * N + 48 * B + 16 * C
+    b += 48 * N + 4 * Eff + 48 * N + 4 * Efb + 48 * B + 4 * Ebb
+    b += 36 * B + 4 * Ebb
+    b += 2 * (28 * N + 4 * Ef) + 24 * N
+    b += (J_df + J_cd) * (44 * N + 4 * Ef) + (A_df + A_cd) * (48 * N + 4 * Ef)
+    b += (36 * N + 4 * Eff) + (56 * N + 4 * Eff)
+    b += (1 + I_cg) * (68 * N + 4 * Eff) + I_cg * 132 * N + 96 * N
+    b += 48 * N
+    b += (28 * N + 96 * N + 28 * B + 128 * B) / 50.0
+    return b
+
+
+def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
+    """FP32 CPU oracle on the same workload: one untimed step, then timed steps until the
+    budget is used (at least one)."""
+    from mps_cuda_amd import scenes
+    from oracle import binding
+
+    threads = binding.set_threads(0)
+    o = binding.OracleSim(scenes.make_config(scene), 32)
+    scenes.load_into(o, scene)
+    o.step()
+    n_done, t_used = 0, 0.0
+    while n_done < max_steps and (n_done == 0 or t_used < budget_s):
+        t0 = time.perf_counter()
+        o.step()
+        t_used += time.perf_counter() - t0
+        n_done += 1
+    rate = scene.n * n_done / t_used
+    return {
+        "value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        "sample": f"FP32 oracle (oracle/sph_oracle.cpp, OpenMP {threads} threads) on the same "
+                  f"{scene.n}-particle workload: steps 1..{n_done} after one untimed step, {t_used:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-side", type=int, default=100, help="fluid block edge in particles (default 100 -> 1M)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
+    ap.add_argument("--cpu-max-steps", type=int, default=3)
+    ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
+    ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+
+    from mps_cuda_amd import HipSim, scenes
+
+    scene = scenes.dam_break_1m(n_side=args.n_side, steps=args.steps)
+    cfg = scenes.make_config(scene)
+    sim = HipSim(cfg)
+    scenes.load_into(sim, scene)
+    for _ in range(args.warmup):
+        sim.step()
+    scenes.load_into(sim, scene)
+    sim.synchronize()
+    if not args.no_profile:
+        sim.profile(True)
+
+    N = scene.n
+    stats = []
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = sim.step()
+        stats.append((st.l_df, st.l_cd, st.l_cg, st.nnz_ff, st.nnz_fb, st.nnz_bb, st.hit_cap_cd,
+                      st.err_cd[max(0, min(st.l_cd, 128) - 1)], st.nonfinite, st.n_boundary))
+    sim.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = N * args.steps / elapsed
+    prof = sim.profile_read() if not args.no_profile else {}
+
+    cap = cfg.max_iter
+    C = cfg.hash_size[0] * cfg.hash_size[1] * cfg.hash_size[2]
+    B = stats[0][9] if stats else 0
+    tot_step_bytes = 0.0
+    kbytes = {}
+    for (l_df, l_cd, l_cg, eff, efb, ebb, _, _, _, _) in stats:
+        tot_step_bytes += step_bytes(N, B, eff, efb, ebb, l_df, l_cd, l_cg, cap, C)
+        uses = {
+            "k_cg_apply<1>": max(min(l_cg - 1, cfg.cg_max_iter), 0), "k_cg_apply<0>": 1,
+            "k_pressure_eval<DF>": min(l_df, cap), "k_pressure_eval<CD>": min(l_cd, cap),
+            "k_pressure_apply": (l_df - 1) + (l_cd - 1), "k_density_alpha": 1, "k_st_factor": 1,
+            "k_st_rhs": 1, "k_neighbors_fluid": 1, "k_advect": 1,
+        }
+        for name, u in uses.items():
+            kb = kernel_bytes(name, N, eff, efb)
+            kbytes[name] = kbytes.get(name, 0.0) + u * kb
+
+    roofline = None
+    kernels = {}
+    for name, p in prof.items():
+        k = dict(p)
+        k["avg_us"] = 1000.0 * p["total_ms"] / max(p["launches"], 1)
+        if name in kbytes and p["total_ms"] > 0:
+            k["algorithmic_GBps"] = kbytes[name] / (p["total_ms"] * 1e-3) / 1e9
+        kernels[name] = k
+    if prof:
+        dom = max(prof.items(), key=lambda kv: kv[1]["total_ms"])[0]
+        if dom not in kbytes:
+            dom = max((kv for kv in prof.items() if kv[0] in kbytes), key=lambda kv: kv[1]["total_ms"])[0]
+        p = prof[dom]
+        achieved = kbytes[dom] / (p["total_ms"] * 1e-3) / 1e9
+        roofline = {
+            "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_us": 1000.0 * p["total_ms"] / max(p["launches"], 1),
+            "launches": p["launches"], "useful_launches": p["useful"],
+            "share_of_gpu_time": p["total_ms"] / max(sum(q["total_ms"] for q in prof.values()), 1e-9),
+            "step_algorithmic_GBps": tot_step_bytes / elapsed / 1e9,
+            "step_frac": tot_step_bytes / elapsed / 1e9 / HBM_PEAK_GBS,
+        }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
+
+    n = len(stats)
+    iters = {
+        "l_df_mean": sum(s[0] for s in stats) / n, "l_cd_mean": sum(s[1] for s in stats) / n,
+        "l_cg_mean": sum(s[2] for s in stats) / n, "cd_capped_steps": sum(s[6] for s in stats),
+        "cd_final_err_mean": sum(s[7] for s in stats) / n, "nonfinite_steps": sum(s[8] for s in stats),
+        "nnz_ff_last": stats[-1][3], "nnz_fb_last": stats[-1][4], "nnz_bb": stats[-1][5],
+    }
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1000.0 * elapsed / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": f"plain DFSPH dam break, {N} fluid + {B} wall particles, 0.01 m lattice, "
+                        f"3.0x1.5x1.5 m tank, cubic kernel h=0.02 m, dt=0.5 ms, CD tol 1e-3, DF tol 1e-2, "
+                        f"viscosity 0.01 + surface tension 0.05 (CG), steps 0..{args.steps - 1}",
+            "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
+        },
+        "roofline": roofline, "cpu_baseline": cpu, "iterations": iters,
+    }
+    if args.kernels:
+        for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["total_ms"]):
+            print(f"{name:24s} launches={k['launches']:6d} useful={k['useful']:6d} total={k['total_ms']:9.2f} ms "
+                  f"avg={k['avg_us']:8.1f} us GB/s={k.get('algorithmic_GBps', float('nan')):8.1f}", file=sys.stderr)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -246,6 +246,19 @@ int mps_fence_wait(mps_ctx* ctx, mps_fence* fence);
 int mps_host_alloc(mps_ctx* ctx, size_t bytes, void** out);
 int mps_host_free(mps_ctx* ctx, void* p);
 
+/* ---- per-kernel timing (replaces MTimer's debugger prints, MTimer.h:12-80) ---------------
+ * When enabled, every kernel launch is bracketed by HIP events on the context's stream.
+ * `launches` counts every launch; `useful` counts launches that did work (solver kernels
+ * enqueued past convergence exit at once and are not useful). Enabling resets the counters. */
+typedef struct mps_kernel_stat {
+    char name[48];
+    uint64_t launches;
+    uint64_t useful;
+    double total_ms;
+} mps_kernel_stat;
+int mps_profile_enable(mps_ctx* ctx, int32_t on);
+int mps_profile_read(mps_ctx* ctx, mps_kernel_stat* out, size_t capacity, size_t* count);
+
 #ifdef __cplusplus
 }
 #endif

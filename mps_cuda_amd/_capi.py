@@ -84,6 +84,10 @@ class StepStats(C.Structure):
         return d
 
 
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("useful", C.c_uint64), ("total_ms", C.c_double)]
+
+
 class Fence(C.Structure):
     _fields_ = [("event", C.c_void_p), ("frame", C.c_uint64)]
 
@@ -112,6 +116,8 @@ _SIGS = {
     "mps_fence_wait": (C.c_int, [_P, C.POINTER(Fence)]),
     "mps_host_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "mps_host_free": (C.c_int, [_P, _P]),
+    "mps_profile_enable": (C.c_int, [_P, C.c_int32]),
+    "mps_profile_read": (C.c_int, [_P, C.POINTER(KernelStat), C.c_size_t, C.POINTER(C.c_size_t)]),
 }
 
 _lib = None
@@ -269,3 +275,18 @@ class HipSim:
 
     def fence_wait(self, f: Fence):
         self._chk(self.lib.mps_fence_wait(self.h, C.byref(f)))
+
+    # ---- per-kernel timing
+    def profile(self, on: bool = True):
+        self._chk(self.lib.mps_profile_enable(self.h, int(bool(on))))
+
+    def profile_read(self) -> dict:
+        cap = 64
+        arr = (KernelStat * cap)()
+        n = C.c_size_t()
+        self._chk(self.lib.mps_profile_read(self.h, arr, cap, C.byref(n)))
+        out = {}
+        for k in range(min(n.value, cap)):
+            st = arr[k]
+            out[st.name.decode()] = dict(launches=int(st.launches), useful=int(st.useful), total_ms=float(st.total_ms))
+        return out

@@ -120,6 +120,69 @@ struct mps_ctx {
     DBuf<float4> rb_rgba;
     uint32_t pred[3] = {2, 3, 8};
 
+    // ------------------------------------------------------------------ kernel timing
+    enum KId {
+        K_KEYS, K_SORT, K_CELLS, K_NEI_FLUID, K_NEI_BND, K_BSUM, K_BVOL, K_DENSITY, K_EVAL_DF, K_EVAL_CD,
+        K_APPLY, K_ST_FACTOR, K_ST_RHS, K_CG_INIT, K_CG_APPLY, K_CG_UPDATE, K_CG_DIR, K_COPY_OUT, K_ADVECT,
+        K_ZSORT, K_JACOBI, K_COUNT
+    };
+    static const char* kname(int k) {
+        static const char* names[K_COUNT] = {
+            "k_cell_keys", "radix_sort", "k_cell_table", "k_neighbors_fluid", "k_neighbors_boundary",
+            "k_boundary_sum", "k_boundary_volume", "k_density_alpha", "k_pressure_eval<DF>", "k_pressure_eval<CD>",
+            "k_pressure_apply", "k_st_factor", "k_st_rhs", "k_cg_apply<0>", "k_cg_apply<1>", "k_cg_update",
+            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi"};
+        return names[k];
+    }
+    struct Rec {
+        int k;
+        hipEvent_t a, b;
+    };
+    bool prof_on = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> evpool;
+    double kms[K_COUNT] = {};
+    uint64_t kn[K_COUNT] = {}, kuse[K_COUNT] = {};
+    hipEvent_t ev_get() {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    // Run launch(s) f; when profiling, bracket them with events on the context stream.
+    template <class F>
+    void T_(int k, F&& f, bool always_useful = true) {
+        if (!prof_on) {
+            f();
+            return;
+        }
+        hipEvent_t a = ev_get(), b = ev_get();
+        HIPCHK(hipEventRecord(a, stream));
+        f();
+        HIPCHK(hipEventRecord(b, stream));
+        recs.push_back(Rec{k, a, b});
+        kn[k]++;
+        if (always_useful) kuse[k]++;
+    }
+    void prof_collect() {
+        for (const Rec& r : recs) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            kms[r.k] += ms;
+            evpool.push_back(r.a);
+            evpool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    void prof_reset() {
+        prof_collect();
+        for (int k = 0; k < K_COUNT; k++) { kms[k] = 0.0; kn[k] = 0; kuse[k] = 0; }
+    }
+
     // ------------------------------------------------------------------ helpers
     HashGeom geom() const {
         const double cell = cfg.cell_size > 0.0 ? double(float(cfg.cell_size)) : double(float(mat.radius));
@@ -168,15 +231,15 @@ struct mps_ctx {
         const HashGeom g = geom();
         zkey.reserve(n); zid.reserve(n); zkey2.reserve(n); zid2.reserve(n);
         ztemp.reserve(mps::launch::radix_temp_words(n));
-        mps::launch::cell_keys(p4.p, n, g, 1, zkey.p, zid.p, stream);
+        T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, zkey.p, zid.p, stream); });
         const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
         const int bits = 3 * std::min(bits_for(hmax), 10);
         uint32_t *k = zkey.p, *v = zid.p, *k2 = zkey2.p, *v2 = zid2.p;
-        mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream);
+        T_(K_ZSORT, [&] { mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream); });
         const uint32_t* perm = v;
         tmp4.reserve(n);
         auto perm4 = [&](DBuf<float4>& a) {
-            mps::launch::gather_f4(perm, n, a.p, tmp4.p, stream);
+            T_(K_ZSORT, [&] { mps::launch::gather_f4(perm, n, a.p, tmp4.p, stream); });
             std::swap(a.p, tmp4.p);
             std::swap(a.cap, tmp4.cap);
         };
@@ -210,21 +273,23 @@ struct mps_ctx {
         h.start.reserve(C);
         h.end.reserve(C);
         if (!h.cells_init) {
-            mps::launch::fill_u32(h.start.p, C, mps::dev::kEmpty, stream);
-            mps::launch::fill_u32(h.end.p, C, mps::dev::kEmpty, stream);
+            T_(K_CELLS, [&] {
+                mps::launch::fill_u32(h.start.p, C, mps::dev::kEmpty, stream);
+                mps::launch::fill_u32(h.end.p, C, mps::dev::kEmpty, stream);
+            });
             h.cells_init = true;
         } else {
-            mps::launch::clear_cells(h.key.p, h.n_valid, h.start.p, h.end.p, stream);
+            T_(K_CELLS, [&] { mps::launch::clear_cells(h.key.p, h.n_valid, h.start.p, h.end.p, stream); });
         }
         h.n_valid = 0;
         if (n == 0) return;
         h.key.reserve(n); h.id.reserve(n); h.key2.reserve(n); h.id2.reserve(n); h.spos.reserve(n);
         h.temp.reserve(mps::launch::radix_temp_words(n));
-        mps::launch::cell_keys(p4, n, geom(), 0, h.key.p, h.id.p, stream);
-        mps::launch::radix_sort(h.key.p, h.id.p, h.key2.p, h.id2.p, h.temp.p, n, bits_for(C), stream);
+        T_(K_KEYS, [&] { mps::launch::cell_keys(p4, n, geom(), 0, h.key.p, h.id.p, stream); });
+        T_(K_SORT, [&] { mps::launch::radix_sort(h.key.p, h.id.p, h.key2.p, h.id2.p, h.temp.p, n, bits_for(C), stream); });
         // key/key2 (id/id2) are always reserved together, so their capacities are equal and
         // the pointer swap inside radix_sort keeps each DBuf consistent.
-        mps::launch::cell_table(h.key.p, h.id.p, n, h.start.p, h.end.p, p4, h.spos.p, stream);
+        T_(K_CELLS, [&] { mps::launch::cell_table(h.key.p, h.id.p, n, h.start.p, h.end.p, p4, h.spos.p, stream); });
         h.n_valid = n;
     }
     void hash() {
@@ -246,14 +311,16 @@ struct mps_ctx {
             if (do_b && B) {
                 nbb.reserve(size_t(Kbb) * B);
                 cbb.reserve(B);
-                mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream);
+                T_(K_NEI_BND, [&] { mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream); });
             }
             nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
             cff.reserve(std::max<uint32_t>(N, 1));
             nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
             cfb.reserve(std::max<uint32_t>(N, 1));
-            mps::launch::neighbors_fluid(pos.p, N, g, view(fh, N), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
-                                         d_ctr, stream);
+            T_(K_NEI_FLUID, [&] {
+                mps::launch::neighbors_fluid(pos.p, N, g, view(fh, N), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
+                                             d_ctr, stream);
+            });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
             sync();
             bool grow = false;
@@ -264,7 +331,7 @@ struct mps_ctx {
                 if (do_b) {
                     nnz_bb = h_ctr->nnz_bb;
                     bS.reserve(std::max<uint32_t>(B, 1));
-                    mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, cbb.p, bS.p, stream);
+                    T_(K_BSUM, [&] { mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, cbb.p, bS.p, stream); });
                     bnd_dirty = false;
                 }
                 nnz_ff = h_ctr->nnz_ff;
@@ -277,9 +344,13 @@ struct mps_ctx {
     }
     unsigned long long nnz_ff = 0, nnz_fb = 0, nnz_bb = 0;
 
-    void boundary_volume() { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); }
+    void boundary_volume() {
+        T_(K_BVOL, [&] { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); });
+    }
     void density_alpha(int flags) {
-        mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p, stream);
+        T_(K_DENSITY, [&] {
+            mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p, stream);
+        });
     }
 
     void fetch_solver(int idx) {
@@ -303,15 +374,21 @@ struct mps_ctx {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[idx], lmin + 1);
             chunk = std::min(chunk, cap - it + 1);
             for (uint32_t c = 0; c < chunk; c++, it++) {
-                mps::launch::pressure_eval(cubic(), df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
-                                           kappa.p, delta.p, sd, partials.p, it, tol, lmin, stream);
-                mps::launch::pressure_apply(cubic(), P, nbr(), pos.p, vel.p, bpos.p, bvol.p, kappa.p, sd, stream);
+                T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
+                    mps::launch::pressure_eval(cubic(), df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
+                                               kappa.p, delta.p, sd, partials.p, it, tol, lmin, stream);
+                }, false);
+                T_(K_APPLY, [&] { mps::launch::pressure_apply(cubic(), P, nbr(), pos.p, vel.p, bpos.p, bvol.p, kappa.p, sd, stream); }, false);
             }
             fetch_solver(idx);
             done = h_solver[idx].done != 0;
         }
         const uint32_t l = done ? h_solver[idx].l : cap + 1;
         pred[idx] = std::max<uint32_t>(l, 1);
+        if (prof_on) {  // eval 1..min(l,cap) and apply 1..l-1 did work
+            kuse[df ? K_EVAL_DF : K_EVAL_CD] += std::min(l, cap);
+            kuse[K_APPLY] += l - 1;
+        }
         HIPCHK(hipMemcpyAsync(&h_solver[idx], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
         sync();
         if (st) {
@@ -330,12 +407,14 @@ struct mps_ctx {
         SolverDev* sd = &d_solver[2];
         ensure_partials();
         const StepParams P = params();
-        mps::launch::st_factor(cubic(), P, nbr(), pos.p, vel.p, fst.p, cgx.p, stream);
-        mps::launch::st_rhs(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, cgb.p, stream);
+        T_(K_ST_FACTOR, [&] { mps::launch::st_factor(cubic(), P, nbr(), pos.p, vel.p, fst.p, cgx.p, stream); });
+        T_(K_ST_RHS, [&] { mps::launch::st_rhs(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, cgb.p, stream); });
         mps::launch::solver_reset(sd, stream);
         const double tol = cfg.cg_tol;
-        mps::launch::cg_apply(cubic(), 0, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgx.p, cgAp.p, cgb.p, cgr.p, cgp.p, sd,
-                              partials.p, 0, tol, stream);
+        T_(K_CG_INIT, [&] {
+            mps::launch::cg_apply(cubic(), 0, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgx.p, cgAp.p, cgb.p, cgr.p, cgp.p, sd,
+                                  partials.p, 0, tol, stream);
+        });
         const uint32_t cap = cfg.cg_max_iter;
         uint32_t it = 1;
         bool done = false;
@@ -343,17 +422,24 @@ struct mps_ctx {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
             chunk = std::min(chunk, cap - it + 1);
             for (uint32_t c = 0; c < chunk; c++, it++) {
-                mps::launch::cg_apply(cubic(), 1, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgp.p, cgAp.p, cgb.p, cgr.p, cgp.p,
-                                      sd, partials.p, it, tol, stream);
-                mps::launch::cg_update(N, cgx.p, cgr.p, cgp.p, cgAp.p, sd, partials.p, it, tol, stream);
-                mps::launch::cg_direction(N, cgr.p, cgp.p, sd, stream);
+                T_(K_CG_APPLY, [&] {
+                    mps::launch::cg_apply(cubic(), 1, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgp.p, cgAp.p, cgb.p, cgr.p, cgp.p,
+                                          sd, partials.p, it, tol, stream);
+                }, false);
+                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(N, cgx.p, cgr.p, cgp.p, cgAp.p, sd, partials.p, it, tol, stream); }, false);
+                T_(K_CG_DIR, [&] { mps::launch::cg_direction(N, cgr.p, cgp.p, sd, stream); }, false);
             }
             fetch_solver(2);
             done = h_solver[2].done != 0;
         }
         const uint32_t l = done ? h_solver[2].l : cap + 1;
         pred[2] = std::max<uint32_t>(l, 1);
-        mps::launch::copy_vel_out(N, cgx.p, vel.p, stream);
+        if (prof_on) {  // A.p and the update ran for iterations 1..l-1, the direction for 1..l-2
+            kuse[K_CG_APPLY] += std::min(l - 1, cap);
+            kuse[K_CG_UPDATE] += std::min(l - 1, cap);
+            kuse[K_CG_DIR] += done ? (l >= 2 ? l - 2 : 0) : cap;
+        }
+        T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(N, cgx.p, vel.p, stream); });
         HIPCHK(hipMemcpyAsync(&h_solver[2], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
         sync();
         if (st) {
@@ -433,7 +519,9 @@ struct mps_ctx {
         if (cfg.visc_solver == MPS_VISC_CG) viscosity_cg(st);
         else if (cfg.visc_solver == MPS_VISC_JACOBI) viscosity_jacobi(st);
     }
-    void advect() { mps::launch::advect(params(), pos.p, vel.p, d_flags, stream); }
+    void advect() {
+        T_(K_ADVECT, [&] { mps::launch::advect(params(), pos.p, vel.p, d_flags, stream); });
+    }
 
     void run_phase(int32_t ph, mps_step_stats* st) {
         switch (ph) {
@@ -480,6 +568,7 @@ struct mps_ctx {
         st->nnz_bb = nnz_bb;
         st->nei_regrow = nei_regrow - regrow0;
         frame++;
+        if (prof_on) prof_collect();
     }
 
     // ------------------------------------------------------------------ setup
@@ -789,6 +878,8 @@ struct mps_ctx {
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
+        for (const Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
         if (d_ctr) (void)hipFree(d_ctr);
         if (d_solver) (void)hipFree(d_solver);
         if (d_flags) (void)hipFree(d_flags);
@@ -951,6 +1042,7 @@ int mps_run_phase(mps_ctx* c, int32_t phase, mps_step_stats* out) {
     return guarded(c, [&] {
         c->run_phase(phase, out);
         c->sync();
+        if (c->prof_on) c->prof_collect();
     });
 }
 
@@ -1036,6 +1128,36 @@ int mps_fence_wait(mps_ctx* c, mps_fence* fence) {
         HIPCHK(hipEventSynchronize(ev));
         HIPCHK(hipEventDestroy(ev));
         fence->event = nullptr;
+    });
+}
+
+int mps_profile_enable(mps_ctx* c, int32_t on) {
+    return guarded(c, [&] {
+        c->sync();
+        c->prof_reset();
+        c->prof_on = on != 0;
+    });
+}
+
+int mps_profile_read(mps_ctx* c, mps_kernel_stat* out, size_t capacity, size_t* count) {
+    if (!c || !count || (capacity && !out)) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        c->sync();
+        c->prof_collect();
+        size_t n = 0;
+        for (int k = 0; k < mps_ctx::K_COUNT; k++) {
+            if (c->kn[k] == 0) continue;
+            if (n < capacity) {
+                mps_kernel_stat& st = out[n];
+                std::memset(&st, 0, sizeof(st));
+                std::strncpy(st.name, mps_ctx::kname(k), sizeof(st.name) - 1);
+                st.launches = c->kn[k];
+                st.useful = c->kuse[k];
+                st.total_ms = c->kms[k];
+            }
+            n++;
+        }
+        *count = n;
     });
 }
 
