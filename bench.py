@@ -50,7 +50,7 @@ def kernel_bytes(name: str, N: int, Eff: float, Efb: float) -> float | None:
         "k_pressure_apply": 48 * N + 4 * Ef,
         "k_density_alpha": 2 * (28 * N + 4 * Ef) + 24 * N,
         "k_st_factor": 36 * N + 4 * Eff,
-        "k_st_rhs": 56 * N + 4 * Eff,
+        "k_cg_setup": 56 * N + 4 * Eff,
         "k_neighbors_fluid": 96 * N + 4 * Ef,
         "k_advect": 48 * N,
     }
@@ -155,7 +155,7 @@ def main() -> int:
             "k_cg_apply<1>": max(min(l_cg - 1, cfg.cg_max_iter), 0), "k_cg_apply<0>": 1,
             "k_pressure_eval<DF>": min(l_df, cap), "k_pressure_eval<CD>": min(l_cd, cap),
             "k_pressure_apply": (l_df - 1) + (l_cd - 1), "k_density_alpha": 1, "k_st_factor": 1,
-            "k_st_rhs": 1, "k_neighbors_fluid": 1, "k_advect": 1,
+            "k_cg_setup": 1, "k_neighbors_fluid": 1, "k_advect": 1,
         }
         for name, u in uses.items():
             kb = kernel_bytes(name, N, eff, efb)

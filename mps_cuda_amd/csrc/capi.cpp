@@ -102,9 +102,17 @@ struct mps_ctx {
     HashSet fh, bh;
     DBuf<uint32_t> zkey, zid, zkey2, zid2, ztemp, tmpu;
     DBuf<float2> tmp2;
+    // The fluid's working arrays are kept in slot order (sorted by cell key, ties by
+    // reference index). perm[s] = reference index of slot s, inv = its inverse.
+    DBuf<uint32_t> perm, inv, keyref, vals, src, lkey, hash_perm;
     // neighbour lists (column-major ELL)
     DBuf<uint32_t> nff, cff, nfb, cfb, nbb, cbb;
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
+    // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
+    DBuf<float> pairG_ff, pairG_fb;
+    DBuf<float2> pairAB;
+    DBuf<float4> cgrecA, cgrecB;  // 32-byte (position, vector) records for the CG gathers
+    bool pairs_valid = false;
     uint32_t nei_regrow = 0;
     // small device records + pinned host mirrors
     NbrCounters* d_ctr = nullptr;
@@ -123,15 +131,15 @@ struct mps_ctx {
     // ------------------------------------------------------------------ kernel timing
     enum KId {
         K_KEYS, K_SORT, K_CELLS, K_NEI_FLUID, K_NEI_BND, K_BSUM, K_BVOL, K_DENSITY, K_EVAL_DF, K_EVAL_CD,
-        K_APPLY, K_ST_FACTOR, K_ST_RHS, K_CG_INIT, K_CG_APPLY, K_CG_UPDATE, K_CG_DIR, K_COPY_OUT, K_ADVECT,
-        K_ZSORT, K_JACOBI, K_COUNT
+        K_APPLY, K_ST_FACTOR, K_CG_SETUP, K_CG_INIT, K_CG_APPLY, K_CG_UPDATE, K_CG_DIR, K_COPY_OUT, K_ADVECT,
+        K_ZSORT, K_JACOBI, K_FOLD, K_REORDER, K_COUNT
     };
     static const char* kname(int k) {
         static const char* names[K_COUNT] = {
             "k_cell_keys", "radix_sort", "k_cell_table", "k_neighbors_fluid", "k_neighbors_boundary",
             "k_boundary_sum", "k_boundary_volume", "k_density_alpha", "k_pressure_eval<DF>", "k_pressure_eval<CD>",
-            "k_pressure_apply", "k_st_factor", "k_st_rhs", "k_cg_apply<0>", "k_cg_apply<1>", "k_cg_update",
-            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi"};
+            "k_pressure_apply", "k_st_factor", "k_cg_setup", "k_cg_apply<0>", "k_cg_apply<1>", "k_cg_update",
+            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi", "k_fold", "reorder"};
         return names[k];
     }
     struct Rec {
@@ -264,9 +272,27 @@ struct mps_ctx {
             perm4(color);
         }
     }
+    // Fluid Z-sort: the reference permutes its arrays by Morton key; here the data stays in
+    // slot order and only the reference labels change (perm[s] <- new index of perm[s]).
+    void zsort_fluid() {
+        if (N == 0) return;
+        const HashGeom g = geom();
+        keyref.reserve(N); vals.reserve(N); zkey2.reserve(N); zid2.reserve(N); zid.reserve(N);
+        ztemp.reserve(mps::launch::radix_temp_words(N));
+        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
+        const int bits = 3 * std::min(bits_for(hmax), 10);
+        T_(K_ZSORT, [&] {
+            mps::launch::cell_keys_slots(pos.p, perm.p, N, g, 1, keyref.p, vals.p, stream);
+            uint32_t *k = keyref.p, *v = vals.p, *k2 = zkey2.p, *v2 = zid2.p;
+            mps::launch::radix_sort(k, v, k2, v2, ztemp.p, N, bits, stream);
+            mps::launch::invert(v, N, zid.p, stream);       // zid[old ref] = new ref
+            mps::launch::relabel(perm.p, zid.p, N, stream);
+            mps::launch::invert(perm.p, N, inv.p, stream);
+        });
+    }
     void zsort() {
         zsort_set(B, bpos, true);
-        zsort_set(N, pos, false);
+        zsort_fluid();
     }
     void hash_set(HashSet& h, uint32_t n, const float4* p4) {
         const size_t C = cells();
@@ -292,11 +318,97 @@ struct mps_ctx {
         T_(K_CELLS, [&] { mps::launch::cell_table(h.key.p, h.id.p, n, h.start.p, h.end.p, p4, h.spos.p, stream); });
         h.n_valid = n;
     }
+    // Fluid UpdateHash with slot-order storage. Slots are sorted by the Morton code of their
+    // cell (values = reference ids, stable => ties by reference id), so every cell is a
+    // contiguous slot range in ascending reference order — the candidate order of the
+    // reference's sorted (key, id) table — while blocks of slots are compact in space. The
+    // cell start/end table is indexed by the reference's linear key. The reference's
+    // linear-order key/id arrays are rebuilt from lkey + perm only when exported.
+    void hash_fluid() {
+        HashSet& h = fh;
+        const size_t C = cells();
+        const HashGeom g = geom();
+        h.start.reserve(C);
+        h.end.reserve(C);
+        if (!h.cells_init) {
+            T_(K_CELLS, [&] {
+                mps::launch::fill_u32(h.start.p, C, mps::dev::kEmpty, stream);
+                mps::launch::fill_u32(h.end.p, C, mps::dev::kEmpty, stream);
+            });
+            h.cells_init = true;
+        } else {
+            T_(K_CELLS, [&] { mps::launch::clear_cells_m(h.key.p, lkey.p, h.n_valid, h.start.p, h.end.p, stream); });
+        }
+        h.n_valid = 0;
+        if (N == 0) return;
+        keyref.reserve(N); vals.reserve(N); src.reserve(N); lkey.reserve(N); hash_perm.reserve(N);
+        h.key.reserve(N); h.id.reserve(N); h.key2.reserve(N); h.id2.reserve(N);
+        h.temp.reserve(mps::launch::radix_temp_words(N));
+        T_(K_KEYS, [&] { mps::launch::cell_morton_slots(pos.p, perm.p, N, g, keyref.p, vals.p, stream); });
+        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
+        const int bits = 3 * std::min(bits_for(hmax), 10);
+        uint32_t *k = keyref.p, *v = vals.p, *k2 = h.key2.p, *v2 = h.id2.p;
+        T_(K_SORT, [&] { mps::launch::radix_sort(k, v, k2, v2, h.temp.p, N, bits, stream); });
+        if (k == keyref.p) {
+            std::swap(h.key, keyref);
+            std::swap(h.id, vals);
+        } else {
+            std::swap(h.key, h.key2);
+            std::swap(h.id, h.id2);
+        }
+        T_(K_REORDER, [&] {
+            mps::launch::compose(h.id.p, inv.p, N, src.p, stream);
+            auto mv4 = [&](DBuf<float4>& a) {
+                mps::launch::gather_f4(src.p, N, a.p, tmp4.p, stream);
+                std::swap(a, tmp4);
+            };
+            auto mv1 = [&](DBuf<float>& a) {
+                mps::launch::gather_f1(src.p, N, a.p, tmp1.p, stream);
+                std::swap(a, tmp1);
+            };
+            tmp4.reserve(N);
+            tmp1.reserve(N);
+            mv4(pos); mv4(vel); mv4(color); mv4(cgb);
+            mv1(rho); mv1(alpha); mv1(kappa); mv1(delta); mv1(fst);
+            HIPCHK(hipMemcpyAsync(perm.p, h.id.p, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(hash_perm.p, h.id.p, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
+            mps::launch::invert(perm.p, N, inv.p, stream);
+        });
+        T_(K_CELLS, [&] { mps::launch::cell_table_m(h.key.p, pos.p, N, g, h.start.p, h.end.p, lkey.p, stream); });
+        h.n_valid = N;
+        pairs_valid = false;
+    }
+    // The reference's fluid hash in its own layout (keys sorted by linear cell index, ties by
+    // reference id; MPSSpatialHash.cu:53-77), rebuilt on the host from the last hash.
+    void fluid_hash_export(std::vector<uint32_t>* keys, std::vector<uint32_t>* ids) {
+        const uint32_t n = fh.n_valid;
+        std::vector<uint32_t> lk = down(lkey.p, n), hp = down(hash_perm.p, n);
+        std::vector<uint64_t> kv(n);
+        for (uint32_t s = 0; s < n; s++) kv[s] = (uint64_t(lk[s]) << 32) | hp[s];
+        std::sort(kv.begin(), kv.end());
+        if (keys) { keys->resize(n); for (uint32_t s = 0; s < n; s++) (*keys)[s] = uint32_t(kv[s] >> 32); }
+        if (ids) { ids->resize(n); for (uint32_t s = 0; s < n; s++) (*ids)[s] = uint32_t(kv[s]); }
+    }
+    std::vector<uint32_t> fluid_cells_export(bool want_start) {
+        std::vector<uint32_t> keys;
+        fluid_hash_export(&keys, nullptr);
+        std::vector<uint32_t> out(fh.cells_init ? cells() : 0, mps::dev::kEmpty);
+        const uint32_t n = uint32_t(keys.size());
+        for (uint32_t s = 0; s < n; s++) {
+            if (s == 0 || keys[s - 1] != keys[s]) {
+                if (want_start) out[keys[s]] = s;
+                else if (s > 0) out[keys[s - 1]] = s;
+            }
+            if (s == n - 1 && !want_start) out[keys[s]] = n;
+        }
+        return out;
+    }
     void hash() {
         if (bnd_dirty || !bh.cells_init) hash_set(bh, B, bpos.p);
-        hash_set(fh, N, pos.p);
+        hash_fluid();
     }
     HashView view(const HashSet& h, uint32_t n) const { return HashView{h.start.p, h.end.p, h.id.p, h.spos.p, n}; }
+    HashView fluid_view() const { return HashView{fh.start.p, fh.end.p, nullptr, pos.p, N}; }
 
     void reset_counters() { mps::launch::zero_words(reinterpret_cast<uint32_t*>(d_ctr), sizeof(NbrCounters) / 4, stream); }
 
@@ -318,7 +430,7 @@ struct mps_ctx {
             nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
             cfb.reserve(std::max<uint32_t>(N, 1));
             T_(K_NEI_FLUID, [&] {
-                mps::launch::neighbors_fluid(pos.p, N, g, view(fh, N), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
+                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
                                              d_ctr, stream);
             });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
@@ -336,6 +448,7 @@ struct mps_ctx {
                 }
                 nnz_ff = h_ctr->nnz_ff;
                 nnz_fb = h_ctr->nnz_fb;
+                pairs_valid = false;
                 return;
             }
             nei_regrow++;
@@ -348,9 +461,20 @@ struct mps_ctx {
         T_(K_BVOL, [&] { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); });
     }
     void density_alpha(int flags) {
+        if (flags & (mps::launch::kDoAlpha | mps::launch::kDoPairs)) {
+            flags |= mps::launch::kDoPairs;
+            const size_t m = std::max<uint32_t>(N, 1);
+            pairG_ff.reserve(size_t(Kff) * m);
+            pairG_fb.reserve(size_t(Kfb) * m);
+        }
         T_(K_DENSITY, [&] {
-            mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p, stream);
+            mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
+                                       pairG_ff.p, pairG_fb.p, stream);
         });
+        if (flags & mps::launch::kDoPairs) pairs_valid = true;
+    }
+    void ensure_pairs() {
+        if (!pairs_valid) density_alpha(mps::launch::kDoPairs);
     }
 
     void fetch_solver(int idx) {
@@ -364,6 +488,7 @@ struct mps_ctx {
         SolverDev* sd = &d_solver[idx];
         mps::launch::solver_reset(sd, stream);
         ensure_partials();
+        ensure_pairs();
         const StepParams P = params();
         const double tol = df ? cfg.df_tol : cfg.cd_tol;
         const uint32_t lmin = df ? cfg.df_min_iter : cfg.cd_min_iter;
@@ -375,18 +500,22 @@ struct mps_ctx {
             chunk = std::min(chunk, cap - it + 1);
             for (uint32_t c = 0; c < chunk; c++, it++) {
                 T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
-                    mps::launch::pressure_eval(cubic(), df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
-                                               kappa.p, delta.p, sd, partials.p, it, tol, lmin, stream);
+                    mps::launch::pressure_eval(df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p, pairG_ff.p,
+                                               pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
                 }, false);
-                T_(K_APPLY, [&] { mps::launch::pressure_apply(cubic(), P, nbr(), pos.p, vel.p, bpos.p, bvol.p, kappa.p, sd, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_pressure(partials.p, N, sd, it, tol, lmin, stream); }, false);
+                T_(K_APPLY, [&] {
+                    mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd, stream);
+                }, false);
             }
             fetch_solver(idx);
             done = h_solver[idx].done != 0;
         }
         const uint32_t l = done ? h_solver[idx].l : cap + 1;
         pred[idx] = std::max<uint32_t>(l, 1);
-        if (prof_on) {  // eval 1..min(l,cap) and apply 1..l-1 did work
+        if (prof_on) {  // eval and fold 1..min(l,cap), apply 1..l-1 did work
             kuse[df ? K_EVAL_DF : K_EVAL_CD] += std::min(l, cap);
+            kuse[K_FOLD] += std::min(l, cap);
             kuse[K_APPLY] += l - 1;
         }
         HIPCHK(hipMemcpyAsync(&h_solver[idx], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
@@ -406,15 +535,23 @@ struct mps_ctx {
         if (N == 0) return 0;
         SolverDev* sd = &d_solver[2];
         ensure_partials();
+        ensure_pairs();
         const StepParams P = params();
-        T_(K_ST_FACTOR, [&] { mps::launch::st_factor(cubic(), P, nbr(), pos.p, vel.p, fst.p, cgx.p, stream); });
-        T_(K_ST_RHS, [&] { mps::launch::st_rhs(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, cgb.p, stream); });
+        const size_t m = std::max<uint32_t>(N, 1);
+        pairAB.reserve(size_t(Kff) * m);
+        cgrecA.reserve(2 * m);
+        cgrecB.reserve(2 * m);
+        T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
+        T_(K_CG_SETUP, [&] {
+            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, pairAB.p, cgb.p, stream);
+        });
         mps::launch::solver_reset(sd, stream);
         const double tol = cfg.cg_tol;
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_apply(cubic(), 0, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgx.p, cgAp.p, cgb.p, cgr.p, cgp.p, sd,
-                                  partials.p, 0, tol, stream);
+            mps::launch::cg_pack(N, pos.p, cgx.p, cgrecA.p, stream);
+            mps::launch::cg_apply(0, P, nbr(), pairAB.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p, stream);
         });
+        T_(K_FOLD, [&] { mps::launch::fold_cg(0, partials.p, N, sd, 0, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
         uint32_t it = 1;
         bool done = false;
@@ -423,11 +560,13 @@ struct mps_ctx {
             chunk = std::min(chunk, cap - it + 1);
             for (uint32_t c = 0; c < chunk; c++, it++) {
                 T_(K_CG_APPLY, [&] {
-                    mps::launch::cg_apply(cubic(), 1, P, nbr(), pos.p, vel.p, rho.p, fst.p, cgp.p, cgAp.p, cgb.p, cgr.p, cgp.p,
-                                          sd, partials.p, it, tol, stream);
+                    mps::launch::cg_apply(1, P, nbr(), pairAB.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd, partials.p,
+                                          stream);
                 }, false);
-                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(N, cgx.p, cgr.p, cgp.p, cgAp.p, sd, partials.p, it, tol, stream); }, false);
-                T_(K_CG_DIR, [&] { mps::launch::cg_direction(N, cgr.p, cgp.p, sd, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_cg(1, partials.p, N, sd, it, tol, stream); }, false);
+                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(N, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_cg(2, partials.p, N, sd, it, tol, stream); }, false);
+                T_(K_CG_DIR, [&] { mps::launch::cg_direction(N, cgr.p, cgrecB.p, sd, stream); }, false);
             }
             fetch_solver(2);
             done = h_solver[2].done != 0;
@@ -437,6 +576,7 @@ struct mps_ctx {
         if (prof_on) {  // A.p and the update ran for iterations 1..l-1, the direction for 1..l-2
             kuse[K_CG_APPLY] += std::min(l - 1, cap);
             kuse[K_CG_UPDATE] += std::min(l - 1, cap);
+            kuse[K_FOLD] += 2 * std::min(l - 1, cap);
             kuse[K_CG_DIR] += done ? (l >= 2 ? l - 2 : 0) : cap;
         }
         T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(N, cgx.p, vel.p, stream); });
@@ -576,7 +716,7 @@ struct mps_ctx {
         const size_t m = std::max<uint32_t>(n, 1);
         pos.reserve(m); vel.reserve(m); color.reserve(m); cgb.reserve(m); cgx.reserve(m); cgr.reserve(m); cgp.reserve(m);
         cgAp.reserve(m); rho.reserve(m); alpha.reserve(m); kappa.reserve(m); delta.reserve(m); fst.reserve(m);
-        cff.reserve(m); cfb.reserve(m);
+        cff.reserve(m); cfb.reserve(m); perm.reserve(m); inv.reserve(m);
     }
     void alloc_boundary(uint32_t n) {
         const size_t m = std::max<uint32_t>(n, 1);
@@ -604,9 +744,12 @@ struct mps_ctx {
             mps::launch::zero_words(reinterpret_cast<uint32_t*>(cgb.p), 4 * n, stream);
             mps::launch::zero_words(cff.p, n, stream);
             mps::launch::zero_words(cfb.p, n, stream);
+            mps::launch::iota(perm.p, uint32_t(n), stream);
+            mps::launch::iota(inv.p, uint32_t(n), stream);
         }
         fh.cells_init = false;
         fh.n_valid = 0;
+        pairs_valid = false;
         frame = 0;
         pred[0] = 2; pred[1] = 3; pred[2] = 8;
         sync();
@@ -756,6 +899,26 @@ struct mps_ctx {
             sync();
         }
     }
+    // CSR in reference order from slot-order ELL lists; fluid neighbour slots are mapped to
+    // reference indices (map_ids), boundary neighbours are already reference indices.
+    std::vector<uint32_t> csr_idx_fluid(const uint32_t* cnt) {
+        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, N), idx(size_t(N) + 1, 0u);
+        for (uint32_t i = 0; i < N; i++) idx[i + 1] = idx[i] + c[iv[i]];
+        return idx;
+    }
+    std::vector<uint32_t> csr_nei_fluid(const uint32_t* nbrs, const uint32_t* cnt, uint32_t K, bool map_ids) {
+        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, N), pm = down(perm.p, N);
+        std::vector<uint32_t> ell = down(nbrs, size_t(K) * N);
+        std::vector<uint32_t> out;
+        for (uint32_t i = 0; i < N; i++) {
+            const uint32_t sl = iv[i];
+            for (uint32_t k = 0; k < c[sl]; k++) {
+                const uint32_t j = ell[size_t(k) * N + sl];
+                out.push_back(map_ids ? pm[j] : j);
+            }
+        }
+        return out;
+    }
     std::vector<uint32_t> csr_idx(const uint32_t* cnt, uint32_t n) {
         std::vector<uint32_t> c = down(cnt, n), idx(size_t(n) + 1, 0u);
         for (uint32_t i = 0; i < n; i++) idx[i + 1] = idx[i] + c[i];
@@ -771,6 +934,54 @@ struct mps_ctx {
     }
     // returns (count, elem_bytes) and optionally the data as bytes
     void array_io(int32_t id, size_t* count, size_t* eb, void* dst, const void* src, size_t bytes) {
+        // fluid arrays live in slot order; the API shows reference order (out[perm[s]] = a[s])
+        std::vector<uint32_t> hperm;
+        auto get_perm = [&]() -> const std::vector<uint32_t>& {
+            if (hperm.empty() && N) hperm = down(perm.p, N);
+            return hperm;
+        };
+        auto fl4 = [&](DBuf<float4>& a, int comp /* -1 = xyz, 3 = w, 4 = all */) {
+            const size_t w = comp == -1 ? 3 : (comp == 4 ? 4 : 1);
+            *count = N;
+            *eb = 4 * w;
+            if (!(dst || src)) return;
+            if (bytes != size_t(N) * 4 * w) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            const auto& P = get_perm();
+            std::vector<float4> h = down(a.p, N);
+            if (dst) {
+                float* d = static_cast<float*>(dst);
+                for (uint32_t s = 0; s < N; s++) {
+                    const float q[4] = {h[s].x, h[s].y, h[s].z, h[s].w};
+                    if (comp == -1) for (int k = 0; k < 3; k++) d[3 * size_t(P[s]) + k] = q[k];
+                    else if (comp == 4) for (int k = 0; k < 4; k++) d[4 * size_t(P[s]) + k] = q[k];
+                    else d[P[s]] = q[3];
+                }
+            } else {
+                const float* q = static_cast<const float*>(src);
+                for (uint32_t s = 0; s < N; s++) {
+                    if (comp == -1) { h[s].x = q[3 * size_t(P[s])]; h[s].y = q[3 * size_t(P[s]) + 1]; h[s].z = q[3 * size_t(P[s]) + 2]; }
+                    else if (comp == 4) { h[s].x = q[4 * size_t(P[s])]; h[s].y = q[4 * size_t(P[s]) + 1]; h[s].z = q[4 * size_t(P[s]) + 2]; h[s].w = q[4 * size_t(P[s]) + 3]; }
+                    else h[s].w = q[P[s]];
+                }
+                up(a.p, h);
+            }
+        };
+        auto fl1 = [&](DBuf<float>& a) {
+            *count = N;
+            *eb = 4;
+            if (!(dst || src)) return;
+            if (bytes != size_t(N) * 4) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            const auto& P = get_perm();
+            std::vector<float> h = down(a.p, N);
+            if (dst) {
+                float* d = static_cast<float*>(dst);
+                for (uint32_t s = 0; s < N; s++) d[P[s]] = h[s];
+            } else {
+                const float* q = static_cast<const float*>(src);
+                for (uint32_t s = 0; s < N; s++) h[s] = q[P[s]];
+                up(a.p, h);
+            }
+        };
         auto f4xyz = [&](DBuf<float4>& a, uint32_t n) {
             *count = n;
             *eb = 12;
@@ -828,17 +1039,23 @@ struct mps_ctx {
             raw(const_cast<uint32_t*>(p), n, 4);
         };
         switch (id) {
-            case MPS_ARR_FLUID_POS: f4xyz(pos, N); break;
-            case MPS_ARR_FLUID_VEL: f4xyz(vel, N); break;
-            case MPS_ARR_FLUID_RADIUS: f4w(pos, N); break;
-            case MPS_ARR_FLUID_MASS: f4w(vel, N); break;
-            case MPS_ARR_FLUID_COLOR: raw(color.p, N, 16); break;
-            case MPS_ARR_FLUID_DENSITY: raw(rho.p, N, 4); break;
-            case MPS_ARR_FLUID_ALPHA: raw(alpha.p, N, 4); break;
-            case MPS_ARR_FLUID_KAPPA: raw(kappa.p, N, 4); break;
-            case MPS_ARR_FLUID_DELTA: raw(delta.p, N, 4); break;
-            case MPS_ARR_FLUID_FACTOR_ST: raw(fst.p, N, 4); break;
-            case MPS_ARR_FLUID_CG_B: f4xyz(cgb, N); break;
+            case MPS_ARR_FLUID_POS:
+                fl4(pos, -1);
+                if (src) pairs_valid = false;
+                break;
+            case MPS_ARR_FLUID_VEL: fl4(vel, -1); break;
+            case MPS_ARR_FLUID_RADIUS:
+                fl4(pos, 3);
+                if (src) pairs_valid = false;
+                break;
+            case MPS_ARR_FLUID_MASS: fl4(vel, 3); break;
+            case MPS_ARR_FLUID_COLOR: fl4(color, 4); break;
+            case MPS_ARR_FLUID_DENSITY: fl1(rho); break;
+            case MPS_ARR_FLUID_ALPHA: fl1(alpha); break;
+            case MPS_ARR_FLUID_KAPPA: fl1(kappa); break;
+            case MPS_ARR_FLUID_DELTA: fl1(delta); break;
+            case MPS_ARR_FLUID_FACTOR_ST: fl1(fst); break;
+            case MPS_ARR_FLUID_CG_B: fl4(cgb, -1); break;
             case MPS_ARR_BND_POS:
                 f4xyz(bpos, B);
                 if (src) bnd_dirty = true;
@@ -851,18 +1068,28 @@ struct mps_ctx {
             case MPS_ARR_BND_MASS: f4w(bvel, B); break;
             case MPS_ARR_BND_VEL: f4xyz(bvel, B); break;
             case MPS_ARR_BND_FACE_ID: raw(bface.p, B, 4); break;
-            case MPS_ARR_FHASH_KEY: ro_raw(fh.key.p, fh.n_valid); break;
-            case MPS_ARR_FHASH_ID: ro_raw(fh.id.p, fh.n_valid); break;
-            case MPS_ARR_FHASH_START: ro_raw(fh.start.p, fh.cells_init ? cells() : 0); break;
-            case MPS_ARR_FHASH_END: ro_raw(fh.end.p, fh.cells_init ? cells() : 0); break;
+            case MPS_ARR_FHASH_KEY: {
+                std::vector<uint32_t> v;
+                if (dst) fluid_hash_export(&v, nullptr); else v.resize(fh.n_valid);
+                ro(v);
+                break;
+            }
+            case MPS_ARR_FHASH_ID: {
+                std::vector<uint32_t> v;
+                if (dst) fluid_hash_export(nullptr, &v); else v.resize(fh.n_valid);
+                ro(v);
+                break;
+            }
+            case MPS_ARR_FHASH_START: ro(dst ? fluid_cells_export(true) : std::vector<uint32_t>(fh.cells_init ? cells() : 0)); break;
+            case MPS_ARR_FHASH_END: ro(dst ? fluid_cells_export(false) : std::vector<uint32_t>(fh.cells_init ? cells() : 0)); break;
             case MPS_ARR_BHASH_KEY: ro_raw(bh.key.p, bh.n_valid); break;
             case MPS_ARR_BHASH_ID: ro_raw(bh.id.p, bh.n_valid); break;
             case MPS_ARR_BHASH_START: ro_raw(bh.start.p, bh.cells_init ? cells() : 0); break;
             case MPS_ARR_BHASH_END: ro_raw(bh.end.p, bh.cells_init ? cells() : 0); break;
-            case MPS_ARR_NEI_FF_IDX: ro(csr_idx(cff.p, N)); break;
-            case MPS_ARR_NEI_FF: ro(csr_nei(nff.p, cff.p, N, Kff)); break;
-            case MPS_ARR_NEI_FB_IDX: ro(csr_idx(cfb.p, N)); break;
-            case MPS_ARR_NEI_FB: ro(csr_nei(nfb.p, cfb.p, N, Kfb)); break;
+            case MPS_ARR_NEI_FF_IDX: ro(csr_idx_fluid(cff.p)); break;
+            case MPS_ARR_NEI_FF: ro(csr_nei_fluid(nff.p, cff.p, Kff, true)); break;
+            case MPS_ARR_NEI_FB_IDX: ro(csr_idx_fluid(cfb.p)); break;
+            case MPS_ARR_NEI_FB: ro(csr_nei_fluid(nfb.p, cfb.p, Kfb, false)); break;
             case MPS_ARR_NEI_BB_IDX: ro(csr_idx(cbb.p, B)); break;
             case MPS_ARR_NEI_BB: ro(csr_nei(nbb.p, cbb.p, B, Kbb)); break;
             default: throw ApiError{MPS_ERR_INVALID, "unknown array id"};
@@ -877,7 +1104,9 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
+        pairG_ff.release(); pairG_fb.release(); pairAB.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
+        perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         for (const Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
         for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
         if (d_ctr) (void)hipFree(d_ctr);
@@ -1102,15 +1331,17 @@ int mps_readback_async(mps_ctx* c, float* host_pos, float* host_radius, float* h
         const uint32_t n = c->N;
         if (host_pos) c->rb_xyz.reserve(3 * size_t(std::max<uint32_t>(n, 1)));
         if (host_radius) c->rb_rad.reserve(std::max<uint32_t>(n, 1));
-        mps::launch::pack_readback(n, c->pos.p, c->color.p, host_pos ? c->rb_xyz.p : nullptr,
-                                   host_radius ? c->rb_rad.p : nullptr, nullptr, c->stream);
+        if (host_rgba) c->rb_rgba.reserve(std::max<uint32_t>(n, 1));
+        mps::launch::pack_readback(n, c->pos.p, c->color.p, c->perm.p, host_pos ? c->rb_xyz.p : nullptr,
+                                   host_radius ? c->rb_rad.p : nullptr,
+                                   host_rgba ? reinterpret_cast<float*>(c->rb_rgba.p) : nullptr, c->stream);
         if (n) {
             if (host_pos)
                 HIPCHK(hipMemcpyAsync(host_pos, c->rb_xyz.p, 3 * sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
             if (host_radius)
                 HIPCHK(hipMemcpyAsync(host_radius, c->rb_rad.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
             if (host_rgba)
-                HIPCHK(hipMemcpyAsync(host_rgba, c->color.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(host_rgba, c->rb_rgba.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
         }
         hipEvent_t ev;
         HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));

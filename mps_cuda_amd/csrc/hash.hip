@@ -39,29 +39,58 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_hist(const uint32_t* __r
     hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of a u32 array of length L in one 1024-thread block (L is small:
-// 256 * number of sort tiles).
-__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t* __restrict__ a, uint32_t L) {
-    __shared__ uint32_t s[1024];
-    const uint32_t per = (L + 1023u) / 1024u;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = min(b + per, L);
-    uint32_t sum = 0u;
-    for (uint32_t k = b; k < e; k++) sum += a[k];
-    s[threadIdx.x] = sum;
+// Exclusive scan of the digit histogram (length L = 256 * tiles), reduce-then-scan over
+// 1024-element chunks: chunk sums, one-block scan of the sums, chunk scans.
+constexpr int kScanChunk = 1024;
+__device__ __forceinline__ uint32_t block_incl_scan_1024(uint32_t v, uint32_t* lds /*[16]*/, uint32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    if (lane == 63) lds[w] = v;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024u; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += v;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+            const uint32_t t = lds[q];
+            lds[q] = run;
+            run += t;
+        }
+        lds[16] = run;
     }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t k = b; k < e; k++) {
-        const uint32_t v = a[k];
-        a[k] = run;
-        run += v;
+    __syncthreads();
+    v += lds[w];
+    *total = lds[16];
+    __syncthreads();
+    return v;
+}
+__global__ void __launch_bounds__(kScanChunk) k_scan_sums(const uint32_t* __restrict__ a, uint32_t L, uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[17];
+    const uint32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+    uint32_t tot;
+    block_incl_scan_1024(i < L ? a[i] : 0u, lds, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ void __launch_bounds__(kScanChunk) k_scan_top(uint32_t* __restrict__ sums, uint32_t n) {
+    __shared__ uint32_t lds[17];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < n; base += kScanChunk) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < n ? sums[i] : 0u;
+        uint32_t tot;
+        const uint32_t inc = block_incl_scan_1024(v, lds, &tot);
+        if (i < n) sums[i] = carry + inc - v;
+        carry += tot;
     }
+}
+__global__ void __launch_bounds__(kScanChunk) k_scan_apply(uint32_t* __restrict__ a, uint32_t L, const uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[17];
+    const uint32_t i = blockIdx.x * kScanChunk + threadIdx.x;
+    const uint32_t v = i < L ? a[i] : 0u;
+    uint32_t tot;
+    const uint32_t inc = block_incl_scan_1024(v, lds, &tot);
+    if (i < L) a[i] = sums[blockIdx.x] + inc - v;
 }
 
 // Stable scatter: items are ranked round by round (256 per round, wave-major, lane-minor),
@@ -123,6 +152,86 @@ __global__ void __launch_bounds__(kBlock) k_cell_keys(const float4* __restrict__
     id[i] = i;
 }
 
+// Keys of particles held in slot order, written at their reference index: keyref[perm[s]].
+__global__ void __launch_bounds__(kBlock) k_cell_keys_slots(const float4* __restrict__ pos, const uint32_t* __restrict__ perm,
+                                                          uint32_t n, HashGeom g, int morton, uint32_t* __restrict__ keyref,
+                                                          uint32_t* __restrict__ iota) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = pos[s];
+    const i3 c = grid_pos(p.x, p.y, p.z, g);
+    keyref[perm[s]] = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
+    iota[s] = s;
+}
+// Morton code of the (wrapped) cell of each slot, written at the slot's reference index;
+// sorting by it keeps each cell contiguous and blocks of slots spatially compact.
+__device__ __forceinline__ uint32_t cell_morton(i3 c, const HashGeom& g) {
+    const uint32_t x = static_cast<uint32_t>(c.x) & (g.hx - 1u);
+    const uint32_t y = static_cast<uint32_t>(c.y) & (g.hy - 1u);
+    const uint32_t z = static_cast<uint32_t>(c.z) & (g.hz - 1u);
+    return split_by3(x) | (split_by3(y) << 1u) | (split_by3(z) << 2u);
+}
+__global__ void __launch_bounds__(kBlock) k_cell_morton_slots(const float4* __restrict__ pos, const uint32_t* __restrict__ perm,
+                                                            uint32_t n, HashGeom g, uint32_t* __restrict__ keyref,
+                                                            uint32_t* __restrict__ iota) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = pos[s];
+    keyref[perm[s]] = cell_morton(grid_pos(p.x, p.y, p.z, g), g);
+    iota[s] = s;
+}
+// Cell table over Morton-sorted slots: start/end are indexed by the reference's linear cell
+// key; lkey[s] keeps each slot's linear key (for clearing and for exporting the hash).
+__global__ void __launch_bounds__(kBlock) k_cell_table_m(const uint32_t* __restrict__ mkey, const float4* __restrict__ pos,
+                                                       uint32_t n, HashGeom g, uint32_t* __restrict__ start,
+                                                       uint32_t* __restrict__ end, uint32_t* __restrict__ lkey) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = pos[s];
+    const i3 c = grid_pos(p.x, p.y, p.z, g);
+    const uint32_t k = grid_index(c.x, c.y, c.z, g);
+    lkey[s] = k;
+    const uint32_t m = mkey[s];
+    if (s == 0 || mkey[s - 1] != m) {
+        start[k] = s;
+        if (s > 0) {
+            const float4 q = pos[s - 1];
+            const i3 cq = grid_pos(q.x, q.y, q.z, g);
+            end[grid_index(cq.x, cq.y, cq.z, g)] = s;
+        }
+    }
+    if (s == n - 1) end[k] = n;
+}
+__global__ void __launch_bounds__(kBlock) k_clear_cells_m(const uint32_t* __restrict__ mkey, const uint32_t* __restrict__ lkey,
+                                                        uint32_t n, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (s == 0 || mkey[s - 1] != mkey[s]) {
+        start[lkey[s]] = kEmpty;
+        end[lkey[s]] = kEmpty;
+    }
+}
+// inv[perm[s]] = s
+__global__ void __launch_bounds__(kBlock) k_invert(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ inv) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) inv[perm[s]] = s;
+}
+// src[s'] = inv[perm_new[s']]: old slot of the particle that moves to slot s'
+__global__ void __launch_bounds__(kBlock) k_compose(const uint32_t* __restrict__ perm_new, const uint32_t* __restrict__ inv,
+                                                  uint32_t n, uint32_t* __restrict__ src) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) src[s] = inv[perm_new[s]];
+}
+// perm[s] = map[perm[s]]  (relabel reference ids after a Z-sort)
+__global__ void __launch_bounds__(kBlock) k_relabel(uint32_t* __restrict__ perm, const uint32_t* __restrict__ map, uint32_t n) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) perm[s] = map[perm[s]];
+}
+__global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ p, uint32_t n) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) p[s] = s;
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -152,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_table(const uint32_t* __restric
         if (s > 0) end[key[s - 1]] = s;
     }
     if (s == n - 1) end[k] = n;
-    spos[s] = pos[id[s]];
+    if (spos) spos[s] = pos[id[s]];
 }
 
 // 27-cell scan (Research, MPSSpatialHash.cuh:47-80): z outer, y, x inner; slots in key order.
@@ -174,10 +283,18 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
                 for (; s < e; s++) {
                     const float4 q = spos[s];
                     const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
-                    const float dist = __builtin_sqrtf((ddx * ddx + ddy * ddy) + ddz * ddz);
+                    const float dd = (ddx * ddx + ddy * ddy) + ddz * ddz;
                     const float rj = q.w * 4.0f;
-                    if (dist < (ri + rj) * 0.5f) {
-                        const uint32_t j = ids[s];
+                    const float thr = (ri + rj) * 0.5f;
+                    const float t2 = thr * thr;
+                    // sqrt is monotone and correctly rounded: far from the cut-off the squared
+                    // distance decides; near it the exact test of the reference runs.
+                    bool in;
+                    if (dd < t2 * 0.9999f) in = true;
+                    else if (dd > t2 * 1.0001f) in = false;
+                    else in = __builtin_sqrtf(dd) < thr;
+                    if (in) {
+                        const uint32_t j = ids ? ids[s] : s;
                         if (SELF && j == i) continue;
                         if (k < K) out[size_t(k) * stride + i] = j;
                         k++;
@@ -260,7 +377,10 @@ using namespace dev;
 
 static inline unsigned nb(size_t n, unsigned b = kBlock) { return unsigned((n + b - 1) / b); }
 
-size_t radix_temp_words(uint32_t n) { return size_t(256) * ((n + kSortTile - 1) / kSortTile) + 1; }
+size_t radix_temp_words(uint32_t n) {
+    const size_t L = size_t(256) * ((n + kSortTile - 1) / kSortTile);
+    return L + (L + kScanChunk - 1) / kScanChunk + 1;
+}
 
 int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,
                int key_bits, hipStream_t s) {
@@ -269,7 +389,11 @@ int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*&
     const uint32_t L = 256u * nblk;
     for (int shift = 0; shift < key_bits; shift += 8) {
         k_radix_hist<<<nblk, kSortThreads, 0, s>>>(keys, n, shift, temp, nblk);
-        k_scan_exclusive<<<1, 1024, 0, s>>>(temp, L);
+        const uint32_t nch = (L + kScanChunk - 1) / kScanChunk;
+        uint32_t* sums = temp + L;
+        k_scan_sums<<<nch, kScanChunk, 0, s>>>(temp, L, sums);
+        k_scan_top<<<1, kScanChunk, 0, s>>>(sums, nch);
+        k_scan_apply<<<nch, kScanChunk, 0, s>>>(temp, L, sums);
         k_radix_scatter<<<nblk, kSortThreads, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, temp, nblk);
         std::swap(keys, keys_alt);
         std::swap(vals, vals_alt);
@@ -279,6 +403,33 @@ int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*&
 
 void cell_keys(const float4* pos, uint32_t n, const HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s) {
     if (n) k_cell_keys<<<nb(n), kBlock, 0, s>>>(pos, n, g, morton, key, id);
+}
+void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, int morton, uint32_t* keyref,
+                     uint32_t* iota, hipStream_t s) {
+    if (n) k_cell_keys_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, morton, keyref, iota);
+}
+void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, uint32_t* keyref,
+                       uint32_t* iota, hipStream_t s) {
+    if (n) k_cell_morton_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, keyref, iota);
+}
+void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const HashGeom& g, uint32_t* start, uint32_t* end,
+                  uint32_t* lkey, hipStream_t s) {
+    if (n) k_cell_table_m<<<nb(n), kBlock, 0, s>>>(mkey, pos, n, g, start, end, lkey);
+}
+void clear_cells_m(const uint32_t* mkey, const uint32_t* lkey, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s) {
+    if (n) k_clear_cells_m<<<nb(n), kBlock, 0, s>>>(mkey, lkey, n, start, end);
+}
+void invert(const uint32_t* perm, uint32_t n, uint32_t* inv, hipStream_t s) {
+    if (n) k_invert<<<nb(n), kBlock, 0, s>>>(perm, n, inv);
+}
+void compose(const uint32_t* perm_new, const uint32_t* inv, uint32_t n, uint32_t* src, hipStream_t s) {
+    if (n) k_compose<<<nb(n), kBlock, 0, s>>>(perm_new, inv, n, src);
+}
+void relabel(uint32_t* perm, const uint32_t* map, uint32_t n, hipStream_t s) {
+    if (n) k_relabel<<<nb(n), kBlock, 0, s>>>(perm, map, n);
+}
+void iota(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n) k_iota<<<nb(n), kBlock, 0, s>>>(p, n);
 }
 void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
     if (n) k_fill_u32<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n, v);

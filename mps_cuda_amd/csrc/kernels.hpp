@@ -13,7 +13,8 @@ namespace dev {
 struct HashGeom;
 }
 
-// Read-only view of one set's hash (cell table + ids and positions in key order).
+// Read-only view of one set's hash (cell table + ids and positions in key order). For the
+// fluid, whose working arrays are kept in key order, `id` is null and candidates are slots.
 struct HashView {
     const uint32_t* start;
     const uint32_t* end;
@@ -28,12 +29,12 @@ struct NbrCounters {
     unsigned long long nnz_ff, nnz_fb, nnz_bb;
 };
 
-// Device-resident state of one iterative solve. The last workgroup of each reduction
-// kernel finalises the sums here, so the host only needs to look at `done`/`l`.
+// Device-resident state of one iterative solve. The fold kernels finalise the sums here,
+// so the host only needs to look at `done`/`l`.
 struct SolverDev {
     uint32_t done;     // loop finished (converged); later kernels of the chunk exit at once
     uint32_t l;        // reference loop variable when the loop ended (or next iteration)
-    uint32_t counter;  // last-workgroup ticket
+    uint32_t counter;  // unused (kept for layout)
     uint32_t pad;
     double rr;         // CG: r.r of the current residual
     double pAp;        // CG: p.Ap
@@ -65,6 +66,17 @@ size_t radix_temp_words(uint32_t n);
 int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,
                int key_bits, hipStream_t s);
 void cell_keys(const float4* pos, uint32_t n, const dev::HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s);
+void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, int morton, uint32_t* keyref,
+                     uint32_t* iota, hipStream_t s);
+void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, uint32_t* keyref,
+                       uint32_t* iota, hipStream_t s);
+void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const dev::HashGeom& g, uint32_t* start, uint32_t* end,
+                  uint32_t* lkey, hipStream_t s);
+void clear_cells_m(const uint32_t* mkey, const uint32_t* lkey, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
+void invert(const uint32_t* perm, uint32_t n, uint32_t* inv, hipStream_t s);
+void compose(const uint32_t* perm_new, const uint32_t* inv, uint32_t n, uint32_t* src, hipStream_t s);
+void relabel(uint32_t* perm, const uint32_t* map, uint32_t n, hipStream_t s);
+void iota(uint32_t* p, uint32_t n, hipStream_t s);
 void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
 void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
@@ -80,29 +92,36 @@ void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, 
 void gather_f1(const uint32_t* perm, uint32_t n, const float* in, float* out, hipStream_t s);
 
 // ---- sph.hip
-enum DensityFlags { kDoDensity = 1, kDoAlpha = 2, kDoGravity = 4 };
+enum DensityFlags { kDoDensity = 1, kDoAlpha = 2, kDoGravity = 4, kDoPairs = 8 };
 void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, const uint32_t* cbb, float* S,
                   hipStream_t s);
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
+// density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
-                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, hipStream_t s);
-void pressure_eval(int cubic, int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
-                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
-                   float* kappa, float* delta, SolverDev* st, double* partials, uint32_t it, double tol,
-                   uint32_t lmin, hipStream_t s);
-void pressure_apply(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
-                    const float* bvol, const float* kappa, const SolverDev* st, hipStream_t s);
-void st_factor(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, float* fst,
+                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
+                   float* pG_fb, hipStream_t s);
+void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+                   const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
+                   const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
+void fold_pressure(const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, uint32_t lmin, hipStream_t s);
+void pressure_apply(const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+                    const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
+                    hipStream_t s);
+void st_factor(const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s);
-void st_rhs(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
-            const float* fst, float4* cgb, hipStream_t s);
-// mode 0: A x0 with f = 0, then r = b - A x0, p = r, r.r; mode 1: A p with f, p.Ap, alpha
-void cg_apply(int cubic, int mode, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
-              const float* rho, const float* fst, const float4* in, float4* Ap, const float4* b, float4* r, float4* p,
-              SolverDev* st, double* partials, uint32_t it, double tol, hipStream_t s);
-void cg_update(uint32_t N, float4* x, float4* r, const float4* p, const float4* Ap, SolverDev* st, double* partials,
-               uint32_t it, double tol, hipStream_t s);
-void cg_direction(uint32_t N, const float4* r, float4* p, const SolverDev* st, hipStream_t s);
+void cg_setup(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+              const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s);
+// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
+// mode 0: A x0 with f = 0 (x0 in `in`), then r = b - A x0, p = r (into `out`), partial r.r;
+// mode 1: A p with f (p in `in`), partial p.Ap
+void cg_apply(int mode, const StepParams& P, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
+              const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
+// mode 0: initial r.r + test; 1: alpha; 2: new r.r, beta + test
+void fold_cg(int mode, const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, hipStream_t s);
+void cg_update(uint32_t N, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st, double* partials,
+               hipStream_t s);
+void cg_direction(uint32_t N, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);
 void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s);
 void jacobi_candidate(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
                       const float* rho, const float4* u, float4* out, hipStream_t s);
@@ -115,7 +134,8 @@ void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* p
 void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s);
 void solver_reset(SolverDev* st, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
-void pack_readback(uint32_t n, const float4* pos, const float4* color, float* xyz, float* rad, float* rgba, hipStream_t s);
+void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
+                   float* rgba, hipStream_t s);
 
 // ---- sampling.hip
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,

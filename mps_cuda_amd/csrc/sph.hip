@@ -7,11 +7,17 @@
 // same neighbour list are fused (density + alpha + gravity; delta + stiffness + error
 // reduction; pressure force + velocity update; ST + XSPH + viscosity + final A.p).
 //
-// Convergence sums: each workgroup reduces its T-rounded terms in double, writes one
-// partial, and the last workgroup to finish (ticket counter) adds the partials in a fixed
-// order — deterministic, no FP64 atomics — and updates a SolverDev record on the device.
-// Kernels of later iterations read SolverDev::done first and exit at once, so the host can
-// enqueue several iterations before it looks at the result.
+// Quantities that stay fixed while a solver iterates are computed once per step or per
+// solve and kept per neighbour slot: the kernel gradient factor G_ij (written by the alpha
+// pass, read by every DF/CD iteration) and the two scalar coefficients of the implicit
+// viscosity/surface-tension operator (written by the CG set-up pass), so an iteration
+// streams 4-8 bytes per pair instead of re-evaluating square roots and divisions.
+//
+// Convergence sums: each workgroup reduces its T-rounded terms in double and writes one
+// partial; a one-workgroup fold kernel adds the partials in index order (deterministic, no
+// FP64 atomics) and updates a SolverDev record on the device. Kernels of later iterations
+// read SolverDev::done first and exit at once, so the host can enqueue several iterations
+// before it looks at the result.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -27,32 +33,20 @@ __device__ __forceinline__ uint32_t nbr_fb(const Nbr& nb, uint32_t k, uint32_t i
     return nb.fb[size_t(k) * N + i];
 }
 
-// Block-reduce `v` (double), publish the partial, and let the last workgroup fold all
-// partials in index order and call fin(total) in thread 0.
-template <class Fin>
-__device__ __forceinline__ void reduce_last(double v, double* __restrict__ partials, uint32_t* counter, Fin fin) {
+// Block-reduce `v` (double) in a fixed order and publish the workgroup's partial.
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partials) {
     __shared__ double lds[kBlock / 64];
-    __shared__ int am_last;
     const double bs = block_sum_d(v, lds);
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = bs;
-        __threadfence();
-        const uint32_t t = atomicAdd(counter, 1u);
-        am_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (am_last) {
-        __threadfence();
-        double s = 0.0;
-        for (uint32_t k = threadIdx.x; k < gridDim.x; k += blockDim.x)
-            s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s = block_sum_d(s, lds);
-        if (threadIdx.x == 0) {
-            fin(s);
-            __threadfence();
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+// Sum of partials[0..n) in index-strided order, valid in thread 0 (one workgroup of
+// kFoldThreads).
+constexpr int kFoldThreads = 1024;
+__device__ __forceinline__ double fold(const double* __restrict__ partials, uint32_t n) {
+    __shared__ double lds[kFoldThreads / 64];
+    double s = 0.0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k];
+    return block_sum_d(s, lds);
 }
 
 // ----------------------------------------------------------------- boundary volume
@@ -89,15 +83,18 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ComputeDensity_kernel ff/fb + Final + DensityColorTest (MPSSPHBaseUtil.cuh:88-221),
 // ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
+// Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams P, Nbr nb, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
-                                                        float* __restrict__ alpha, float4* __restrict__ color) {
+                                                        float* __restrict__ alpha, float4* __restrict__ color,
+                                                        float* __restrict__ pG_ff, float* __restrict__ pG_fb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
     const bool dens = flags & launch::kDoDensity, alph = flags & launch::kDoAlpha;
+    const bool pairs = flags & launch::kDoPairs;
     const float4 xi4 = pos[i];
     const f3 xi = mk(xi4);
     const float hi = xi4.w * 4.0f;
@@ -105,19 +102,32 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
     float sf = dens ? P.V * Wk<CUBIC>(0.0f, invHi) : 0.0f;
     f3 gf{0.f, 0.f, 0.f};
     float s = 0.0f;
-    if (dens || alph) {
+    if (dens || alph || pairs) {
         const uint32_t c = nb.cff[i];
-        for (uint32_t k = 0; k < c; k++) {
-            const uint32_t j = nbr_ff(nb, k, i, N);
-            const f3 xij = xi - mk(pos[j]);
+        auto pair = [&](uint32_t k, float4 xj4) {
+            const f3 xij = xi - mk(xj4);
             const float dist = len3(xij);
             if (dens) sf += P.V * Wk<CUBIC>(dist, invHi);
-            if (alph) {
-                const f3 grad = (P.V * Gk<CUBIC>(dist, invHi)) * xij;
+            if (alph || pairs) {
+                const float G = Gk<CUBIC>(dist, invHi);
+                if (pairs) pG_ff[size_t(k) * N + i] = G;
+                const f3 grad = (P.V * G) * xij;
                 gf = gf + grad;
                 s += dot3(grad, grad);
             }
+        };
+        uint32_t k = 0;
+        for (; k + 4 <= c; k += 4) {
+            uint32_t j[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) j[q] = nbr_ff(nb, k + q, i, N);
+            float4 xj[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) xj[q] = pos[j[q]];
+#pragma unroll
+            for (int q = 0; q < 4; q++) pair(k + q, xj[q]);
         }
+        for (; k < c; k++) pair(k, pos[nbr_ff(nb, k, i, N)]);
         float sb = 0.0f;
         f3 gb{0.f, 0.f, 0.f};
         const uint32_t cb = nb.cfb[i];
@@ -127,7 +137,11 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
             const float dist = len3(xij);
             const float vb = bvol[j];
             if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
-            if (alph) gb = gb + (vb * Gk<CUBIC>(dist, invHi)) * xij;
+            if (alph || pairs) {
+                const float G = Gk<CUBIC>(dist, invHi);
+                if (pairs) pG_fb[size_t(k) * N + i] = G;
+                gb = gb + (vb * G) * xij;
+            }
         }
         if (dens) {
             float d = 0.0f;
@@ -167,38 +181,58 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
 
 // ----------------------------------------------------------------- pressure solves
 // ComputeDensityDelta_kernel ff/fb (MPSSPHDFSPHUtil.cuh:139-257) fused with
-// ComputeDFStiffness_kernel / ComputeCDStiffness_kernel (:259-340) and the mean-error
-// readback of the drivers (MPSSPHDFSPHUtil.cu:331-333,417-419).
-template <bool CUBIC, bool DF>
+// ComputeDFStiffness_kernel / ComputeCDStiffness_kernel (:259-340); G_ij comes from the
+// alpha pass. Each workgroup publishes its sum of positive errors for k_fold_pressure.
+template <bool DF>
 __global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, const float4* __restrict__ pos,
                                                         const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, const float4* __restrict__ bvel,
                                                         const float* __restrict__ rho, const float* __restrict__ alpha,
-                                                        float* __restrict__ kappa, float* __restrict__ delta, SolverDev* st,
-                                                        double* __restrict__ partials, uint32_t it, double tol, uint32_t lmin) {
+                                                        const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
+                                                        float* __restrict__ kappa, float* __restrict__ delta,
+                                                        const SolverDev* st, double* __restrict__ partials) {
     if (st->done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     double epos = 0.0;
     if (i < N) {
-        const float4 xi4 = pos[i];
-        const f3 xi = mk(xi4), vi = mk(vel[i]);
-        const float invHi = 1.0f / (xi4.w * 4.0f);
+        const f3 xi = mk(pos[i]), vi = mk(vel[i]);
         float df = 0.0f;
         const uint32_t c = nb.cff[i];
-        for (uint32_t k = 0; k < c; k++) {
-            const uint32_t j = nbr_ff(nb, k, i, N);
+        uint32_t k = 0;
+        for (; k + 4 <= c; k += 4) {
+            uint32_t j[4];
+            float G[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const size_t slot = size_t(k + q) * N + i;
+                j[q] = nb.ff[slot];
+                G[q] = pG_ff[slot];
+            }
+            float4 xj[4], vj[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                xj[q] = pos[j[q]];
+                vj[q] = vel[j[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) df += P.V * G[q] * dot3(xi - mk(xj[q]), vi - mk(vj[q]));
+        }
+        for (; k < c; k++) {
+            const size_t slot = size_t(k) * N + i;
+            const uint32_t j = nb.ff[slot];
             const f3 xij = xi - mk(pos[j]);
             const f3 vij = vi - mk(vel[j]);
-            df += P.V * Gk<CUBIC>(len3(xij), invHi) * dot3(xij, vij);
+            df += P.V * pG_ff[slot] * dot3(xij, vij);
         }
         float db = 0.0f;
         const uint32_t cb = nb.cfb[i];
         for (uint32_t k = 0; k < cb; k++) {
-            const uint32_t j = nbr_fb(nb, k, i, N);
+            const size_t slot = size_t(k) * N + i;
+            const uint32_t j = nb.fb[slot];
             const f3 xij = xi - mk(bpos[j]);
             const f3 vij = vi - mk(bvel[j]);
-            db += bvol[j] * Gk<CUBIC>(len3(xij), invHi) * dot3(xij, vij);
+            db += bvol[j] * pG_fb[slot] * dot3(xij, vij);
         }
         float d = 0.0f;
         d += df;
@@ -220,7 +254,16 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, 
         }
         kappa[i] = e;
     }
-    reduce_last(epos, partials, &st->counter, [&](double s) {
+    block_partial(epos, partials);
+}
+
+// Mean positive error and the loop decision of the DF/CD drivers
+// (MPSSPHDFSPHUtil.cu:331-333,417-419): break when E < tol and l > lmin.
+__global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(const double* __restrict__ partials, uint32_t nblk, SolverDev* st,
+                                                        uint32_t it, double tol, uint32_t lmin, uint32_t N) {
+    if (st->done) return;
+    const double s = fold(partials, nblk);
+    if (threadIdx.x == 0) {
         const double E = s / double(N);
         if (it - 1 < MPS_MAX_ITER_LOG) st->log[it - 1] = E;
         if (E < tol && it > lmin) {
@@ -229,36 +272,56 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, 
         } else {
             st->l = it + 1;
         }
-    });
+    }
 }
 
 // ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
-template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_pressure_apply(StepParams P, Nbr nb, const float4* __restrict__ pos, float4* __restrict__ vel,
                                                          const float4* __restrict__ bpos, const float* __restrict__ bvol,
+                                                         const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
                                                          const float* __restrict__ kappa, const SolverDev* st) {
     if (st->done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
-    const float4 xi4 = pos[i];
-    const f3 xi = mk(xi4);
-    const float invHi = 1.0f / (xi4.w * 4.0f);
+    const f3 xi = mk(pos[i]);
     const float pi = kappa[i];
     f3 ff{0.f, 0.f, 0.f};
     const uint32_t c = nb.cff[i];
-    for (uint32_t k = 0; k < c; k++) {
-        const uint32_t j = nbr_ff(nb, k, i, N);
+    uint32_t k = 0;
+    for (; k + 4 <= c; k += 4) {
+        uint32_t j[4];
+        float G[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const size_t slot = size_t(k + q) * N + i;
+            j[q] = nb.ff[slot];
+            G[q] = pG_ff[slot];
+        }
+        float4 xj[4];
+        float kj[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xj[q] = pos[j[q]];
+            kj[q] = kappa[j[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) ff = ff - 0.5f * P.V * (pi + kj[q]) * G[q] * (xi - mk(xj[q]));
+    }
+    for (; k < c; k++) {
+        const size_t slot = size_t(k) * N + i;
+        const uint32_t j = nb.ff[slot];
         const f3 xij = xi - mk(pos[j]);
-        const f3 fij = 0.5f * P.V * (pi + kappa[j]) * Gk<CUBIC>(len3(xij), invHi) * xij;
+        const f3 fij = 0.5f * P.V * (pi + kappa[j]) * pG_ff[slot] * xij;
         ff = ff - fij;
     }
     f3 fb{0.f, 0.f, 0.f};
     const uint32_t cb = nb.cfb[i];
     for (uint32_t k = 0; k < cb; k++) {
-        const uint32_t j = nbr_fb(nb, k, i, N);
+        const size_t slot = size_t(k) * N + i;
+        const uint32_t j = nb.fb[slot];
         const f3 xij = xi - mk(bpos[j]);
-        const f3 fij = 0.5f * bvol[j] * pi * Gk<CUBIC>(len3(xij), invHi) * xij;
+        const f3 fij = 0.5f * bvol[j] * pi * pG_fb[slot] * xij;
         fb = fb - fij;
     }
     f3 F{0.f, 0.f, 0.f};
@@ -277,26 +340,44 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(StepParams P, Nbr nb,
 }
 
 // ----------------------------------------------------------------- viscosity + surface tension CG
-// ComputeSurfaceTensionFactor_kernel (MPSSPHNonPressureUtil.cuh:425-465); also x0 = v
-// (the copy into pPredictVel, MPSSPHNonPressureUtil.cu:144-147).
-template <bool CUBIC>
+// ComputeSurfaceTensionFactor_kernel (MPSSPHNonPressureUtil.cuh:425-465) with the stored
+// G_ij; also x0 = v (the copy into pPredictVel, MPSSPHNonPressureUtil.cu:144-147).
 __global__ void __launch_bounds__(kBlock) k_st_factor(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                    float* __restrict__ fst, float4* __restrict__ cgx) {
+                                                    const float* __restrict__ pG_ff, float* __restrict__ fst,
+                                                    float4* __restrict__ cgx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
-    const float4 xi4 = pos[i];
     const float4 vi4 = vel[i];
-    const f3 xi = mk(xi4), vi = mk(vi4);
-    const float invHi = 1.0f / (xi4.w * 4.0f);
+    const f3 xi = mk(pos[i]), vi = mk(vi4);
     float d = 0.0f;
     const uint32_t c = nb.cff[i];
-    for (uint32_t k = 0; k < c; k++) {
-        const uint32_t j = nbr_ff(nb, k, i, N);
+    uint32_t k = 0;
+    for (; k + 4 <= c; k += 4) {
+        uint32_t j[4];
+        float G[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const size_t slot = size_t(k + q) * N + i;
+            j[q] = nb.ff[slot];
+            G[q] = pG_ff[slot];
+        }
+        float4 xj[4], vj[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xj[q] = pos[j[q]];
+            vj[q] = vel[j[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) d += vj[q].w * dot3(vi - mk(vj[q]), G[q] * (xi - mk(xj[q])));
+    }
+    for (; k < c; k++) {
+        const size_t slot = size_t(k) * N + i;
+        const uint32_t j = nb.ff[slot];
         const f3 xij = xi - mk(pos[j]);
         const float4 vj4 = vel[j];
         const f3 vij = vi - mk(vj4);
-        const f3 grad = Gk<CUBIC>(len3(xij), invHi) * xij;
+        const f3 grad = pG_ff[slot] * xij;
         d += vj4.w * dot3(vij, grad);
     }
     fst[i] = d;
@@ -313,10 +394,16 @@ __device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float
     return (stW + P.dt * (dot3(vij, stG) - (fi + fj) * invDij * 0.5f * stW)) * invDij;
 }
 
-// ComputeSurfaceTensionCGb_kernel (MPSSPHNonPressureUtil.cuh:467-521)
+// Once per solve: right-hand side (ComputeSurfaceTensionCGb_kernel, :467-521) and the pair
+// coefficients of the implicit operator (ST :523-585, viscosity :587-635, final :637-648):
+//   a_ij = dt (cs_i m_ij g_ij + cx (m_j/rho_j) W_ij),   b_ij = -dt cv_i c_ij,
+//   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON),
+//   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_st_rhs(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                 const float* __restrict__ rho, const float* __restrict__ fst, float4* __restrict__ cgb) {
+__global__ void __launch_bounds__(kBlock) k_cg_setup(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                   const float* __restrict__ rho, const float* __restrict__ fst,
+                                                   const float* __restrict__ pG_ff, float2* __restrict__ cab,
+                                                   float4* __restrict__ cgb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
@@ -324,111 +411,178 @@ __global__ void __launch_bounds__(kBlock) k_st_rhs(StepParams P, Nbr nb, const f
     const float4 vi4 = vel[i];
     const f3 xi = mk(xi4), vi = mk(vi4);
     const float mi = vi4.w, di = rho[i], fi = fst[i];
-    const float invHi = 1.0f / (xi4.w * 4.0f);
+    const float hi = xi4.w * 4.0f;
+    const float invHi = 1.0f / hi;
+    const float onePercentHSq = 0.01f * hi * hi;
+    const float cs = P.dt * P.sigma / mi;
+    const float cx = P.xsph / P.dt;
+    const float cv = 10.0f * P.rho0 * P.mu / di;
     f3 bi{0.f, 0.f, 0.f};
     const uint32_t c = nb.cff[i];
-    for (uint32_t k = 0; k < c; k++) {
-        const uint32_t j = nbr_ff(nb, k, i, N);
-        const f3 xij = xi - mk(pos[j]);
-        const float4 vj4 = vel[j];
+    auto pair = [&](size_t slot, float4 xj4, float4 vj4, float dj, float fj, float G) {
+        const f3 xij = xi - mk(xj4);
+        const float mj = vj4.w;
         const float dist = len3(xij);
         float mij;
-        const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, vj4.w, di, rho[j], fi, fst[j], mij);
-        bi = bi - (mij * gij) * xij;
+        const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, mj, di, dj, fi, fj, mij);
+        const float sij = mij * gij;
+        const float wij = mj / dj * Wk<CUBIC>(dist, invHi);
+        const float cij = 0.5f * (mi + mj) / dj / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
+        cab[slot] = make_float2(P.dt * (cs * sij + cx * wij), -P.dt * cv * cij);
+        bi = bi - sij * xij;
+    };
+    uint32_t k = 0;
+    for (; k + 4 <= c; k += 4) {
+        uint32_t j[4];
+        float G[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const size_t slot = size_t(k + q) * N + i;
+            j[q] = nb.ff[slot];
+            G[q] = pG_ff[slot];
+        }
+        float4 xj[4], vj[4];
+        float dj[4], fj[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xj[q] = pos[j[q]];
+            vj[q] = vel[j[q]];
+            dj[q] = rho[j[q]];
+            fj[q] = fst[j[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) pair(size_t(k + q) * N + i, xj[q], vj[q], dj[q], fj[q], G[q]);
+    }
+    for (; k < c; k++) {
+        const size_t slot = size_t(k) * N + i;
+        const uint32_t j = nb.ff[slot];
+        pair(slot, pos[j], vel[j], rho[j], fst[j], pG_ff[slot]);
     }
     const f3 b = (P.dt * P.sigma / mi) * bi + vi;
     cgb[i] = make_float4(b.x, b.y, b.z, 0.0f);
 }
 
-// A.p = (1+f) p + dt [ ST + XSPH + viscosity ]
-// (ComputeSurfaceTensionCGAp_kernel :523-585, ComputeCGViscosityAp_kernel :587-635,
-//  ComputeFinalCGAp_kernel :637-648). MODE 0 = initial residual (x0, f = 0) fused with
-// InitCGResidual_kernel (MPSCGKernel.cuh:5-19); MODE 1 = iteration, fused with the p.Ap
-// half of ComputeCGAlphaParam_kernel (MPSCGKernel.cuh:21-67).
-template <bool CUBIC, int MODE>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const float4* __restrict__ pos,
-                                                   const float4* __restrict__ vel, const float* __restrict__ rho,
-                                                   const float* __restrict__ fst, const float4* __restrict__ in,
+// CG vectors that neighbours read are kept in 32-byte records with the position, so one
+// gather per pair brings both x_j and p_j (one cache line): rec[2j] = (x, y, z, p.x),
+// rec[2j+1] = (p.y, p.z, 0, 0).
+__device__ __forceinline__ f3 rec_p(float4 a, float4 b) { return {a.w, b.x, b.y}; }
+__global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __restrict__ pos, const float4* __restrict__ x0,
+                                                  float4* __restrict__ rec) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 x = pos[i], v = x0[i];
+    rec[2 * size_t(i)] = make_float4(x.x, x.y, x.z, v.x);
+    rec[2 * size_t(i) + 1] = make_float4(v.y, v.z, 0.0f, 0.0f);
+}
+
+// (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ].
+// MODE 0: A x0 with f = 0 (x0 in `in`) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
+//         r = p = b - A x0 (p written to `out`), partial sums of r.r.
+// MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const float2* __restrict__ cab,
+                                                   const float4* __restrict__ in, float4* __restrict__ out,
                                                    float4* __restrict__ Ap, const float4* __restrict__ b,
-                                                   float4* __restrict__ r, float4* __restrict__ p, SolverDev* st,
-                                                   double* __restrict__ partials, uint32_t it, double tol) {
+                                                   float4* __restrict__ r, const SolverDev* st,
+                                                   double* __restrict__ partials) {
     if (MODE == 1 && st->done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     double part = 0.0;
     if (i < N) {
-        const float4 xi4 = pos[i];
-        const float4 vi4 = vel[i];
-        const f3 xi = mk(xi4), vi = mk(vi4);
-        const float mi = vi4.w, di = rho[i], fi = fst[i];
-        const float hi = xi4.w * 4.0f;
-        const float invHi = 1.0f / hi;
-        const float4 pi4 = in[i];
-        const f3 pi = mk(pi4);
-        f3 st_{0.f, 0.f, 0.f}, xs{0.f, 0.f, 0.f}, vis{0.f, 0.f, 0.f};
-        const float onePercentHSq = 0.01f * hi * hi;
+        const float4 ra = in[2 * size_t(i)], rb = in[2 * size_t(i) + 1];
+        const f3 xi = mk(ra);
+        const f3 pi = rec_p(ra, rb);
+        f3 acc{0.f, 0.f, 0.f};
         const uint32_t c = nb.cff[i];
-        for (uint32_t k = 0; k < c; k++) {
-            const uint32_t j = nbr_ff(nb, k, i, N);
-            const f3 xij = xi - mk(pos[j]);
-            const float4 vj4 = vel[j];
-            const float mj = vj4.w, dj = rho[j];
-            const float dist = len3(xij);
-            float mij;
-            const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, mj, di, dj, fi, fst[j], mij);
-            const f3 pij = pi - mk(in[j]);
-            st_ = st_ + (mij * gij) * pij;
-            xs = xs + (mj / dj * pij) * Wk<CUBIC>(dist, invHi);
-            const f3 dpij = 0.5f * (mi + mj) / dj * dot3(pij, xij) / (dist * dist + onePercentHSq) * Gk<CUBIC>(dist, invHi) /
-                            (dist + kFltEps) * xij;
-            vis = vis - dpij;
+        uint32_t k = 0;
+        // eight neighbours per trip: all index/coefficient loads, then all gathers, then the
+        // sums in list order (same result as the one-at-a-time loop)
+        constexpr int U = 8;
+        for (; k + U <= c; k += U) {
+            uint32_t j[U];
+            float2 ab[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const size_t slot = size_t(k + q) * N + i;
+                j[q] = nb.ff[slot];
+                ab[q] = cab[slot];
+            }
+            float4 ja[U], jb[U];
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                ja[q] = in[2 * size_t(j[q])];
+                jb[q] = in[2 * size_t(j[q]) + 1];
+            }
+#pragma unroll
+            for (int q = 0; q < U; q++) {
+                const f3 xij = xi - mk(ja[q]);
+                const f3 pij = pi - rec_p(ja[q], jb[q]);
+                acc = acc + (ab[q].x * pij + (ab[q].y * dot3(pij, xij)) * xij);
+            }
         }
-        st_ = st_ * (P.dt * P.sigma / mi);
-        xs = xs * (P.xsph / P.dt);
-        vis = vis * (10.0f * P.rho0 * P.mu / di);
-        f3 a{0.f, 0.f, 0.f};
-        a = a + (st_ + xs);
-        a = a + vis;
+        for (; k < c; k++) {
+            const size_t slot = size_t(k) * N + i;
+            const uint32_t j = nb.ff[slot];
+            const float2 ab = cab[slot];
+            const float4 ja = in[2 * size_t(j)], jb = in[2 * size_t(j) + 1];
+            const f3 xij = xi - mk(ja);
+            const f3 pij = pi - rec_p(ja, jb);
+            acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
+        }
         const float f = MODE == 0 ? 0.0f : P.cg_factor;
-        const f3 out = pi * (1.0f + f) + P.dt * a;
+        const f3 o = pi * (1.0f + f) + acc;
         if (MODE == 0) {
             const float4 bi = b[i];
-            const f3 ri{bi.x - out.x, bi.y - out.y, bi.z - out.z};
+            const f3 ri{bi.x - o.x, bi.y - o.y, bi.z - o.z};
             r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
-            p[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
+            out[2 * size_t(i)] = make_float4(xi.x, xi.y, xi.z, ri.x);
+            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, 0.0f, 0.0f);
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
-            Ap[i] = make_float4(out.x, out.y, out.z, 0.0f);
-            part = double(pi.x * out.x) + double(pi.y * out.y) + double(pi.z * out.z);
+            Ap[i] = make_float4(o.x, o.y, o.z, 0.0f);
+            part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
     }
-    reduce_last(part, partials, &st->counter, [&](double s) {
-        if (MODE == 0) {
-            st->rr = s;
-            st->log[0] = s;
-            if (s < tol) {
-                st->done = 1u;
-                st->l = 1u;
-            } else {
-                st->l = 1u;
-            }
-        } else {
-            st->pAp = s;
-            st->alpha = float(st->rr / s);
-        }
-    });
+    block_partial(part, partials);
 }
 
-// UpdateCGResidual_kernel + ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121)
+// Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
+// first convergence test; 1: alpha = r.r / p.Ap; 2: r.r of the new residual, beta, test.
+template <int MODE>
+__global__ void __launch_bounds__(kFoldThreads) k_fold_cg(const double* __restrict__ partials, uint32_t nblk, SolverDev* st, uint32_t it,
+                                                  double tol) {
+    if (MODE != 0 && st->done) return;
+    const double s = fold(partials, nblk);
+    if (threadIdx.x != 0) return;
+    if (MODE == 0) {
+        st->rr = s;
+        st->log[0] = s;
+        st->l = 1u;
+        if (s < tol) st->done = 1u;
+    } else if (MODE == 1) {
+        st->pAp = s;
+        st->alpha = float(st->rr / s);
+    } else {
+        st->beta = float(s / st->rr);
+        st->rr = s;
+        if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
+        st->l = it + 1;
+        if (s < tol) st->done = 1u;
+    }
+}
+
+// UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121)
 __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t N, float4* __restrict__ x, float4* __restrict__ r,
-                                                    const float4* __restrict__ p, const float4* __restrict__ Ap, SolverDev* st,
-                                                    double* __restrict__ partials, uint32_t it, double tol) {
+                                                    const float4* __restrict__ prec, const float4* __restrict__ Ap,
+                                                    const SolverDev* st, double* __restrict__ partials) {
     if (st->done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const float a = st->alpha;
     double part = 0.0;
     if (i < N) {
-        const float4 pi = p[i], api = Ap[i];
+        const f3 pi = rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]);
+        const float4 api = Ap[i];
         float4 xi = x[i], ri = r[i];
         xi.x = xi.x + a * pi.x;
         xi.y = xi.y + a * pi.y;
@@ -440,28 +594,23 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t N, float4* __rest
         r[i] = ri;
         part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
     }
-    reduce_last(part, partials, &st->counter, [&](double s) {
-        st->beta = float(s / st->rr);
-        st->rr = s;
-        if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
-        st->l = it + 1;
-        if (s < tol) st->done = 1u;
-    });
+    block_partial(part, partials);
 }
 
-// UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133)
-__global__ void __launch_bounds__(kBlock) k_cg_direction(uint32_t N, const float4* __restrict__ r, float4* __restrict__ p,
+// UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133), p kept in the position records
+__global__ void __launch_bounds__(kBlock) k_cg_direction(uint32_t N, const float4* __restrict__ r, float4* __restrict__ prec,
                                                        const SolverDev* st) {
     if (st->done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const float bta = st->beta;
     const float4 ri = r[i];
-    float4 pi = p[i];
-    pi.x = ri.x + bta * pi.x;
-    pi.y = ri.y + bta * pi.y;
-    pi.z = ri.z + bta * pi.z;
-    p[i] = pi;
+    float4 a = prec[2 * size_t(i)], b = prec[2 * size_t(i) + 1];
+    a.w = ri.x + bta * a.w;
+    b.x = ri.y + bta * b.x;
+    b.y = ri.z + bta * b.y;
+    prec[2 * size_t(i)] = a;
+    prec[2 * size_t(i) + 1] = b;
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy_vel_out(uint32_t N, const float4* __restrict__ x, float4* __restrict__ vel) {
@@ -653,17 +802,19 @@ __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n) {
 // Render feed: positions and radii de-interleaved for the host (replaces the GL buffers
 // SPHRenderer binds as SSBOs, MPSSPHRenderer.cpp:22-29).
 __global__ void __launch_bounds__(kBlock) k_pack_readback(uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ color,
-                                                        float* __restrict__ xyz, float* __restrict__ rad, float* __restrict__ rgba) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
+                                                        const uint32_t* __restrict__ perm, float* __restrict__ xyz,
+                                                        float* __restrict__ rad, float* __restrict__ rgba) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];  // reference index of slot s
+    const float4 p = pos[s];
     if (xyz) {
         xyz[3 * size_t(i)] = p.x;
         xyz[3 * size_t(i) + 1] = p.y;
         xyz[3 * size_t(i) + 2] = p.z;
     }
     if (rad) rad[i] = p.w;
-    if (rgba) reinterpret_cast<float4*>(rgba)[i] = color[i];
+    if (rgba) reinterpret_cast<float4*>(rgba)[i] = color[s];
 }
 
 }  // namespace dev
@@ -683,62 +834,60 @@ void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipSt
     if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
 }
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
-                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, hipStream_t s) {
+                   const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
+                   float* pG_fb, hipStream_t s) {
     if (!P.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color);
-    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color);
+    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
-void pressure_eval(int cubic, int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
-                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
-                   float* kappa, float* delta, SolverDev* st, double* partials, uint32_t it, double tol, uint32_t lmin,
-                   hipStream_t s) {
+void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+                   const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
+                   const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s) {
     if (!P.N) return;
     const unsigned g = nb_(P.N);
-    if (cubic) {
-        if (df) k_pressure_eval<true, true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
-        else k_pressure_eval<true, false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
-    } else {
-        if (df) k_pressure_eval<false, true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
-        else k_pressure_eval<false, false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, kappa, delta, st, partials, it, tol, lmin);
-    }
+    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
-void pressure_apply(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
-                    const float* bvol, const float* kappa, const SolverDev* st, hipStream_t s) {
-    if (!P.N) return;
-    if (cubic) k_pressure_apply<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, kappa, st);
-    else k_pressure_apply<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, kappa, st);
+void fold_pressure(const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, uint32_t lmin, hipStream_t s) {
+    if (N) k_fold_pressure<<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol, lmin, N);
 }
-void st_factor(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, float* fst,
+void pressure_apply(const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+                    const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
+                    hipStream_t s) {
+    if (P.N) k_pressure_apply<<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
+}
+void st_factor(const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {
-    if (!P.N) return;
-    if (cubic) k_st_factor<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, fst, cgx);
-    else k_st_factor<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, fst, cgx);
+    if (P.N) k_st_factor<<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, pG_ff, fst, cgx);
 }
-void st_rhs(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
-            const float* fst, float4* cgb, hipStream_t s) {
+void cg_setup(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+              const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s) {
     if (!P.N) return;
-    if (cubic) k_st_rhs<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, cgb);
-    else k_st_rhs<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, cgb);
+    if (cubic) k_cg_setup<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
+    else k_cg_setup<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
 }
-void cg_apply(int cubic, int mode, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
-              const float* rho, const float* fst, const float4* in, float4* Ap, const float4* b, float4* r, float4* p,
-              SolverDev* st, double* partials, uint32_t it, double tol, hipStream_t s) {
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
+    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
+}
+void cg_apply(int mode, const StepParams& P, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
+              const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
     if (!P.N) return;
     const unsigned g = nb_(P.N);
-    if (cubic) {
-        if (mode == 0) k_cg_apply<true, 0><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
-        else k_cg_apply<true, 1><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
-    } else {
-        if (mode == 0) k_cg_apply<false, 0><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
-        else k_cg_apply<false, 1><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, in, Ap, b, r, p, st, partials, it, tol);
-    }
+    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(P, nb, cab, in, out, Ap, b, r, st, partials);
+    else k_cg_apply<1><<<g, kBlock, 0, s>>>(P, nb, cab, in, out, Ap, b, r, st, partials);
 }
-void cg_update(uint32_t N, float4* x, float4* r, const float4* p, const float4* Ap, SolverDev* st, double* partials,
-               uint32_t it, double tol, hipStream_t s) {
-    if (N) k_cg_update<<<nb_(N), kBlock, 0, s>>>(N, x, r, p, Ap, st, partials, it, tol);
+void fold_cg(int mode, const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, hipStream_t s) {
+    if (!N) return;
+    if (mode == 0) k_fold_cg<0><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
+    else if (mode == 1) k_fold_cg<1><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
+    else k_fold_cg<2><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
 }
-void cg_direction(uint32_t N, const float4* r, float4* p, const SolverDev* st, hipStream_t s) {
-    if (N) k_cg_direction<<<nb_(N), kBlock, 0, s>>>(N, r, p, st);
+void cg_update(uint32_t N, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st, double* partials,
+               hipStream_t s) {
+    if (N) k_cg_update<<<nb_(N), kBlock, 0, s>>>(N, x, r, prec, Ap, st, partials);
+}
+void cg_direction(uint32_t N, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
+    if (N) k_cg_direction<<<nb_(N), kBlock, 0, s>>>(N, r, prec, st);
 }
 void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s) {
     if (N) k_copy_vel_out<<<nb_(N), kBlock, 0, s>>>(N, x, vel);
@@ -769,8 +918,9 @@ void solver_reset(SolverDev* st, hipStream_t s) { k_solver_reset<<<1, kBlock, 0,
 void zero_words(uint32_t* p, size_t n, hipStream_t s) {
     if (n) k_zero_words<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n);
 }
-void pack_readback(uint32_t n, const float4* pos, const float4* color, float* xyz, float* rad, float* rgba, hipStream_t s) {
-    if (n) k_pack_readback<<<nb_(n), kBlock, 0, s>>>(n, pos, color, xyz, rad, rgba);
+void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
+                   float* rgba, hipStream_t s) {
+    if (n) k_pack_readback<<<nb_(n), kBlock, 0, s>>>(n, pos, color, perm, xyz, rad, rgba);
 }
 
 }  // namespace launch

@@ -759,60 +759,64 @@ struct Ctx final : Base {
         const V3<T> stG = STGk(dist, invHi) * xij;
         return (stW + dt() * (dot(vij, stG) - (fst[i] + fst[j]) * invDij * T(0.5) * stW)) * invDij;
     }
-    // ComputeSurfaceTensionCGb_kernel (MPSSPHNonPressureUtil.cuh:467-521)
-    void st_cgb() {
-#pragma omp parallel for
-        for (long long ii = 0; ii < (long long)N; ii++) {
-            const size_t i = size_t(ii);
-            const T invHi = T(1) / (r[i] * T(4));
-            const V3<T> xi = ld3(x, i);
-            V3<T> bi{0, 0, 0};
-            nsearch(ff, i, [&](uint32_t j) {
-                const V3<T> xij = xi - ld3(x, j);
-                const T dist = length(xij);
-                T mij;
-                const T gij = st_g(i, j, invHi, xij, dist, mij);
-                bi = bi - mij * gij * xij;
-            });
-            st3(cgb, i, dt() * mSigma / m[i] * bi + ld3(v, i));
-        }
-    }
-    // A.p = (1+f) p + dt [ST + XSPH + viscosity] (MPSSPHNonPressureUtil.cuh:523-648)
-    void cg_apply(const std::vector<T>& p, std::vector<T>& Ap, T f) {
+    // Pair coefficients of the implicit operator, fixed for one solve (positions, v, rho and
+    // Delta do not change while CG runs). With p_ij = p_i - p_j:
+    //   (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ]
+    //   a_ij = dt (cs_i s_ij + cx w_ij),  b_ij = -dt cv_i c_ij,
+    //   s_ij = m_ij g_ij (surface tension), w_ij = (m_j/rho_j) W_ij (XSPH),
+    //   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON) (viscosity),
+    //   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
+    // This is the operator of ComputeSurfaceTensionCGAp_kernel + ComputeCGViscosityAp_kernel
+    // + ComputeFinalCGAp_kernel (MPSSPHNonPressureUtil.cuh:523-648) with the per-particle
+    // scale factors distributed over the pair sums. The same pass forms the right-hand side
+    // of ComputeSurfaceTensionCGb_kernel (:467-521): b_i = (dt sigma/m_i)(-sum s_ij x_ij) + v_i.
+    std::vector<T> ca, cb;
+    void cg_setup() {
         const T dtv = dt();
+        const T cx = mXsph / dtv;
+        ca.assign(ff.nei.size(), T(0));
+        cb.assign(ff.nei.size(), T(0));
 #pragma omp parallel for
         for (long long ii = 0; ii < (long long)N; ii++) {
             const size_t i = size_t(ii);
             const T hi = r[i] * T(4);
             const T invHi = T(1) / hi;
-            const V3<T> xi = ld3(x, i), pi = ld3(p, i);
+            const T onePercentHSq = T(0.01) * hi * hi;
+            const V3<T> xi = ld3(x, i);
             const T mi = m[i], di = rho[i];
-            V3<T> st{0, 0, 0}, xs{0, 0, 0}, vis{0, 0, 0};
-            nsearch(ff, i, [&](uint32_t j) {
+            const T cs = dtv * mSigma / mi;
+            const T cv = T(10) * mRho0 * mMu / di;
+            V3<T> bi{0, 0, 0};
+            for (uint32_t k = ff.idx[i]; k < ff.idx[i + 1]; k++) {
+                const uint32_t j = ff.nei[k];
                 const V3<T> xij = xi - ld3(x, j);
                 const T dist = length(xij);
                 T mij;
                 const T gij = st_g(i, j, invHi, xij, dist, mij);
-                const V3<T> pj = ld3(p, j);
-                st = st + mij * gij * (pi - pj);
-                xs = xs + m[j] / rho[j] * (pi - pj) * Wk(dist, invHi, cubic);
-            });
-            st = st * (dtv * mSigma / mi);
-            xs = xs * (mXsph / dtv);
-            nsearch(ff, i, [&](uint32_t j) {
-                const T onePercentHSq = T(0.01) * hi * hi;
+                const T sij = mij * gij;
+                const T wij = m[j] / rho[j] * Wk(dist, invHi, cubic);
+                const T cij = T(0.5) * (mi + m[j]) / rho[j] / (dist * dist + onePercentHSq) * Gk(dist, invHi, cubic) /
+                              (dist + T(FLT_EPSILON));
+                ca[k] = dtv * (cs * sij + cx * wij);
+                cb[k] = -dtv * cv * cij;
+                bi = bi - sij * xij;
+            }
+            st3(cgb, i, dt() * mSigma / m[i] * bi + ld3(v, i));
+        }
+    }
+    void cg_apply(const std::vector<T>& p, std::vector<T>& Ap, T f) {
+#pragma omp parallel for
+        for (long long ii = 0; ii < (long long)N; ii++) {
+            const size_t i = size_t(ii);
+            const V3<T> xi = ld3(x, i), pi = ld3(p, i);
+            V3<T> acc{0, 0, 0};
+            for (uint32_t k = ff.idx[i]; k < ff.idx[i + 1]; k++) {
+                const uint32_t j = ff.nei[k];
                 const V3<T> xij = xi - ld3(x, j);
                 const V3<T> pij = pi - ld3(p, j);
-                const T dist = length(xij);
-                const V3<T> dpij = T(0.5) * (mi + m[j]) / rho[j] * dot(pij, xij) / (dist * dist + onePercentHSq) *
-                                   Gk(dist, invHi, cubic) / (dist + T(FLT_EPSILON)) * xij;
-                vis = vis - dpij;
-            });
-            vis = vis * (T(10) * mRho0 * mMu / di);
-            V3<T> a{0, 0, 0};
-            a = a + (st + xs);
-            a = a + vis;
-            st3(Ap, i, pi * (T(1) + f) + dtv * a);
+                acc = acc + (ca[k] * pij + (cb[k] * dot(pij, xij)) * xij);
+            }
+            st3(Ap, i, pi * (T(1) + f) + acc);
         }
     }
     static double dotd(const std::vector<T>& a, const std::vector<T>& b) {
@@ -827,7 +831,7 @@ struct Ctx final : Base {
         if (N == 0) return 0;
         predv = v;
         st_factor();
-        st_cgb();
+        cg_setup();
         const size_t n3 = 3 * N;
         std::vector<T> rr(n3), p(n3), Ap(n3);
         cg_apply(predv, Ap, T(0));
