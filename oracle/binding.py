@@ -37,6 +37,7 @@ _SIGS = {
     "orc_set_array": (C.c_int, [_P, C.c_int32, _P, C.c_size_t]),
     "orc_get_frame": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "orc_set_frame": (C.c_int, [_P, C.c_uint32]),
+    "orc_cg_operator": (C.c_int, [_P, _P, _P, C.c_double]),
     "orc_kernel_eval": (C.c_int, [C.c_int, C.c_int, C.c_size_t, _P, C.c_double, _P, _P, _P, _P]),
     "orc_cell_keys": (C.c_int, [C.c_int, C.c_size_t, _P, C.c_double, _P, C.c_int, _P]),
     "orc_mesh_rtri": (C.c_int, [C.c_size_t, _P, C.c_size_t, _P]),
@@ -166,6 +167,12 @@ class OracleSim:
         aid, kind, w = capi.ARRAYS[name]
         a = np.ascontiguousarray(arr, self._dtype(kind))
         self._chk(self.lib.orc_set_array(self.h, aid, _ptr(a), a.nbytes))
+
+    def cg_operator(self, p, f: float):
+        p = np.ascontiguousarray(p, self.real).reshape(-1, 3)
+        out = np.empty_like(p)
+        self._chk(self.lib.orc_cg_operator(self.h, _ptr(p), _ptr(out), float(f)))
+        return out
 
     @property
     def frame(self) -> int:

@@ -368,6 +368,7 @@ struct Base {
     virtual int array_info(int32_t id, size_t* count, size_t* eb) = 0;
     virtual int get_array(int32_t id, void* dst, size_t bytes) = 0;
     virtual int set_array(int32_t id, const void* src, size_t bytes) = 0;
+    virtual int cg_operator(const void* p, void* out, double f) = 0;
     uint32_t frame = 0;
 };
 
@@ -1011,6 +1012,14 @@ struct Ctx final : Base {
             }
         }
     }
+    int cg_operator(const void* pin, void* pout, double f) override {
+        st_factor();
+        cg_setup();
+        std::vector<T> p(static_cast<const T*>(pin), static_cast<const T*>(pin) + 3 * N), Ap(3 * N);
+        cg_apply(p, Ap, T(f));
+        std::memcpy(pout, Ap.data(), sizeof(T) * 3 * N);
+        return MPS_OK;
+    }
     bool finite_state() const {
         for (size_t k = 0; k < 3 * N; k++)
             if (!std::isfinite(double(x[k])) || !std::isfinite(double(v[k]))) return false;
@@ -1208,6 +1217,11 @@ int orc_set_frame(orc_ctx* c, uint32_t f) {
     if (!c) return MPS_ERR_INVALID;
     c->impl->frame = f;
     return MPS_OK;
+}
+
+int orc_cg_operator(orc_ctx* c, const void* p, void* out, double f) {
+    if (!c || !p || !out) return MPS_ERR_INVALID;
+    return c->impl->cg_operator(p, out, f);
 }
 
 int orc_kernel_eval(int precision, int kernel, size_t n, const double* dist, double invh, double* w, double* g,

@@ -54,6 +54,11 @@ int orc_set_array(orc_ctx* c, int32_t id, const void* src, size_t bytes);
 int orc_get_frame(orc_ctx* c, uint32_t* frame);
 int orc_set_frame(orc_ctx* c, uint32_t frame);
 
+/* Test hook: out = A(f) p with the implicit viscosity/surface-tension operator of the
+ * current state (density and the pair lists must be current; p, out: N*3 in the working
+ * precision). Runs the surface-tension factor and the pair-coefficient set-up first. */
+int orc_cg_operator(orc_ctx* c, const void* p, void* out, double f);
+
 /* Scalar helpers for known-answer tests (double in/out, evaluated in `precision`). */
 int orc_kernel_eval(int precision, int kernel, size_t n, const double* dist, double invh,
                     double* w, double* g, double* stw, double* stg);
