@@ -111,8 +111,8 @@ def main() -> int:
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="after the run, copy a 512 MiB buffer 3 times (torch) so hardware byte counters "
-                         "can be scaled against a copy of known size")
+                    help="after the run, launch the library's plain copy kernel (k_calib_copy) 3 times on "
+                         "512 MiB so hardware byte counters can be scaled against a copy of known size")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -212,13 +212,8 @@ def main() -> int:
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters,
     }
     if args.copy_calibration:
-        import torch
-        a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print("copy calibration: 3 x copy of 536870912 bytes", file=sys.stderr)
+        sim.calibration_copy(512 * 1024 * 1024, 3)
+        print("copy calibration: 3 x k_calib_copy of 536870912 bytes", file=sys.stderr)
     if args.kernels:
         for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["total_ms"]):
             print(f"{name:24s} launches={k['launches']:6d} useful={k['useful']:6d} total={k['total_ms']:9.2f} ms "

@@ -117,6 +117,7 @@ _SIGS = {
     "mps_host_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "mps_host_free": (C.c_int, [_P, _P]),
     "mps_profile_enable": (C.c_int, [_P, C.c_int32]),
+    "mps_calibration_copy": (C.c_int, [_P, C.c_size_t, C.c_int32]),
     "mps_profile_read": (C.c_int, [_P, C.POINTER(KernelStat), C.c_size_t, C.POINTER(C.c_size_t)]),
 }
 
@@ -279,6 +280,9 @@ class HipSim:
     # ---- per-kernel timing
     def profile(self, on: bool = True):
         self._chk(self.lib.mps_profile_enable(self.h, int(bool(on))))
+
+    def calibration_copy(self, nbytes: int, reps: int = 3):
+        self._chk(self.lib.mps_calibration_copy(self.h, int(nbytes), int(reps)))
 
     def profile_read(self) -> dict:
         cap = 64

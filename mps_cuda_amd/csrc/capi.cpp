@@ -107,6 +107,7 @@ struct mps_ctx {
     DBuf<uint32_t> perm, inv, keyref, vals, src, lkey, hash_perm;
     // neighbour lists (column-major ELL)
     DBuf<uint32_t> nff, cff, nfb, cfb, nbb, cbb;
+    DBuf<uint32_t> nffr, nfbr;  // row-major search output
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
@@ -224,7 +225,7 @@ struct mps_ctx {
         P.faithful = cfg.faithful_quirks;
         return P;
     }
-    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p}; }
+    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p, nffr.p, nfbr.p, Kff, Kfb}; }
     int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
     void sync() { HIPCHK(hipStreamSynchronize(stream)); }
 
@@ -414,9 +415,10 @@ struct mps_ctx {
 
     void neighbors() {
         const HashGeom g = geom();
-        if (Kff == 0) Kff = std::max<uint32_t>(cfg.nei_capacity, 8);
-        if (Kfb == 0) Kfb = std::max<uint32_t>(cfg.nei_capacity / 2, 8);
-        if (Kbb == 0) Kbb = std::max<uint32_t>(cfg.nei_capacity / 2, 8);
+        // capacities are multiples of 4 so every row of the row-major lists is 16-byte aligned
+        if (Kff == 0) Kff = (std::max<uint32_t>(cfg.nei_capacity, 8) + 3u) & ~3u;
+        if (Kfb == 0) Kfb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
+        if (Kbb == 0) Kbb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
         for (int attempt = 0; attempt < 8; attempt++) {
             reset_counters();
             const bool do_b = bnd_dirty;
@@ -425,12 +427,12 @@ struct mps_ctx {
                 cbb.reserve(B);
                 T_(K_NEI_BND, [&] { mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream); });
             }
-            nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
+            nffr.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
             cff.reserve(std::max<uint32_t>(N, 1));
-            nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
+            nfbr.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
             cfb.reserve(std::max<uint32_t>(N, 1));
             T_(K_NEI_FLUID, [&] {
-                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
+                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nffr.p, Kff, cff.p, nfbr.p, Kfb, cfb.p,
                                              d_ctr, stream);
             });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
@@ -443,7 +445,7 @@ struct mps_ctx {
                 if (do_b) {
                     nnz_bb = h_ctr->nnz_bb;
                     bS.reserve(std::max<uint32_t>(B, 1));
-                    T_(K_BSUM, [&] { mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, cbb.p, bS.p, stream); });
+                    T_(K_BSUM, [&] { mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, Kbb, cbb.p, bS.p, stream); });
                     bnd_dirty = false;
                 }
                 nnz_ff = h_ctr->nnz_ff;
@@ -466,10 +468,12 @@ struct mps_ctx {
             const size_t m = std::max<uint32_t>(N, 1);
             pairG_ff.reserve(size_t(Kff) * m);
             pairG_fb.reserve(size_t(Kfb) * m);
+            nff.reserve(size_t(Kff) * m);
+            nfb.reserve(size_t(Kfb) * m);
         }
         T_(K_DENSITY, [&] {
             mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
-                                       pairG_ff.p, pairG_fb.p, stream);
+                                       pairG_ff.p, pairG_fb.p, nff.p, nfb.p, stream);
         });
         if (flags & mps::launch::kDoPairs) pairs_valid = true;
     }
@@ -594,6 +598,7 @@ struct mps_ctx {
     uint32_t viscosity_jacobi(mps_step_stats* st) {
         if (N == 0) return 0;
         ensure_partials();
+        ensure_pairs();  // the column-major fluid list is written by the pairs pass
         const StepParams P = params();
         jprev.reserve(N); jcand.reserve(N);
         for (auto& t : jt) t.reserve(N);
@@ -913,10 +918,18 @@ struct mps_ctx {
         for (uint32_t i = 0; i < N; i++) {
             const uint32_t sl = iv[i];
             for (uint32_t k = 0; k < c[sl]; k++) {
-                const uint32_t j = ell[size_t(k) * N + sl];
+                const uint32_t j = ell[size_t(sl) * K + k];
                 out.push_back(map_ids ? pm[j] : j);
             }
         }
+        return out;
+    }
+    std::vector<uint32_t> csr_nei_rows(const uint32_t* nbrs, const uint32_t* cnt, uint32_t n, uint32_t K) {
+        std::vector<uint32_t> c = down(cnt, n);
+        std::vector<uint32_t> rows = down(nbrs, size_t(K) * n);
+        std::vector<uint32_t> out;
+        for (uint32_t i = 0; i < n; i++)
+            for (uint32_t k = 0; k < c[i]; k++) out.push_back(rows[size_t(i) * K + k]);
         return out;
     }
     std::vector<uint32_t> csr_idx(const uint32_t* cnt, uint32_t n) {
@@ -1087,11 +1100,11 @@ struct mps_ctx {
             case MPS_ARR_BHASH_START: ro_raw(bh.start.p, bh.cells_init ? cells() : 0); break;
             case MPS_ARR_BHASH_END: ro_raw(bh.end.p, bh.cells_init ? cells() : 0); break;
             case MPS_ARR_NEI_FF_IDX: ro(csr_idx_fluid(cff.p)); break;
-            case MPS_ARR_NEI_FF: ro(csr_nei_fluid(nff.p, cff.p, Kff, true)); break;
+            case MPS_ARR_NEI_FF: ro(csr_nei_fluid(nffr.p, cff.p, Kff, true)); break;
             case MPS_ARR_NEI_FB_IDX: ro(csr_idx_fluid(cfb.p)); break;
-            case MPS_ARR_NEI_FB: ro(csr_nei_fluid(nfb.p, cfb.p, Kfb, false)); break;
+            case MPS_ARR_NEI_FB: ro(csr_nei_fluid(nfbr.p, cfb.p, Kfb, false)); break;
             case MPS_ARR_NEI_BB_IDX: ro(csr_idx(cbb.p, B)); break;
-            case MPS_ARR_NEI_BB: ro(csr_nei(nbb.p, cbb.p, B, Kbb)); break;
+            case MPS_ARR_NEI_BB: ro(csr_nei_rows(nbb.p, cbb.p, B, Kbb)); break;
             default: throw ApiError{MPS_ERR_INVALID, "unknown array id"};
         }
     }
@@ -1104,6 +1117,7 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
+        nffr.release(); nfbr.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
@@ -1367,6 +1381,21 @@ int mps_profile_enable(mps_ctx* c, int32_t on) {
         c->sync();
         c->prof_reset();
         c->prof_on = on != 0;
+    });
+}
+
+int mps_calibration_copy(mps_ctx* c, size_t bytes, int32_t reps) {
+    if (!c || reps < 0) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        DBuf<float4> a, b;
+        const size_t n = bytes / sizeof(float4);
+        a.reserve(std::max<size_t>(n, 1));
+        b.reserve(std::max<size_t>(n, 1));
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(a.p), 4 * n, c->stream);
+        for (int32_t r = 0; r < reps; r++) mps::launch::calib_copy(a.p, b.p, n, c->stream);
+        c->sync();
+        a.release();
+        b.release();
     });
 }
 

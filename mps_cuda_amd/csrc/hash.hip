@@ -7,8 +7,8 @@
 //     keep ascending particle id — the order the reference's neighbour lists depend on;
 //   * cell start/end tables cleared incrementally (only cells occupied last step), plus a
 //     copy of the positions in key order so the 27-cell scan reads contiguous memory;
-//   * one search pass per particle (no count/scan/fill double scan): neighbour ids go to
-//     column-major ELL storage nbr[k*N + i] (a wave reads 256 contiguous bytes per k), the
+//   * one search pass per particle (no count/scan/fill double scan): neighbour ids go to a
+//     row of K slots per particle (row-major, so each thread fills whole cache lines), the
 //     per-particle count to cnt[i]; overflow past the capacity is counted and reported so
 //     the host can grow the capacity and search again.
 #include "device_common.hpp"
@@ -269,8 +269,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_table(const uint32_t* __restric
 template <bool SELF>
 __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const HashGeom& g, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, const uint32_t* __restrict__ ids,
-                                                 const float4* __restrict__ spos, uint32_t* __restrict__ out, uint32_t K,
-                                                 uint32_t stride) {
+                                                 const float4* __restrict__ spos, uint32_t* __restrict__ out, uint32_t K) {
     const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
     const float ri = xi.w * 4.0f;
     uint32_t k = 0u;
@@ -296,7 +295,7 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
                     if (in) {
                         const uint32_t j = ids ? ids[s] : s;
                         if (SELF && j == i) continue;
-                        if (k < K) out[size_t(k) * stride + i] = j;
+                        if (k < K) out[size_t(i) * K + k] = j;
                         k++;
                     }
                 }
@@ -326,8 +325,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     uint32_t a = 0u, b = 0u;
     if (i < N) {
         const float4 xi = pos[i];
-        a = search_cells<true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff, N);
-        if (bd.n) b = search_cells<false>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb, N);
+        a = search_cells<true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff);
+        if (bd.n) b = search_cells<false>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb);
         cff[i] = a;
         cfb[i] = b;
     }
@@ -341,7 +340,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __r
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t a = 0u;
     if (i < B) {
-        a = search_cells<true>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb, B);
+        a = search_cells<true>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb);
         cbb[i] = a;
     }
     report_counts(a, &ctr->max_bb, &ctr->nnz_bb);

@@ -52,12 +52,17 @@ struct StepParams {
     int faithful;
 };
 
-// Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i].
+// Neighbour lists. The search writes them row-major (rows of K slots per particle, so each
+// thread fills whole cache lines); the alpha pass transposes them once per step into the
+// column-major ELL form nbr[k*N + i] (k < cnt[i]) that every solver iteration reads.
 struct Nbr {
-    const uint32_t* ff;
+    const uint32_t* ff;   // column-major
     const uint32_t* cff;
-    const uint32_t* fb;
+    const uint32_t* fb;   // column-major
     const uint32_t* cfb;
+    const uint32_t* ffr;  // row-major, stride Kff
+    const uint32_t* fbr;  // row-major, stride Kfb
+    uint32_t Kff, Kfb;
 };
 
 namespace launch {
@@ -93,13 +98,13 @@ void gather_f1(const uint32_t* perm, uint32_t n, const float* in, float* out, hi
 
 // ---- sph.hip
 enum DensityFlags { kDoDensity = 1, kDoAlpha = 2, kDoGravity = 4, kDoPairs = 8 };
-void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, const uint32_t* cbb, float* S,
+void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb, float* S,
                   hipStream_t s);
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
 // density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
-                   float* pG_fb, hipStream_t s);
+                   float* pG_fb, uint32_t* ff_col, uint32_t* fb_col, hipStream_t s);
 void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
@@ -133,6 +138,7 @@ void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* p
                   double* err3, hipStream_t s);
 void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s);
 void solver_reset(SolverDev* st, hipStream_t s);
+void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s);

@@ -54,7 +54,7 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 // S_b depends only on boundary positions, so it is computed when they change.
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_boundary_sum(const float4* __restrict__ bpos, uint32_t B, const uint32_t* __restrict__ nbb,
-                                                       const uint32_t* __restrict__ cbb, float* __restrict__ S) {
+                                                       uint32_t Kbb, const uint32_t* __restrict__ cbb, float* __restrict__ S) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B) return;
     const float4 xi4 = bpos[i];
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_sum(const float4* __restric
     float vol = Wk<CUBIC>(0.0f, invHi);
     const uint32_t c = cbb[i];
     for (uint32_t k = 0; k < c; k++) {
-        const uint32_t j = nbb[size_t(k) * B + i];
+        const uint32_t j = nbb[size_t(i) * Kbb + k];
         vol += Wk<CUBIC>(len3(xi - mk(bpos[j])), invHi);
     }
     S[i] = vol;
@@ -84,12 +84,14 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
+// Reads the row-major lists; with kDoPairs it also writes the column-major copies.
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams P, Nbr nb, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
                                                         float* __restrict__ alpha, float4* __restrict__ color,
-                                                        float* __restrict__ pG_ff, float* __restrict__ pG_fb) {
+                                                        float* __restrict__ pG_ff, float* __restrict__ pG_fb,
+                                                        uint32_t* __restrict__ ff_col, uint32_t* __restrict__ fb_col) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
@@ -104,42 +106,52 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
     float s = 0.0f;
     if (dens || alph || pairs) {
         const uint32_t c = nb.cff[i];
-        auto pair = [&](uint32_t k, float4 xj4) {
+        const uint32_t* row = nb.ffr + size_t(i) * nb.Kff;
+        auto pair = [&](uint32_t k, uint32_t j, float4 xj4) {
             const f3 xij = xi - mk(xj4);
             const float dist = len3(xij);
             if (dens) sf += P.V * Wk<CUBIC>(dist, invHi);
             if (alph || pairs) {
                 const float G = Gk<CUBIC>(dist, invHi);
-                if (pairs) pG_ff[size_t(k) * N + i] = G;
+                if (pairs) {
+                    pG_ff[size_t(k) * N + i] = G;
+                    ff_col[size_t(k) * N + i] = j;
+                }
                 const f3 grad = (P.V * G) * xij;
                 gf = gf + grad;
                 s += dot3(grad, grad);
             }
         };
         uint32_t k = 0;
-        for (; k + 4 <= c; k += 4) {
-            uint32_t j[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) j[q] = nbr_ff(nb, k + q, i, N);
+        for (; k + 4 <= c; k += 4) {  // rows are 16-byte aligned (K is a multiple of 4)
+            const uint4 jj = *reinterpret_cast<const uint4*>(row + k);
+            const uint32_t j[4] = {jj.x, jj.y, jj.z, jj.w};
             float4 xj[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) xj[q] = pos[j[q]];
 #pragma unroll
-            for (int q = 0; q < 4; q++) pair(k + q, xj[q]);
+            for (int q = 0; q < 4; q++) pair(k + q, j[q], xj[q]);
         }
-        for (; k < c; k++) pair(k, pos[nbr_ff(nb, k, i, N)]);
+        for (; k < c; k++) {
+            const uint32_t j = row[k];
+            pair(k, j, pos[j]);
+        }
         float sb = 0.0f;
         f3 gb{0.f, 0.f, 0.f};
         const uint32_t cb = nb.cfb[i];
+        const uint32_t* brow = nb.fbr + size_t(i) * nb.Kfb;
         for (uint32_t k = 0; k < cb; k++) {
-            const uint32_t j = nbr_fb(nb, k, i, N);
+            const uint32_t j = brow[k];
             const f3 xij = xi - mk(bpos[j]);
             const float dist = len3(xij);
             const float vb = bvol[j];
             if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
             if (alph || pairs) {
                 const float G = Gk<CUBIC>(dist, invHi);
-                if (pairs) pG_fb[size_t(k) * N + i] = G;
+                if (pairs) {
+                    pG_fb[size_t(k) * N + i] = G;
+                    fb_col[size_t(k) * N + i] = j;
+                }
                 gb = gb + (vb * G) * xij;
             }
         }
@@ -795,6 +807,10 @@ __global__ void k_solver_reset(SolverDev* st) {
     }
     for (uint32_t k = t; k < MPS_MAX_ITER_LOG; k += blockDim.x) st->log[k] = 0.0;
 }
+// Plain streaming copy (16-byte vectors), used only to calibrate byte counters.
+__global__ void __launch_bounds__(kBlock) k_calib_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) b[i] = a[i];
+}
 __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -824,21 +840,21 @@ namespace launch {
 using namespace dev;
 static inline unsigned nb_(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
 
-void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, const uint32_t* cbb, float* S,
+void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb, float* S,
                   hipStream_t s) {
     if (!B) return;
-    if (cubic) k_boundary_sum<true><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, cbb, S);
-    else k_boundary_sum<false><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, cbb, S);
+    if (cubic) k_boundary_sum<true><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, Kbb, cbb, S);
+    else k_boundary_sum<false><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, Kbb, cbb, S);
 }
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
     if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
 }
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
-                   float* pG_fb, hipStream_t s) {
+                   float* pG_fb, uint32_t* ff_col, uint32_t* fb_col, hipStream_t s) {
     if (!P.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
-    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb, ff_col, fb_col);
+    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb, ff_col, fb_col);
 }
 void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
@@ -915,6 +931,9 @@ void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, 
     if (P.N) k_advect<<<nb_(P.N), kBlock, 0, s>>>(P, pos, vel, nonfinite);
 }
 void solver_reset(SolverDev* st, hipStream_t s) { k_solver_reset<<<1, kBlock, 0, s>>>(st); }
+void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s) {
+    if (n) k_calib_copy<<<4096, kBlock, 0, s>>>(a, b, n);
+}
 void zero_words(uint32_t* p, size_t n, hipStream_t s) {
     if (n) k_zero_words<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n);
 }
