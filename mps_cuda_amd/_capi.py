@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmps_sph.so")
+# MPS_SPH_LIB names another build of the same library (e.g. one compiled with other tile
+# parameters); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("MPS_SPH_LIB") or os.path.join(HERE, "libmps_sph.so")
 
 MPS_OK, MPS_ERR_INVALID, MPS_ERR_HIP, MPS_ERR_NOMEM, MPS_ERR_STATE, MPS_ERR_NONFINITE = range(6)
 MAX_ITER_LOG = 128

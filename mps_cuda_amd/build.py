@@ -36,31 +36,32 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+def _compile(src: str, csrc: str = CSRC, out_dir: str = OUT_DIR) -> str:
+    obj = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC, *CFLAGS]
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
-    cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd += ["-c", os.path.join(csrc, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(OUT_DIR, exist_ok=True)
+def build(verbose: bool = True, csrc: str = CSRC, lib: str = LIB, out_dir: str = OUT_DIR) -> str:
+    """Compile `csrc` (default: the package's sources) into the shared library `lib`."""
+    os.makedirs(out_dir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda f: _compile(f, csrc, out_dir), SOURCES))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"[mps_cuda_amd] built {LIB}", file=sys.stderr)
-    return LIB
+        print(f"[mps_cuda_amd] built {lib}", file=sys.stderr)
+    return lib
 
 
 if __name__ == "__main__":

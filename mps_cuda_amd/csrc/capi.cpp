@@ -107,7 +107,6 @@ struct mps_ctx {
     DBuf<uint32_t> perm, inv, keyref, vals, src, lkey, hash_perm;
     // neighbour lists (column-major ELL)
     DBuf<uint32_t> nff, cff, nfb, cfb, nbb, cbb;
-    DBuf<uint32_t> nffr, nfbr;  // row-major search output
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
@@ -225,7 +224,7 @@ struct mps_ctx {
         P.faithful = cfg.faithful_quirks;
         return P;
     }
-    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p, nffr.p, nfbr.p, Kff, Kfb}; }
+    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p}; }
     int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
     void sync() { HIPCHK(hipStreamSynchronize(stream)); }
 
@@ -415,7 +414,7 @@ struct mps_ctx {
 
     void neighbors() {
         const HashGeom g = geom();
-        // capacities are multiples of 4 so every row of the row-major lists is 16-byte aligned
+        // capacities rounded to a multiple of 4
         if (Kff == 0) Kff = (std::max<uint32_t>(cfg.nei_capacity, 8) + 3u) & ~3u;
         if (Kfb == 0) Kfb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
         if (Kbb == 0) Kbb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
@@ -427,12 +426,12 @@ struct mps_ctx {
                 cbb.reserve(B);
                 T_(K_NEI_BND, [&] { mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream); });
             }
-            nffr.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
+            nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
             cff.reserve(std::max<uint32_t>(N, 1));
-            nfbr.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
+            nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
             cfb.reserve(std::max<uint32_t>(N, 1));
             T_(K_NEI_FLUID, [&] {
-                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nffr.p, Kff, cff.p, nfbr.p, Kfb, cfb.p,
+                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
                                              d_ctr, stream);
             });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
@@ -468,12 +467,10 @@ struct mps_ctx {
             const size_t m = std::max<uint32_t>(N, 1);
             pairG_ff.reserve(size_t(Kff) * m);
             pairG_fb.reserve(size_t(Kfb) * m);
-            nff.reserve(size_t(Kff) * m);
-            nfb.reserve(size_t(Kfb) * m);
         }
         T_(K_DENSITY, [&] {
             mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
-                                       pairG_ff.p, pairG_fb.p, nff.p, nfb.p, stream);
+                                       pairG_ff.p, pairG_fb.p, stream);
         });
         if (flags & mps::launch::kDoPairs) pairs_valid = true;
     }
@@ -598,7 +595,6 @@ struct mps_ctx {
     uint32_t viscosity_jacobi(mps_step_stats* st) {
         if (N == 0) return 0;
         ensure_partials();
-        ensure_pairs();  // the column-major fluid list is written by the pairs pass
         const StepParams P = params();
         jprev.reserve(N); jcand.reserve(N);
         for (auto& t : jt) t.reserve(N);
@@ -918,7 +914,7 @@ struct mps_ctx {
         for (uint32_t i = 0; i < N; i++) {
             const uint32_t sl = iv[i];
             for (uint32_t k = 0; k < c[sl]; k++) {
-                const uint32_t j = ell[size_t(sl) * K + k];
+                const uint32_t j = ell[size_t(k) * N + sl];
                 out.push_back(map_ids ? pm[j] : j);
             }
         }
@@ -1100,9 +1096,9 @@ struct mps_ctx {
             case MPS_ARR_BHASH_START: ro_raw(bh.start.p, bh.cells_init ? cells() : 0); break;
             case MPS_ARR_BHASH_END: ro_raw(bh.end.p, bh.cells_init ? cells() : 0); break;
             case MPS_ARR_NEI_FF_IDX: ro(csr_idx_fluid(cff.p)); break;
-            case MPS_ARR_NEI_FF: ro(csr_nei_fluid(nffr.p, cff.p, Kff, true)); break;
+            case MPS_ARR_NEI_FF: ro(csr_nei_fluid(nff.p, cff.p, Kff, true)); break;
             case MPS_ARR_NEI_FB_IDX: ro(csr_idx_fluid(cfb.p)); break;
-            case MPS_ARR_NEI_FB: ro(csr_nei_fluid(nfbr.p, cfb.p, Kfb, false)); break;
+            case MPS_ARR_NEI_FB: ro(csr_nei_fluid(nfb.p, cfb.p, Kfb, false)); break;
             case MPS_ARR_NEI_BB_IDX: ro(csr_idx(cbb.p, B)); break;
             case MPS_ARR_NEI_BB: ro(csr_nei_rows(nbb.p, cbb.p, B, Kbb)); break;
             default: throw ApiError{MPS_ERR_INVALID, "unknown array id"};
@@ -1117,7 +1113,6 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
-        nffr.release(); nfbr.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();

@@ -112,6 +112,16 @@ __device__ __forceinline__ uint32_t split_by3(uint32_t x) {
     x = (x | x << 2u) & 0x09249249u;
     return x;
 }
+// Workgroups of a launch go to the 8 XCDs round-robin (workgroup b runs on XCD b % 8), and
+// each XCD has its own L2. Renumbering so that XCD x runs one contiguous range of logical
+// workgroups keeps the particles a workgroup gathers, which are near it in slot order, in
+// the L2 that already holds them. A bijection on [0, n) for every n.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8u;
+    const uint32_t per = n / X, rem = n % X, x = b % X;
+    return x * per + min(x, rem) + b / X;
+}
+
 // Morton key; adds H/2 on top of grid_pos's H/2 (MPSSpatialHash.cuh:29-45)
 __device__ __forceinline__ uint32_t grid_zindex(i3 p, const HashGeom& g) {
     const uint32_t x = static_cast<uint32_t>(p.x + static_cast<int>(g.hx >> 1u)) & (g.hx - 1u);

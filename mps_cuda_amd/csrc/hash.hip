@@ -7,10 +7,11 @@
 //     keep ascending particle id — the order the reference's neighbour lists depend on;
 //   * cell start/end tables cleared incrementally (only cells occupied last step), plus a
 //     copy of the positions in key order so the 27-cell scan reads contiguous memory;
-//   * one search pass per particle (no count/scan/fill double scan): neighbour ids go to a
-//     row of K slots per particle (row-major, so each thread fills whole cache lines), the
-//     per-particle count to cnt[i]; overflow past the capacity is counted and reported so
-//     the host can grow the capacity and search again.
+//   * one search pass per particle (no count/scan/fill double scan): fluid neighbour ids go
+//     to column-major ELL storage nbr[k*N + i] (a wave reads 256 contiguous bytes per k),
+//     the boundary-boundary list to rows of K slots (it is read once, when the walls move),
+//     the per-particle count to cnt[i]; overflow past the capacity is counted and reported
+//     so the host can grow the capacity and search again.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -266,40 +267,59 @@ __global__ void __launch_bounds__(kBlock) k_cell_table(const uint32_t* __restric
 
 // 27-cell scan (Research, MPSSpatialHash.cuh:47-80): z outer, y, x inner; slots in key order.
 // Neighbour when |x_i - x_j| < (4 r_i + 4 r_j) * 0.5, self excluded when SELF.
-template <bool SELF>
+// The 27 (start, end) pairs are loaded up front, so their latencies overlap instead of
+// adding up cell after cell; candidates are read two at a time for the same reason. Lists
+// are written column-major (out[k*stride + i]) when COLS, else as rows of K slots.
+template <bool SELF, bool COLS>
 __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const HashGeom& g, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, const uint32_t* __restrict__ ids,
-                                                 const float4* __restrict__ spos, uint32_t* __restrict__ out, uint32_t K) {
+                                                 const float4* __restrict__ spos, uint32_t* __restrict__ out, uint32_t K,
+                                                 uint32_t stride) {
     const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
     const float ri = xi.w * 4.0f;
+    uint32_t cs[27], ce[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const uint32_t c = grid_index(p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1, g);
+        cs[t] = start[c];
+        ce[t] = end[c];
+    }
     uint32_t k = 0u;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const uint32_t c = grid_index(p.x + dx, p.y + dy, p.z + dz, g);
-                uint32_t s = start[c];
-                const uint32_t e = end[c];
-                for (; s < e; s++) {
-                    const float4 q = spos[s];
-                    const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
-                    const float dd = (ddx * ddx + ddy * ddy) + ddz * ddz;
-                    const float rj = q.w * 4.0f;
-                    const float thr = (ri + rj) * 0.5f;
-                    const float t2 = thr * thr;
-                    // sqrt is monotone and correctly rounded: far from the cut-off the squared
-                    // distance decides; near it the exact test of the reference runs.
-                    bool in;
-                    if (dd < t2 * 0.9999f) in = true;
-                    else if (dd > t2 * 1.0001f) in = false;
-                    else in = __builtin_sqrtf(dd) < thr;
-                    if (in) {
-                        const uint32_t j = ids ? ids[s] : s;
-                        if (SELF && j == i) continue;
-                        if (k < K) out[size_t(i) * K + k] = j;
-                        k++;
-                    }
-                }
+    auto test = [&](float4 q, uint32_t s) {
+        const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
+        const float dd = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        const float rj = q.w * 4.0f;
+        const float thr = (ri + rj) * 0.5f;
+        const float t2 = thr * thr;
+        // sqrt is monotone and correctly rounded: far from the cut-off the squared distance
+        // decides; near it the exact test of the reference runs.
+        bool in;
+        if (dd < t2 * 0.9999f) in = true;
+        else if (dd > t2 * 1.0001f) in = false;
+        else in = __builtin_sqrtf(dd) < thr;
+        if (in) {
+            const uint32_t j = ids ? ids[s] : s;
+            if (!(SELF && j == i)) {
+                if (k < K) out[COLS ? size_t(k) * stride + i : size_t(i) * K + k] = j;
+                k++;
             }
+        }
+    };
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        uint32_t s = cs[t];
+        const uint32_t e = ce[t];
+        // an empty cell has start = end = 0xFFFFFFFF: only e - s (never s + 1) is formed,
+        // and only once s < e is known
+        if (s < e) {
+            for (; e - s >= 2u; s += 2) {
+                const float4 q0 = spos[s], q1 = spos[s + 1];
+                test(q0, s);
+                test(q1, s + 1);
+            }
+            if (s < e) test(spos[s], s);
+        }
+    }
     return k;
 }
 
@@ -325,8 +345,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     uint32_t a = 0u, b = 0u;
     if (i < N) {
         const float4 xi = pos[i];
-        a = search_cells<true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff);
-        if (bd.n) b = search_cells<false>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb);
+        a = search_cells<true, true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff, N);
+        if (bd.n) b = search_cells<false, true>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb, N);
         cff[i] = a;
         cfb[i] = b;
     }
@@ -340,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __r
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t a = 0u;
     if (i < B) {
-        a = search_cells<true>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb);
+        a = search_cells<true, false>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb, B);
         cbb[i] = a;
     }
     report_counts(a, &ctr->max_bb, &ctr->nnz_bb);

@@ -52,17 +52,12 @@ struct StepParams {
     int faithful;
 };
 
-// Neighbour lists. The search writes them row-major (rows of K slots per particle, so each
-// thread fills whole cache lines); the alpha pass transposes them once per step into the
-// column-major ELL form nbr[k*N + i] (k < cnt[i]) that every solver iteration reads.
+// Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i].
 struct Nbr {
-    const uint32_t* ff;   // column-major
+    const uint32_t* ff;
     const uint32_t* cff;
-    const uint32_t* fb;   // column-major
+    const uint32_t* fb;
     const uint32_t* cfb;
-    const uint32_t* ffr;  // row-major, stride Kff
-    const uint32_t* fbr;  // row-major, stride Kfb
-    uint32_t Kff, Kfb;
 };
 
 namespace launch {
@@ -104,7 +99,7 @@ void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipSt
 // density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
-                   float* pG_fb, uint32_t* ff_col, uint32_t* fb_col, hipStream_t s);
+                   float* pG_fb, hipStream_t s);
 void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);

@@ -34,10 +34,13 @@ __device__ __forceinline__ uint32_t nbr_fb(const Nbr& nb, uint32_t k, uint32_t i
 }
 
 // Block-reduce `v` (double) in a fixed order and publish the workgroup's partial.
-__device__ __forceinline__ void block_partial(double v, double* __restrict__ partials) {
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partials, uint32_t blk) {
     __shared__ double lds[kBlock / 64];
     const double bs = block_sum_d(v, lds);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (threadIdx.x == 0) partials[blk] = bs;
+}
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partials) {
+    block_partial(v, partials, blockIdx.x);
 }
 // Sum of partials[0..n) in index-strided order, valid in thread 0 (one workgroup of
 // kFoldThreads).
@@ -84,14 +87,12 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
-// Reads the row-major lists; with kDoPairs it also writes the column-major copies.
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams P, Nbr nb, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
                                                         float* __restrict__ alpha, float4* __restrict__ color,
-                                                        float* __restrict__ pG_ff, float* __restrict__ pG_fb,
-                                                        uint32_t* __restrict__ ff_col, uint32_t* __restrict__ fb_col) {
+                                                        float* __restrict__ pG_ff, float* __restrict__ pG_fb) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     if (i >= N) return;
@@ -106,52 +107,42 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
     float s = 0.0f;
     if (dens || alph || pairs) {
         const uint32_t c = nb.cff[i];
-        const uint32_t* row = nb.ffr + size_t(i) * nb.Kff;
-        auto pair = [&](uint32_t k, uint32_t j, float4 xj4) {
+        auto pair = [&](uint32_t k, float4 xj4) {
             const f3 xij = xi - mk(xj4);
             const float dist = len3(xij);
             if (dens) sf += P.V * Wk<CUBIC>(dist, invHi);
             if (alph || pairs) {
                 const float G = Gk<CUBIC>(dist, invHi);
-                if (pairs) {
-                    pG_ff[size_t(k) * N + i] = G;
-                    ff_col[size_t(k) * N + i] = j;
-                }
+                if (pairs) pG_ff[size_t(k) * N + i] = G;
                 const f3 grad = (P.V * G) * xij;
                 gf = gf + grad;
                 s += dot3(grad, grad);
             }
         };
         uint32_t k = 0;
-        for (; k + 4 <= c; k += 4) {  // rows are 16-byte aligned (K is a multiple of 4)
-            const uint4 jj = *reinterpret_cast<const uint4*>(row + k);
-            const uint32_t j[4] = {jj.x, jj.y, jj.z, jj.w};
+        for (; k + 4 <= c; k += 4) {
+            uint32_t j[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) j[q] = nbr_ff(nb, k + q, i, N);
             float4 xj[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) xj[q] = pos[j[q]];
 #pragma unroll
-            for (int q = 0; q < 4; q++) pair(k + q, j[q], xj[q]);
+            for (int q = 0; q < 4; q++) pair(k + q, xj[q]);
         }
-        for (; k < c; k++) {
-            const uint32_t j = row[k];
-            pair(k, j, pos[j]);
-        }
+        for (; k < c; k++) pair(k, pos[nbr_ff(nb, k, i, N)]);
         float sb = 0.0f;
         f3 gb{0.f, 0.f, 0.f};
         const uint32_t cb = nb.cfb[i];
-        const uint32_t* brow = nb.fbr + size_t(i) * nb.Kfb;
         for (uint32_t k = 0; k < cb; k++) {
-            const uint32_t j = brow[k];
+            const uint32_t j = nbr_fb(nb, k, i, N);
             const f3 xij = xi - mk(bpos[j]);
             const float dist = len3(xij);
             const float vb = bvol[j];
             if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
             if (alph || pairs) {
                 const float G = Gk<CUBIC>(dist, invHi);
-                if (pairs) {
-                    pG_fb[size_t(k) * N + i] = G;
-                    fb_col[size_t(k) * N + i] = j;
-                }
+                if (pairs) pG_fb[size_t(k) * N + i] = G;
                 gb = gb + (vb * G) * xij;
             }
         }
@@ -498,7 +489,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const
                                                    float4* __restrict__ r, const SolverDev* st,
                                                    double* __restrict__ partials) {
     if (MODE == 1 && st->done) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     double part = 0.0;
     if (i < N) {
@@ -508,9 +500,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const
         f3 acc{0.f, 0.f, 0.f};
         const uint32_t c = nb.cff[i];
         uint32_t k = 0;
-        // eight neighbours per trip: all index/coefficient loads, then all gathers, then the
-        // sums in list order (same result as the one-at-a-time loop)
-        constexpr int U = 8;
+        // four neighbours per trip: all index/coefficient loads, then all gathers, then the
+        // sums in list order (same result as the one-at-a-time loop). Four keeps the kernel at
+        // 56 VGPRs (8 waves per SIMD); eight (96 VGPRs) and sixteen measured slower.
+        constexpr int U = 4;
         for (; k + U <= c; k += U) {
             uint32_t j[U];
             float2 ab[U];
@@ -556,7 +549,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const
             part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
     }
-    block_partial(part, partials);
+    block_partial(part, partials, blk);
 }
 
 // Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
@@ -851,10 +844,10 @@ void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipSt
 }
 void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
-                   float* pG_fb, uint32_t* ff_col, uint32_t* fb_col, hipStream_t s) {
+                   float* pG_fb, hipStream_t s) {
     if (!P.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb, ff_col, fb_col);
-    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb, ff_col, fb_col);
+    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
 void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
