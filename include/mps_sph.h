@@ -246,6 +246,41 @@ int mps_fence_wait(mps_ctx* ctx, mps_fence* fence);
 int mps_host_alloc(mps_ctx* ctx, size_t bytes, void** out);
 int mps_host_free(mps_ctx* ctx, void* p);
 
+/* ---- batched scenes ---------------------------------------------------------------------
+ * No counterpart in the reference, which steps one scene per System. A context can hold S
+ * independent scenes (own particles, walls, physics and material) stored back to back; every
+ * phase of mps_step is then one launch over all of them, each solver loop keeps a record and a
+ * convergence flag per scene, and a scene whose loop has converged sits out the remaining
+ * iterations. Each scene's results are those of the same scene stepped alone in a context of
+ * its own with the same configuration. mps_upload_* and mps_sample_boundary_mesh are
+ * single-scene calls and fail with MPS_ERR_STATE while a batch is held; mps_set_physics and
+ * mps_set_fluid_material then apply to every scene. Fluid arrays are exported scene after
+ * scene (mps_scene_info gives each scene's range); hash tables only for one scene. */
+typedef struct mps_scene_range {
+    size_t fluid_offset;    /* reference index of the scene's first fluid particle */
+    size_t n_fluid;
+    size_t boundary_offset; /* index of its first boundary particle */
+    size_t n_boundary;
+} mps_scene_range;
+
+/* Drop every scene and all particles. */
+int mps_batch_clear(mps_ctx* ctx);
+/* Stage one scene (copied; uploaded by mps_batch_commit). pos/vel n_fluid*3, radius/mass
+ * n_fluid (vel NULL = 0, mass NULL = material mass); boundary bpos n_boundary*3, bradius and
+ * bmass n_boundary (bmass NULL = 0). n_fluid must be > 0. */
+int mps_batch_add_scene(mps_ctx* ctx, const mps_physics* phys, const mps_sph_material* mat,
+                        size_t n_fluid, const float* pos, const float* vel, const float* radius,
+                        const float* mass, size_t n_boundary, const float* bpos,
+                        const float* bradius, const float* bmass);
+/* Upload the staged scenes in the order added; replaces the context's state, frame = 0. */
+int mps_batch_commit(mps_ctx* ctx);
+int mps_scene_count(mps_ctx* ctx, uint32_t* n);
+int mps_scene_info(mps_ctx* ctx, uint32_t scene, mps_scene_range* out);
+/* Statistics of one scene in the last step (iteration counts, error logs, caps). */
+int mps_scene_stats(mps_ctx* ctx, uint32_t scene, mps_step_stats* out);
+/* Change one scene's material (viscosity, surface tension, ...) between steps. */
+int mps_scene_set_material(mps_ctx* ctx, uint32_t scene, const mps_sph_material* m);
+
 /* ---- per-kernel timing (replaces MTimer's debugger prints, MTimer.h:12-80) ---------------
  * When enabled, every kernel launch is bracketed by HIP events on the context's stream.
  * `launches` counts every launch; `useful` counts launches that did work (solver kernels

@@ -94,6 +94,11 @@ class Fence(C.Structure):
     _fields_ = [("event", C.c_void_p), ("frame", C.c_uint64)]
 
 
+class SceneRange(C.Structure):
+    _fields_ = [("fluid_offset", C.c_size_t), ("n_fluid", C.c_size_t), ("boundary_offset", C.c_size_t),
+                ("n_boundary", C.c_size_t)]
+
+
 _P = C.c_void_p
 _SIGS = {
     "mps_config_init": (C.c_int, [C.POINTER(Config)]),
@@ -121,6 +126,14 @@ _SIGS = {
     "mps_profile_enable": (C.c_int, [_P, C.c_int32]),
     "mps_calibration_copy": (C.c_int, [_P, C.c_size_t, C.c_int32]),
     "mps_profile_read": (C.c_int, [_P, C.POINTER(KernelStat), C.c_size_t, C.POINTER(C.c_size_t)]),
+    "mps_batch_clear": (C.c_int, [_P]),
+    "mps_batch_add_scene": (C.c_int, [_P, C.POINTER(Physics), C.POINTER(SPHMaterial), C.c_size_t, _P, _P, _P, _P,
+                                      C.c_size_t, _P, _P, _P]),
+    "mps_batch_commit": (C.c_int, [_P]),
+    "mps_scene_count": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "mps_scene_info": (C.c_int, [_P, C.c_uint32, C.POINTER(SceneRange)]),
+    "mps_scene_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(StepStats)]),
+    "mps_scene_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(SPHMaterial)]),
 }
 
 _lib = None
@@ -225,6 +238,56 @@ class HipSim:
         m = MeshMaterial(radius, density)
         self._chk(self.lib.mps_sample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
                                                     _ptr(c), _ptr(s), C.byref(m)))
+
+    # ---- batched scenes
+    def batch_clear(self):
+        self._chk(self.lib.mps_batch_clear(self.h))
+
+    def batch_add_scene(self, physics: dict, material: dict, pos, radius, vel=None, mass=None, bpos=None,
+                        bradius=None, bmass=None):
+        """Stage one scene. physics: dt, gravity, box_min, box_max; material: radius, density,
+        viscosity, xsph, surface_tension."""
+        p = Physics(physics["dt"], (C.c_double * 3)(*physics["gravity"]), (C.c_double * 3)(*physics["box_min"]),
+                    (C.c_double * 3)(*physics["box_max"]))
+        m = SPHMaterial(material["radius"], material["density"], material["viscosity"], material["xsph"],
+                        material["surface_tension"])
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        radius = np.ascontiguousarray(np.broadcast_to(np.asarray(radius, np.float32), (n,)))
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float32).reshape(n, 3)
+        mass = None if mass is None else np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (n,)))
+        nb = 0
+        if bpos is not None:
+            bpos = np.ascontiguousarray(bpos, np.float32).reshape(-1, 3)
+            nb = bpos.shape[0]
+            bradius = np.ascontiguousarray(np.broadcast_to(np.asarray(bradius, np.float32), (nb,)))
+            bmass = None if bmass is None else np.ascontiguousarray(bmass, np.float32).reshape(nb)
+        self._chk(self.lib.mps_batch_add_scene(self.h, C.byref(p), C.byref(m), n, _ptr(pos), _ptr(vel), _ptr(radius),
+                                               _ptr(mass), nb, _ptr(bpos), _ptr(bradius), _ptr(bmass)))
+
+    def batch_commit(self):
+        self._chk(self.lib.mps_batch_commit(self.h))
+
+    @property
+    def scene_count(self) -> int:
+        n = C.c_uint32()
+        self._chk(self.lib.mps_scene_count(self.h, C.byref(n)))
+        return n.value
+
+    def scene_info(self, scene: int) -> dict:
+        r = SceneRange()
+        self._chk(self.lib.mps_scene_info(self.h, int(scene), C.byref(r)))
+        return dict(fluid_offset=r.fluid_offset, n_fluid=r.n_fluid, boundary_offset=r.boundary_offset,
+                    n_boundary=r.n_boundary)
+
+    def scene_stats(self, scene: int) -> StepStats:
+        st = StepStats()
+        self._chk(self.lib.mps_scene_stats(self.h, int(scene), C.byref(st)))
+        return st
+
+    def scene_set_material(self, scene: int, radius, density, viscosity, xsph, surface_tension):
+        m = SPHMaterial(radius, density, viscosity, xsph, surface_tension)
+        self._chk(self.lib.mps_scene_set_material(self.h, int(scene), C.byref(m)))
 
     # ---- stepping
     def step(self) -> StepStats:

@@ -1,7 +1,9 @@
 // capi.cpp — the C ABI of include/mps_sph.h on top of the gfx950 kernels.
 //
-// One context = one HIP stream + all device buffers of one scene. The step runs the
-// phases of System::OnUpdate (MPSSystem.cpp:191-291) in order. Solver loops are driven
+// One context = one HIP stream + all device buffers of S >= 1 independent scenes stored
+// back to back (one scene unless the batch entry points define several). The step runs the
+// phases of System::OnUpdate (MPSSystem.cpp:191-291) in order, each phase as one launch
+// over every scene. Solver loops are driven
 // from the host in chunks: a chunk of iterations is enqueued back to back, the kernels
 // keep the loop state on the device (SolverDev) and exit at once after convergence, and
 // the host reads the 8-byte loop header once per chunk. The chunk length follows the
@@ -89,7 +91,31 @@ struct mps_ctx {
     hipStream_t stream = nullptr;
     std::string err;
 
-    uint32_t N = 0, B = 0, frame = 0;
+    // N = fluid slots (scenes back to back, each starting on a kBlock boundary), Nref = fluid
+    // particles (reference ids 0..Nref), B = boundary particles. With one scene N == Nref.
+    uint32_t N = 0, Nref = 0, B = 0, frame = 0;
+
+    // ------------------------------------------------------------------ scenes
+    struct SceneHost {
+        mps_physics phys{};
+        mps_sph_material mat{};
+        double V = 0.0, mass = 0.0;
+        uint32_t nf = 0, nb = 0;  // fluid / boundary particles
+        uint32_t begin = 0;       // first fluid slot
+        uint32_t ref0 = 0;        // first reference id
+        uint32_t b0 = 0;          // first boundary particle
+    };
+    struct StagedScene {
+        SceneHost h;
+        std::vector<float> pos, vel, rad, mass, bpos, brad, bmass;
+    };
+    std::vector<SceneHost> scenes;      // batch layout; empty = one scene made of everything uploaded
+    std::vector<StagedScene> staged;    // scenes added since the last commit
+    std::vector<mps_step_stats> sstats; // per-scene statistics of the last step
+    uint32_t S = 1;
+    DBuf<StepParams> d_params;
+    DBuf<uint32_t> blk_scene, bscene, d_delta, mkey;
+    bool params_dirty = true;
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
     DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
@@ -117,8 +143,11 @@ struct mps_ctx {
     // small device records + pinned host mirrors
     NbrCounters* d_ctr = nullptr;
     NbrCounters* h_ctr = nullptr;
-    SolverDev* d_solver = nullptr;  // [3]: DF, CD, CG
+    SolverDev* d_solver = nullptr;  // [3][S]: DF, CD, CG records of every scene
     SolverDev* h_solver = nullptr;
+    uint32_t solver_cap = 0;        // scenes the two arrays above hold
+    uint32_t* d_active = nullptr;   // [3]: scenes whose DF / CD / CG loop is still running
+    uint32_t* h_active = nullptr;
     uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch
     uint32_t* h_flags = nullptr;
     double* d_err3 = nullptr;
@@ -202,28 +231,86 @@ struct mps_ctx {
         return g;
     }
     size_t cells() const { return size_t(cfg.hash_size[0]) * cfg.hash_size[1] * cfg.hash_size[2]; }
-    StepParams params() const {
+    // The scene table: the batch layout, or one scene covering everything uploaded.
+    std::vector<SceneHost> scene_list() const {
+        if (!scenes.empty()) return scenes;
+        SceneHost h;
+        h.phys = phys;
+        h.mat = mat;
+        h.V = mV;
+        h.mass = mMass;
+        h.nf = N;
+        h.nb = B;
+        return {h};
+    }
+    StepParams params_of(const SceneHost& h) const {
         StepParams P{};
         P.N = N;
-        P.B = B;
-        P.V = float(mV);
-        P.rho0 = float(mat.density);
-        P.mass = float(mMass);
-        P.mu = float(mat.viscosity);
-        P.xsph = float(mat.xsph);
-        P.sigma = float(mat.surface_tension);
-        P.dt = float(phys.dt);
-        P.gx = float(phys.gravity[0]);
-        P.gy = float(phys.gravity[1]);
-        P.gz = float(phys.gravity[2]);
+        P.begin = h.begin;
+        P.end = h.begin + h.nf;
+        P.count = h.nf;
+        P.blk0 = h.begin / mps::dev::kBlock;
+        P.nblk = (h.nf + mps::dev::kBlock - 1) / mps::dev::kBlock;
+        P.V = float(h.V);
+        P.rho0 = float(h.mat.density);
+        P.mass = float(h.mass);
+        P.mu = float(h.mat.viscosity);
+        P.xsph = float(h.mat.xsph);
+        P.sigma = float(h.mat.surface_tension);
+        P.dt = float(h.phys.dt);
+        P.gx = float(h.phys.gravity[0]);
+        P.gy = float(h.phys.gravity[1]);
+        P.gz = float(h.phys.gravity[2]);
         for (int k = 0; k < 3; k++) {
-            P.bmin[k] = float(phys.box_min[k]);
-            P.bmax[k] = float(phys.box_max[k]);
+            P.bmin[k] = float(h.phys.box_min[k]);
+            P.bmax[k] = float(h.phys.box_max[k]);
         }
         P.cg_factor = float(cfg.cg_factor);
         P.faithful = cfg.faithful_quirks;
         return P;
     }
+    // Upload the per-scene parameters (and the workgroup -> scene map for one scene) when
+    // something they depend on changed.
+    void sync_params() {
+        if (!params_dirty) return;
+        const std::vector<SceneHost> sl = scene_list();
+        S = uint32_t(sl.size());
+        std::vector<StepParams> hp;
+        for (const SceneHost& h : sl) hp.push_back(params_of(h));
+        d_params.reserve(hp.size());
+        HIPCHK(hipMemcpyAsync(d_params.p, hp.data(), hp.size() * sizeof(StepParams), hipMemcpyHostToDevice, stream));
+        if (scenes.empty()) {
+            const size_t nblk = std::max<size_t>((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock, 1);
+            blk_scene.reserve(nblk);
+            mps::launch::zero_words(blk_scene.p, nblk, stream);
+            d_delta.reserve(1);
+            mps::launch::zero_words(d_delta.p, 1, stream);
+        }
+        ensure_solvers();
+        sync();
+        params_dirty = false;
+    }
+    void ensure_solvers() {
+        if (solver_cap >= S) return;
+        if (d_solver) HIPCHK(hipFree(d_solver));
+        if (h_solver) HIPCHK(hipHostFree(h_solver));
+        d_solver = nullptr;
+        h_solver = nullptr;
+        solver_cap = 0;
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&d_solver), 3 * size_t(S) * sizeof(SolverDev)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_solver), 3 * size_t(S) * sizeof(SolverDev)));
+        solver_cap = S;
+        for (int k = 0; k < 3; k++) mps::launch::solver_reset(d_solver + size_t(k) * S, S, d_active + k, stream);
+    }
+    mps::Scenes scn() const { return mps::Scenes{d_params.p, blk_scene.p, N, S}; }
+    // bits of the scene index placed above the cell bits of a sort key
+    int scene_bits() const { return S > 1 ? bits_for(S) : 0; }
+    int morton_bits() const {
+        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
+        return 3 * std::min(bits_for(hmax), 10);
+    }
+    int cell_bits() const { return bits_for(cells()); }
+    const uint32_t* bnd_scene() const { return S > 1 ? bscene.p : nullptr; }
     mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p}; }
     int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
     void sync() { HIPCHK(hipStreamSynchronize(stream)); }
@@ -232,6 +319,7 @@ struct mps_ctx {
         const size_t nblk = (std::max<size_t>(std::max<size_t>(N, B), 1) + mps::dev::kBlock - 1) / mps::dev::kBlock;
         partials.reserve(3 * nblk + 8);
     }
+    size_t table_cells() const { return size_t(S) * cells(); }
 
     // ------------------------------------------------------------------ phases
     void zsort_set(uint32_t n, DBuf<float4>& p4, bool boundary) {
@@ -239,9 +327,9 @@ struct mps_ctx {
         const HashGeom g = geom();
         zkey.reserve(n); zid.reserve(n); zkey2.reserve(n); zid2.reserve(n);
         ztemp.reserve(mps::launch::radix_temp_words(n));
-        T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, zkey.p, zid.p, stream); });
-        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
-        const int bits = 3 * std::min(bits_for(hmax), 10);
+        const int zb = morton_bits();
+        T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, bnd_scene(), zb, zkey.p, zid.p, stream); });
+        const int bits = zb + scene_bits();
         uint32_t *k = zkey.p, *v = zid.p, *k2 = zkey2.p, *v2 = zid2.p;
         T_(K_ZSORT, [&] { mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream); });
         const uint32_t* perm = v;
@@ -274,18 +362,19 @@ struct mps_ctx {
     }
     // Fluid Z-sort: the reference permutes its arrays by Morton key; here the data stays in
     // slot order and only the reference labels change (perm[s] <- new index of perm[s]).
+    // Keys sort scene by scene, so scene s keeps the reference ids [ref0, ref0 + nf).
     void zsort_fluid() {
         if (N == 0) return;
         const HashGeom g = geom();
         keyref.reserve(N); vals.reserve(N); zkey2.reserve(N); zid2.reserve(N); zid.reserve(N);
         ztemp.reserve(mps::launch::radix_temp_words(N));
-        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
-        const int bits = 3 * std::min(bits_for(hmax), 10);
+        const int zb = morton_bits();
+        const int bits = zb + scene_bits();
         T_(K_ZSORT, [&] {
-            mps::launch::cell_keys_slots(pos.p, perm.p, N, g, 1, keyref.p, vals.p, stream);
+            mps::launch::cell_keys_slots(pos.p, perm.p, N, g, 1, blk_scene.p, zb, keyref.p, vals.p, stream);
             uint32_t *k = keyref.p, *v = vals.p, *k2 = zkey2.p, *v2 = zid2.p;
-            mps::launch::radix_sort(k, v, k2, v2, ztemp.p, N, bits, stream);
-            mps::launch::invert(v, N, zid.p, stream);       // zid[old ref] = new ref
+            mps::launch::radix_sort(k, v, k2, v2, ztemp.p, Nref, bits, stream);
+            mps::launch::invert(v, Nref, zid.p, stream);    // zid[old ref] = new ref
             mps::launch::relabel(perm.p, zid.p, N, stream);
             mps::launch::invert(perm.p, N, inv.p, stream);
         });
@@ -295,7 +384,7 @@ struct mps_ctx {
         zsort_fluid();
     }
     void hash_set(HashSet& h, uint32_t n, const float4* p4) {
-        const size_t C = cells();
+        const size_t C = table_cells();
         h.start.reserve(C);
         h.end.reserve(C);
         if (!h.cells_init) {
@@ -311,8 +400,9 @@ struct mps_ctx {
         if (n == 0) return;
         h.key.reserve(n); h.id.reserve(n); h.key2.reserve(n); h.id2.reserve(n); h.spos.reserve(n);
         h.temp.reserve(mps::launch::radix_temp_words(n));
-        T_(K_KEYS, [&] { mps::launch::cell_keys(p4, n, geom(), 0, h.key.p, h.id.p, stream); });
-        T_(K_SORT, [&] { mps::launch::radix_sort(h.key.p, h.id.p, h.key2.p, h.id2.p, h.temp.p, n, bits_for(C), stream); });
+        const int cb = cell_bits();
+        T_(K_KEYS, [&] { mps::launch::cell_keys(p4, n, geom(), 0, bnd_scene(), cb, h.key.p, h.id.p, stream); });
+        T_(K_SORT, [&] { mps::launch::radix_sort(h.key.p, h.id.p, h.key2.p, h.id2.p, h.temp.p, n, cb + scene_bits(), stream); });
         // key/key2 (id/id2) are always reserved together, so their capacities are equal and
         // the pointer swap inside radix_sort keeps each DBuf consistent.
         T_(K_CELLS, [&] { mps::launch::cell_table(h.key.p, h.id.p, n, h.start.p, h.end.p, p4, h.spos.p, stream); });
@@ -326,7 +416,7 @@ struct mps_ctx {
     // linear-order key/id arrays are rebuilt from lkey + perm only when exported.
     void hash_fluid() {
         HashSet& h = fh;
-        const size_t C = cells();
+        const size_t C = table_cells();
         const HashGeom g = geom();
         h.start.reserve(C);
         h.end.reserve(C);
@@ -337,18 +427,18 @@ struct mps_ctx {
             });
             h.cells_init = true;
         } else {
-            T_(K_CELLS, [&] { mps::launch::clear_cells_m(h.key.p, lkey.p, h.n_valid, h.start.p, h.end.p, stream); });
+            T_(K_CELLS, [&] { mps::launch::clear_cells_m(mkey.p, lkey.p, h.n_valid, h.start.p, h.end.p, stream); });
         }
         h.n_valid = 0;
         if (N == 0) return;
         keyref.reserve(N); vals.reserve(N); src.reserve(N); lkey.reserve(N); hash_perm.reserve(N);
         h.key.reserve(N); h.id.reserve(N); h.key2.reserve(N); h.id2.reserve(N);
         h.temp.reserve(mps::launch::radix_temp_words(N));
-        T_(K_KEYS, [&] { mps::launch::cell_morton_slots(pos.p, perm.p, N, g, keyref.p, vals.p, stream); });
-        const uint32_t hmax = std::max(cfg.hash_size[0], std::max(cfg.hash_size[1], cfg.hash_size[2]));
-        const int bits = 3 * std::min(bits_for(hmax), 10);
+        const int mb = morton_bits();
+        T_(K_KEYS, [&] { mps::launch::cell_morton_slots(pos.p, perm.p, N, g, blk_scene.p, mb, keyref.p, vals.p, stream); });
+        const int bits = mb + scene_bits();
         uint32_t *k = keyref.p, *v = vals.p, *k2 = h.key2.p, *v2 = h.id2.p;
-        T_(K_SORT, [&] { mps::launch::radix_sort(k, v, k2, v2, h.temp.p, N, bits, stream); });
+        T_(K_SORT, [&] { mps::launch::radix_sort(k, v, k2, v2, h.temp.p, Nref, bits, stream); });
         if (k == keyref.p) {
             std::swap(h.key, keyref);
             std::swap(h.id, vals);
@@ -357,7 +447,8 @@ struct mps_ctx {
             std::swap(h.id, h.id2);
         }
         T_(K_REORDER, [&] {
-            mps::launch::compose(h.id.p, inv.p, N, src.p, stream);
+            // sorted position q -> slot; perm, src (old slot) and the key per slot
+            mps::launch::place(h.key.p, h.id.p, Nref, mb, d_delta.p, inv.p, perm.p, src.p, mkey.p, stream);
             auto mv4 = [&](DBuf<float4>& a) {
                 mps::launch::gather_f4(src.p, N, a.p, tmp4.p, stream);
                 std::swap(a, tmp4);
@@ -370,11 +461,12 @@ struct mps_ctx {
             tmp1.reserve(N);
             mv4(pos); mv4(vel); mv4(color); mv4(cgb);
             mv1(rho); mv1(alpha); mv1(kappa); mv1(delta); mv1(fst);
-            HIPCHK(hipMemcpyAsync(perm.p, h.id.p, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(hash_perm.p, h.id.p, sizeof(uint32_t) * N, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(hash_perm.p, h.id.p, sizeof(uint32_t) * Nref, hipMemcpyDeviceToDevice, stream));
             mps::launch::invert(perm.p, N, inv.p, stream);
         });
-        T_(K_CELLS, [&] { mps::launch::cell_table_m(h.key.p, pos.p, N, g, h.start.p, h.end.p, lkey.p, stream); });
+        T_(K_CELLS, [&] {
+            mps::launch::cell_table_m(mkey.p, pos.p, N, g, mb, uint32_t(cells()), h.start.p, h.end.p, lkey.p, stream);
+        });
         h.n_valid = N;
         pairs_valid = false;
     }
@@ -424,15 +516,18 @@ struct mps_ctx {
             if (do_b && B) {
                 nbb.reserve(size_t(Kbb) * B);
                 cbb.reserve(B);
-                T_(K_NEI_BND, [&] { mps::launch::neighbors_boundary(bpos.p, B, g, view(bh, B), nbb.p, Kbb, cbb.p, d_ctr, stream); });
+                T_(K_NEI_BND, [&] {
+                    mps::launch::neighbors_boundary(bpos.p, B, bnd_scene(), uint32_t(cells()), g, view(bh, B), nbb.p, Kbb, cbb.p,
+                                                    d_ctr, stream);
+                });
             }
             nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
             cff.reserve(std::max<uint32_t>(N, 1));
             nfb.reserve(size_t(Kfb) * std::max<uint32_t>(N, 1));
             cfb.reserve(std::max<uint32_t>(N, 1));
             T_(K_NEI_FLUID, [&] {
-                mps::launch::neighbors_fluid(pos.p, N, g, fluid_view(), view(bh, B), nff.p, Kff, cff.p, nfb.p, Kfb, cfb.p,
-                                             d_ctr, stream);
+                mps::launch::neighbors_fluid(pos.p, scn(), uint32_t(cells()), g, fluid_view(), view(bh, B), nff.p, Kff, cff.p,
+                                             nfb.p, Kfb, cfb.p, d_ctr, stream);
             });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
             sync();
@@ -469,7 +564,7 @@ struct mps_ctx {
             pairG_fb.reserve(size_t(Kfb) * m);
         }
         T_(K_DENSITY, [&] {
-            mps::launch::density_alpha(cubic(), flags, params(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
+            mps::launch::density_alpha(cubic(), flags, scn(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
                                        pairG_ff.p, pairG_fb.p, stream);
         });
         if (flags & mps::launch::kDoPairs) pairs_valid = true;
@@ -478,19 +573,35 @@ struct mps_ctx {
         if (!pairs_valid) density_alpha(mps::launch::kDoPairs);
     }
 
-    void fetch_solver(int idx) {
-        HIPCHK(hipMemcpyAsync(&h_solver[idx], &d_solver[idx], 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    // number of scenes whose loop `idx` is still running
+    uint32_t fetch_active(int idx) {
+        HIPCHK(hipMemcpyAsync(&h_active[idx], &d_active[idx], sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
+        return h_active[idx];
+    }
+    // Copy the S records of loop `idx` back and return the reference loop variable of each
+    // scene (cap + 1 when it hit the cap), the largest one in *lmax.
+    std::vector<uint32_t> fetch_records(int idx, uint32_t cap, uint32_t* lmax) {
+        SolverDev* h = h_solver + size_t(idx) * S;
+        HIPCHK(hipMemcpyAsync(h, d_solver + size_t(idx) * S, S * sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<uint32_t> l(S);
+        *lmax = 0;
+        for (uint32_t q = 0; q < S; q++) {
+            l[q] = h[q].done ? h[q].l : cap + 1;
+            *lmax = std::max(*lmax, l[q]);
+        }
+        return l;
     }
     // DF / CD loop (MPSSPHDFSPHUtil.cu:270-441)
     uint32_t pressure_solve(bool df, mps_step_stats* st) {
         const int idx = df ? 0 : 1;
         if (N == 0) return 0;
-        SolverDev* sd = &d_solver[idx];
-        mps::launch::solver_reset(sd, stream);
+        SolverDev* sd = d_solver + size_t(idx) * S;
+        mps::launch::solver_reset(sd, S, d_active + idx, stream);
         ensure_partials();
         ensure_pairs();
-        const StepParams P = params();
+        const mps::Scenes P = scn();
         const double tol = df ? cfg.df_tol : cfg.cd_tol;
         const uint32_t lmin = df ? cfg.df_min_iter : cfg.cd_min_iter;
         const uint32_t cap = cfg.max_iter;
@@ -504,29 +615,38 @@ struct mps_ctx {
                     mps::launch::pressure_eval(df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p, pairG_ff.p,
                                                pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
                 }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_pressure(partials.p, N, sd, it, tol, lmin, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, it, tol, lmin, stream); }, false);
                 T_(K_APPLY, [&] {
                     mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd, stream);
                 }, false);
             }
-            fetch_solver(idx);
-            done = h_solver[idx].done != 0;
+            done = fetch_active(idx) == 0;
         }
-        const uint32_t l = done ? h_solver[idx].l : cap + 1;
+        uint32_t l = 0;
+        const std::vector<uint32_t> ls = fetch_records(idx, cap, &l);
         pred[idx] = std::max<uint32_t>(l, 1);
-        if (prof_on) {  // eval and fold 1..min(l,cap), apply 1..l-1 did work
+        if (prof_on) {  // eval and fold 1..min(l,cap), apply 1..l-1 did work (slowest scene)
             kuse[df ? K_EVAL_DF : K_EVAL_CD] += std::min(l, cap);
             kuse[K_FOLD] += std::min(l, cap);
             kuse[K_APPLY] += l - 1;
         }
-        HIPCHK(hipMemcpyAsync(&h_solver[idx], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
-        sync();
+        for (uint32_t q = 0; q < S && q < sstats.size(); q++) {
+            mps_step_stats& o = sstats[q];
+            const SolverDev& r = h_solver[size_t(idx) * S + q];
+            double* log = df ? o.err_df : o.err_cd;
+            const uint32_t nlog = std::min<uint32_t>(std::min(ls[q], cap), MPS_MAX_ITER_LOG);
+            for (uint32_t k = 0; k < nlog; k++) log[k] = r.log[k];
+            if (df) { o.l_df = ls[q]; o.hit_cap_df = ls[q] > cap; }
+            else { o.l_cd = ls[q]; o.hit_cap_cd = ls[q] > cap; }
+        }
         if (st) {
-            double* log = df ? st->err_df : st->err_cd;
-            const uint32_t nlog = std::min<uint32_t>(std::min(l, cap), MPS_MAX_ITER_LOG);
-            for (uint32_t k = 0; k < nlog; k++) log[k] = h_solver[idx].log[k];
             if (df) { st->l_df = l; st->hit_cap_df = l > cap; }
             else { st->l_cd = l; st->hit_cap_cd = l > cap; }
+            if (S == 1) {
+                double* log = df ? st->err_df : st->err_cd;
+                const double* src = df ? sstats[0].err_df : sstats[0].err_cd;
+                std::memcpy(log, src, sizeof(double) * MPS_MAX_ITER_LOG);
+            }
         }
         return l;
     }
@@ -534,10 +654,11 @@ struct mps_ctx {
     // MPSSolver.cuh:12-73)
     uint32_t viscosity_cg(mps_step_stats* st) {
         if (N == 0) return 0;
-        SolverDev* sd = &d_solver[2];
+        SolverDev* sd = d_solver + 2 * size_t(S);
+        uint32_t* act = d_active + 2;
         ensure_partials();
         ensure_pairs();
-        const StepParams P = params();
+        const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
         pairAB.reserve(size_t(Kff) * m);
         cgrecA.reserve(2 * m);
@@ -546,13 +667,13 @@ struct mps_ctx {
         T_(K_CG_SETUP, [&] {
             mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, pairAB.p, cgb.p, stream);
         });
-        mps::launch::solver_reset(sd, stream);
+        mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         T_(K_CG_INIT, [&] {
             mps::launch::cg_pack(N, pos.p, cgx.p, cgrecA.p, stream);
             mps::launch::cg_apply(0, P, nbr(), pairAB.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p, stream);
         });
-        T_(K_FOLD, [&] { mps::launch::fold_cg(0, partials.p, N, sd, 0, tol, stream); });
+        T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
         uint32_t it = 1;
         bool done = false;
@@ -564,15 +685,15 @@ struct mps_ctx {
                     mps::launch::cg_apply(1, P, nbr(), pairAB.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd, partials.p,
                                           stream);
                 }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_cg(1, partials.p, N, sd, it, tol, stream); }, false);
-                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(N, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_cg(2, partials.p, N, sd, it, tol, stream); }, false);
-                T_(K_CG_DIR, [&] { mps::launch::cg_direction(N, cgr.p, cgrecB.p, sd, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, it, tol, stream); }, false);
+                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
+                T_(K_FOLD, [&] { mps::launch::fold_cg(2, P, partials.p, sd, act, it, tol, stream); }, false);
+                T_(K_CG_DIR, [&] { mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream); }, false);
             }
-            fetch_solver(2);
-            done = h_solver[2].done != 0;
+            done = fetch_active(2) == 0;
         }
-        const uint32_t l = done ? h_solver[2].l : cap + 1;
+        uint32_t l = 0;
+        const std::vector<uint32_t> ls = fetch_records(2, cap, &l);
         pred[2] = std::max<uint32_t>(l, 1);
         if (prof_on) {  // A.p and the update ran for iterations 1..l-1, the direction for 1..l-2
             kuse[K_CG_APPLY] += std::min(l - 1, cap);
@@ -580,12 +701,17 @@ struct mps_ctx {
             kuse[K_FOLD] += 2 * std::min(l - 1, cap);
             kuse[K_CG_DIR] += done ? (l >= 2 ? l - 2 : 0) : cap;
         }
-        T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(N, cgx.p, vel.p, stream); });
-        HIPCHK(hipMemcpyAsync(&h_solver[2], sd, sizeof(SolverDev), hipMemcpyDeviceToHost, stream));
-        sync();
+        T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(P, cgx.p, vel.p, stream); });
+        for (uint32_t q = 0; q < S && q < sstats.size(); q++) {
+            mps_step_stats& o = sstats[q];
+            const SolverDev& r = h_solver[2 * size_t(S) + q];
+            const uint32_t nlog = std::min<uint32_t>(std::min(ls[q], cap), MPS_MAX_ITER_LOG);
+            for (uint32_t k = 0; k < nlog; k++) o.rr_cg[k] = r.log[k];
+            o.l_cg = ls[q];
+            o.hit_cap_cg = ls[q] > cap;
+        }
         if (st) {
-            const uint32_t nlog = std::min<uint32_t>(std::min(l, cap), MPS_MAX_ITER_LOG);
-            for (uint32_t k = 0; k < nlog; k++) st->rr_cg[k] = h_solver[2].log[k];
+            if (S == 1) std::memcpy(st->rr_cg, sstats[0].rr_cg, sizeof(double) * MPS_MAX_ITER_LOG);
             st->l_cg = l;
             st->hit_cap_cg = l > cap;
         }
@@ -594,8 +720,9 @@ struct mps_ctx {
     // ApplyImplicitJacobiViscosity (SPHUtil.cu:344-453, commented out in the reference)
     uint32_t viscosity_jacobi(mps_step_stats* st) {
         if (N == 0) return 0;
+        if (S > 1) throw ApiError{MPS_ERR_STATE, "the Jacobi viscosity solve runs one scene per context"};
         ensure_partials();
-        const StepParams P = params();
+        const mps::Scenes P = scn();
         jprev.reserve(N); jcand.reserve(N);
         for (auto& t : jt) t.reserve(N);
         HIPCHK(hipMemcpyAsync(cgx.p, vel.p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
@@ -649,7 +776,11 @@ struct mps_ctx {
             if (e < cfg.jacobi_tol) { conv = true; break; }
             l++;
         }
-        mps::launch::copy_vel_out(N, cgx.p, vel.p, stream);
+        mps::launch::copy_vel_out(P, cgx.p, vel.p, stream);
+        if (!sstats.empty()) {
+            sstats[0].l_cg = l;
+            sstats[0].hit_cap_cg = conv ? 0u : 1u;
+        }
         if (st) {
             st->l_cg = l;
             st->hit_cap_cg = conv ? 0u : 1u;
@@ -661,10 +792,21 @@ struct mps_ctx {
         else if (cfg.visc_solver == MPS_VISC_JACOBI) viscosity_jacobi(st);
     }
     void advect() {
-        T_(K_ADVECT, [&] { mps::launch::advect(params(), pos.p, vel.p, d_flags, stream); });
+        T_(K_ADVECT, [&] { mps::launch::advect(scn(), pos.p, vel.p, d_flags, stream); });
     }
 
+    void begin_stats() {
+        sstats.assign(S, mps_step_stats{});
+        const std::vector<SceneHost> sl = scene_list();
+        for (uint32_t q = 0; q < S; q++) {
+            sstats[q].frame = frame;
+            sstats[q].n_fluid = sl[q].nf;
+            sstats[q].n_boundary = sl[q].nb;
+        }
+    }
     void run_phase(int32_t ph, mps_step_stats* st) {
+        sync_params();
+        if (sstats.size() != S) begin_stats();
         switch (ph) {
             case MPS_PHASE_ZSORT: zsort(); break;
             case MPS_PHASE_HASH: hash(); break;
@@ -684,8 +826,10 @@ struct mps_ctx {
         mps_step_stats local;
         if (!st) st = &local;
         std::memset(st, 0, sizeof(*st));
+        sync_params();
+        begin_stats();
         st->frame = frame;
-        st->n_fluid = N;
+        st->n_fluid = Nref;
         st->n_boundary = B;
         const uint32_t regrow0 = nei_regrow;
         mps::launch::zero_words(d_flags, 1, stream);
@@ -708,6 +852,16 @@ struct mps_ctx {
         st->nnz_fb = nnz_fb;
         st->nnz_bb = nnz_bb;
         st->nei_regrow = nei_regrow - regrow0;
+        for (mps_step_stats& o : sstats) {
+            o.zsorted = st->zsorted;
+            o.nonfinite = st->nonfinite;
+            o.nei_regrow = st->nei_regrow;
+        }
+        if (S == 1) {  // totals of the one scene
+            sstats[0].nnz_ff = nnz_ff;
+            sstats[0].nnz_fb = nnz_fb;
+            sstats[0].nnz_bb = nnz_bb;
+        }
         frame++;
         if (prof_on) prof_collect();
     }
@@ -717,7 +871,34 @@ struct mps_ctx {
         const size_t m = std::max<uint32_t>(n, 1);
         pos.reserve(m); vel.reserve(m); color.reserve(m); cgb.reserve(m); cgx.reserve(m); cgr.reserve(m); cgp.reserve(m);
         cgAp.reserve(m); rho.reserve(m); alpha.reserve(m); kappa.reserve(m); delta.reserve(m); fst.reserve(m);
-        cff.reserve(m); cfb.reserve(m); perm.reserve(m); inv.reserve(m);
+        cff.reserve(m); cfb.reserve(m); perm.reserve(m); inv.reserve(m); mkey.reserve(m);
+    }
+    void require_single(const char* what) const {
+        if (!scenes.empty()) throw ApiError{MPS_ERR_STATE, std::string(what) + ": the context holds a batch of scenes"};
+    }
+    // Zero the derived fluid state of slots [0, n) and start at frame 0.
+    void reset_fluid_state(uint32_t n) {
+        if (n) {
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(rho.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(alpha.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(kappa.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(delta.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(fst.p), n, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(cgb.p), 4 * size_t(n), stream);
+            mps::launch::zero_words(cff.p, n, stream);
+            mps::launch::zero_words(cfb.p, n, stream);
+            // Padding slots between scenes keep kEmpty in mkey and src for good: the hash
+            // writes both only at live slots, and the reorder gathers skip kEmpty sources.
+            mps::launch::fill_u32(mkey.p, n, mps::dev::kEmpty, stream);
+            src.reserve(n);
+            mps::launch::fill_u32(src.p, n, mps::dev::kEmpty, stream);
+        }
+        fh.cells_init = false;
+        fh.n_valid = 0;
+        pairs_valid = false;
+        frame = 0;
+        pred[0] = 2; pred[1] = 3; pred[2] = 8;
+        params_dirty = true;
     }
     void alloc_boundary(uint32_t n) {
         const size_t m = std::max<uint32_t>(n, 1);
@@ -726,37 +907,138 @@ struct mps_ctx {
     }
     void upload_fluid(size_t n, const float* p, const float* v, const float* r, const float* m) {
         if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
+        require_single("mps_upload_fluid");
         alloc_fluid(uint32_t(n));
         std::vector<float4> hp(n), hv(n), hc(n, make_float4(0.f, 0.f, 1.f, 1.f));
         for (size_t i = 0; i < n; i++) {
             hp[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], r[i]);
             hv[i] = make_float4(v ? v[3 * i] : 0.f, v ? v[3 * i + 1] : 0.f, v ? v[3 * i + 2] : 0.f, m ? m[i] : float(mMass));
         }
-        N = uint32_t(n);
+        N = Nref = uint32_t(n);
         if (n) {
             HIPCHK(hipMemcpyAsync(pos.p, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(vel.p, hv.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
             HIPCHK(hipMemcpyAsync(color.p, hc.data(), n * sizeof(float4), hipMemcpyHostToDevice, stream));
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(rho.p), n, stream);
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(alpha.p), n, stream);
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(kappa.p), n, stream);
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(delta.p), n, stream);
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(fst.p), n, stream);
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(cgb.p), 4 * n, stream);
-            mps::launch::zero_words(cff.p, n, stream);
-            mps::launch::zero_words(cfb.p, n, stream);
             mps::launch::iota(perm.p, uint32_t(n), stream);
             mps::launch::iota(inv.p, uint32_t(n), stream);
         }
-        fh.cells_init = false;
-        fh.n_valid = 0;
+        reset_fluid_state(uint32_t(n));
+        sync();
+    }
+
+    // ------------------------------------------------------------------ batched scenes
+    void batch_clear() {
+        sync();
+        staged.clear();
+        scenes.clear();
+        sstats.clear();
+        N = Nref = B = 0;
+        S = 1;
+        fh.cells_init = bh.cells_init = false;
+        fh.n_valid = bh.n_valid = 0;
+        bnd_dirty = true;
         pairs_valid = false;
         frame = 0;
-        pred[0] = 2; pred[1] = 3; pred[2] = 8;
+        params_dirty = true;
+    }
+    void batch_add(const mps_physics& ph, const mps_sph_material& mt, size_t nf, const float* p, const float* v,
+                   const float* r, const float* m, size_t nb, const float* bp, const float* br, const float* bm) {
+        if (nf == 0) throw ApiError{MPS_ERR_INVALID, "a scene needs fluid particles"};
+        if (!(ph.dt > 0.0) || !(mt.radius > 0.0) || !(mt.density > 0.0)) throw ApiError{MPS_ERR_INVALID, "bad physics or material"};
+        StagedScene sc;
+        sc.h.phys = ph;
+        sc.h.mat = mt;
+        sc.h.V = mt.radius * mt.radius * mt.radius / 48.0 * 3.141592;
+        sc.h.mass = mt.density * sc.h.V;
+        sc.h.nf = uint32_t(nf);
+        sc.h.nb = uint32_t(nb);
+        sc.pos.assign(p, p + 3 * nf);
+        sc.rad.assign(r, r + nf);
+        sc.vel = v ? std::vector<float>(v, v + 3 * nf) : std::vector<float>(3 * nf, 0.0f);
+        sc.mass = m ? std::vector<float>(m, m + nf) : std::vector<float>(nf, float(sc.h.mass));
+        if (nb) {
+            sc.bpos.assign(bp, bp + 3 * nb);
+            sc.brad.assign(br, br + nb);
+            sc.bmass = bm ? std::vector<float>(bm, bm + nb) : std::vector<float>(nb, 0.0f);
+        }
+        staged.push_back(std::move(sc));
+    }
+    // Lay the staged scenes out back to back and upload them (replaces the current state).
+    void batch_commit() {
+        if (staged.empty()) throw ApiError{MPS_ERR_STATE, "no scenes were added"};
+        const uint32_t blk = mps::dev::kBlock;
+        uint64_t n = 0, nref = 0, nb = 0;
+        std::vector<SceneHost> lay;
+        for (StagedScene& sc : staged) {
+            SceneHost h = sc.h;
+            h.begin = uint32_t((n + blk - 1) / blk * blk);
+            h.ref0 = uint32_t(nref);
+            h.b0 = uint32_t(nb);
+            n = uint64_t(h.begin) + h.nf;
+            nref += h.nf;
+            nb += h.nb;
+            if (n > 0x7FFFFFFFu || nb > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
+            lay.push_back(h);
+        }
+        const uint32_t nsc = uint32_t(lay.size());
+        if (morton_bits() + bits_for(nsc) > 31 || cell_bits() + bits_for(nsc) > 31 ||
+            uint64_t(nsc) * cells() > 0xFFFFFFFFull)
+            throw ApiError{MPS_ERR_INVALID, "too many scenes for the hash size"};
+        N = uint32_t(n);
+        Nref = uint32_t(nref);
+        B = uint32_t(nb);
+        alloc_fluid(N);
+        alloc_boundary(B);
+        std::vector<float4> hp(N, make_float4(0.f, 0.f, 0.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
+        std::vector<float4> hc(N, make_float4(0.f, 0.f, 1.f, 1.f));
+        std::vector<uint32_t> hperm(N, mps::dev::kEmpty), hinv(std::max<uint32_t>(Nref, 1)), hbs(std::max<uint32_t>(B, 1), 0u);
+        std::vector<float4> hbp(std::max<uint32_t>(B, 1)), hbv(std::max<uint32_t>(B, 1));
+        const size_t nblk = std::max<size_t>((size_t(N) + blk - 1) / blk, 1);
+        std::vector<uint32_t> hblk(nblk, 0u), hdelta(nsc);
+        for (uint32_t q = 0; q < nsc; q++) {
+            const SceneHost& h = lay[q];
+            const StagedScene& sc = staged[q];
+            for (uint32_t i = 0; i < h.nf; i++) {
+                const uint32_t sl = h.begin + i;
+                hp[sl] = make_float4(sc.pos[3 * i], sc.pos[3 * i + 1], sc.pos[3 * i + 2], sc.rad[i]);
+                hv[sl] = make_float4(sc.vel[3 * i], sc.vel[3 * i + 1], sc.vel[3 * i + 2], sc.mass[i]);
+                hperm[sl] = h.ref0 + i;
+                hinv[h.ref0 + i] = sl;
+            }
+            for (uint32_t b = h.begin / blk; b < (h.begin + h.nf + blk - 1) / blk; b++) hblk[b] = q;
+            hdelta[q] = h.begin - h.ref0;
+            for (uint32_t i = 0; i < h.nb; i++) {
+                hbp[h.b0 + i] = make_float4(sc.bpos[3 * i], sc.bpos[3 * i + 1], sc.bpos[3 * i + 2], sc.brad[i]);
+                hbv[h.b0 + i] = make_float4(0.f, 0.f, 0.f, sc.bmass[i]);
+                hbs[h.b0 + i] = q;
+            }
+        }
+        blk_scene.reserve(nblk);
+        d_delta.reserve(nsc);
+        bscene.reserve(std::max<uint32_t>(B, 1));
+        up(pos.p, hp); up(vel.p, hv); up(color.p, hc); up(perm.p, hperm); up(inv.p, hinv);
+        up(blk_scene.p, hblk); up(d_delta.p, hdelta);
+        if (B) {
+            up(bpos.p, hbp); up(bvel.p, hbv); up(bscene.p, hbs);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bcolor.p), 4 * size_t(B), stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bvol.p), B, stream);
+            mps::launch::zero_words(bface.p, B, stream);
+            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bbcc.p), 2 * size_t(B), stream);
+            mps::launch::zero_words(cbb.p, B, stream);
+        }
+        scenes = lay;
+        S = nsc;
+        staged.clear();
+        bh.cells_init = false;
+        bh.n_valid = 0;
+        bnd_dirty = true;
+        reset_fluid_state(N);
+        sstats.clear();
         sync();
     }
     void upload_boundary(size_t n, const float* p, const float* r, const float* m) {
         if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
+        require_single("mps_upload_boundary");
         alloc_boundary(uint32_t(n));
         std::vector<float4> hp(n), hv(n);
         for (size_t i = 0; i < n; i++) {
@@ -776,6 +1058,7 @@ struct mps_ctx {
         bh.cells_init = false;
         bh.n_valid = 0;
         bnd_dirty = true;
+        params_dirty = true;
         sync();
     }
     // Face ownership bits: RTriangleBuild_kernel over per-vertex face lists sorted by face id
@@ -816,6 +1099,8 @@ struct mps_ctx {
     void sample_mesh(const double* verts, size_t nv, const uint32_t* tris, size_t nt, const double* center,
                      const double* size, const mps_mesh_material* m) {
         if (!verts || !tris || nv == 0 || nt == 0) throw ApiError{MPS_ERR_INVALID, "empty mesh"};
+        require_single("mps_sample_boundary_mesh");
+        params_dirty = true;
         for (size_t i = 0; i < 3 * nt; i++)
             if (tris[i] >= nv) throw ApiError{MPS_ERR_INVALID, "triangle index out of range"};
         std::vector<double> V(verts, verts + 3 * nv);
@@ -903,15 +1188,15 @@ struct mps_ctx {
     // CSR in reference order from slot-order ELL lists; fluid neighbour slots are mapped to
     // reference indices (map_ids), boundary neighbours are already reference indices.
     std::vector<uint32_t> csr_idx_fluid(const uint32_t* cnt) {
-        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, N), idx(size_t(N) + 1, 0u);
-        for (uint32_t i = 0; i < N; i++) idx[i + 1] = idx[i] + c[iv[i]];
+        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, Nref), idx(size_t(Nref) + 1, 0u);
+        for (uint32_t i = 0; i < Nref; i++) idx[i + 1] = idx[i] + c[iv[i]];
         return idx;
     }
     std::vector<uint32_t> csr_nei_fluid(const uint32_t* nbrs, const uint32_t* cnt, uint32_t K, bool map_ids) {
-        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, N), pm = down(perm.p, N);
+        std::vector<uint32_t> c = down(cnt, N), iv = down(inv.p, Nref), pm = down(perm.p, N);
         std::vector<uint32_t> ell = down(nbrs, size_t(K) * N);
         std::vector<uint32_t> out;
-        for (uint32_t i = 0; i < N; i++) {
+        for (uint32_t i = 0; i < Nref; i++) {
             const uint32_t sl = iv[i];
             for (uint32_t k = 0; k < c[sl]; k++) {
                 const uint32_t j = ell[size_t(k) * N + sl];
@@ -951,15 +1236,16 @@ struct mps_ctx {
         };
         auto fl4 = [&](DBuf<float4>& a, int comp /* -1 = xyz, 3 = w, 4 = all */) {
             const size_t w = comp == -1 ? 3 : (comp == 4 ? 4 : 1);
-            *count = N;
+            *count = Nref;
             *eb = 4 * w;
             if (!(dst || src)) return;
-            if (bytes != size_t(N) * 4 * w) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            if (bytes != size_t(Nref) * 4 * w) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
             const auto& P = get_perm();
             std::vector<float4> h = down(a.p, N);
             if (dst) {
                 float* d = static_cast<float*>(dst);
                 for (uint32_t s = 0; s < N; s++) {
+                    if (P[s] == mps::dev::kEmpty) continue;
                     const float q[4] = {h[s].x, h[s].y, h[s].z, h[s].w};
                     if (comp == -1) for (int k = 0; k < 3; k++) d[3 * size_t(P[s]) + k] = q[k];
                     else if (comp == 4) for (int k = 0; k < 4; k++) d[4 * size_t(P[s]) + k] = q[k];
@@ -968,6 +1254,7 @@ struct mps_ctx {
             } else {
                 const float* q = static_cast<const float*>(src);
                 for (uint32_t s = 0; s < N; s++) {
+                    if (P[s] == mps::dev::kEmpty) continue;
                     if (comp == -1) { h[s].x = q[3 * size_t(P[s])]; h[s].y = q[3 * size_t(P[s]) + 1]; h[s].z = q[3 * size_t(P[s]) + 2]; }
                     else if (comp == 4) { h[s].x = q[4 * size_t(P[s])]; h[s].y = q[4 * size_t(P[s]) + 1]; h[s].z = q[4 * size_t(P[s]) + 2]; h[s].w = q[4 * size_t(P[s]) + 3]; }
                     else h[s].w = q[P[s]];
@@ -976,18 +1263,20 @@ struct mps_ctx {
             }
         };
         auto fl1 = [&](DBuf<float>& a) {
-            *count = N;
+            *count = Nref;
             *eb = 4;
             if (!(dst || src)) return;
-            if (bytes != size_t(N) * 4) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
+            if (bytes != size_t(Nref) * 4) throw ApiError{MPS_ERR_INVALID, "size mismatch"};
             const auto& P = get_perm();
             std::vector<float> h = down(a.p, N);
             if (dst) {
                 float* d = static_cast<float*>(dst);
-                for (uint32_t s = 0; s < N; s++) d[P[s]] = h[s];
+                for (uint32_t s = 0; s < N; s++)
+                    if (P[s] != mps::dev::kEmpty) d[P[s]] = h[s];
             } else {
                 const float* q = static_cast<const float*>(src);
-                for (uint32_t s = 0; s < N; s++) h[s] = q[P[s]];
+                for (uint32_t s = 0; s < N; s++)
+                    if (P[s] != mps::dev::kEmpty) h[s] = q[P[s]];
                 up(a.p, h);
             }
         };
@@ -1047,6 +1336,8 @@ struct mps_ctx {
             if (src) throw ApiError{MPS_ERR_INVALID, "array is read-only"};
             raw(const_cast<uint32_t*>(p), n, 4);
         };
+        const bool hash_id = id >= MPS_ARR_FHASH_KEY && id <= MPS_ARR_BHASH_END;
+        if (hash_id && S > 1) throw ApiError{MPS_ERR_STATE, "hash tables are exported for a single scene only"};
         switch (id) {
             case MPS_ARR_FLUID_POS:
                 fl4(pos, -1);
@@ -1116,6 +1407,9 @@ struct mps_ctx {
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
+        d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();
+        if (d_active) (void)hipFree(d_active);
+        if (h_active) (void)hipHostFree(h_active);
         for (const Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
         for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
         if (d_ctr) (void)hipFree(d_ctr);
@@ -1197,16 +1491,16 @@ int mps_create(const mps_config* cfg, mps_ctx** out) {
     const int rc = guarded(c, [&] {
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_ctr), sizeof(NbrCounters)));
-        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_solver), 3 * sizeof(SolverDev)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_active), 4 * sizeof(uint32_t)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_active), 4 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_flags), 4 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_err3), 4 * sizeof(double)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ctr), sizeof(NbrCounters)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_solver), 3 * sizeof(SolverDev)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 4 * sizeof(uint32_t)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_err3), 4 * sizeof(double)));
         std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
         mps::launch::zero_words(c->d_flags, 4, c->stream);
-        for (int k = 0; k < 3; k++) mps::launch::solver_reset(&c->d_solver[k], c->stream);
+        c->ensure_solvers();
         c->sync();
     });
     if (rc != MPS_OK) {
@@ -1233,6 +1527,8 @@ int mps_set_physics(mps_ctx* c, const mps_physics* p) {
         return MPS_ERR_INVALID;
     }
     c->phys = *p;
+    for (auto& h : c->scenes) h.phys = *p;
+    c->params_dirty = true;
     return MPS_OK;
 }
 
@@ -1246,6 +1542,12 @@ int mps_set_fluid_material(mps_ctx* c, const mps_sph_material* m) {
     c->mat = *m;
     c->mV = m->radius * m->radius * m->radius / 48.0 * 3.141592;
     c->mMass = m->density * c->mV;
+    for (auto& h : c->scenes) {
+        h.mat = *m;
+        h.V = c->mV;
+        h.mass = c->mMass;
+    }
+    c->params_dirty = true;
     return MPS_OK;
 }
 
@@ -1337,11 +1639,11 @@ int mps_set_frame(mps_ctx* c, uint32_t f) {
 int mps_readback_async(mps_ctx* c, float* host_pos, float* host_radius, float* host_rgba, mps_fence* fence) {
     if (!c || !fence) return MPS_ERR_INVALID;
     return guarded(c, [&] {
-        const uint32_t n = c->N;
+        const uint32_t n = c->Nref;
         if (host_pos) c->rb_xyz.reserve(3 * size_t(std::max<uint32_t>(n, 1)));
         if (host_radius) c->rb_rad.reserve(std::max<uint32_t>(n, 1));
         if (host_rgba) c->rb_rgba.reserve(std::max<uint32_t>(n, 1));
-        mps::launch::pack_readback(n, c->pos.p, c->color.p, c->perm.p, host_pos ? c->rb_xyz.p : nullptr,
+        mps::launch::pack_readback(c->N, c->pos.p, c->color.p, c->perm.p, host_pos ? c->rb_xyz.p : nullptr,
                                    host_radius ? c->rb_rad.p : nullptr,
                                    host_rgba ? reinterpret_cast<float*>(c->rb_rgba.p) : nullptr, c->stream);
         if (n) {
@@ -1414,6 +1716,62 @@ int mps_profile_read(mps_ctx* c, mps_kernel_stat* out, size_t capacity, size_t* 
         }
         *count = n;
     });
+}
+
+int mps_batch_clear(mps_ctx* c) {
+    return guarded(c, [&] { c->batch_clear(); });
+}
+
+int mps_batch_add_scene(mps_ctx* c, const mps_physics* phys, const mps_sph_material* mat, size_t n_fluid, const float* pos,
+                        const float* vel, const float* radius, const float* mass, size_t n_boundary, const float* bpos,
+                        const float* bradius, const float* bmass) {
+    if (!c || !phys || !mat || (n_fluid && (!pos || !radius)) || (n_boundary && (!bpos || !bradius))) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        c->batch_add(*phys, *mat, n_fluid, pos, vel, radius, mass, n_boundary, bpos, bradius, bmass);
+    });
+}
+
+int mps_batch_commit(mps_ctx* c) {
+    return guarded(c, [&] { c->batch_commit(); });
+}
+
+int mps_scene_count(mps_ctx* c, uint32_t* n) {
+    if (!c || !n) return MPS_ERR_INVALID;
+    *n = c->scenes.empty() ? 1u : uint32_t(c->scenes.size());
+    return MPS_OK;
+}
+
+int mps_scene_info(mps_ctx* c, uint32_t scene, mps_scene_range* out) {
+    if (!c || !out) return MPS_ERR_INVALID;
+    const std::vector<mps_ctx::SceneHost> sl = c->scene_list();
+    if (scene >= sl.size()) return MPS_ERR_INVALID;
+    out->fluid_offset = sl[scene].ref0;
+    out->n_fluid = sl[scene].nf;
+    out->boundary_offset = sl[scene].b0;
+    out->n_boundary = sl[scene].nb;
+    return MPS_OK;
+}
+
+int mps_scene_stats(mps_ctx* c, uint32_t scene, mps_step_stats* out) {
+    if (!c || !out) return MPS_ERR_INVALID;
+    if (scene >= c->sstats.size()) {
+        c->err = "no statistics for this scene (no step yet?)";
+        return MPS_ERR_STATE;
+    }
+    *out = c->sstats[scene];
+    return MPS_OK;
+}
+
+int mps_scene_set_material(mps_ctx* c, uint32_t scene, const mps_sph_material* m) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    if (c->scenes.empty()) return scene == 0 ? mps_set_fluid_material(c, m) : MPS_ERR_INVALID;
+    if (scene >= c->scenes.size() || !(m->radius > 0.0) || !(m->density > 0.0)) return MPS_ERR_INVALID;
+    mps_ctx::SceneHost& h = c->scenes[scene];
+    h.mat = *m;
+    h.V = m->radius * m->radius * m->radius / 48.0 * 3.141592;
+    h.mass = m->density * h.V;
+    c->params_dirty = true;
+    return MPS_OK;
 }
 
 int mps_host_alloc(mps_ctx* c, size_t bytes, void** out) {

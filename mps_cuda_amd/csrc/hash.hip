@@ -143,26 +143,34 @@ __global__ void __launch_bounds__(kSortThreads) k_radix_scatter(const uint32_t* 
 }
 
 // ===================================================================== hashing
+// Keys of a set held in id order. With several scenes the scene index sits above the cell
+// bits (scene << shift), so a sort keeps every scene contiguous; `scene` is null for one.
 __global__ void __launch_bounds__(kBlock) k_cell_keys(const float4* __restrict__ pos, uint32_t n, HashGeom g, int morton,
+                                                    const uint32_t* __restrict__ scene, int shift,
                                                     uint32_t* __restrict__ key, uint32_t* __restrict__ id) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = pos[i];
     const i3 c = grid_pos(p.x, p.y, p.z, g);
-    key[i] = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
+    const uint32_t k = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
+    key[i] = scene ? (scene[i] << shift) | k : k;
     id[i] = i;
 }
 
-// Keys of particles held in slot order, written at their reference index: keyref[perm[s]].
+// Keys of particles held in slot order, written at their reference index: keyref[perm[s]]
+// (scene index above the cell bits). Padding slots between scenes (perm = kEmpty) are skipped.
 __global__ void __launch_bounds__(kBlock) k_cell_keys_slots(const float4* __restrict__ pos, const uint32_t* __restrict__ perm,
-                                                          uint32_t n, HashGeom g, int morton, uint32_t* __restrict__ keyref,
-                                                          uint32_t* __restrict__ iota) {
+                                                          uint32_t n, HashGeom g, int morton, const uint32_t* __restrict__ blk_scene,
+                                                          int shift, uint32_t* __restrict__ keyref, uint32_t* __restrict__ iota) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    iota[s] = s;
+    const uint32_t ref = perm[s];
+    if (ref == kEmpty) return;
     const float4 p = pos[s];
     const i3 c = grid_pos(p.x, p.y, p.z, g);
-    keyref[perm[s]] = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
-    iota[s] = s;
+    const uint32_t k = morton ? grid_zindex(c, g) : grid_index(c.x, c.y, c.z, g);
+    keyref[ref] = (blk_scene[blockIdx.x] << shift) | k;
 }
 // Morton code of the (wrapped) cell of each slot, written at the slot's reference index;
 // sorting by it keeps each cell contiguous and blocks of slots spatially compact.
@@ -173,49 +181,65 @@ __device__ __forceinline__ uint32_t cell_morton(i3 c, const HashGeom& g) {
     return split_by3(x) | (split_by3(y) << 1u) | (split_by3(z) << 2u);
 }
 __global__ void __launch_bounds__(kBlock) k_cell_morton_slots(const float4* __restrict__ pos, const uint32_t* __restrict__ perm,
-                                                            uint32_t n, HashGeom g, uint32_t* __restrict__ keyref,
-                                                            uint32_t* __restrict__ iota) {
+                                                            uint32_t n, HashGeom g, const uint32_t* __restrict__ blk_scene,
+                                                            int shift, uint32_t* __restrict__ keyref, uint32_t* __restrict__ iota) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const float4 p = pos[s];
-    keyref[perm[s]] = cell_morton(grid_pos(p.x, p.y, p.z, g), g);
     iota[s] = s;
+    const uint32_t ref = perm[s];
+    if (ref == kEmpty) return;
+    const float4 p = pos[s];
+    keyref[ref] = (blk_scene[blockIdx.x] << shift) | cell_morton(grid_pos(p.x, p.y, p.z, g), g);
 }
-// Cell table over Morton-sorted slots: start/end are indexed by the reference's linear cell
-// key; lkey[s] keeps each slot's linear key (for clearing and for exporting the hash).
+// Sorted position q (key[q], reference id id[q]) -> slot q + delta[scene]: scene s's sorted
+// range [cum_s, cum_s + count_s) lands on its slots [begin_s, end_s). Writes the new perm, the
+// old slot of each particle (src, for the gathers) and the key per slot.
+__global__ void __launch_bounds__(kBlock) k_place(const uint32_t* __restrict__ key, const uint32_t* __restrict__ id, uint32_t nref,
+                                                int shift, const uint32_t* __restrict__ delta, const uint32_t* __restrict__ inv,
+                                                uint32_t* __restrict__ perm, uint32_t* __restrict__ src,
+                                                uint32_t* __restrict__ mkey) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nref) return;
+    const uint32_t k = key[q];
+    const uint32_t slot = q + delta[k >> shift];
+    const uint32_t ref = id[q];
+    perm[slot] = ref;
+    src[slot] = inv[ref];
+    mkey[slot] = k;
+}
+// Cell table over Morton-sorted slots: start/end are indexed by scene * C + the reference's
+// linear cell key; lkey[s] keeps that index (for clearing and for exporting the hash). mkey
+// is kEmpty on padding slots, which therefore end every run.
 __global__ void __launch_bounds__(kBlock) k_cell_table_m(const uint32_t* __restrict__ mkey, const float4* __restrict__ pos,
-                                                       uint32_t n, HashGeom g, uint32_t* __restrict__ start,
-                                                       uint32_t* __restrict__ end, uint32_t* __restrict__ lkey) {
+                                                       uint32_t n, HashGeom g, int shift, uint32_t C,
+                                                       uint32_t* __restrict__ start, uint32_t* __restrict__ end,
+                                                       uint32_t* __restrict__ lkey) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    const uint32_t m = mkey[s];
+    if (m == kEmpty) return;
     const float4 p = pos[s];
     const i3 c = grid_pos(p.x, p.y, p.z, g);
-    const uint32_t k = grid_index(c.x, c.y, c.z, g);
-    lkey[s] = k;
-    const uint32_t m = mkey[s];
-    if (s == 0 || mkey[s - 1] != m) {
-        start[k] = s;
-        if (s > 0) {
-            const float4 q = pos[s - 1];
-            const i3 cq = grid_pos(q.x, q.y, q.z, g);
-            end[grid_index(cq.x, cq.y, cq.z, g)] = s;
-        }
-    }
-    if (s == n - 1) end[k] = n;
+    const uint32_t t = (m >> shift) * C + grid_index(c.x, c.y, c.z, g);
+    lkey[s] = t;
+    if (s == 0 || mkey[s - 1] != m) start[t] = s;
+    if (s == n - 1 || mkey[s + 1] != m) end[t] = s + 1;
 }
 __global__ void __launch_bounds__(kBlock) k_clear_cells_m(const uint32_t* __restrict__ mkey, const uint32_t* __restrict__ lkey,
                                                         uint32_t n, uint32_t* __restrict__ start, uint32_t* __restrict__ end) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    if (s == 0 || mkey[s - 1] != mkey[s]) {
+    const uint32_t m = mkey[s];
+    if (m == kEmpty) return;
+    if (s == 0 || mkey[s - 1] != m) {
         start[lkey[s]] = kEmpty;
         end[lkey[s]] = kEmpty;
     }
 }
-// inv[perm[s]] = s
+// inv[perm[s]] = s (padding slots skipped)
 __global__ void __launch_bounds__(kBlock) k_invert(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ inv) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) inv[perm[s]] = s;
+    if (s < n && perm[s] != kEmpty) inv[perm[s]] = s;
 }
 // src[s'] = inv[perm_new[s']]: old slot of the particle that moves to slot s'
 __global__ void __launch_bounds__(kBlock) k_compose(const uint32_t* __restrict__ perm_new, const uint32_t* __restrict__ inv,
@@ -226,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_compose(const uint32_t* __restrict__
 // perm[s] = map[perm[s]]  (relabel reference ids after a Z-sort)
 __global__ void __launch_bounds__(kBlock) k_relabel(uint32_t* __restrict__ perm, const uint32_t* __restrict__ map, uint32_t n) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) perm[s] = map[perm[s]];
+    if (s < n && perm[s] != kEmpty) perm[s] = map[perm[s]];
 }
 __global__ void __launch_bounds__(kBlock) k_iota(uint32_t* __restrict__ p, uint32_t n) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,16 +361,22 @@ __device__ __forceinline__ void report_counts(uint32_t c, uint32_t* maxc, unsign
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __restrict__ pos, uint32_t N, HashGeom g,
+// Scene s's cells are entries [s*C, (s+1)*C) of both tables.
+__global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __restrict__ pos, Scenes sc, uint32_t C, HashGeom g,
                                                           HashView fl, HashView bd, uint32_t* __restrict__ nff, uint32_t Kff,
                                                           uint32_t* __restrict__ cff, uint32_t* __restrict__ nfb, uint32_t Kfb,
                                                           uint32_t* __restrict__ cfb, NbrCounters* __restrict__ ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = sc.N;
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
     uint32_t a = 0u, b = 0u;
     if (i < N) {
-        const float4 xi = pos[i];
-        a = search_cells<true, true>(xi, i, g, fl.start, fl.end, fl.id, fl.spos, nff, Kff, N);
-        if (bd.n) b = search_cells<false, true>(xi, i, g, bd.start, bd.end, bd.id, bd.spos, nfb, Kfb, N);
+        if (i < sc.P[sid].end) {
+            const float4 xi = pos[i];
+            const size_t base = size_t(sid) * C;
+            a = search_cells<true, true>(xi, i, g, fl.start + base, fl.end + base, fl.id, fl.spos, nff, Kff, N);
+            if (bd.n) b = search_cells<false, true>(xi, i, g, bd.start + base, bd.end + base, bd.id, bd.spos, nfb, Kfb, N);
+        }
         cff[i] = a;
         cfb[i] = b;
     }
@@ -354,13 +384,15 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     report_counts(b, &ctr->max_fb, &ctr->nnz_fb);
 }
 
-__global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __restrict__ pos, uint32_t B, HashGeom g, HashView bd,
-                                                             uint32_t* __restrict__ nbb, uint32_t Kbb, uint32_t* __restrict__ cbb,
-                                                             NbrCounters* __restrict__ ctr) {
+__global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __restrict__ pos, uint32_t B,
+                                                             const uint32_t* __restrict__ scene, uint32_t C, HashGeom g,
+                                                             HashView bd, uint32_t* __restrict__ nbb, uint32_t Kbb,
+                                                             uint32_t* __restrict__ cbb, NbrCounters* __restrict__ ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t a = 0u;
     if (i < B) {
-        a = search_cells<true, false>(pos[i], i, g, bd.start, bd.end, bd.id, bd.spos, nbb, Kbb, B);
+        const size_t base = scene ? size_t(scene[i]) * C : 0;
+        a = search_cells<true, false>(pos[i], i, g, bd.start + base, bd.end + base, bd.id, bd.spos, nbb, Kbb, B);
         cbb[i] = a;
     }
     report_counts(a, &ctr->max_bb, &ctr->nnz_bb);
@@ -370,22 +402,22 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __r
 __global__ void __launch_bounds__(kBlock) k_gather_f4(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ in,
                                                     float4* __restrict__ out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) out[s] = in[perm[s]];
+    if (s < n && perm[s] != kEmpty) out[s] = in[perm[s]];
 }
 __global__ void __launch_bounds__(kBlock) k_gather_u32(const uint32_t* __restrict__ perm, uint32_t n, const uint32_t* __restrict__ in,
                                                      uint32_t* __restrict__ out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) out[s] = in[perm[s]];
+    if (s < n && perm[s] != kEmpty) out[s] = in[perm[s]];
 }
 __global__ void __launch_bounds__(kBlock) k_gather_f2(const uint32_t* __restrict__ perm, uint32_t n, const float2* __restrict__ in,
                                                     float2* __restrict__ out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) out[s] = in[perm[s]];
+    if (s < n && perm[s] != kEmpty) out[s] = in[perm[s]];
 }
 __global__ void __launch_bounds__(kBlock) k_gather_f1(const uint32_t* __restrict__ perm, uint32_t n, const float* __restrict__ in,
                                                     float* __restrict__ out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) out[s] = in[perm[s]];
+    if (s < n && perm[s] != kEmpty) out[s] = in[perm[s]];
 }
 
 }  // namespace dev
@@ -420,20 +452,25 @@ int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*&
     return 0;
 }
 
-void cell_keys(const float4* pos, uint32_t n, const HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s) {
-    if (n) k_cell_keys<<<nb(n), kBlock, 0, s>>>(pos, n, g, morton, key, id);
+void cell_keys(const float4* pos, uint32_t n, const HashGeom& g, int morton, const uint32_t* scene, int shift, uint32_t* key,
+               uint32_t* id, hipStream_t s) {
+    if (n) k_cell_keys<<<nb(n), kBlock, 0, s>>>(pos, n, g, morton, scene, shift, key, id);
 }
-void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, int morton, uint32_t* keyref,
-                     uint32_t* iota, hipStream_t s) {
-    if (n) k_cell_keys_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, morton, keyref, iota);
+void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, int morton,
+                     const uint32_t* blk_scene, int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s) {
+    if (n) k_cell_keys_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, morton, blk_scene, shift, keyref, iota);
 }
-void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, uint32_t* keyref,
-                       uint32_t* iota, hipStream_t s) {
-    if (n) k_cell_morton_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, keyref, iota);
+void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, const uint32_t* blk_scene,
+                       int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s) {
+    if (n) k_cell_morton_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, blk_scene, shift, keyref, iota);
 }
-void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const HashGeom& g, uint32_t* start, uint32_t* end,
-                  uint32_t* lkey, hipStream_t s) {
-    if (n) k_cell_table_m<<<nb(n), kBlock, 0, s>>>(mkey, pos, n, g, start, end, lkey);
+void place(const uint32_t* key, const uint32_t* id, uint32_t nref, int shift, const uint32_t* delta, const uint32_t* inv,
+           uint32_t* perm, uint32_t* src, uint32_t* mkey, hipStream_t s) {
+    if (nref) k_place<<<nb(nref), kBlock, 0, s>>>(key, id, nref, shift, delta, inv, perm, src, mkey);
+}
+void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const HashGeom& g, int shift, uint32_t C, uint32_t* start,
+                  uint32_t* end, uint32_t* lkey, hipStream_t s) {
+    if (n) k_cell_table_m<<<nb(n), kBlock, 0, s>>>(mkey, pos, n, g, shift, C, start, end, lkey);
 }
 void clear_cells_m(const uint32_t* mkey, const uint32_t* lkey, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s) {
     if (n) k_clear_cells_m<<<nb(n), kBlock, 0, s>>>(mkey, lkey, n, start, end);
@@ -460,13 +497,14 @@ void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* s
                 float4* spos, hipStream_t s) {
     if (n) k_cell_table<<<nb(n), kBlock, 0, s>>>(key, id, n, start, end, pos, spos);
 }
-void neighbors_fluid(const float4* pos, uint32_t N, const HashGeom& g, const HashView& fl, const HashView& bd, uint32_t* nff,
-                     uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr, hipStream_t s) {
-    if (N) k_neighbors_fluid<<<nb(N), kBlock, 0, s>>>(pos, N, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, ctr);
+void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const HashGeom& g, const HashView& fl, const HashView& bd,
+                     uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr,
+                     hipStream_t s) {
+    if (sc.N) k_neighbors_fluid<<<nb(sc.N), kBlock, 0, s>>>(pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, ctr);
 }
-void neighbors_boundary(const float4* pos, uint32_t B, const HashGeom& g, const HashView& bd, uint32_t* nbb, uint32_t Kbb,
-                        uint32_t* cbb, NbrCounters* ctr, hipStream_t s) {
-    if (B) k_neighbors_boundary<<<nb(B), kBlock, 0, s>>>(pos, B, g, bd, nbb, Kbb, cbb, ctr);
+void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const HashGeom& g, const HashView& bd,
+                        uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s) {
+    if (B) k_neighbors_boundary<<<nb(B), kBlock, 0, s>>>(pos, B, scene, C, g, bd, nbb, Kbb, cbb, ctr);
 }
 void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s) {
     if (n) k_gather_f4<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);

@@ -42,14 +42,28 @@ struct SolverDev {
     double log[MPS_MAX_ITER_LOG];
 };
 
-// Everything a particle kernel needs by value.
+// Parameters of one scene. A context holds S >= 1 scenes stored back to back: scene s owns
+// the fluid slots [begin, end), and `begin` is a multiple of the workgroup size, so every
+// workgroup of a fluid kernel belongs to exactly one scene (its partial sums, and their fold,
+// are then the same as when the scene runs alone).
 struct StepParams {
-    uint32_t N, B;
+    uint32_t N;           // fluid slots of the context (the column stride of the ELL lists)
+    uint32_t begin, end;  // this scene's slots
+    uint32_t count;       // its particles (end - begin)
+    uint32_t blk0, nblk;  // its workgroups: [begin / kBlock, blk0 + ceil(count / kBlock))
     float V, rho0, mass, mu, xsph, sigma;  // SPH material (MPSSPHMaterial.h:9-21)
     float dt, gx, gy, gz;
     float bmin[3], bmax[3];
     float cg_factor;
     int faithful;
+};
+
+// Scene lookup handed to the fluid kernels by value.
+struct Scenes {
+    const StepParams* P;        // [S], device
+    const uint32_t* blk_scene;  // [ceil(N / kBlock)]: scene of each workgroup of slots
+    uint32_t N;                 // fluid slots of the context
+    uint32_t S;                 // scenes
 };
 
 // Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i].
@@ -65,13 +79,18 @@ namespace launch {
 size_t radix_temp_words(uint32_t n);
 int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,
                int key_bits, hipStream_t s);
-void cell_keys(const float4* pos, uint32_t n, const dev::HashGeom& g, int morton, uint32_t* key, uint32_t* id, hipStream_t s);
-void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, int morton, uint32_t* keyref,
-                     uint32_t* iota, hipStream_t s);
-void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, uint32_t* keyref,
-                       uint32_t* iota, hipStream_t s);
-void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const dev::HashGeom& g, uint32_t* start, uint32_t* end,
-                  uint32_t* lkey, hipStream_t s);
+// Keys carry the scene index above `shift` cell bits (scene / blk_scene may be null for one scene).
+void cell_keys(const float4* pos, uint32_t n, const dev::HashGeom& g, int morton, const uint32_t* scene, int shift,
+               uint32_t* key, uint32_t* id, hipStream_t s);
+void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, int morton,
+                     const uint32_t* blk_scene, int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s);
+void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const dev::HashGeom& g, const uint32_t* blk_scene,
+                       int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s);
+// sorted position q -> slot q + delta[key[q] >> shift]: new perm, src (old slot) and key per slot
+void place(const uint32_t* key, const uint32_t* id, uint32_t nref, int shift, const uint32_t* delta, const uint32_t* inv,
+           uint32_t* perm, uint32_t* src, uint32_t* mkey, hipStream_t s);
+void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const dev::HashGeom& g, int shift, uint32_t C,
+                  uint32_t* start, uint32_t* end, uint32_t* lkey, hipStream_t s);
 void clear_cells_m(const uint32_t* mkey, const uint32_t* lkey, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
 void invert(const uint32_t* perm, uint32_t n, uint32_t* inv, hipStream_t s);
 void compose(const uint32_t* perm_new, const uint32_t* inv, uint32_t n, uint32_t* src, hipStream_t s);
@@ -81,11 +100,11 @@ void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
 void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
                 float4* spos, hipStream_t s);
-void neighbors_fluid(const float4* pos, uint32_t N, const dev::HashGeom& g, const HashView& fl, const HashView& bd,
-                     uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr,
-                     hipStream_t s);
-void neighbors_boundary(const float4* pos, uint32_t B, const dev::HashGeom& g, const HashView& bd, uint32_t* nbb,
-                        uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s);
+void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const dev::HashGeom& g, const HashView& fl,
+                     const HashView& bd, uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb,
+                     NbrCounters* ctr, hipStream_t s);
+void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const dev::HashGeom& g,
+                        const HashView& bd, uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s);
 void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s);
 void gather_u32(const uint32_t* perm, uint32_t n, const uint32_t* in, uint32_t* out, hipStream_t s);
 void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, hipStream_t s);
@@ -97,42 +116,46 @@ void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb
                   hipStream_t s);
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
 // density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
-void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
+void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s);
-void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
-void fold_pressure(const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, uint32_t lmin, hipStream_t s);
-void pressure_apply(const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+// one workgroup per scene; `active` counts the scenes whose loop has not finished
+void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
+                   uint32_t lmin, hipStream_t s);
+void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s);
-void st_factor(const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
+void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s);
-void cg_setup(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s);
 // CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
 // mode 0: A x0 with f = 0 (x0 in `in`), then r = b - A x0, p = r (into `out`), partial r.r;
 // mode 1: A p with f (p in `in`), partial p.Ap
-void cg_apply(int mode, const StepParams& P, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
+void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
               const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
 // mode 0: initial r.r + test; 1: alpha; 2: new r.r, beta + test
-void fold_cg(int mode, const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, hipStream_t s);
-void cg_update(uint32_t N, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st, double* partials,
-               hipStream_t s);
-void cg_direction(uint32_t N, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);
-void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s);
-void jacobi_candidate(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
+             hipStream_t s);
+void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
+               double* partials, hipStream_t s);
+void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);
+void copy_vel_out(const Scenes& sc, const float4* x, float4* vel, hipStream_t s);
+void jacobi_candidate(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                       const float* rho, const float4* u, float4* out, hipStream_t s);
 void jacobi_relax(uint32_t N, const float4* cand, const float4* cur, const float4* prev, float ur, const float* w3,
                   float4* o0, float4* o1, float4* o2, hipStream_t s);
 // sums |u - a sum(...) - v0| for up to three candidates into err3 (double, deterministic)
-void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+void jacobi_error(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                   const float* rho, const float4* u0, const float4* u1, const float4* u2, int ncand, double* partials,
                   double* err3, hipStream_t s);
-void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s);
-void solver_reset(SolverDev* st, hipStream_t s);
+void advect(const Scenes& sc, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s);
+// resets the S solver records st[0..S) and sets *active = S
+void solver_reset(SolverDev* st, uint32_t S, uint32_t* active, hipStream_t s);
 void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,

@@ -88,14 +88,16 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams P, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
                                                         float* __restrict__ alpha, float4* __restrict__ color,
                                                         float* __restrict__ pG_ff, float* __restrict__ pG_fb) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= N) return;
+    if (i >= P.end) return;
     const bool dens = flags & launch::kDoDensity, alph = flags & launch::kDoAlpha;
     const bool pairs = flags & launch::kDoPairs;
     const float4 xi4 = pos[i];
@@ -187,18 +189,20 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, StepParams 
 // ComputeDFStiffness_kernel / ComputeCDStiffness_kernel (:259-340); G_ij comes from the
 // alpha pass. Each workgroup publishes its sum of positive errors for k_fold_pressure.
 template <bool DF>
-__global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                         const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, const float4* __restrict__ bvel,
                                                         const float* __restrict__ rho, const float* __restrict__ alpha,
                                                         const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
                                                         float* __restrict__ kappa, float* __restrict__ delta,
                                                         const SolverDev* st, double* __restrict__ partials) {
-    if (st->done) return;
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
+    if (st[sid].done) return;  // this scene's loop has finished
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     double epos = 0.0;
-    if (i < N) {
+    if (i < P.end) {
         const f3 xi = mk(pos[i]), vi = mk(vel[i]);
         float df = 0.0f;
         const uint32_t c = nb.cff[i];
@@ -261,32 +265,40 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(StepParams P, Nbr nb, 
 }
 
 // Mean positive error and the loop decision of the DF/CD drivers
-// (MPSSPHDFSPHUtil.cu:331-333,417-419): break when E < tol and l > lmin.
-__global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(const double* __restrict__ partials, uint32_t nblk, SolverDev* st,
-                                                        uint32_t it, double tol, uint32_t lmin, uint32_t N) {
-    if (st->done) return;
-    const double s = fold(partials, nblk);
+// (MPSSPHDFSPHUtil.cu:331-333,417-419): break when E < tol and l > lmin. Workgroup s folds
+// scene s's partials; a scene that finishes leaves the count of active scenes.
+__global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(Scenes sc, const double* __restrict__ partials,
+                                                        SolverDev* st, uint32_t* active, uint32_t it, double tol,
+                                                        uint32_t lmin) {
+    const uint32_t sid = blockIdx.x;
+    SolverDev* sd = st + sid;
+    if (sd->done) return;
+    const StepParams P = sc.P[sid];
+    const double s = fold(partials + P.blk0, P.nblk);
     if (threadIdx.x == 0) {
-        const double E = s / double(N);
-        if (it - 1 < MPS_MAX_ITER_LOG) st->log[it - 1] = E;
+        const double E = s / double(P.count);
+        if (it - 1 < MPS_MAX_ITER_LOG) sd->log[it - 1] = E;
         if (E < tol && it > lmin) {
-            st->done = 1u;
-            st->l = it;
+            sd->done = 1u;
+            sd->l = it;
+            atomicSub(active, 1u);
         } else {
-            st->l = it + 1;
+            sd->l = it + 1;
         }
     }
 }
 
 // ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
-__global__ void __launch_bounds__(kBlock) k_pressure_apply(StepParams P, Nbr nb, const float4* __restrict__ pos, float4* __restrict__ vel,
+__global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, const float4* __restrict__ pos, float4* __restrict__ vel,
                                                          const float4* __restrict__ bpos, const float* __restrict__ bvol,
                                                          const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
                                                          const float* __restrict__ kappa, const SolverDev* st) {
-    if (st->done) return;
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
+    if (st[sid].done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= N) return;
+    if (i >= P.end) return;
     const f3 xi = mk(pos[i]);
     const float pi = kappa[i];
     f3 ff{0.f, 0.f, 0.f};
@@ -345,12 +357,14 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(StepParams P, Nbr nb,
 // ----------------------------------------------------------------- viscosity + surface tension CG
 // ComputeSurfaceTensionFactor_kernel (MPSSPHNonPressureUtil.cuh:425-465) with the stored
 // G_ij; also x0 = v (the copy into pPredictVel, MPSSPHNonPressureUtil.cu:144-147).
-__global__ void __launch_bounds__(kBlock) k_st_factor(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
+__global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                     const float* __restrict__ pG_ff, float* __restrict__ fst,
                                                     float4* __restrict__ cgx) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= N) return;
+    if (i >= P.end) return;
     const float4 vi4 = vel[i];
     const f3 xi = mk(pos[i]), vi = mk(vi4);
     float d = 0.0f;
@@ -403,13 +417,15 @@ __device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float
 //   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON),
 //   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_cg_setup(StepParams P, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
+__global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                    const float* __restrict__ rho, const float* __restrict__ fst,
                                                    const float* __restrict__ pG_ff, float2* __restrict__ cab,
                                                    float4* __restrict__ cgb) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= N) return;
+    if (i >= P.end) return;
     const float4 xi4 = pos[i];
     const float4 vi4 = vel[i];
     const f3 xi = mk(xi4), vi = mk(vi4);
@@ -483,17 +499,19 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 //         r = p = b - A x0 (p written to `out`), partial sums of r.r.
 // MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
 template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const float2* __restrict__ cab,
+__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const float2* __restrict__ cab,
                                                    const float4* __restrict__ in, float4* __restrict__ out,
                                                    float4* __restrict__ Ap, const float4* __restrict__ b,
                                                    float4* __restrict__ r, const SolverDev* st,
                                                    double* __restrict__ partials) {
-    if (MODE == 1 && st->done) return;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
+    if (MODE == 1 && st[sid].done) return;
+    const StepParams P = sc.P[sid];
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     double part = 0.0;
-    if (i < N) {
+    if (i < P.end) {
         const float4 ra = in[2 * size_t(i)], rb = in[2 * size_t(i) + 1];
         const f3 xi = mk(ra);
         const f3 pi = rec_p(ra, rb);
@@ -554,17 +572,24 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(StepParams P, Nbr nb, const
 
 // Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
 // first convergence test; 1: alpha = r.r / p.Ap; 2: r.r of the new residual, beta, test.
+// Workgroup s folds scene s's partials.
 template <int MODE>
-__global__ void __launch_bounds__(kFoldThreads) k_fold_cg(const double* __restrict__ partials, uint32_t nblk, SolverDev* st, uint32_t it,
-                                                  double tol) {
+__global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const double* __restrict__ partials, SolverDev* stv,
+                                                  uint32_t* active, uint32_t it, double tol) {
+    const uint32_t sid = blockIdx.x;
+    SolverDev* st = stv + sid;
     if (MODE != 0 && st->done) return;
-    const double s = fold(partials, nblk);
+    const StepParams P = sc.P[sid];
+    const double s = fold(partials + P.blk0, P.nblk);
     if (threadIdx.x != 0) return;
     if (MODE == 0) {
         st->rr = s;
         st->log[0] = s;
         st->l = 1u;
-        if (s < tol) st->done = 1u;
+        if (s < tol) {
+            st->done = 1u;
+            atomicSub(active, 1u);
+        }
     } else if (MODE == 1) {
         st->pAp = s;
         st->alpha = float(st->rr / s);
@@ -573,19 +598,24 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(const double* __restri
         st->rr = s;
         if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
         st->l = it + 1;
-        if (s < tol) st->done = 1u;
+        if (s < tol) {
+            st->done = 1u;
+            atomicSub(active, 1u);
+        }
     }
 }
 
 // UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121)
-__global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t N, float4* __restrict__ x, float4* __restrict__ r,
+__global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restrict__ x, float4* __restrict__ r,
                                                     const float4* __restrict__ prec, const float4* __restrict__ Ap,
                                                     const SolverDev* st, double* __restrict__ partials) {
-    if (st->done) return;
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    if (st[sid].done) return;
+    const uint32_t end = sc.P[sid].end;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float a = st->alpha;
+    const float a = st[sid].alpha;
     double part = 0.0;
-    if (i < N) {
+    if (i < end) {
         const f3 pi = rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]);
         const float4 api = Ap[i];
         float4 xi = x[i], ri = r[i];
@@ -603,12 +633,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(uint32_t N, float4* __rest
 }
 
 // UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133), p kept in the position records
-__global__ void __launch_bounds__(kBlock) k_cg_direction(uint32_t N, const float4* __restrict__ r, float4* __restrict__ prec,
+__global__ void __launch_bounds__(kBlock) k_cg_direction(Scenes sc, const float4* __restrict__ r, float4* __restrict__ prec,
                                                        const SolverDev* st) {
-    if (st->done) return;
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    if (st[sid].done) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const float bta = st->beta;
+    if (i >= sc.P[sid].end) return;
+    const float bta = st[sid].beta;
     const float4 ri = r[i];
     float4 a = prec[2 * size_t(i)], b = prec[2 * size_t(i) + 1];
     a.w = ri.x + bta * a.w;
@@ -618,9 +649,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(uint32_t N, const float
     prec[2 * size_t(i) + 1] = b;
 }
 
-__global__ void __launch_bounds__(kBlock) k_copy_vel_out(uint32_t N, const float4* __restrict__ x, float4* __restrict__ vel) {
+__global__ void __launch_bounds__(kBlock) k_copy_vel_out(Scenes sc, const float4* __restrict__ x, float4* __restrict__ vel) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    if (i >= sc.P[sc.blk_scene[blockIdx.x]].end) return;
     const float4 xi = x[i];
     float4 v = vel[i];
     v.x = xi.x;
@@ -633,12 +664,14 @@ __global__ void __launch_bounds__(kBlock) k_copy_vel_out(uint32_t N, const float
 // ComputeJacobiViscosity kernels (MPSSPHNonPressureUtil.cuh:90-194, commented out in the
 // reference): per particle a 3x3 block with A_ij = V G/((d^2+0.01h^2)(d+eps)) x_ij x_ij^T.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_jacobi_candidate(StepParams P, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_jacobi_candidate(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                            const float4* __restrict__ vel, const float* __restrict__ rho,
                                                            const float4* __restrict__ u, float4* __restrict__ out) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= N) return;
+    if (i >= P.end) return;
     const float4 xi4 = pos[i];
     const f3 xi = mk(xi4);
     const float h = xi4.w * 4.0f;
@@ -712,17 +745,19 @@ __global__ void __launch_bounds__(kBlock) k_jacobi_relax(uint32_t N, const float
 // ComputeJacobiError_0/1_kernel (MPSSPHNonPressureUtil.cuh:195-333): per-block partial sums
 // of |u_i - a_i sum_j (x_ij.(u_i-u_j)) grad_ij - v0_i| for up to three candidates.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_jacobi_error(StepParams P, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_jacobi_error(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                        const float4* __restrict__ vel, const float* __restrict__ rho,
                                                        const float4* __restrict__ u0, const float4* __restrict__ u1,
                                                        const float4* __restrict__ u2, int ncand, double* __restrict__ partials) {
     __shared__ double lds[kBlock / 64];
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     const float4* U[3] = {u0, u1, u2};
     for (int q = 0; q < ncand; q++) {
         double e = 0.0;
-        if (i < N) {
+        if (i < P.end) {
             const float4 xi4 = pos[i];
             const f3 xi = mk(xi4);
             const float h = xi4.w * 4.0f;
@@ -760,10 +795,12 @@ __global__ void __launch_bounds__(kBlock) k_fold_partials(const double* __restri
 
 // ----------------------------------------------------------------- advection
 // BoundaryCollision_kernel + UpdatePosition_kernel (MPSAdvectUtil.cuh:36-77)
-__global__ void __launch_bounds__(kBlock) k_advect(StepParams P, float4* __restrict__ pos, float4* __restrict__ vel,
+__global__ void __launch_bounds__(kBlock) k_advect(Scenes sc, float4* __restrict__ pos, float4* __restrict__ vel,
                                                  uint32_t* __restrict__ nonfinite) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.N) return;
+    if (i >= P.end) return;
     float4 x4 = pos[i];
     float4 v4 = vel[i];
     float v[3] = {v4.x, v4.y, v4.z};
@@ -787,7 +824,9 @@ __global__ void __launch_bounds__(kBlock) k_advect(StepParams P, float4* __restr
     if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(nonfinite, 1u);
 }
 
-__global__ void k_solver_reset(SolverDev* st) {
+// One workgroup per scene: st[blockIdx.x] starts a new loop; *active = S.
+__global__ void k_solver_reset(SolverDev* stv, uint32_t S, uint32_t* active) {
+    SolverDev* st = stv + blockIdx.x;
     const uint32_t t = threadIdx.x;
     if (t == 0) {
         st->done = 0u;
@@ -797,6 +836,7 @@ __global__ void k_solver_reset(SolverDev* st) {
         st->pAp = 0.0;
         st->alpha = 0.0f;
         st->beta = 0.0f;
+        if (blockIdx.x == 0) *active = S;
     }
     for (uint32_t k = t; k < MPS_MAX_ITER_LOG; k += blockDim.x) st->log[k] = 0.0;
 }
@@ -816,6 +856,7 @@ __global__ void __launch_bounds__(kBlock) k_pack_readback(uint32_t n, const floa
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm[s];  // reference index of slot s
+    if (i == kEmpty) return;     // padding between scenes
     const float4 p = pos[s];
     if (xyz) {
         xyz[3 * size_t(i)] = p.x;
@@ -842,88 +883,92 @@ void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
     if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
 }
-void density_alpha(int cubic, int flags, const StepParams& P, const Nbr& nb, const float4* pos, float4* vel,
+void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s) {
-    if (!P.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
-    else k_density_alpha<false><<<nb_(P.N), kBlock, 0, s>>>(flags, P, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    if (!sc.N) return;
+    if (cubic) k_density_alpha<true><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    else k_density_alpha<false><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
-void pressure_eval(int df, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s) {
-    if (!P.N) return;
-    const unsigned g = nb_(P.N);
-    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
-    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    if (!sc.N) return;
+    const unsigned g = nb_(sc.N);
+    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
-void fold_pressure(const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, uint32_t lmin, hipStream_t s) {
-    if (N) k_fold_pressure<<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol, lmin, N);
+void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
+                   uint32_t lmin, hipStream_t s) {
+    if (sc.N) k_fold_pressure<<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol, lmin);
 }
-void pressure_apply(const StepParams& P, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s) {
-    if (P.N) k_pressure_apply<<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
+    if (sc.N) k_pressure_apply<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
-void st_factor(const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
+void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {
-    if (P.N) k_st_factor<<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, pG_ff, fst, cgx);
+    if (sc.N) k_st_factor<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, pG_ff, fst, cgx);
 }
-void cg_setup(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s) {
-    if (!P.N) return;
-    if (cubic) k_cg_setup<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
-    else k_cg_setup<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
+    if (!sc.N) return;
+    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
+    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
 }
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
     if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
 }
-void cg_apply(int mode, const StepParams& P, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
+void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
               const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
-    if (!P.N) return;
-    const unsigned g = nb_(P.N);
-    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(P, nb, cab, in, out, Ap, b, r, st, partials);
-    else k_cg_apply<1><<<g, kBlock, 0, s>>>(P, nb, cab, in, out, Ap, b, r, st, partials);
+    if (!sc.N) return;
+    const unsigned g = nb_(sc.N);
+    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, cab, in, out, Ap, b, r, st, partials);
+    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, cab, in, out, Ap, b, r, st, partials);
 }
-void fold_cg(int mode, const double* partials, uint32_t N, SolverDev* st, uint32_t it, double tol, hipStream_t s) {
-    if (!N) return;
-    if (mode == 0) k_fold_cg<0><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
-    else if (mode == 1) k_fold_cg<1><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
-    else k_fold_cg<2><<<1, kFoldThreads, 0, s>>>(partials, nb_(N), st, it, tol);
+void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
+             hipStream_t s) {
+    if (!sc.N) return;
+    if (mode == 0) k_fold_cg<0><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
+    else if (mode == 1) k_fold_cg<1><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
+    else k_fold_cg<2><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
 }
-void cg_update(uint32_t N, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st, double* partials,
-               hipStream_t s) {
-    if (N) k_cg_update<<<nb_(N), kBlock, 0, s>>>(N, x, r, prec, Ap, st, partials);
+void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
+               double* partials, hipStream_t s) {
+    if (sc.N) k_cg_update<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
 }
-void cg_direction(uint32_t N, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
-    if (N) k_cg_direction<<<nb_(N), kBlock, 0, s>>>(N, r, prec, st);
+void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
+    if (sc.N) k_cg_direction<<<nb_(sc.N), kBlock, 0, s>>>(sc, r, prec, st);
 }
-void copy_vel_out(uint32_t N, const float4* x, float4* vel, hipStream_t s) {
-    if (N) k_copy_vel_out<<<nb_(N), kBlock, 0, s>>>(N, x, vel);
+void copy_vel_out(const Scenes& sc, const float4* x, float4* vel, hipStream_t s) {
+    if (sc.N) k_copy_vel_out<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, vel);
 }
-void jacobi_candidate(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+void jacobi_candidate(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                       const float* rho, const float4* u, float4* out, hipStream_t s) {
-    if (!P.N) return;
-    if (cubic) k_jacobi_candidate<true><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, u, out);
-    else k_jacobi_candidate<false><<<nb_(P.N), kBlock, 0, s>>>(P, nb, pos, vel, rho, u, out);
+    if (!sc.N) return;
+    if (cubic) k_jacobi_candidate<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, u, out);
+    else k_jacobi_candidate<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, u, out);
 }
 void jacobi_relax(uint32_t N, const float4* cand, const float4* cur, const float4* prev, float ur, const float* w3,
                   float4* o0, float4* o1, float4* o2, hipStream_t s) {
     if (N) k_jacobi_relax<<<nb_(N), kBlock, 0, s>>>(N, cand, cur, prev, ur, w3[0], w3[1], w3[2], o0, o1, o2);
 }
-void jacobi_error(int cubic, const StepParams& P, const Nbr& nb, const float4* pos, const float4* vel,
+void jacobi_error(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                   const float* rho, const float4* u0, const float4* u1, const float4* u2, int ncand, double* partials,
                   double* err3, hipStream_t s) {
-    if (!P.N) return;
-    const unsigned g = nb_(P.N);
-    if (cubic) k_jacobi_error<true><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
-    else k_jacobi_error<false><<<g, kBlock, 0, s>>>(P, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    if (!sc.N) return;
+    const unsigned g = nb_(sc.N);
+    if (cubic) k_jacobi_error<true><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    else k_jacobi_error<false><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
     k_fold_partials<<<1, kBlock, 0, s>>>(partials, g, ncand, err3);
 }
-void advect(const StepParams& P, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s) {
-    if (P.N) k_advect<<<nb_(P.N), kBlock, 0, s>>>(P, pos, vel, nonfinite);
+void advect(const Scenes& sc, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s) {
+    if (sc.N) k_advect<<<nb_(sc.N), kBlock, 0, s>>>(sc, pos, vel, nonfinite);
 }
-void solver_reset(SolverDev* st, hipStream_t s) { k_solver_reset<<<1, kBlock, 0, s>>>(st); }
+void solver_reset(SolverDev* st, uint32_t S, uint32_t* active, hipStream_t s) {
+    if (S) k_solver_reset<<<S, kBlock, 0, s>>>(st, S, active);
+}
 void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s) {
     if (n) k_calib_copy<<<4096, kBlock, 0, s>>>(a, b, n);
 }

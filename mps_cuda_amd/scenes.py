@@ -196,3 +196,82 @@ def droplet(n: int = 65536, radius: float = 0.25, seed: int = 0, steps: int = 10
         boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=[0.5, 0.5, 0.5], size=[1.0, 1.0, 1.0], radius=h, density=1.0),
         steps=steps,
     )
+
+
+# ---------------------------------------------------------------------------- batched scenes
+def batch_scenes(n_scenes: int = 64, seed: int = 2, n_min: int = 8192, n_max: int = 262144, spacing: float = 0.02,
+                 dt: float = 1.0e-3, steps: int = 300) -> list[Scene]:
+    """Independent scenes for the batched step: sizes log-uniform in [n_min, n_max] (seeded);
+    each a jittered fluid block in the corner or a sphere falling from near the top of its
+    own box (coin flip); viscosity ~ U(0.005, 0.05), surface tension ~ U(0.01, 0.5). The box
+    edge is 1 m, grown (and recorded in `extra`) when the fluid would not fit. All scenes
+    share one configuration (cubic kernel, h = 2*spacing, hash 64^3, DF tol 1e-2, CD tol 1e-3),
+    as the scenes of one context must."""
+    rng = np.random.default_rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(np.log(n_min), np.log(n_max), n_scenes))).astype(np.int64)
+    kinds = rng.integers(0, 2, n_scenes)
+    mus = rng.uniform(0.005, 0.05, n_scenes)
+    sigmas = rng.uniform(0.01, 0.5, n_scenes)
+    h = 2.0 * spacing
+    out = []
+    for s in range(n_scenes):
+        n = int(sizes[s])
+        if kinds[s] == 0:  # block in the lower corner, filled layer by layer
+            a = int(np.ceil(n ** (1.0 / 3.0)))
+            k, j, i = np.meshgrid(np.arange(a), np.arange(a), np.arange(a), indexing="ij")
+            lat = np.stack([i.ravel(), k.ravel(), j.ravel()], axis=1)[:n].astype(np.float64)  # y (height) last
+            base = spacing * (lat + 1.0)
+            L = max(1.0, spacing * (a + 2))
+            kind = "block"
+        else:  # sphere, nearest lattice points to its centre
+            R = spacing * (3.0 * n / (4.0 * np.pi)) ** (1.0 / 3.0)
+            m = int(np.ceil(2 * R / spacing)) + 4
+            g = (np.arange(m) - (m - 1) / 2.0) * spacing
+            X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+            pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+            pts = pts[np.argsort(np.linalg.norm(pts, axis=1), kind="stable")[:n]]
+            L = max(1.0, 2 * R + 8 * spacing)
+            base = pts + np.array([L / 2, L - R - 3 * spacing, L / 2])
+            kind = "sphere"
+        pos = _f32(base + rng.uniform(-0.1 * spacing, 0.1 * spacing, size=base.shape))
+        out.append(Scene(
+            name=f"batch{s}_{kind}_{n}", pos=pos, radius=np.full(n, spacing * 0.5), vel=None,
+            physics=dict(dt=dt, gravity=(0.0, -9.81, 0.0), box_min=[0.0, 0.0, 0.0], box_max=[L, L, L]),
+            material=dict(radius=h, density=1000.0, viscosity=float(mus[s]), xsph=0.1, surface_tension=float(sigmas[s])),
+            config=dict(kernel=capi.KERNEL_CUBIC, hash_size=(64, 64, 64), cell_size=h, df_tol=1.0e-2, cd_tol=1.0e-3),
+            boundary_mesh=dict(verts=CUBE_VERTS, tris=CUBE_TRIS, center=[L / 2] * 3, size=[L] * 3, radius=h, density=1.0),
+            steps=steps, extra=dict(kind=kind, box_edge=L),
+        ))
+    return out
+
+
+def sample_walls(sim_factory, scene: Scene):
+    """Boundary particles of a scene's wall mesh, sampled by a fresh single-scene context:
+    (positions (B, 3), radii (B,), masses (B,)) as float32."""
+    sim = sim_factory(make_config(scene))
+    p = scene.physics
+    sim.set_physics(p["dt"], p["gravity"], p["box_min"], p["box_max"])
+    b = scene.boundary_mesh
+    sim.sample_boundary_mesh(b["verts"], b["tris"], b["radius"], b.get("density", 1.0), b.get("center"), b.get("size"))
+    out = (sim.get("BND_POS").astype(np.float32), sim.get("BND_RADIUS").astype(np.float32),
+           sim.get("BND_MASS").astype(np.float32))
+    if hasattr(sim, "close"):
+        sim.close()
+    return out
+
+
+def load_plain(sim, scene: Scene, walls):
+    """One scene into a single-scene context from pre-sampled walls (same inputs as a batch)."""
+    p, m = scene.physics, scene.material
+    sim.set_physics(p["dt"], p["gravity"], p["box_min"], p["box_max"])
+    sim.set_fluid_material(m["radius"], m["density"], m["viscosity"], m["xsph"], m["surface_tension"])
+    sim.upload_boundary(walls[0], walls[1], walls[2])
+    sim.upload_fluid(scene.pos, scene.radius, scene.vel)
+
+
+def load_batch(sim, scenes: list[Scene], walls: list):
+    """All scenes into one context (the batch entry points), in order."""
+    sim.batch_clear()
+    for sc, w in zip(scenes, walls):
+        sim.batch_add_scene(sc.physics, sc.material, sc.pos, sc.radius, vel=sc.vel, bpos=w[0], bradius=w[1], bmass=w[2])
+    sim.batch_commit()
