@@ -99,6 +99,59 @@ def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
     }
 
 
+def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
+    """The batched form against the plain path on the same GPU: `n_scenes` independent scenes
+    (sizes log-uniform 8k-256k, blocks or falling spheres, per-scene viscosity and surface
+    tension) stepped together in one context, then each scene stepped alone in a context of
+    its own. Both start from the same state and run the same steps (after `warmup` untimed
+    steps); particle-steps/s = sum of scene sizes x steps / wall time."""
+    from mps_cuda_amd import HipSim, scenes
+
+    sc = scenes.batch_scenes(n_scenes=n_scenes, seed=seed)
+    walls = [scenes.sample_walls(HipSim, s) for s in sc]
+    total = sum(s.n for s in sc)
+    cfg = scenes.make_config(sc[0])
+    print(f"batched leg: {n_scenes} scenes, {total} particles", file=sys.stderr, flush=True)
+
+    bat = HipSim(cfg)
+    scenes.load_batch(bat, sc, walls)
+    for _ in range(warmup):
+        bat.step()
+    bat.synchronize()
+    l_cg = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = bat.step()
+        l_cg.append(st.l_cg)
+    bat.synchronize()
+    t_bat = time.perf_counter() - t0
+    bat.close()
+    print(f"batched: {1000.0 * t_bat / steps:.2f} ms/step", file=sys.stderr, flush=True)
+
+    t_plain = 0.0
+    for s, w in zip(sc, walls):
+        g = HipSim(cfg)
+        scenes.load_plain(g, s, w)
+        for _ in range(warmup):
+            g.step()
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.step()
+        g.synchronize()
+        t_plain += time.perf_counter() - t0
+        g.close()
+    print(f"plain, scene by scene: {1000.0 * t_plain / steps:.2f} ms/step", file=sys.stderr, flush=True)
+    v_bat, v_plain = total * steps / t_bat, total * steps / t_plain
+    return {
+        "scenes": n_scenes, "particles": total, "steps": steps, "warmup": warmup,
+        "sizes": "log-uniform 8192..262144, seed %d" % seed,
+        "value": v_bat, "ms_per_step": 1000.0 * t_bat / steps,
+        "plain_value": v_plain, "plain_ms_per_step": 1000.0 * t_plain / steps,
+        "speedup": v_bat / v_plain, "l_cg_max_mean": sum(l_cg) / len(l_cg),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +163,8 @@ def main() -> int:
     ap.add_argument("--cpu-max-steps", type=int, default=3)
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
+    ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip it)")
+    ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch the library's plain copy kernel (k_calib_copy) 3 times on "
                          "512 MiB so hardware byte counters can be scaled against a copy of known size")
@@ -188,6 +243,10 @@ def main() -> int:
             "step_frac": tot_step_bytes / elapsed / 1e9 / HBM_PEAK_GBS,
         }
 
+    sim.profile(False)
+    sim.close()
+    batched = batched_leg(args.batch_scenes, args.batch_steps, args.warmup) if args.batch_scenes > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
@@ -209,10 +268,12 @@ def main() -> int:
                         f"viscosity 0.01 + surface tension 0.05 (CG), steps 0..{args.steps - 1}",
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
         },
-        "roofline": roofline, "cpu_baseline": cpu, "iterations": iters,
+        "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,
     }
     if args.copy_calibration:
+        sim = HipSim(cfg)
         sim.calibration_copy(512 * 1024 * 1024, 3)
+        sim.close()
         print("copy calibration: 3 x k_calib_copy of 536870912 bytes", file=sys.stderr)
     if args.kernels:
         for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["total_ms"]):
