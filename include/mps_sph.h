@@ -152,7 +152,7 @@ typedef struct mps_config {
     double cd_tol;               /* constant-density tolerance; reference 1e-4 */
     uint32_t cd_min_iter;        /* reference 2 */
     uint32_t max_iter;           /* DF/CD cap; reference 100 */
-    double cg_tol;               /* absolute r.r threshold; reference 1e-2 */
+    double cg_tol;               /* absolute r.r threshold (reference 1e-2); < 0: -cg_tol * r0.r0 */
     uint32_t cg_max_iter;        /* reference 100 */
     double cg_factor;            /* diagonal shift f used after the first A.x; reference 0.01 */
     uint32_t jacobi_max_iter;    /* Jacobi cap; reference 100 */
@@ -280,6 +280,23 @@ int mps_scene_info(mps_ctx* ctx, uint32_t scene, mps_scene_range* out);
 int mps_scene_stats(mps_ctx* ctx, uint32_t scene, mps_step_stats* out);
 /* Change one scene's material (viscosity, surface tension, ...) between steps. */
 int mps_scene_set_material(mps_ctx* ctx, uint32_t scene, const mps_sph_material* m);
+
+/* ---- adjoint ------------------------------------------------------------------------------
+ * No counterpart in the reference. Reverse mode through mps_step with each step's control
+ * flow frozen: its neighbour lists, the DF/CD iteration counts (differentiated through their
+ * iterations), the per-particle branches; the viscosity / surface-tension solve differentiated
+ * implicitly (transposed solve with the same operator). Needs the cubic kernel, the CG solve
+ * and equal fluid radii within a scene; a relative CG tolerance (cg_tol < 0) makes the forward
+ * solve match the implicit derivative. */
+/* Step n_steps times from the current state, keeping a checkpoint every checkpoint_every
+ * steps. loop_counts (may be NULL): n_steps * scenes * 3 values (l_df, l_cg, l_cd per scene). */
+int mps_adj_rollout(mps_ctx* ctx, uint32_t n_steps, uint32_t checkpoint_every, uint32_t* loop_counts);
+/* Gradients of a loss L(final positions, final velocities) given dL/dpos (n*3, reference
+ * order) and dL/dvel (n*3 or NULL): dL/d(velocities at the rollout's start) into dloss_dvel0
+ * (n*3), and per scene dL/dviscosity and dL/dsurface_tension (may be NULL). Steps are
+ * recomputed from the checkpoints; the context is left at the rollout's final state. */
+int mps_adj_backward(mps_ctx* ctx, const float* dloss_dpos, const float* dloss_dvel, float* dloss_dvel0,
+                     double* dloss_dviscosity, double* dloss_dsurface_tension);
 
 /* ---- per-kernel timing (replaces MTimer's debugger prints, MTimer.h:12-80) ---------------
  * When enabled, every kernel launch is bracketed by HIP events on the context's stream.

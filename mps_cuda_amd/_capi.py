@@ -134,6 +134,8 @@ _SIGS = {
     "mps_scene_info": (C.c_int, [_P, C.c_uint32, C.POINTER(SceneRange)]),
     "mps_scene_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(StepStats)]),
     "mps_scene_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(SPHMaterial)]),
+    "mps_adj_rollout": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
+    "mps_adj_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -288,6 +290,25 @@ class HipSim:
     def scene_set_material(self, scene: int, radius, density, viscosity, xsph, surface_tension):
         m = SPHMaterial(radius, density, viscosity, xsph, surface_tension)
         self._chk(self.lib.mps_scene_set_material(self.h, int(scene), C.byref(m)))
+
+    # ---- adjoint
+    def adj_rollout(self, n_steps: int, checkpoint_every: int = 10) -> np.ndarray:
+        """Step n_steps keeping checkpoints; returns the loop counts (n_steps, scenes, 3) =
+        (l_df, l_cg, l_cd) of every step."""
+        counts = np.zeros((n_steps, self.scene_count, 3), np.uint32)
+        self._chk(self.lib.mps_adj_rollout(self.h, int(n_steps), int(checkpoint_every), _ptr(counts)))
+        return counts
+
+    def adj_backward(self, dloss_dpos, dloss_dvel=None):
+        """(dL/dv0 (n, 3) float32, dL/dviscosity (scenes,), dL/dsurface_tension (scenes,))."""
+        n = self.get("FLUID_RADIUS").shape[0]
+        gx = np.ascontiguousarray(dloss_dpos, np.float32).reshape(n, 3)
+        gv = None if dloss_dvel is None else np.ascontiguousarray(dloss_dvel, np.float32).reshape(n, 3)
+        out = np.zeros((n, 3), np.float32)
+        gmu = np.zeros(self.scene_count, np.float64)
+        gsig = np.zeros(self.scene_count, np.float64)
+        self._chk(self.lib.mps_adj_backward(self.h, _ptr(gx), _ptr(gv), _ptr(out), _ptr(gmu), _ptr(gsig)))
+        return out, gmu, gsig
 
     # ---- stepping
     def step(self) -> StepStats:

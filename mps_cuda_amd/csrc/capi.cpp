@@ -116,6 +116,46 @@ struct mps_ctx {
     DBuf<StepParams> d_params;
     DBuf<uint32_t> blk_scene, bscene, d_delta, mkey;
     bool params_dirty = true;
+
+    // ------------------------------------------------------------------ adjoint state
+    // One recorded step: what its reverse needs (positions, frozen lists, walls, the velocity
+    // after gravity and after every DF/CD force and the CG solve, per-scene loop counts).
+    struct TapeStep {
+        uint32_t Kff = 0, Kfb = 0;
+        DBuf<float4> pos, bpos;
+        DBuf<float> bvol;
+        DBuf<uint32_t> src, nff, cff, nfb, cfb;
+        std::vector<DBuf<float4>> snaps;
+        uint32_t nsnap = 0, df_last = 0, cg_out = 0;
+        std::vector<uint32_t> ldf, lcd;
+        void release() {
+            pos.release(); bpos.release(); bvol.release(); src.release(); nff.release(); cff.release();
+            nfb.release(); cfb.release();
+            for (auto& b : snaps) b.release();
+        }
+    };
+    // Everything a step reads that earlier steps wrote.
+    struct Checkpoint {
+        DBuf<float4> pos, vel, bpos, bvel, bcolor;
+        DBuf<float> bvol;
+        DBuf<uint32_t> perm, bface;
+        DBuf<float2> bbcc;
+        uint32_t frame = 0;
+        void release() {
+            pos.release(); vel.release(); bpos.release(); bvel.release(); bcolor.release(); bvol.release();
+            perm.release(); bface.release(); bbcc.release();
+        }
+    };
+    std::vector<TapeStep> tape;
+    TapeStep* rec = nullptr;
+    std::vector<Checkpoint> ckpts;
+    Checkpoint ck_end;
+    uint32_t adj_n = 0, adj_every = 0;
+    DBuf<uint32_t> perm0, d_L, rev;
+    DBuf<float4> adx, adv, adx2, adv2, aux4, gvec, avin;
+    DBuf<float> abar, rbar, dbar, fbar, fstr;
+    DBuf<float2> cabT;
+    DBuf<double> acc_mu, acc_sig;
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
     DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
@@ -464,6 +504,7 @@ struct mps_ctx {
             HIPCHK(hipMemcpyAsync(hash_perm.p, h.id.p, sizeof(uint32_t) * Nref, hipMemcpyDeviceToDevice, stream));
             mps::launch::invert(perm.p, N, inv.p, stream);
         });
+        if (rec) dcopy(rec->src, src.p, N);
         T_(K_CELLS, [&] {
             mps::launch::cell_table_m(mkey.p, pos.p, N, g, mb, uint32_t(cells()), h.start.p, h.end.p, lkey.p, stream);
         });
@@ -619,12 +660,21 @@ struct mps_ctx {
                 T_(K_APPLY, [&] {
                     mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd, stream);
                 }, false);
+                rec_snap();
             }
             done = fetch_active(idx) == 0;
         }
         uint32_t l = 0;
         const std::vector<uint32_t> ls = fetch_records(idx, cap, &l);
         pred[idx] = std::max<uint32_t>(l, 1);
+        if (rec) {
+            if (df) {
+                rec->ldf = ls;
+                rec->df_last = rec->nsnap - 1;
+            } else {
+                rec->lcd = ls;
+            }
+        }
         if (prof_on) {  // eval and fold 1..min(l,cap), apply 1..l-1 did work (slowest scene)
             kuse[df ? K_EVAL_DF : K_EVAL_CD] += std::min(l, cap);
             kuse[K_FOLD] += std::min(l, cap);
@@ -795,6 +845,218 @@ struct mps_ctx {
         T_(K_ADVECT, [&] { mps::launch::advect(scn(), pos.p, vel.p, d_flags, stream); });
     }
 
+    template <class T>
+    void dcopy(DBuf<T>& dst, const T* from, size_t n) {
+        dst.reserve(std::max<size_t>(n, 1));
+        if (n) HIPCHK(hipMemcpyAsync(dst.p, from, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    }
+    void rec_snap() {
+        if (!rec) return;
+        if (rec->snaps.size() <= rec->nsnap) rec->snaps.emplace_back();
+        dcopy(rec->snaps[rec->nsnap++], vel.p, N);
+    }
+
+    // ------------------------------------------------------------------ adjoint
+    void save_ck(Checkpoint& c) {
+        dcopy(c.pos, pos.p, N); dcopy(c.vel, vel.p, N); dcopy(c.perm, perm.p, N);
+        dcopy(c.bpos, bpos.p, B); dcopy(c.bvel, bvel.p, B); dcopy(c.bcolor, bcolor.p, B); dcopy(c.bvol, bvol.p, B);
+        dcopy(c.bface, bface.p, B); dcopy(c.bbcc, bbcc.p, B);
+        c.frame = frame;
+    }
+    void restore_ck(const Checkpoint& c) {
+        auto back = [&](auto& dst, const auto& from, size_t n) {
+            if (n) HIPCHK(hipMemcpyAsync(dst.p, from.p, n * sizeof(*dst.p), hipMemcpyDeviceToDevice, stream));
+        };
+        back(pos, c.pos, N); back(vel, c.vel, N); back(perm, c.perm, N);
+        back(bpos, c.bpos, B); back(bvel, c.bvel, B); back(bcolor, c.bcolor, B); back(bvol, c.bvol, B);
+        back(bface, c.bface, B); back(bbcc, c.bbcc, B);
+        mps::launch::invert(perm.p, N, inv.p, stream);
+        frame = c.frame;
+        pairs_valid = false;
+        bnd_dirty = true;
+    }
+    // Forward n steps from the current state, keeping a checkpoint every `every` steps.
+    void adj_rollout(uint32_t n, uint32_t every, uint32_t* counts) {
+        if (cfg.kernel != MPS_KERNEL_CUBIC) throw ApiError{MPS_ERR_STATE, "the adjoint needs the cubic kernel"};
+        if (cfg.visc_solver != MPS_VISC_CG) throw ApiError{MPS_ERR_STATE, "the adjoint needs the CG viscosity solve"};
+        if (N == 0 || n == 0) throw ApiError{MPS_ERR_INVALID, "nothing to roll out"};
+        sync_params();
+        {  // equal fluid radii within a scene (symmetric lists, G_ij = G_ji)
+            std::vector<float4> hp = down(pos.p, N);
+            std::vector<uint32_t> hm = down(perm.p, N);
+            for (const SceneHost& h : scene_list()) {
+                const float r0 = hp[h.begin].w;
+                for (uint32_t q = h.begin; q < h.begin + h.nf; q++)
+                    if (hp[q].w != r0) throw ApiError{MPS_ERR_STATE, "the adjoint needs equal fluid radii within a scene"};
+            }
+        }
+        every = std::max<uint32_t>(every, 1);
+        dcopy(perm0, perm.p, N);
+        const uint32_t nseg = (n + every - 1) / every;
+        if (ckpts.size() < nseg) ckpts.resize(nseg);
+        mps_step_stats st;
+        for (uint32_t t = 0; t < n; t++) {
+            if (t % every == 0) save_ck(ckpts[t / every]);
+            step(&st);
+            if (st.nonfinite) throw ApiError{MPS_ERR_NONFINITE, "non-finite state during the rollout"};
+            if (counts)
+                for (uint32_t q = 0; q < S; q++) {
+                    uint32_t* o = counts + 3 * (size_t(t) * S + q);
+                    o[0] = sstats[q].l_df;
+                    o[1] = sstats[q].l_cg;
+                    o[2] = sstats[q].l_cd;
+                }
+        }
+        save_ck(ck_end);
+        sync();
+        adj_n = n;
+        adj_every = every;
+    }
+    // Transposed solve A_f^T lambda = adv (lambda in cgx), same chunked loop as the forward CG.
+    void adj_solveT(const mps::Nbr& nb, const float4* p0) {
+        const mps::Scenes P = scn();
+        SolverDev* sd = d_solver + 2 * size_t(S);
+        uint32_t* act = d_active + 2;
+        const double tol = cfg.cg_tol < 0.0 ? cfg.cg_tol : -1.0e-10;
+        mps::launch::solver_reset(sd, S, act, stream);
+        mps::launch::adj_cg_init(P, p0, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p, stream);
+        mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream);
+        const uint32_t cap = cfg.cg_max_iter;
+        uint32_t it = 1;
+        bool done = fetch_active(2) == 0;
+        while (it <= cap && !done) {
+            const uint32_t chunk = std::min(std::max<uint32_t>(pred[2], 1), cap - it + 1);
+            for (uint32_t c = 0; c < chunk; c++, it++) {
+                mps::launch::adj_cg_applyT(P, nb, pairAB.p, cabT.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
+                mps::launch::fold_cg(1, P, partials.p, sd, act, it, tol, stream);
+                mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
+                mps::launch::fold_cg(2, P, partials.p, sd, act, it, tol, stream);
+                mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream);
+            }
+            done = fetch_active(2) == 0;
+        }
+    }
+    // Reverse of one recorded step: (adx, adv) from the step's end state to its start, in the
+    // slot order of the previous step.
+    void reverse_step(TapeStep& T) {
+        const mps::Scenes P = scn();
+        const mps::Nbr nb{T.nff.p, T.cff.p, T.nfb.p, T.cfb.p};
+        const size_t m = std::max<uint32_t>(N, 1);
+        pairG_ff.reserve(size_t(T.Kff) * m);
+        pairG_fb.reserve(size_t(T.Kfb) * m);
+        pairs_valid = false;
+        mps::launch::density_alpha(1, mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb, T.pos.p,
+                                   T.snaps[0].p, T.bpos.p, T.bvol.p, rho.p, alpha.p, color.p, pairG_ff.p, pairG_fb.p, stream);
+        abar.reserve(m); rbar.reserve(m); dbar.reserve(m); fbar.reserve(m); fstr.reserve(m);
+        aux4.reserve(m); gvec.reserve(m); avin.reserve(m); adx2.reserve(m); adv2.reserve(m);
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(abar.p), N, stream);
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(rbar.p), N, stream);
+        mps::launch::adj_advect(P, T.pos.p, T.snaps[T.nsnap - 1].p, adx.p, adv.p, stream);
+        auto loop_rev = [&](int df, const std::vector<uint32_t>& L, uint32_t base) {
+            uint32_t lmax = 0;
+            for (uint32_t l : L) lmax = std::max(lmax, l);
+            if (lmax < 2) return;
+            d_L.reserve(S);
+            up(d_L.p, L);
+            for (uint32_t it = lmax - 1; it >= 1; it--) {
+                const float4* vk = T.snaps[base + it - 1].p;
+                mps::launch::adj_peval(df, P, nb, T.pos.p, vk, T.bpos.p, T.bvol.p, bvel.p, rho.p, alpha.p, pairG_ff.p,
+                                       pairG_fb.p, aux4.p, stream);
+                mps::launch::adj_kbar(df, P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, aux4.p, alpha.p, adv.p,
+                                      d_L.p, it, dbar.p, adx.p, abar.p, rbar.p, stream);
+                mps::launch::adj_dbar(P, nb, T.pos.p, vk, T.bpos.p, T.bvol.p, bvel.p, pairG_ff.p, pairG_fb.p, dbar.p, d_L.p, it,
+                                      adv.p, adx.p, stream);
+            }
+        };
+        loop_rev(0, T.lcd, T.cg_out);
+        {  // the implicit viscosity / surface-tension solve
+            const float4* vin = T.snaps[T.df_last].p;
+            const float4* vst = T.snaps[T.cg_out].p;
+            pairAB.reserve(size_t(T.Kff) * m); cabT.reserve(size_t(T.Kff) * m); rev.reserve(size_t(T.Kff) * m);
+            cgrecB.reserve(2 * m);
+            mps::launch::st_factor(P, nb, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
+            mps::launch::cg_setup(1, P, nb, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, pairAB.p, cgb.p, stream);
+            mps::launch::zero_words(d_flags + 2, 1, stream);
+            mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
+            mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);
+            adj_solveT(nb, T.pos.p);
+            mps::launch::adj_cg_grad(P, nb, T.pos.p, vin, vst, rho.p, fstr.p, cgx.p, avin.p, adx.p, rbar.p, fbar.p, partials.p,
+                                     stream);
+            mps::launch::adj_fold2(P, partials.p, acc_mu.p, acc_sig.p, stream);
+            mps::launch::adj_fst(P, nb, T.pos.p, vin, pairG_ff.p, fbar.p, avin.p, adx.p, stream);
+            std::swap(adv, avin);
+        }
+        loop_rev(1, T.ldf, 0);
+        mps::launch::adj_gvec(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, abar.p, gvec.p, stream);
+        mps::launch::adj_density(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, rbar.p, gvec.p, adx.p, stream);
+        mps::launch::adj_unpermute(N, T.src.p, adx.p, adv.p, adx2.p, adv2.p, stream);
+        std::swap(adx, adx2);
+        std::swap(adv, adv2);
+    }
+    // Gradients of a loss of the rollout's final positions (and velocities) with respect to the
+    // velocities at its start and each scene's viscosity and surface-tension coefficient.
+    void adj_backward(const float* gx, const float* gv, float* gv0, double* gmu, double* gsig) {
+        if (!adj_n) throw ApiError{MPS_ERR_STATE, "no rollout to differentiate"};
+        sync_params();
+        restore_ck(ck_end);
+        ensure_partials();
+        const size_t m = std::max<uint32_t>(N, 1);
+        adx.reserve(m); adv.reserve(m);
+        {
+            std::vector<uint32_t> hp = down(perm.p, N);
+            std::vector<float4> hx(N, make_float4(0.f, 0.f, 0.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
+            for (uint32_t q = 0; q < N; q++) {
+                const uint32_t r = hp[q];
+                if (r == mps::dev::kEmpty) continue;
+                hx[q] = make_float4(gx[3 * size_t(r)], gx[3 * size_t(r) + 1], gx[3 * size_t(r) + 2], 0.f);
+                if (gv) hv[q] = make_float4(gv[3 * size_t(r)], gv[3 * size_t(r) + 1], gv[3 * size_t(r) + 2], 0.f);
+            }
+            up(adx.p, hx);
+            up(adv.p, hv);
+        }
+        acc_mu.reserve(S); acc_sig.reserve(S);
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(acc_mu.p), 2 * size_t(S), stream);
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(acc_sig.p), 2 * size_t(S), stream);
+        const uint32_t nseg = (adj_n + adj_every - 1) / adj_every;
+        mps_step_stats st;
+        for (uint32_t sg = nseg; sg-- > 0;) {
+            restore_ck(ckpts[sg]);
+            const uint32_t t0 = sg * adj_every, t1 = std::min(adj_n, t0 + adj_every);
+            if (tape.size() < t1 - t0) tape.resize(t1 - t0);
+            for (uint32_t t = t0; t < t1; t++) {
+                rec = &tape[t - t0];
+                rec->nsnap = 0;
+                try {
+                    step(&st);
+                } catch (...) {
+                    rec = nullptr;
+                    throw;
+                }
+                rec = nullptr;
+            }
+            for (uint32_t t = t1; t-- > t0;) reverse_step(tape[t - t0]);
+        }
+        HIPCHK(hipMemcpyAsync(h_flags + 2, d_flags + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        if (h_flags[2]) throw ApiError{MPS_ERR_STATE, "asymmetric neighbour lists in the adjoint"};
+        std::vector<float4> hv = down(adv.p, N);
+        std::vector<uint32_t> hp0 = down(perm0.p, N);
+        for (uint32_t q = 0; q < N; q++) {
+            const uint32_t r = hp0[q];
+            if (r == mps::dev::kEmpty) continue;
+            gv0[3 * size_t(r)] = hv[q].x;
+            gv0[3 * size_t(r) + 1] = hv[q].y;
+            gv0[3 * size_t(r) + 2] = hv[q].z;
+        }
+        std::vector<double> hm = down(acc_mu.p, S), hs = down(acc_sig.p, S);
+        for (uint32_t q = 0; q < S; q++) {
+            if (gmu) gmu[q] = hm[q];
+            if (gsig) gsig[q] = hs[q];
+        }
+        restore_ck(ck_end);
+        sync();
+    }
+
     void begin_stats() {
         sstats.assign(S, mps_step_stats{});
         const std::vector<SceneHost> sl = scene_list();
@@ -840,9 +1102,25 @@ struct mps_ctx {
         hash();
         neighbors();
         boundary_volume();
+        if (rec) {
+            rec->Kff = Kff;
+            rec->Kfb = Kfb;
+            dcopy(rec->pos, pos.p, N);
+            dcopy(rec->nff, nff.p, size_t(Kff) * N);
+            dcopy(rec->cff, cff.p, N);
+            dcopy(rec->nfb, nfb.p, size_t(Kfb) * N);
+            dcopy(rec->cfb, cfb.p, N);
+            dcopy(rec->bpos, bpos.p, B);
+            dcopy(rec->bvol, bvol.p, B);
+        }
         density_alpha(mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoGravity);
+        rec_snap();
         pressure_solve(true, st);
         viscosity(st);
+        if (rec) {
+            rec_snap();
+            rec->cg_out = rec->nsnap - 1;
+        }
         pressure_solve(false, st);
         advect();
         HIPCHK(hipMemcpyAsync(h_flags, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
@@ -1408,6 +1686,12 @@ struct mps_ctx {
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();
+        for (auto& t : tape) t.release();
+        for (auto& c : ckpts) c.release();
+        ck_end.release();
+        perm0.release(); d_L.release(); rev.release(); adx.release(); adv.release(); adx2.release(); adv2.release();
+        aux4.release(); gvec.release(); avin.release(); abar.release(); rbar.release(); dbar.release(); fbar.release();
+        fstr.release(); cabT.release(); acc_mu.release(); acc_sig.release();
         if (d_active) (void)hipFree(d_active);
         if (h_active) (void)hipHostFree(h_active);
         for (const Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -1772,6 +2056,16 @@ int mps_scene_set_material(mps_ctx* c, uint32_t scene, const mps_sph_material* m
     h.mass = m->density * h.V;
     c->params_dirty = true;
     return MPS_OK;
+}
+
+int mps_adj_rollout(mps_ctx* c, uint32_t n_steps, uint32_t checkpoint_every, uint32_t* loop_counts) {
+    return guarded(c, [&] { c->adj_rollout(n_steps, checkpoint_every, loop_counts); });
+}
+
+int mps_adj_backward(mps_ctx* c, const float* dloss_dpos, const float* dloss_dvel, float* dloss_dvel0, double* dloss_dviscosity,
+                     double* dloss_dsurface_tension) {
+    if (!c || !dloss_dpos || !dloss_dvel0) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->adj_backward(dloss_dpos, dloss_dvel, dloss_dvel0, dloss_dviscosity, dloss_dsurface_tension); });
 }
 
 int mps_host_alloc(mps_ctx* c, size_t bytes, void** out) {

@@ -38,6 +38,7 @@ struct SolverDev {
     uint32_t pad;
     double rr;         // CG: r.r of the current residual
     double pAp;        // CG: p.Ap
+    double thr;        // CG: stop when r.r < thr (tol, or -tol * initial r.r when tol < 0)
     float alpha, beta; // CG step scalars (rounded to float like the FP32 reference)
     double log[MPS_MAX_ITER_LOG];
 };
@@ -160,6 +161,38 @@ void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s);
+
+// ---- adjoint.hip (reverse mode of one step with frozen control flow; cubic kernel)
+void adj_advect(const Scenes& sc, const float4* pos, const float4* vin, const float4* ax, float4* av, hipStream_t s);
+// aux = (kappa, e, mask, rho branch) recomputed from the velocity an iteration started from
+void adj_peval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+               const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
+               const float* pG_fb, float4* aux, hipStream_t s);
+void adj_kbar(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
+              const float* pG_ff, const float* pG_fb, const float4* aux, const float* alpha, const float4* av,
+              const uint32_t* L, uint32_t it, float* dbar, float4* ax, float* abar, float* rbar, hipStream_t s);
+void adj_dbar(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos, const float* bvol,
+              const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar, const uint32_t* L, uint32_t it,
+              float4* av, float4* ax, hipStream_t s);
+void adj_rev_index(const Scenes& sc, const Nbr& nb, uint32_t* rev, uint32_t* bad, hipStream_t s);
+void adj_transpose_coeffs(const Scenes& sc, const Nbr& nb, const uint32_t* rev, const float2* cab, float2* cabT, hipStream_t s);
+void adj_cg_init(const Scenes& sc, const float4* pos, const float4* b, float4* x, float4* r, float4* rec, double* partials,
+                 hipStream_t s);
+void adj_cg_applyT(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* rec, float4* Ap,
+                   const SolverDev* st, double* partials, hipStream_t s);
+// partials: 2 * ceil(N / kBlock) doubles (mubar, sigmabar per workgroup)
+void adj_cg_grad(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar, const float* rho,
+                 const float* fst, const float4* lam, float4* avin, float4* ax, float* rbar, float* fbar, double* partials,
+                 hipStream_t s);
+void adj_fst(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff, const float* fbar,
+             float4* avin, float4* ax, hipStream_t s);
+void adj_fold2(const Scenes& sc, const double* partials, double* acc0, double* acc1, hipStream_t s);
+void adj_gvec(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
+              const float* pG_fb, const float* abar, float4* gv, hipStream_t s);
+void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
+                 const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s);
+void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, float4* ax_out, float4* av_out,
+                   hipStream_t s);
 
 // ---- sampling.hip
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,

@@ -586,7 +586,8 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const doubl
         st->rr = s;
         st->log[0] = s;
         st->l = 1u;
-        if (s < tol) {
+        st->thr = tol < 0.0 ? -tol * s : tol;  // a negative tolerance is relative to r0.r0
+        if (s < st->thr || (tol < 0.0 && s == 0.0)) {
             st->done = 1u;
             atomicSub(active, 1u);
         }
@@ -598,7 +599,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const doubl
         st->rr = s;
         if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
         st->l = it + 1;
-        if (s < tol) {
+        if (s < st->thr || (tol < 0.0 && s == 0.0)) {
             st->done = 1u;
             atomicSub(active, 1u);
         }
@@ -834,6 +835,7 @@ __global__ void k_solver_reset(SolverDev* stv, uint32_t S, uint32_t* active) {
         st->counter = 0u;
         st->rr = 0.0;
         st->pAp = 0.0;
+        st->thr = 0.0;
         st->alpha = 0.0f;
         st->beta = 0.0f;
         if (blockIdx.x == 0) *active = S;

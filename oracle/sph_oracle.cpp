@@ -839,12 +839,14 @@ struct Ctx final : Base {
         for (size_t k = 0; k < n3; k++) { rr[k] = cgb[k] - Ap[k]; p[k] = rr[k]; }
         uint32_t l = 1u;
         *hit_cap = 1;
+        // a negative tolerance is relative to the initial r.r (a zero residual then ends the loop)
+        const double thr = cfg.cg_tol < 0.0 ? -cfg.cg_tol * dotd(rr, rr) : cfg.cg_tol;
         while (l <= cfg.cg_max_iter) {
             cg_apply(p, Ap, T(cfg.cg_factor));
             const double r2 = dotd(rr, rr);
             const double pAp = dotd(p, Ap);
             if (log && l - 1 < MPS_MAX_ITER_LOG) log[l - 1] = r2;
-            if (r2 < cfg.cg_tol) { *hit_cap = 0; break; }
+            if (r2 < thr || (cfg.cg_tol < 0.0 && r2 == 0.0)) { *hit_cap = 0; break; }
             const T a = T(r2 / pAp);
             for (size_t k = 0; k < n3; k++) { predv[k] = predv[k] + a * p[k]; rr[k] = rr[k] - a * Ap[k]; }
             const double r2n = dotd(rr, rr);
