@@ -24,6 +24,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -152,6 +154,76 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
     }
 
 
+def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
+    """The differentiable droplet: a sphere of n particles falling for `steps` steps; target
+    from viscosity 0.02, surface tension 0.3 and a smooth seeded initial velocity; loss = MSE of
+    the final positions. At the guess (0.025, 0.1, same initial velocity) time one plain forward,
+    the checkpointed rollout and the backward pass, and compare the gradients with central
+    finite differences of the FP32 forward itself (whose rounding noise is about 1e-3 of these
+    small loss differences; the FP64 check is a GPU test)."""
+    from mps_cuda_amd import HipSim, scenes
+
+    sc = scenes.diff_droplet(n=n, steps=steps, seed=seed)
+    g = HipSim(scenes.make_config(sc))
+    v0 = sc.extra["v0"]
+
+    def reset(mu, sig, v):
+        scenes.load_into(g, sc)
+        g.set_fluid_material(**dict(sc.material, viscosity=mu, surface_tension=sig))
+        g.set("FLUID_VEL", v)
+
+    def forward(mu, sig, v):
+        reset(mu, sig, v)
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.step()
+        g.synchronize()
+        return g.get("FLUID_POS").astype(np.float64), time.perf_counter() - t0
+
+    target, _ = forward(0.02, 0.3, v0)
+    mu0, sig0 = 0.025, 0.1
+    x, t_fwd = forward(mu0, sig0, v0)
+
+    def loss(xx):
+        return float(np.mean((xx - target) ** 2))
+
+    reset(mu0, sig0, v0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    counts = g.adj_rollout(steps, every)
+    t_roll = time.perf_counter() - t0
+    xr = g.get("FLUID_POS").astype(np.float64)
+    gx = (2.0 / xr.size) * (xr - target)
+    t0 = time.perf_counter()
+    gv0, gmu, gsig = g.adj_backward(gx.astype(np.float32))
+    t_back = time.perf_counter() - t0
+    print(f"adjoint leg: forward {1000 * t_fwd / steps:.2f} ms/step, rollout {1000 * t_roll / steps:.2f}, "
+          f"backward {1000 * t_back / steps:.2f}", file=sys.stderr, flush=True)
+    # central differences of the FP32 forward: eps = 1e-3 relative for the coefficients, 1e-3 m/s
+    # along random unit vectors (one per particle) for the initial velocities
+    eps = 1.0e-3
+    dv = np.random.default_rng(seed + 7).normal(size=v0.shape)
+    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
+    em, es = eps * mu0, eps * sig0
+    fd = {
+        "viscosity": (loss(forward(mu0 + em, sig0, v0)[0]) - loss(forward(mu0 - em, sig0, v0)[0])) / (2 * em),
+        "surface_tension": (loss(forward(mu0, sig0 + es, v0)[0]) - loss(forward(mu0, sig0 - es, v0)[0])) / (2 * es),
+        "v0_direction": (loss(forward(mu0, sig0, (v0 + eps * dv).astype(np.float32))[0])
+                         - loss(forward(mu0, sig0, (v0 - eps * dv).astype(np.float32))[0])) / (2 * eps),
+    }
+    adj = {"viscosity": float(gmu[0]), "surface_tension": float(gsig[0]),
+           "v0_direction": float(np.sum(gv0.astype(np.float64) * dv))}
+    checks = {k: {"adjoint": adj[k], "fd": fd[k], "rel_err": abs(adj[k] - fd[k]) / max(abs(fd[k]), 1e-30)} for k in fd}
+    return {
+        "particles": n, "steps": steps, "checkpoint_every": every, "loss": loss(xr),
+        "forward_ms_per_step": 1000 * t_fwd / steps, "rollout_ms_per_step": 1000 * t_roll / steps,
+        "backward_ms_per_step": 1000 * t_back / steps, "adjoint_over_forward": t_back / t_fwd,
+        "rollout_plus_backward_over_forward": (t_roll + t_back) / t_fwd,
+        "l_cg_mean": float(counts[:, 0, 1].mean()), "fd_eps": eps, "gradient_checks": checks,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +237,8 @@ def main() -> int:
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched leg")
+    ap.add_argument("--adjoint-particles", type=int, default=65536, help="droplet of the adjoint leg (0 = skip it)")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch the library's plain copy kernel (k_calib_copy) 3 times on "
                          "512 MiB so hardware byte counters can be scaled against a copy of known size")
@@ -246,6 +320,7 @@ def main() -> int:
     sim.profile(False)
     sim.close()
     batched = batched_leg(args.batch_scenes, args.batch_steps, args.warmup) if args.batch_scenes > 0 else None
+    adjoint = adjoint_leg(args.adjoint_particles, args.adjoint_steps, 10) if args.adjoint_particles > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -269,6 +344,7 @@ def main() -> int:
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
         },
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,
+        "adjoint": adjoint,
     }
     if args.copy_calibration:
         sim = HipSim(cfg)

@@ -161,7 +161,9 @@ typedef struct mps_config {
     uint32_t nei_capacity;       /* initial per-particle neighbour slots (grows on overflow) */
     uint32_t host_sync_every;    /* HIP library: solver iterations enqueued between
                                     convergence checks on the host (0 = adaptive) */
-    uint32_t reserved[7];
+    uint32_t adj_cg_max_iter;    /* adjoint: cap of the transposed viscosity solve (300) */
+    uint32_t reserved[6];
+    double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
 } mps_config;
 
 /* Per-step statistics (the reference prints these to the debugger only:

@@ -50,7 +50,8 @@ class Config(C.Structure):
         ("cd_min_iter", C.c_uint32), ("max_iter", C.c_uint32), ("cg_tol", C.c_double),
         ("cg_max_iter", C.c_uint32), ("cg_factor", C.c_double), ("jacobi_max_iter", C.c_uint32),
         ("jacobi_tol", C.c_double), ("zsort_interval", C.c_uint32), ("nei_capacity", C.c_uint32),
-        ("host_sync_every", C.c_uint32), ("reserved", C.c_uint32 * 7),
+        ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("reserved", C.c_uint32 * 6),
+        ("adj_cg_tol", C.c_double),
     ]
 
 

@@ -310,6 +310,75 @@ __global__ void __launch_bounds__(kBlock) k_adj_cg_init(Scenes sc, const float4*
     const double s = block_sum_d(part, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
+// Warm start: r = b - A_f^T x0 (x0 packed in recA), p = r into recB, x = x0; partials of r.r
+// at partials[blk] and of b.b at partials[nblk + blk].
+__global__ void __launch_bounds__(kBlock) k_adj_cg_init_warm(Scenes sc, Nbr nb, const float2* __restrict__ cab,
+                                                           const float2* __restrict__ cabT, const float4* __restrict__ recA,
+                                                           const float4* __restrict__ b, float4* __restrict__ x,
+                                                           float4* __restrict__ r, float4* __restrict__ recB,
+                                                           double* __restrict__ partials, uint32_t nblk) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const StepParams P = sc.P[sid];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = P.N;
+    double prr = 0.0, pbb = 0.0;
+    if (i < N) {
+        const float4 ra = recA[2 * size_t(i)], rb = recA[2 * size_t(i) + 1];
+        const f3 xi = mk(ra), x0 = rec_pv(ra, rb);
+        f3 ri = f3z();
+        if (i < P.end) {
+            f3 acc = f3z();
+            const uint32_t c = nb.cff[i];
+            for (uint32_t k = 0; k < c; k++) {
+                const size_t slot = size_t(k) * N + i;
+                const uint32_t j = nb.ff[slot];
+                const float2 ab = cab[slot], abT = cabT[slot];
+                const float4 ja = recA[2 * size_t(j)], jb = recA[2 * size_t(j) + 1];
+                const f3 xij = xi - mk(ja);
+                const f3 pj = rec_pv(ja, jb);
+                acc = acc + (ab.x * x0 - abT.x * pj) + (ab.y * dot3(x0, xij) - abT.y * dot3(pj, xij)) * xij;
+            }
+            const f3 Ax = x0 * (1.0f + P.cg_factor) + acc;
+            const f3 bi = mk(b[i]);
+            ri = bi - Ax;
+            prr = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
+            pbb = double(bi.x * bi.x) + double(bi.y * bi.y) + double(bi.z * bi.z);
+            x[i] = make_float4(x0.x, x0.y, x0.z, 0.0f);
+        } else {
+            x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
+        recB[2 * size_t(i)] = make_float4(xi.x, xi.y, xi.z, ri.x);
+        recB[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, 0.f, 0.f);
+    }
+    block_partial2(prr, pbb, partials, nblk);
+}
+// Start of the transposed loop: r.r, and the stop threshold -tol * b.b (relative to the right-
+// hand side, so a good starting guess does not tighten it) or tol.
+__global__ void __launch_bounds__(kBlock) k_adj_fold_start(Scenes sc, const double* __restrict__ partials, uint32_t nblk,
+                                                         SolverDev* stv, uint32_t* active, double tol) {
+    __shared__ double lds[kBlock / 64];
+    const StepParams P = sc.P[blockIdx.x];
+    SolverDev* st = stv + blockIdx.x;
+    double a = 0.0, b = 0.0;
+    for (uint32_t k = threadIdx.x; k < P.nblk; k += blockDim.x) {
+        a += partials[P.blk0 + k];
+        b += partials[nblk + P.blk0 + k];
+    }
+    a = block_sum_d(a, lds);
+    b = block_sum_d(b, lds);
+    if (threadIdx.x == 0) {
+        st->rr = a;
+        st->log[0] = a;
+        st->l = 1u;
+        st->thr = tol < 0.0 ? -tol * b : tol;
+        if (a < st->thr || a == 0.0) {
+            st->done = 1u;
+            atomicSub(active, 1u);
+        }
+    }
+}
+
 // (A_f^T p)_i = (1+f) p_i + sum_j [ (a_ij p_i - a_ji p_j) + (b_ij (p_i.x_ij) - b_ji (p_j.x_ij)) x_ij ]
 // and partials of p.Ap, for scenes whose solve is still running.
 __global__ void __launch_bounds__(kBlock) k_adj_cg_applyT(Scenes sc, Nbr nb, const float2* __restrict__ cab,
@@ -594,14 +663,16 @@ __global__ void __launch_bounds__(kBlock) k_adj_density(Scenes sc, Nbr nb, const
 // ----------------------------------------------------------------- slot order between steps
 // The hash of a step moved slot src[s] of the previous step to slot s: send the adjoints back.
 __global__ void __launch_bounds__(kBlock) k_adj_unpermute(uint32_t n, const uint32_t* __restrict__ src, const float4* __restrict__ ax,
-                                                        const float4* __restrict__ av, float4* __restrict__ ax_out,
-                                                        float4* __restrict__ av_out) {
+                                                        const float4* __restrict__ av, const float4* __restrict__ al,
+                                                        float4* __restrict__ ax_out, float4* __restrict__ av_out,
+                                                        float4* __restrict__ al_out) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t o = src[s];
     if (o == kEmpty) return;
     ax_out[o] = ax[s];
     av_out[o] = av[s];
+    if (al) al_out[o] = al[s];
 }
 
 }  // namespace dev
@@ -673,9 +744,16 @@ void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float
                  const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s) {
     if (sc.N) k_adj_density<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, bpos, bvol, pG_ff, pG_fb, rbar, gv, ax);
 }
-void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, float4* ax_out, float4* av_out,
-                   hipStream_t s) {
-    if (n) k_adj_unpermute<<<nbk(n), kBlock, 0, s>>>(n, src, ax, av, ax_out, av_out);
+void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, const float4* al, float4* ax_out,
+                   float4* av_out, float4* al_out, hipStream_t s) {
+    if (n) k_adj_unpermute<<<nbk(n), kBlock, 0, s>>>(n, src, ax, av, al, ax_out, av_out, al_out);
+}
+void adj_cg_init_warm(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* recA,
+                      const float4* b, float4* x, float4* r, float4* recB, double* partials, hipStream_t s) {
+    if (sc.N) k_adj_cg_init_warm<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, cab, cabT, recA, b, x, r, recB, partials, nbk(sc.N));
+}
+void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
+    if (sc.N) k_adj_fold_start<<<sc.S, kBlock, 0, s>>>(sc, partials, nbk(sc.N), st, active, tol);
 }
 
 }  // namespace launch

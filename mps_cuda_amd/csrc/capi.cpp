@@ -152,7 +152,8 @@ struct mps_ctx {
     Checkpoint ck_end;
     uint32_t adj_n = 0, adj_every = 0;
     DBuf<uint32_t> perm0, d_L, rev;
-    DBuf<float4> adx, adv, adx2, adv2, aux4, gvec, avin;
+    DBuf<float4> adx, adv, adx2, adv2, aux4, gvec, avin, lam, lam2;
+    bool lam_valid = false;  // lam holds the later step's transposed solution (warm start)
     DBuf<float> abar, rbar, dbar, fbar, fstr;
     DBuf<float2> cabT;
     DBuf<double> acc_mu, acc_sig;
@@ -912,16 +913,25 @@ struct mps_ctx {
         adj_n = n;
         adj_every = every;
     }
-    // Transposed solve A_f^T lambda = adv (lambda in cgx), same chunked loop as the forward CG.
+    // Transposed solve A_f^T lambda = adv (lambda in cgx), same chunked loop as the forward CG,
+    // started from the later step's solution when there is one.
     void adj_solveT(const mps::Nbr& nb, const float4* p0) {
         const mps::Scenes P = scn();
         SolverDev* sd = d_solver + 2 * size_t(S);
         uint32_t* act = d_active + 2;
-        const double tol = cfg.cg_tol < 0.0 ? cfg.cg_tol : -1.0e-10;
+        const double tol = cfg.adj_cg_tol != 0.0 ? cfg.adj_cg_tol : -1.0e-12;
         mps::launch::solver_reset(sd, S, act, stream);
-        mps::launch::adj_cg_init(P, p0, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p, stream);
-        mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream);
-        const uint32_t cap = cfg.cg_max_iter;
+        if (lam_valid) {
+            cgrecA.reserve(2 * std::max<uint32_t>(N, 1));
+            mps::launch::cg_pack(N, p0, lam.p, cgrecA.p, stream);
+            mps::launch::adj_cg_init_warm(P, nb, pairAB.p, cabT.p, cgrecA.p, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p,
+                                          stream);
+            mps::launch::adj_fold_start(P, partials.p, sd, act, tol, stream);
+        } else {
+            mps::launch::adj_cg_init(P, p0, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p, stream);
+            mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream);
+        }
+        const uint32_t cap = cfg.adj_cg_max_iter ? cfg.adj_cg_max_iter : 300u;
         uint32_t it = 1;
         bool done = fetch_active(2) == 0;
         while (it <= cap && !done) {
@@ -980,6 +990,9 @@ struct mps_ctx {
             mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
             mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);
             adj_solveT(nb, T.pos.p);
+            lam.reserve(m);
+            HIPCHK(hipMemcpyAsync(lam.p, cgx.p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
+            lam_valid = true;
             mps::launch::adj_cg_grad(P, nb, T.pos.p, vin, vst, rho.p, fstr.p, cgx.p, avin.p, adx.p, rbar.p, fbar.p, partials.p,
                                      stream);
             mps::launch::adj_fold2(P, partials.p, acc_mu.p, acc_sig.p, stream);
@@ -989,9 +1002,11 @@ struct mps_ctx {
         loop_rev(1, T.ldf, 0);
         mps::launch::adj_gvec(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, abar.p, gvec.p, stream);
         mps::launch::adj_density(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, rbar.p, gvec.p, adx.p, stream);
-        mps::launch::adj_unpermute(N, T.src.p, adx.p, adv.p, adx2.p, adv2.p, stream);
+        lam2.reserve(m);
+        mps::launch::adj_unpermute(N, T.src.p, adx.p, adv.p, lam_valid ? lam.p : nullptr, adx2.p, adv2.p, lam2.p, stream);
         std::swap(adx, adx2);
         std::swap(adv, adv2);
+        std::swap(lam, lam2);
     }
     // Gradients of a loss of the rollout's final positions (and velocities) with respect to the
     // velocities at its start and each scene's viscosity and surface-tension coefficient.
@@ -1014,6 +1029,7 @@ struct mps_ctx {
             up(adx.p, hx);
             up(adv.p, hv);
         }
+        lam_valid = false;
         acc_mu.reserve(S); acc_sig.reserve(S);
         mps::launch::zero_words(reinterpret_cast<uint32_t*>(acc_mu.p), 2 * size_t(S), stream);
         mps::launch::zero_words(reinterpret_cast<uint32_t*>(acc_sig.p), 2 * size_t(S), stream);
@@ -1691,7 +1707,7 @@ struct mps_ctx {
         ck_end.release();
         perm0.release(); d_L.release(); rev.release(); adx.release(); adv.release(); adx2.release(); adv2.release();
         aux4.release(); gvec.release(); avin.release(); abar.release(); rbar.release(); dbar.release(); fbar.release();
-        fstr.release(); cabT.release(); acc_mu.release(); acc_sig.release();
+        fstr.release(); cabT.release(); acc_mu.release(); acc_sig.release(); lam.release(); lam2.release();
         if (d_active) (void)hipFree(d_active);
         if (h_active) (void)hipHostFree(h_active);
         for (const Rec& r : recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
@@ -1756,6 +1772,8 @@ int mps_config_init(mps_config* cfg) {
     cfg->zsort_interval = 50;                   // MPSSystem.cpp:220
     cfg->nei_capacity = 64;
     cfg->host_sync_every = 0;
+    cfg->adj_cg_max_iter = 300;
+    cfg->adj_cg_tol = -1.0e-12;
     return MPS_OK;
 }
 

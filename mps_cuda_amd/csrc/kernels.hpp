@@ -191,8 +191,13 @@ void adj_gvec(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* 
               const float* pG_fb, const float* abar, float4* gv, hipStream_t s);
 void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
                  const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s);
-void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, float4* ax_out, float4* av_out,
-                   hipStream_t s);
+// also carries the transposed solve's solution (al, may be null) back, as the next warm start
+void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, const float4* al, float4* ax_out,
+                   float4* av_out, float4* al_out, hipStream_t s);
+// warm start of A_f^T lambda = b from x0 (packed in recA with the positions): partials of r.r and b.b
+void adj_cg_init_warm(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* recA,
+                      const float4* b, float4* x, float4* r, float4* recB, double* partials, hipStream_t s);
+void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
 
 // ---- sampling.hip
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,

@@ -298,7 +298,8 @@ def diff_droplet(n: int = 65536, radius: float = 0.25, seed: int = 0, steps: int
     radius, support h) falling in a 1 m box at dt = 1 ms, with a seeded smooth initial velocity
     (3 Fourier modes, 0.5 m/s) in `extra["v0"]`. Configured for the adjoint: cubic kernel,
     no Z-sort (with the reference's volume quirk it reorders wall volumes), and a relative CG
-    tolerance so the viscosity solve matches its implicit derivative."""
+    tolerance (r.r below 1e-8 of its initial value, reachable in FP32) so the viscosity solve
+    matches its implicit derivative."""
     sc = droplet(n=n, radius=radius, seed=seed, steps=steps)
     if h != 0.02:
         sc.radius = np.full(n, h / 4)
@@ -308,6 +309,6 @@ def diff_droplet(n: int = 65536, radius: float = 0.25, seed: int = 0, steps: int
     sc.name = f"diff_droplet_{n}"
     sc.material["viscosity"] = viscosity
     sc.material["surface_tension"] = surface_tension
-    sc.config.update(zsort_interval=0, cg_tol=-1.0e-10)
+    sc.config.update(zsort_interval=0, cg_tol=-1.0e-8, cg_max_iter=200)
     sc.extra["v0"] = fourier_velocity(sc.pos, seed + 1)
     return sc
