@@ -162,7 +162,9 @@ typedef struct mps_config {
     uint32_t host_sync_every;    /* HIP library: solver iterations enqueued between
                                     convergence checks on the host (0 = adaptive) */
     uint32_t adj_cg_max_iter;    /* adjoint: cap of the transposed viscosity solve (300) */
-    uint32_t reserved[6];
+    uint32_t use_graphs;         /* HIP library: replay solver iteration chunks from cached HIP
+                                    graphs (same results; not while timing kernels) */
+    uint32_t reserved[5];
     double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
 } mps_config;
 
@@ -299,6 +301,19 @@ int mps_adj_rollout(mps_ctx* ctx, uint32_t n_steps, uint32_t checkpoint_every, u
  * recomputed from the checkpoints; the context is left at the rollout's final state. */
 int mps_adj_backward(mps_ctx* ctx, const float* dloss_dpos, const float* dloss_dvel, float* dloss_dvel0,
                      double* dloss_dviscosity, double* dloss_dsurface_tension);
+
+/* ---- autotuning ----------------------------------------------------------------------------
+ * Times `steps` steps of the current state under each launch setting that leaves results
+ * unchanged (graph replay of solver chunks on/off; host polling every 1, 4 or 16 iterations or
+ * adaptive), keeps the fastest in the context's configuration and restores the state. */
+typedef struct mps_tune_result {
+    uint32_t use_graphs;        /* chosen */
+    uint32_t host_sync_every;   /* chosen (0 = adaptive) */
+    uint32_t n_tried;
+    uint32_t pad;
+    double ms_per_step[8];      /* per candidate: graphs off with polling {0,1,4,16}, then on */
+} mps_tune_result;
+int mps_autotune(mps_ctx* ctx, uint32_t steps, mps_tune_result* out);
 
 /* ---- per-kernel timing (replaces MTimer's debugger prints, MTimer.h:12-80) ---------------
  * When enabled, every kernel launch is bracketed by HIP events on the context's stream.

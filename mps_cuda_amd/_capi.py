@@ -50,7 +50,8 @@ class Config(C.Structure):
         ("cd_min_iter", C.c_uint32), ("max_iter", C.c_uint32), ("cg_tol", C.c_double),
         ("cg_max_iter", C.c_uint32), ("cg_factor", C.c_double), ("jacobi_max_iter", C.c_uint32),
         ("jacobi_tol", C.c_double), ("zsort_interval", C.c_uint32), ("nei_capacity", C.c_uint32),
-        ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("reserved", C.c_uint32 * 6),
+        ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
+        ("reserved", C.c_uint32 * 5),
         ("adj_cg_tol", C.c_double),
     ]
 
@@ -95,6 +96,11 @@ class Fence(C.Structure):
     _fields_ = [("event", C.c_void_p), ("frame", C.c_uint64)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("use_graphs", C.c_uint32), ("host_sync_every", C.c_uint32), ("n_tried", C.c_uint32),
+                ("pad", C.c_uint32), ("ms_per_step", C.c_double * 8)]
+
+
 class SceneRange(C.Structure):
     _fields_ = [("fluid_offset", C.c_size_t), ("n_fluid", C.c_size_t), ("boundary_offset", C.c_size_t),
                 ("n_boundary", C.c_size_t)]
@@ -136,6 +142,7 @@ _SIGS = {
     "mps_scene_stats": (C.c_int, [_P, C.c_uint32, C.POINTER(StepStats)]),
     "mps_scene_set_material": (C.c_int, [_P, C.c_uint32, C.POINTER(SPHMaterial)]),
     "mps_adj_rollout": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
+    "mps_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(TuneResult)]),
     "mps_adj_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
 }
 
@@ -291,6 +298,13 @@ class HipSim:
     def scene_set_material(self, scene: int, radius, density, viscosity, xsph, surface_tension):
         m = SPHMaterial(radius, density, viscosity, xsph, surface_tension)
         self._chk(self.lib.mps_scene_set_material(self.h, int(scene), C.byref(m)))
+
+    # ---- autotuning
+    def autotune(self, steps: int = 5) -> dict:
+        r = TuneResult()
+        self._chk(self.lib.mps_autotune(self.h, int(steps), C.byref(r)))
+        return dict(use_graphs=r.use_graphs, host_sync_every=r.host_sync_every,
+                    ms_per_step=list(r.ms_per_step[:r.n_tried]))
 
     # ---- adjoint
     def adj_rollout(self, n_steps: int, checkpoint_every: int = 10) -> np.ndarray:

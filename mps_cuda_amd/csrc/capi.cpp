@@ -11,9 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -160,6 +162,10 @@ struct mps_ctx {
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
     DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
+    // second buffer of each array the hash reorders: the two alternate, so the arrays a solver
+    // chunk reads take one of two addresses (cached solver graphs stay valid)
+    DBuf<float4> pos_alt, vel_alt, color_alt, cgb_alt;
+    DBuf<float> rho_alt, alpha_alt, kappa_alt, delta_alt, fst_alt;
     // boundary: bpos = (x, y, z, r), bvel = (vx, vy, vz, m)
     DBuf<float4> bpos, bvel, bcolor;
     DBuf<float> bvol, bS;
@@ -177,7 +183,8 @@ struct mps_ctx {
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
-    DBuf<float2> pairAB;
+    DBuf<float2> pairAB;  // (a_ij, b_ij) for the adjoint's transposed operator
+    DBuf<float> pairA;    // a_ij for the forward operator (b_ij is recomputed in A.p)
     DBuf<float4> cgrecA, cgrecB;  // 32-byte (position, vector) records for the CG gathers
     bool pairs_valid = false;
     uint32_t nei_regrow = 0;
@@ -259,6 +266,50 @@ struct mps_ctx {
     void prof_reset() {
         prof_collect();
         for (int k = 0; k < K_COUNT; k++) { kms[k] = 0.0; kn[k] = 0; kuse[k] = 0; }
+    }
+
+    // ------------------------------------------------------------------ graph replay
+    // A solver chunk is a fixed sequence of launches whose arguments do not depend on the
+    // iteration, so it is captured once per argument set and replayed.
+    struct GraphEntry {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::map<std::vector<uint64_t>, GraphEntry> graphs;
+    bool graphs_on() const { return cfg.use_graphs && !prof_on && !rec; }
+    void clear_graphs() {
+        for (auto& kv : graphs) {
+            if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
+            if (kv.second.graph) (void)hipGraphDestroy(kv.second.graph);
+        }
+        graphs.clear();
+    }
+    template <class F>
+    void launch_chunk(std::vector<uint64_t> key, F&& f) {
+        if (!graphs_on()) {
+            f();
+            return;
+        }
+        auto it = graphs.find(key);
+        if (it == graphs.end()) {
+            if (graphs.size() >= 64) clear_graphs();
+            GraphEntry e;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            f();
+            HIPCHK(hipStreamEndCapture(stream, &e.graph));
+            HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
+            it = graphs.emplace(std::move(key), e).first;
+        }
+        HIPCHK(hipGraphLaunch(it->second.exec, stream));
+    }
+    static uint64_t u(const void* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)); }
+    static uint64_t u(double d) {
+        uint64_t b;
+        std::memcpy(&b, &d, sizeof(b));
+        return b;
+    }
+    std::vector<uint64_t> key_base(uint64_t kind, uint32_t chunk, const mps::Scenes& P, const mps::Nbr& nb) const {
+        return {kind, chunk, u(P.P), u(P.blk_scene), P.N, P.S, u(nb.ff), u(nb.cff), u(nb.fb), u(nb.cfb)};
     }
 
     // ------------------------------------------------------------------ helpers
@@ -490,18 +541,18 @@ struct mps_ctx {
         T_(K_REORDER, [&] {
             // sorted position q -> slot; perm, src (old slot) and the key per slot
             mps::launch::place(h.key.p, h.id.p, Nref, mb, d_delta.p, inv.p, perm.p, src.p, mkey.p, stream);
-            auto mv4 = [&](DBuf<float4>& a) {
-                mps::launch::gather_f4(src.p, N, a.p, tmp4.p, stream);
-                std::swap(a, tmp4);
+            auto mv4 = [&](DBuf<float4>& a, DBuf<float4>& alt) {
+                alt.reserve(N);
+                mps::launch::gather_f4(src.p, N, a.p, alt.p, stream);
+                std::swap(a, alt);
             };
-            auto mv1 = [&](DBuf<float>& a) {
-                mps::launch::gather_f1(src.p, N, a.p, tmp1.p, stream);
-                std::swap(a, tmp1);
+            auto mv1 = [&](DBuf<float>& a, DBuf<float>& alt) {
+                alt.reserve(N);
+                mps::launch::gather_f1(src.p, N, a.p, alt.p, stream);
+                std::swap(a, alt);
             };
-            tmp4.reserve(N);
-            tmp1.reserve(N);
-            mv4(pos); mv4(vel); mv4(color); mv4(cgb);
-            mv1(rho); mv1(alpha); mv1(kappa); mv1(delta); mv1(fst);
+            mv4(pos, pos_alt); mv4(vel, vel_alt); mv4(color, color_alt); mv4(cgb, cgb_alt);
+            mv1(rho, rho_alt); mv1(alpha, alpha_alt); mv1(kappa, kappa_alt); mv1(delta, delta_alt); mv1(fst, fst_alt);
             HIPCHK(hipMemcpyAsync(hash_perm.p, h.id.p, sizeof(uint32_t) * Nref, hipMemcpyDeviceToDevice, stream));
             mps::launch::invert(perm.p, N, inv.p, stream);
         });
@@ -652,17 +703,29 @@ struct mps_ctx {
         while (it <= cap && !done) {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[idx], lmin + 1);
             chunk = std::min(chunk, cap - it + 1);
-            for (uint32_t c = 0; c < chunk; c++, it++) {
-                T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
-                    mps::launch::pressure_eval(df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p, pairG_ff.p,
-                                               pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
-                }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, it, tol, lmin, stream); }, false);
-                T_(K_APPLY, [&] {
-                    mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd, stream);
-                }, false);
-                rec_snap();
-            }
+            std::vector<uint64_t> key = key_base(df ? 1 : 2, chunk, P, nbr());
+            for (const void* q : {(const void*)pos.p, (const void*)vel.p, (const void*)bpos.p, (const void*)bvol.p,
+                                  (const void*)bvel.p, (const void*)rho.p, (const void*)alpha.p, (const void*)pairG_ff.p,
+                                  (const void*)pairG_fb.p, (const void*)kappa.p, (const void*)delta.p, (const void*)sd,
+                                  (const void*)partials.p})
+                key.push_back(u(q));
+            key.push_back(u(tol));
+            key.push_back(lmin);
+            launch_chunk(std::move(key), [&] {
+                for (uint32_t c = 0; c < chunk; c++) {
+                    T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
+                        mps::launch::pressure_eval(df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
+                                                   pairG_ff.p, pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
+                    }, false);
+                    T_(K_FOLD, [&] { mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, tol, lmin, stream); }, false);
+                    T_(K_APPLY, [&] {
+                        mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd,
+                                                    stream);
+                    }, false);
+                    rec_snap();
+                }
+            });
+            it += chunk;
             done = fetch_active(idx) == 0;
         }
         uint32_t l = 0;
@@ -711,36 +774,47 @@ struct mps_ctx {
         ensure_pairs();
         const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
-        pairAB.reserve(size_t(Kff) * m);
+        pairA.reserve(size_t(Kff) * m);
         cgrecA.reserve(2 * m);
         cgrecB.reserve(2 * m);
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
         T_(K_CG_SETUP, [&] {
-            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, pairAB.p, cgb.p, stream);
+            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, pairA.p, nullptr, cgb.p, stream);
         });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_pack(N, pos.p, cgx.p, cgrecA.p, stream);
-            mps::launch::cg_apply(0, P, nbr(), pairAB.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p, stream);
+            mps::launch::cg_pack(N, pos.p, cgx.p, vel.p, rho.p, cgrecA.p, stream);
+            mps::launch::cg_apply(0, cubic(), P, nbr(), pairA.p, pos.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p,
+                                  stream);
         });
-        T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream); });
+        T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
         uint32_t it = 1;
         bool done = false;
         while (it <= cap && !done) {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
             chunk = std::min(chunk, cap - it + 1);
-            for (uint32_t c = 0; c < chunk; c++, it++) {
-                T_(K_CG_APPLY, [&] {
-                    mps::launch::cg_apply(1, P, nbr(), pairAB.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd, partials.p,
-                                          stream);
-                }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, it, tol, stream); }, false);
-                T_(K_CG_UPDATE, [&] { mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
-                T_(K_FOLD, [&] { mps::launch::fold_cg(2, P, partials.p, sd, act, it, tol, stream); }, false);
-                T_(K_CG_DIR, [&] { mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream); }, false);
-            }
+            std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
+            for (const void* q : {(const void*)pairA.p, (const void*)pos.p, (const void*)cgrecB.p, (const void*)cgAp.p,
+                                  (const void*)cgb.p, (const void*)cgr.p, (const void*)cgx.p, (const void*)sd,
+                                  (const void*)partials.p, (const void*)act})
+                key.push_back(u(q));
+            key.push_back(u(tol));
+            key.push_back(uint64_t(cubic()));
+            launch_chunk(std::move(key), [&] {
+                for (uint32_t c = 0; c < chunk; c++) {
+                    T_(K_CG_APPLY, [&] {
+                        mps::launch::cg_apply(1, cubic(), P, nbr(), pairA.p, pos.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd,
+                                              partials.p, stream);
+                    }, false);
+                    T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream); }, false);
+                    T_(K_CG_UPDATE, [&] { mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
+                    T_(K_FOLD, [&] { mps::launch::fold_cg(2, P, partials.p, sd, act, tol, stream); }, false);
+                    T_(K_CG_DIR, [&] { mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream); }, false);
+                }
+            });
+            it += chunk;
             done = fetch_active(2) == 0;
         }
         uint32_t l = 0;
@@ -923,26 +997,35 @@ struct mps_ctx {
         mps::launch::solver_reset(sd, S, act, stream);
         if (lam_valid) {
             cgrecA.reserve(2 * std::max<uint32_t>(N, 1));
-            mps::launch::cg_pack(N, p0, lam.p, cgrecA.p, stream);
+            mps::launch::cg_pack(N, p0, lam.p, nullptr, nullptr, cgrecA.p, stream);
             mps::launch::adj_cg_init_warm(P, nb, pairAB.p, cabT.p, cgrecA.p, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p,
                                           stream);
             mps::launch::adj_fold_start(P, partials.p, sd, act, tol, stream);
         } else {
             mps::launch::adj_cg_init(P, p0, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p, stream);
-            mps::launch::fold_cg(0, P, partials.p, sd, act, 0, tol, stream);
+            mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream);
         }
         const uint32_t cap = cfg.adj_cg_max_iter ? cfg.adj_cg_max_iter : 300u;
         uint32_t it = 1;
         bool done = fetch_active(2) == 0;
         while (it <= cap && !done) {
             const uint32_t chunk = std::min(std::max<uint32_t>(pred[2], 1), cap - it + 1);
-            for (uint32_t c = 0; c < chunk; c++, it++) {
-                mps::launch::adj_cg_applyT(P, nb, pairAB.p, cabT.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
-                mps::launch::fold_cg(1, P, partials.p, sd, act, it, tol, stream);
-                mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
-                mps::launch::fold_cg(2, P, partials.p, sd, act, it, tol, stream);
-                mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream);
-            }
+            std::vector<uint64_t> key = key_base(4, chunk, P, nb);
+            for (const void* q : {(const void*)pairAB.p, (const void*)cabT.p, (const void*)cgrecB.p, (const void*)cgAp.p,
+                                  (const void*)cgx.p, (const void*)cgr.p, (const void*)sd, (const void*)partials.p,
+                                  (const void*)act})
+                key.push_back(u(q));
+            key.push_back(u(tol));
+            launch_chunk(std::move(key), [&] {
+                for (uint32_t c = 0; c < chunk; c++) {
+                    mps::launch::adj_cg_applyT(P, nb, pairAB.p, cabT.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
+                    mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream);
+                    mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream);
+                    mps::launch::fold_cg(2, P, partials.p, sd, act, tol, stream);
+                    mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream);
+                }
+            });
+            it += chunk;
             done = fetch_active(2) == 0;
         }
     }
@@ -985,7 +1068,7 @@ struct mps_ctx {
             pairAB.reserve(size_t(T.Kff) * m); cabT.reserve(size_t(T.Kff) * m); rev.reserve(size_t(T.Kff) * m);
             cgrecB.reserve(2 * m);
             mps::launch::st_factor(P, nb, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
-            mps::launch::cg_setup(1, P, nb, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, pairAB.p, cgb.p, stream);
+            mps::launch::cg_setup(1, P, nb, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
             mps::launch::zero_words(d_flags + 2, 1, stream);
             mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
             mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);
@@ -1071,6 +1154,48 @@ struct mps_ctx {
         }
         restore_ck(ck_end);
         sync();
+    }
+
+    // ------------------------------------------------------------------ autotuning
+    void autotune(uint32_t steps, mps_tune_result* out) {
+        if (N == 0 || steps == 0) throw ApiError{MPS_ERR_INVALID, "nothing to tune"};
+        sync_params();
+        Checkpoint c;
+        save_ck(c);
+        const uint32_t polls[4] = {0u, 1u, 4u, 16u};
+        double best = 1e300;
+        uint32_t bg = cfg.use_graphs, bh = cfg.host_sync_every, n = 0;
+        mps_step_stats st;
+        for (uint32_t g = 0; g < 2; g++)
+            for (uint32_t k = 0; k < 4; k++) {
+                cfg.use_graphs = g;
+                cfg.host_sync_every = polls[k];
+                restore_ck(c);
+                step(&st);  // captures graphs, sizes buffers
+                restore_ck(c);
+                sync();
+                const auto t0 = std::chrono::steady_clock::now();
+                for (uint32_t q = 0; q < steps; q++) step(&st);
+                sync();
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+                if (out) out->ms_per_step[n] = ms;
+                n++;
+                if (ms < best) {
+                    best = ms;
+                    bg = g;
+                    bh = polls[k];
+                }
+            }
+        cfg.use_graphs = bg;
+        cfg.host_sync_every = bh;
+        restore_ck(c);
+        sync();
+        c.release();
+        if (out) {
+            out->use_graphs = bg;
+            out->host_sync_every = bh;
+            out->n_tried = n;
+        }
     }
 
     void begin_stats() {
@@ -1694,14 +1819,17 @@ struct mps_ctx {
         pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release();
         cgAp.release(); jprev.release(); for (auto& t : jt) t.release(); jcand.release(); tmp4.release();
         rho.release(); alpha.release(); kappa.release(); delta.release(); fst.release(); tmp1.release();
+        pos_alt.release(); vel_alt.release(); color_alt.release(); cgb_alt.release(); rho_alt.release();
+        alpha_alt.release(); kappa_alt.release(); delta_alt.release(); fst_alt.release();
         bpos.release(); bvel.release(); bcolor.release(); bvol.release(); bS.release(); bface.release(); bbcc.release();
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
-        pairG_ff.release(); pairG_fb.release(); pairAB.release(); cgrecA.release(); cgrecB.release();
+        pairG_ff.release(); pairG_fb.release(); pairAB.release(); pairA.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();
+        clear_graphs();
         for (auto& t : tape) t.release();
         for (auto& c : ckpts) c.release();
         ck_end.release();
@@ -2074,6 +2202,10 @@ int mps_scene_set_material(mps_ctx* c, uint32_t scene, const mps_sph_material* m
     h.mass = m->density * h.V;
     c->params_dirty = true;
     return MPS_OK;
+}
+
+int mps_autotune(mps_ctx* c, uint32_t steps, mps_tune_result* out) {
+    return guarded(c, [&] { c->autotune(steps, out); });
 }
 
 int mps_adj_rollout(mps_ctx* c, uint32_t n_steps, uint32_t checkpoint_every, uint32_t* loop_counts) {

@@ -124,24 +124,25 @@ void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, c
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
 // one workgroup per scene; `active` counts the scenes whose loop has not finished
-void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
-                   uint32_t lmin, hipStream_t s);
+void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
+                   hipStream_t s);
 void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s);
 void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s);
+// pair coefficients of the implicit operator: a_ij into ca and/or (a_ij, b_ij) into cab (either may be null)
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
-              const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s);
-// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
-void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
+              const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
+// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x),
+// rec[2i+1] = (p.y, p.z, m, rho) (m from vel.w, rho; zero where those are null)
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, const float4* vel, const float* rho, float4* rec, hipStream_t s);
 // mode 0: A x0 with f = 0 (x0 in `in`), then r = b - A x0, p = r (into `out`), partial r.r;
-// mode 1: A p with f (p in `in`), partial p.Ap
-void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
-              const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
+// mode 1: A p with f (p in `in`), partial p.Ap. b_ij is recomputed from the records.
+void cg_apply(int mode, int cubic, const Scenes& sc, const Nbr& nb, const float* ca, const float4* pos, const float4* in,
+              float4* out, float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
 // mode 0: initial r.r + test; 1: alpha; 2: new r.r, beta + test
-void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
-             hipStream_t s);
+void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s);
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);

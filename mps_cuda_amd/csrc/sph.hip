@@ -267,12 +267,14 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, con
 // Mean positive error and the loop decision of the DF/CD drivers
 // (MPSSPHDFSPHUtil.cu:331-333,417-419): break when E < tol and l > lmin. Workgroup s folds
 // scene s's partials; a scene that finishes leaves the count of active scenes.
+// The iteration number is the record's l (1 after the reset, it + 1 after iteration it), so the
+// launches of a chunk do not depend on where the chunk starts.
 __global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(Scenes sc, const double* __restrict__ partials,
-                                                        SolverDev* st, uint32_t* active, uint32_t it, double tol,
-                                                        uint32_t lmin) {
+                                                        SolverDev* st, uint32_t* active, double tol, uint32_t lmin) {
     const uint32_t sid = blockIdx.x;
     SolverDev* sd = st + sid;
     if (sd->done) return;
+    const uint32_t it = sd->l;
     const StepParams P = sc.P[sid];
     const double s = fold(partials + P.blk0, P.nblk);
     if (threadIdx.x == 0) {
@@ -419,8 +421,8 @@ __device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                    const float* __restrict__ rho, const float* __restrict__ fst,
-                                                   const float* __restrict__ pG_ff, float2* __restrict__ cab,
-                                                   float4* __restrict__ cgb) {
+                                                   const float* __restrict__ pG_ff, float* __restrict__ ca,
+                                                   float2* __restrict__ cab, float4* __restrict__ cgb) {
     const uint32_t sid = sc.blk_scene[blockIdx.x];
     const StepParams P = sc.P[sid];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -447,7 +449,9 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const fl
         const float sij = mij * gij;
         const float wij = mj / dj * Wk<CUBIC>(dist, invHi);
         const float cij = 0.5f * (mi + mj) / dj / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
-        cab[slot] = make_float2(P.dt * (cs * sij + cx * wij), -P.dt * cv * cij);
+        const float aij = P.dt * (cs * sij + cx * wij);
+        if (ca) ca[slot] = aij;
+        if (cab) cab[slot] = make_float2(aij, -P.dt * cv * cij);
         bi = bi - sij * xij;
     };
     uint32_t k = 0;
@@ -482,27 +486,31 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const fl
 }
 
 // CG vectors that neighbours read are kept in 32-byte records with the position, so one
-// gather per pair brings both x_j and p_j (one cache line): rec[2j] = (x, y, z, p.x),
-// rec[2j+1] = (p.y, p.z, 0, 0).
+// gather per pair brings x_j, p_j and what the viscosity coefficient needs of j (one 32-byte
+// sector): rec[2j] = (x, y, z, p.x), rec[2j+1] = (p.y, p.z, m_j, rho_j).
 __device__ __forceinline__ f3 rec_p(float4 a, float4 b) { return {a.w, b.x, b.y}; }
 __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __restrict__ pos, const float4* __restrict__ x0,
+                                                  const float4* __restrict__ vel, const float* __restrict__ rho,
                                                   float4* __restrict__ rec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const float4 x = pos[i], v = x0[i];
     rec[2 * size_t(i)] = make_float4(x.x, x.y, x.z, v.x);
-    rec[2 * size_t(i) + 1] = make_float4(v.y, v.z, 0.0f, 0.0f);
+    rec[2 * size_t(i) + 1] = make_float4(v.y, v.z, vel ? vel[i].w : 0.0f, rho ? rho[i] : 0.0f);
 }
 
 // (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ].
+// a_ij is streamed (written once per solve by k_cg_setup, read with non-temporal loads so it
+// does not evict the gathered records); b_ij = -dt cv_i c_ij is recomputed from the record of j
+// by k_cg_setup's own expression (same bits), which saves 4 bytes of stream per pair.
 // MODE 0: A x0 with f = 0 (x0 in `in`) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
 //         r = p = b - A x0 (p written to `out`), partial sums of r.r.
 // MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
-template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const float2* __restrict__ cab,
-                                                   const float4* __restrict__ in, float4* __restrict__ out,
-                                                   float4* __restrict__ Ap, const float4* __restrict__ b,
-                                                   float4* __restrict__ r, const SolverDev* st,
+template <int MODE, bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const float* __restrict__ ca,
+                                                   const float4* __restrict__ pos, const float4* __restrict__ in,
+                                                   float4* __restrict__ out, float4* __restrict__ Ap,
+                                                   const float4* __restrict__ b, float4* __restrict__ r, const SolverDev* st,
                                                    double* __restrict__ partials) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t sid = sc.blk_scene[blk];
@@ -515,21 +523,35 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
         const float4 ra = in[2 * size_t(i)], rb = in[2 * size_t(i) + 1];
         const f3 xi = mk(ra);
         const f3 pi = rec_p(ra, rb);
+        const float mi = rb.z, di = rb.w;
+        const float hi = pos[i].w * 4.0f;
+        const float invHi = 1.0f / hi;
+        const float onePercentHSq = 0.01f * hi * hi;
+        const float cv = 10.0f * P.rho0 * P.mu / di;
+        const float bfac = -P.dt * cv;
         f3 acc{0.f, 0.f, 0.f};
+        auto pair = [&](float a, float4 ja, float4 jb) {
+            const f3 xij = xi - mk(ja);
+            const f3 pij = pi - rec_p(ja, jb);
+            const float dist = len3(xij);
+            const float G = Gk<CUBIC>(dist, invHi);
+            const float cij = 0.5f * (mi + jb.z) / jb.w / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
+            const float bij = bfac * cij;
+            acc = acc + (a * pij + (bij * dot3(pij, xij)) * xij);
+        };
         const uint32_t c = nb.cff[i];
         uint32_t k = 0;
         // four neighbours per trip: all index/coefficient loads, then all gathers, then the
-        // sums in list order (same result as the one-at-a-time loop). Four keeps the kernel at
-        // 56 VGPRs (8 waves per SIMD); eight (96 VGPRs) and sixteen measured slower.
+        // sums in list order (same result as the one-at-a-time loop)
         constexpr int U = 4;
         for (; k + U <= c; k += U) {
             uint32_t j[U];
-            float2 ab[U];
+            float a[U];
 #pragma unroll
             for (int q = 0; q < U; q++) {
                 const size_t slot = size_t(k + q) * N + i;
-                j[q] = nb.ff[slot];
-                ab[q] = cab[slot];
+                j[q] = __builtin_nontemporal_load(nb.ff + slot);
+                a[q] = __builtin_nontemporal_load(ca + slot);
             }
             float4 ja[U], jb[U];
 #pragma unroll
@@ -538,20 +560,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
                 jb[q] = in[2 * size_t(j[q]) + 1];
             }
 #pragma unroll
-            for (int q = 0; q < U; q++) {
-                const f3 xij = xi - mk(ja[q]);
-                const f3 pij = pi - rec_p(ja[q], jb[q]);
-                acc = acc + (ab[q].x * pij + (ab[q].y * dot3(pij, xij)) * xij);
-            }
+            for (int q = 0; q < U; q++) pair(a[q], ja[q], jb[q]);
         }
         for (; k < c; k++) {
             const size_t slot = size_t(k) * N + i;
             const uint32_t j = nb.ff[slot];
-            const float2 ab = cab[slot];
-            const float4 ja = in[2 * size_t(j)], jb = in[2 * size_t(j) + 1];
-            const f3 xij = xi - mk(ja);
-            const f3 pij = pi - rec_p(ja, jb);
-            acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
+            pair(ca[slot], in[2 * size_t(j)], in[2 * size_t(j) + 1]);
         }
         const float f = MODE == 0 ? 0.0f : P.cg_factor;
         const f3 o = pi * (1.0f + f) + acc;
@@ -560,7 +574,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
             const f3 ri{bi.x - o.x, bi.y - o.y, bi.z - o.z};
             r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
             out[2 * size_t(i)] = make_float4(xi.x, xi.y, xi.z, ri.x);
-            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, 0.0f, 0.0f);
+            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, mi, di);
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
             Ap[i] = make_float4(o.x, o.y, o.z, 0.0f);
@@ -573,12 +587,14 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
 // Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
 // first convergence test; 1: alpha = r.r / p.Ap; 2: r.r of the new residual, beta, test.
 // Workgroup s folds scene s's partials.
+// MODE 2 takes the iteration number from the record's l (it, when iteration it is folded).
 template <int MODE>
 __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const double* __restrict__ partials, SolverDev* stv,
-                                                  uint32_t* active, uint32_t it, double tol) {
+                                                  uint32_t* active, double tol) {
     const uint32_t sid = blockIdx.x;
     SolverDev* st = stv + sid;
     if (MODE != 0 && st->done) return;
+    const uint32_t it = st->l;
     const StepParams P = sc.P[sid];
     const double s = fold(partials + P.blk0, P.nblk);
     if (threadIdx.x != 0) return;
@@ -900,9 +916,9 @@ void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, c
     if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
     else k_pressure_eval<false><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
-void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
-                   uint32_t lmin, hipStream_t s) {
-    if (sc.N) k_fold_pressure<<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol, lmin);
+void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
+                   hipStream_t s) {
+    if (sc.N) k_fold_pressure<<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol, lmin);
 }
 void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
@@ -914,27 +930,31 @@ void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4*
     if (sc.N) k_st_factor<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, pG_ff, fst, cgx);
 }
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
-              const float* fst, const float* pG_ff, float2* cab, float4* cgb, hipStream_t s) {
+              const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
-    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, cab, cgb);
+    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
 }
-void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
-    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, const float4* vel, const float* rho, float4* rec, hipStream_t s) {
+    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, vel, rho, rec);
 }
-void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const float2* cab, const float4* in, float4* out, float4* Ap,
-              const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
+void cg_apply(int mode, int cubic, const Scenes& sc, const Nbr& nb, const float* ca, const float4* pos, const float4* in,
+              float4* out, float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, cab, in, out, Ap, b, r, st, partials);
-    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, cab, in, out, Ap, b, r, st, partials);
+    if (cubic) {
+        if (mode == 0) k_cg_apply<0, true><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
+        else k_cg_apply<1, true><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
+    } else {
+        if (mode == 0) k_cg_apply<0, false><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
+        else k_cg_apply<1, false><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
+    }
 }
-void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, uint32_t it, double tol,
-             hipStream_t s) {
+void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;
-    if (mode == 0) k_fold_cg<0><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
-    else if (mode == 1) k_fold_cg<1><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
-    else k_fold_cg<2><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, it, tol);
+    if (mode == 0) k_fold_cg<0><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
+    else if (mode == 1) k_fold_cg<1><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
+    else k_fold_cg<2><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
 }
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s) {

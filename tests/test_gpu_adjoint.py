@@ -159,3 +159,19 @@ def test_adjoint_matches_fp64_finite_differences():
     say(f"[adjoint] 4096 x {steps} steps vs FP64 FD: mu {gmu[0]:.6e}/{fd_mu:.6e}  sigma {gsig[0]:.6e}/{fd_sig:.6e}  "
         f"v0 {ad_v:.6e}/{fd_v:.6e}  rel errs {errs}")
     assert max(errs) < 1e-3, errs
+
+
+def test_adjoint_with_graph_replay():
+    sc = scenes.diff_droplet(n=300, radius=0.08, seed=3, steps=STEPS, h=0.04)
+    w = np.random.default_rng(4).normal(size=(sc.n, 3)).astype(np.float32)
+    out = []
+    for graphs in (0, 1):
+        cfg = scenes.make_config(sc)
+        cfg.use_graphs = graphs
+        g = HipSim(cfg)
+        scenes.load_into(g, sc)
+        g.set("FLUID_VEL", sc.extra["v0"])
+        g.adj_rollout(STEPS, 3)
+        out.append(g.adj_backward(w))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
