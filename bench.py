@@ -263,13 +263,14 @@ def main() -> int:
         sim.profile(True)
 
     N = scene.n
-    stats = []
+    stats, halo = [], []
     sim.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         st = sim.step()
         stats.append((st.l_df, st.l_cd, st.l_cg, st.nnz_ff, st.nnz_fb, st.nnz_bb, st.hit_cap_cd,
                       st.err_cd[max(0, min(st.l_cd, 128) - 1)], st.nonfinite, st.n_boundary))
+        halo.append((st.halo_max, st.halo_spill, st.halo_blocks))
     sim.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -332,6 +333,10 @@ def main() -> int:
         "l_cg_mean": sum(s[2] for s in stats) / n, "cd_capped_steps": sum(s[6] for s in stats),
         "cd_final_err_mean": sum(s[7] for s in stats) / n, "nonfinite_steps": sum(s[8] for s in stats),
         "nnz_ff_last": stats[-1][3], "nnz_fb_last": stats[-1][4], "nnz_bb": stats[-1][5],
+        # LDS halos of the viscosity operator: largest staged halo, and the share of 256-slot
+        # blocks over the LDS cap (those gather from global memory)
+        "halo_max": max(h[0] for h in halo),
+        "halo_spill_frac": sum(h[1] for h in halo) / max(sum(h[2] for h in halo), 1),
     }
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,

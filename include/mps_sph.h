@@ -164,7 +164,10 @@ typedef struct mps_config {
     uint32_t adj_cg_max_iter;    /* adjoint: cap of the transposed viscosity solve (300) */
     uint32_t use_graphs;         /* HIP library: replay solver iteration chunks from cached HIP
                                     graphs (same results; not while timing kernels) */
-    uint32_t reserved[5];
+    uint32_t halo_limit;         /* HIP library: largest block halo of the viscosity operator staged in
+                                    LDS; blocks with more gather from global memory. 0 = the LDS
+                                    capacity (1264); smaller values exist to test the global path */
+    uint32_t reserved[4];
     double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
 } mps_config;
 
@@ -189,6 +192,13 @@ typedef struct mps_step_stats {
     double err_df[MPS_MAX_ITER_LOG];  /* mean positive error of each DF evaluation */
     double err_cd[MPS_MAX_ITER_LOG];
     double rr_cg[MPS_MAX_ITER_LOG];   /* r.r of each CG iteration (or Jacobi error) */
+    /* block halos of the viscosity operator (no reference counterpart): largest halo staged in
+     * LDS, 256-slot blocks over the LDS cap (these gather from global memory), blocks with
+     * particles; zero when the CG did not run */
+    uint32_t halo_max;
+    uint32_t halo_spill;
+    uint32_t halo_blocks;
+    uint32_t halo_pad;
 } mps_step_stats;
 
 typedef struct mps_ctx mps_ctx;

@@ -51,7 +51,7 @@ class Config(C.Structure):
         ("cg_max_iter", C.c_uint32), ("cg_factor", C.c_double), ("jacobi_max_iter", C.c_uint32),
         ("jacobi_tol", C.c_double), ("zsort_interval", C.c_uint32), ("nei_capacity", C.c_uint32),
         ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("halo_limit", C.c_uint32), ("reserved", C.c_uint32 * 4),
         ("adj_cg_tol", C.c_double),
     ]
 
@@ -78,6 +78,7 @@ class StepStats(C.Structure):
         ("n_boundary", C.c_uint32), ("nei_regrow", C.c_uint32), ("nnz_ff", C.c_uint64),
         ("nnz_fb", C.c_uint64), ("nnz_bb", C.c_uint64), ("err_df", C.c_double * MAX_ITER_LOG),
         ("err_cd", C.c_double * MAX_ITER_LOG), ("rr_cg", C.c_double * MAX_ITER_LOG),
+        ("halo_max", C.c_uint32), ("halo_spill", C.c_uint32), ("halo_blocks", C.c_uint32), ("halo_pad", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmps_sph.so")
 
-SOURCES = ["hash.hip", "sph.hip", "sampling.hip", "adjoint.hip", "capi.cpp"]
+SOURCES = ["hash.hip", "sph.hip", "halo.hip", "sampling.hip", "adjoint.hip", "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPS_OFFLOAD_ARCH", "gfx950")

@@ -183,8 +183,11 @@ struct mps_ctx {
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
-    DBuf<float2> pairAB;  // (a_ij, b_ij) for the adjoint's transposed operator
-    DBuf<float> pairA;    // a_ij for the forward operator (b_ij is recomputed in A.p)
+    DBuf<float2> pairAB;  // (a_ij, b_ij) of the implicit viscosity operator
+    // block halos of the fluid lists for A.p (halo.hip): sorted distinct neighbour slots per
+    // workgroup, their count, and every list entry as a 16-bit position in its block's halo
+    DBuf<uint32_t> halo_idx, halo_n;
+    DBuf<uint16_t> nb16;
     DBuf<float4> cgrecA, cgrecB;  // 32-byte (position, vector) records for the CG gathers
     bool pairs_valid = false;
     uint32_t nei_regrow = 0;
@@ -196,7 +199,7 @@ struct mps_ctx {
     uint32_t solver_cap = 0;        // scenes the two arrays above hold
     uint32_t* d_active = nullptr;   // [3]: scenes whose DF / CD / CG loop is still running
     uint32_t* h_active = nullptr;
-    uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch
+    uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch, [4..6] halo statistics of the step
     uint32_t* h_flags = nullptr;
     double* d_err3 = nullptr;
     double* h_err3 = nullptr;
@@ -209,14 +212,14 @@ struct mps_ctx {
     enum KId {
         K_KEYS, K_SORT, K_CELLS, K_NEI_FLUID, K_NEI_BND, K_BSUM, K_BVOL, K_DENSITY, K_EVAL_DF, K_EVAL_CD,
         K_APPLY, K_ST_FACTOR, K_CG_SETUP, K_CG_INIT, K_CG_APPLY, K_CG_UPDATE, K_CG_DIR, K_COPY_OUT, K_ADVECT,
-        K_ZSORT, K_JACOBI, K_FOLD, K_REORDER, K_COUNT
+        K_ZSORT, K_JACOBI, K_FOLD, K_REORDER, K_HALO, K_COUNT
     };
     static const char* kname(int k) {
         static const char* names[K_COUNT] = {
             "k_cell_keys", "radix_sort", "k_cell_table", "k_neighbors_fluid", "k_neighbors_boundary",
             "k_boundary_sum", "k_boundary_volume", "k_density_alpha", "k_pressure_eval<DF>", "k_pressure_eval<CD>",
             "k_pressure_apply", "k_st_factor", "k_cg_setup", "k_cg_apply<0>", "k_cg_apply<1>", "k_cg_update",
-            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi", "k_fold", "reorder"};
+            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi", "k_fold", "reorder", "k_halo_build"};
         return names[k];
     }
     struct Rec {
@@ -774,19 +777,24 @@ struct mps_ctx {
         ensure_pairs();
         const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
-        pairA.reserve(size_t(Kff) * m);
+        pairAB.reserve(size_t(Kff) * m);
         cgrecA.reserve(2 * m);
         cgrecB.reserve(2 * m);
+        const uint32_t nblk = uint32_t((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock);
+        nb16.reserve(size_t(Kff) * m);
+        halo_idx.reserve(size_t(std::max<uint32_t>(nblk, 1)) * mps::kHaloCap);
+        halo_n.reserve(std::max<uint32_t>(nblk, 1));
+        const mps::Halo hl{halo_idx.p, halo_n.p, nb16.p};
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
         T_(K_CG_SETUP, [&] {
-            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, pairA.p, nullptr, cgb.p, stream);
+            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
         });
+        T_(K_HALO, [&] { mps::launch::halo_build(P, nbr(), nblk, cfg.halo_limit, halo_idx.p, halo_n.p, nb16.p, d_flags + 4, stream); });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_pack(N, pos.p, cgx.p, vel.p, rho.p, cgrecA.p, stream);
-            mps::launch::cg_apply(0, cubic(), P, nbr(), pairA.p, pos.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p,
-                                  stream);
+            mps::launch::cg_pack(N, pos.p, cgx.p, cgrecA.p, stream);
+            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p, stream);
         });
         T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
@@ -796,17 +804,17 @@ struct mps_ctx {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
             chunk = std::min(chunk, cap - it + 1);
             std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
-            for (const void* q : {(const void*)pairA.p, (const void*)pos.p, (const void*)cgrecB.p, (const void*)cgAp.p,
+            for (const void* q : {(const void*)pairAB.p, (const void*)cgrecB.p, (const void*)cgAp.p,
                                   (const void*)cgb.p, (const void*)cgr.p, (const void*)cgx.p, (const void*)sd,
-                                  (const void*)partials.p, (const void*)act})
+                                  (const void*)partials.p, (const void*)act, (const void*)hl.idx, (const void*)hl.n,
+                                  (const void*)hl.nb16})
                 key.push_back(u(q));
             key.push_back(u(tol));
-            key.push_back(uint64_t(cubic()));
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
                     T_(K_CG_APPLY, [&] {
-                        mps::launch::cg_apply(1, cubic(), P, nbr(), pairA.p, pos.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd,
-                                              partials.p, stream);
+                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd, partials.p,
+                                              stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream); }, false);
                     T_(K_CG_UPDATE, [&] { mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
@@ -997,7 +1005,7 @@ struct mps_ctx {
         mps::launch::solver_reset(sd, S, act, stream);
         if (lam_valid) {
             cgrecA.reserve(2 * std::max<uint32_t>(N, 1));
-            mps::launch::cg_pack(N, p0, lam.p, nullptr, nullptr, cgrecA.p, stream);
+            mps::launch::cg_pack(N, p0, lam.p, cgrecA.p, stream);
             mps::launch::adj_cg_init_warm(P, nb, pairAB.p, cabT.p, cgrecA.p, adv.p, cgx.p, cgr.p, cgrecB.p, partials.p,
                                           stream);
             mps::launch::adj_fold_start(P, partials.p, sd, act, tol, stream);
@@ -1236,6 +1244,7 @@ struct mps_ctx {
         st->n_boundary = B;
         const uint32_t regrow0 = nei_regrow;
         mps::launch::zero_words(d_flags, 1, stream);
+        mps::launch::zero_words(d_flags + 4, 4, stream);
         if (cfg.zsort_interval && (frame % cfg.zsort_interval) == 1u) {
             zsort();
             st->zsorted = 1;
@@ -1264,9 +1273,12 @@ struct mps_ctx {
         }
         pressure_solve(false, st);
         advect();
-        HIPCHK(hipMemcpyAsync(h_flags, d_flags, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(h_flags, d_flags, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
         st->nonfinite = h_flags[0] ? 1u : 0u;
+        st->halo_max = h_flags[4];
+        st->halo_spill = h_flags[5];
+        st->halo_blocks = h_flags[6];
         st->nnz_ff = nnz_ff;
         st->nnz_fb = nnz_fb;
         st->nnz_bb = nnz_bb;
@@ -1825,7 +1837,7 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
-        pairG_ff.release(); pairG_fb.release(); pairAB.release(); pairA.release(); cgrecA.release(); cgrecB.release();
+        pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();
@@ -1923,13 +1935,13 @@ int mps_create(const mps_config* cfg, mps_ctx** out) {
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_ctr), sizeof(NbrCounters)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_active), 4 * sizeof(uint32_t)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_active), 4 * sizeof(uint32_t)));
-        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_flags), 4 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_flags), 8 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_err3), 4 * sizeof(double)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ctr), sizeof(NbrCounters)));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 4 * sizeof(uint32_t)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 8 * sizeof(uint32_t)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_err3), 4 * sizeof(double)));
-        std::memset(c->h_flags, 0, 4 * sizeof(uint32_t));
-        mps::launch::zero_words(c->d_flags, 4, c->stream);
+        std::memset(c->h_flags, 0, 8 * sizeof(uint32_t));
+        mps::launch::zero_words(c->d_flags, 8, c->stream);
         c->ensure_solvers();
         c->sync();
     });

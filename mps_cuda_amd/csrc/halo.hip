@@ -1,0 +1,163 @@
+// Block halos of the fluid-fluid neighbour lists.
+//
+// The implicit viscosity operator (A.p) is applied ~30 times per step over the same lists.
+// Gathering the 32-byte CG record of every neighbour from global memory costs most of its
+// time: a block of 256 slots touches ~7k list entries but only ~900 distinct records (its
+// halo), and the caches cannot keep the halos of all resident blocks. This pass, run once
+// per solve, lists for each 256-slot block the distinct neighbour slots in ascending order
+// (at most kHaloCap) and rewrites every list entry of the block as a 16-bit position in that
+// list. A.p then stages the halo's records into LDS with coalesced loads and gathers from
+// LDS. A block with more than kHaloCap distinct neighbours keeps the global path
+// (halo count > kHaloCap marks it).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace mps {
+namespace dev {
+
+constexpr uint32_t kHaloHash = 4096;           // LDS hash slots; occupancy stays below kHaloCap + kBlock
+constexpr uint32_t kHaloSort = 2048;           // power of two >= kHaloCap
+constexpr uint32_t kNoKey = 0xFFFFFFFFu;
+static_assert(kHaloCap <= kHaloSort && kHaloCap + kBlock < kHaloHash, "halo table sizes");
+static_assert(kHaloCap <= 65535u, "local positions are 16-bit");
+
+__device__ __forceinline__ uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32u - 12u); }
+
+// Slot of key j in the table (j must be present).
+__device__ __forceinline__ uint32_t halo_find(const uint32_t* key, uint32_t j) {
+    uint32_t s = halo_hash(j);
+    for (uint32_t probe = 0; probe < kHaloHash; probe++) {
+        if (key[s] == j) break;
+        s = (s + 1u) & (kHaloHash - 1u);
+    }
+    return s;
+}
+
+// Blocks with more than `limit` (<= kHaloCap) distinct neighbours are marked not staged
+// (n = kHaloCap + 1). stats[0] = largest halo among staged blocks, stats[1] = blocks not
+// staged, stats[2] = blocks with particles.
+__global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32_t limit, uint32_t* __restrict__ hidx,
+                                                     uint32_t* __restrict__ hn, uint16_t* __restrict__ n16,
+                                                     uint32_t* __restrict__ stats) {
+    const uint32_t blk = blockIdx.x;
+    const uint32_t sid = sc.blk_scene[blk];
+    const StepParams P = sc.P[sid];
+    const uint32_t N = P.N;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = blk * kBlock + tid;
+    __shared__ uint32_t key[kHaloHash];
+    __shared__ uint16_t at[kHaloHash];
+    __shared__ uint32_t srt[kHaloSort];
+    __shared__ uint32_t scan[kBlock];
+    __shared__ uint32_t cnt;
+    for (uint32_t t = tid; t < kHaloHash; t += kBlock) key[t] = kNoKey;
+    if (tid == 0) cnt = 0;
+    __syncthreads();
+    const uint32_t c = i < P.end ? nb.cff[i] : 0u;
+    // insert; stop as soon as the block is known to exceed the limit (each thread adds at most
+    // one key after the count passes it, so the table never fills and every probe ends).
+    // List entries are loaded kBatch at a time so their latency overlaps.
+    constexpr uint32_t kBatch = 8;
+    for (uint32_t k0 = 0; k0 < c; k0 += kBatch) {
+        uint32_t jb[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; q++)
+            jb[q] = k0 + q < c ? nb.ff[size_t(k0 + q) * N + i] : kNoKey;
+        if (*(volatile uint32_t*)&cnt > limit) break;
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; q++) {
+            const uint32_t j = jb[q];
+            if (j == kNoKey) break;
+            uint32_t s = halo_hash(j);
+            for (uint32_t probe = 0; probe < kHaloHash; probe++) {
+                const uint32_t prev = atomicCAS(&key[s], kNoKey, j);
+                if (prev == kNoKey) {
+                    atomicAdd(&cnt, 1u);
+                    break;
+                }
+                if (prev == j) break;
+                s = (s + 1u) & (kHaloHash - 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t H = cnt;
+    if (H > limit) {
+        if (tid == 0) {
+            hn[blk] = kHaloCap + 1u;
+            atomicAdd(&stats[1], 1u);
+            atomicAdd(&stats[2], 1u);
+        }
+        return;
+    }
+    // compact the occupied slots (each thread owns 16 consecutive slots) ...
+    constexpr uint32_t per = kHaloHash / kBlock;
+    uint32_t mine = 0;
+    for (uint32_t q = 0; q < per; q++) mine += key[tid * per + q] != kNoKey;
+    scan[tid] = mine;
+    __syncthreads();
+    for (uint32_t off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan
+        const uint32_t v = tid >= off ? scan[tid - off] : 0u;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    uint32_t o = scan[tid] - mine;
+    for (uint32_t q = 0; q < per; q++) {
+        const uint32_t kk = key[tid * per + q];
+        if (kk != kNoKey) srt[o++] = kk;
+    }
+    uint32_t P2 = 1;
+    while (P2 < H) P2 <<= 1;
+    for (uint32_t t = H + tid; t < P2; t += kBlock) srt[t] = kNoKey;
+    __syncthreads();
+    // ... and sort them ascending (bitonic network over P2 <= kHaloSort keys)
+    for (uint32_t size = 2; size <= P2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t t = tid; t < P2 / 2; t += kBlock) {
+                const uint32_t lo = 2u * t - (t & (stride - 1u));
+                const uint32_t hi = lo + stride;
+                const uint32_t a = srt[lo], b = srt[hi];
+                if ((a > b) == ((lo & size) == 0u)) {
+                    srt[lo] = b;
+                    srt[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    uint32_t* out = hidx + size_t(blk) * kHaloCap;
+    for (uint32_t t = tid; t < H; t += kBlock) {
+        const uint32_t j = srt[t];
+        out[t] = j;
+        at[halo_find(key, j)] = static_cast<uint16_t>(t);
+    }
+    if (tid == 0) {
+        hn[blk] = H;
+        if (H) {
+            atomicMax(&stats[0], H);
+            atomicAdd(&stats[2], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k0 = 0; k0 < c; k0 += kBatch) {
+        uint32_t jb[kBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; q++)
+            if (k0 + q < c) jb[q] = nb.ff[size_t(k0 + q) * N + i];
+#pragma unroll
+        for (uint32_t q = 0; q < kBatch; q++)
+            if (k0 + q < c) n16[size_t(k0 + q) * N + i] = at[halo_find(key, jb[q])];
+    }
+}
+
+}  // namespace dev
+
+namespace launch {
+void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, uint32_t* hidx, uint32_t* hn, uint16_t* n16,
+                uint32_t* stats, hipStream_t s) {
+    limit = limit == 0u || limit > kHaloCap ? kHaloCap : limit;
+    if (nblk) dev::k_halo_build<<<nblk, dev::kBlock, 0, s>>>(sc, nb, limit, hidx, hn, n16, stats);
+}
+}  // namespace launch
+}  // namespace mps

@@ -75,6 +75,16 @@ struct Nbr {
     const uint32_t* cfb;
 };
 
+// Block halos of the fluid-fluid lists (halo.hip): for workgroup b of slots, idx[b*kHaloCap + t]
+// (t < n[b]) are the distinct neighbour slots in ascending order and nb16[k*N + i] is the
+// position of nff[k*N + i] in its block's list. n[b] > kHaloCap: not staged (use nff).
+constexpr uint32_t kHaloCap = 1264;
+struct Halo {
+    const uint32_t* idx;
+    const uint32_t* n;
+    const uint16_t* nb16;
+};
+
 namespace launch {
 // ---- hash.hip
 size_t radix_temp_words(uint32_t n);
@@ -134,13 +144,17 @@ void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4*
 // pair coefficients of the implicit operator: a_ij into ca and/or (a_ij, b_ij) into cab (either may be null)
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
-// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x),
-// rec[2i+1] = (p.y, p.z, m, rho) (m from vel.w, rho; zero where those are null)
-void cg_pack(uint32_t N, const float4* pos, const float4* x0, const float4* vel, const float* rho, float4* rec, hipStream_t s);
+// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
 // mode 0: A x0 with f = 0 (x0 in `in`), then r = b - A x0, p = r (into `out`), partial r.r;
-// mode 1: A p with f (p in `in`), partial p.Ap. b_ij is recomputed from the records.
-void cg_apply(int mode, int cubic, const Scenes& sc, const Nbr& nb, const float* ca, const float4* pos, const float4* in,
-              float4* out, float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
+// mode 1: A p with f (p in `in`), partial p.Ap. Halo records are staged in LDS (kHaloCap).
+void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* in, float4* out,
+              float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s);
+// ---- halo.hip
+// stats (3 words, accumulated): largest staged halo, blocks over kHaloCap, blocks with particles
+// limit: largest halo staged (0 or > kHaloCap: kHaloCap)
+void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, uint32_t* hidx, uint32_t* hn, uint16_t* n16,
+                uint32_t* stats, hipStream_t s);
 // mode 0: initial r.r + test; 1: alpha; 2: new r.r, beta + test
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,

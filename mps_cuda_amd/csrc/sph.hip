@@ -486,31 +486,35 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const fl
 }
 
 // CG vectors that neighbours read are kept in 32-byte records with the position, so one
-// gather per pair brings x_j, p_j and what the viscosity coefficient needs of j (one 32-byte
-// sector): rec[2j] = (x, y, z, p.x), rec[2j+1] = (p.y, p.z, m_j, rho_j).
+// gather per pair brings both x_j and p_j (one 32-byte sector): rec[2j] = (x, y, z, p.x),
+// rec[2j+1] = (p.y, p.z, 0, 0).
 __device__ __forceinline__ f3 rec_p(float4 a, float4 b) { return {a.w, b.x, b.y}; }
+__device__ __forceinline__ f3 rec_p(float4 a, float2 b) { return {a.w, b.x, b.y}; }
 __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __restrict__ pos, const float4* __restrict__ x0,
-                                                  const float4* __restrict__ vel, const float* __restrict__ rho,
                                                   float4* __restrict__ rec) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const float4 x = pos[i], v = x0[i];
     rec[2 * size_t(i)] = make_float4(x.x, x.y, x.z, v.x);
-    rec[2 * size_t(i) + 1] = make_float4(v.y, v.z, vel ? vel[i].w : 0.0f, rho ? rho[i] : 0.0f);
+    rec[2 * size_t(i) + 1] = make_float4(v.y, v.z, 0.0f, 0.0f);
 }
 
 // (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ].
-// a_ij is streamed (written once per solve by k_cg_setup, read with non-temporal loads so it
-// does not evict the gathered records); b_ij = -dt cv_i c_ij is recomputed from the record of j
-// by k_cg_setup's own expression (same bits), which saves 4 bytes of stream per pair.
+// (a_ij, b_ij) are written once per solve by k_cg_setup and streamed here with non-temporal
+// loads (recomputing b_ij costs more ALU time than the 4 bytes it saves). The block's halo
+// records (x_j, p_j: 24 bytes) are staged in LDS first, with coalesced loads of the sorted
+// halo list, and each pair reads its neighbour there through the 16-bit list; a block whose
+// halo is over kHaloCap gathers from global memory. Both paths visit the same pairs in the
+// same order, so the result does not depend on the path.
 // MODE 0: A x0 with f = 0 (x0 in `in`) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
 //         r = p = b - A x0 (p written to `out`), partial sums of r.r.
 // MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
-template <int MODE, bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const float* __restrict__ ca,
-                                                   const float4* __restrict__ pos, const float4* __restrict__ in,
-                                                   float4* __restrict__ out, float4* __restrict__ Ap,
-                                                   const float4* __restrict__ b, float4* __restrict__ r, const SolverDev* st,
+constexpr uint32_t kApplyBatch = 4;
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
+                                                   const float4* __restrict__ in, float4* __restrict__ out,
+                                                   float4* __restrict__ Ap, const float4* __restrict__ b,
+                                                   float4* __restrict__ r, const SolverDev* st,
                                                    double* __restrict__ partials) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t sid = sc.blk_scene[blk];
@@ -518,54 +522,70 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
     const StepParams P = sc.P[sid];
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
+    const bool live = i < P.end;
+    const uint32_t c = live ? nb.cff[i] : 0u;
+    __shared__ float4 la[kHaloCap];
+    __shared__ float2 lb[kHaloCap];
+    const uint32_t H = hl.n[blk];
+    const bool staged = H <= kHaloCap;  // same for the whole block
+    // LDS bounds the occupancy of this kernel, not registers: each thread keeps a batch of
+    // kApplyBatch list entries in flight, the first issued before the halo is staged
+    constexpr uint32_t U = kApplyBatch;
+    uint32_t jn[U];
+    float2 abn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
+                abn[q] = __builtin_bit_cast(float2, d);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
+            const size_t s = hs[t];
+            const float4 ra = in[2 * s], rb = in[2 * s + 1];
+            la[t] = ra;
+            lb[t] = make_float2(rb.x, rb.y);
+        }
+        __syncthreads();
+    }
     double part = 0.0;
-    if (i < P.end) {
+    if (live) {
         const float4 ra = in[2 * size_t(i)], rb = in[2 * size_t(i) + 1];
         const f3 xi = mk(ra);
         const f3 pi = rec_p(ra, rb);
-        const float mi = rb.z, di = rb.w;
-        const float hi = pos[i].w * 4.0f;
-        const float invHi = 1.0f / hi;
-        const float onePercentHSq = 0.01f * hi * hi;
-        const float cv = 10.0f * P.rho0 * P.mu / di;
-        const float bfac = -P.dt * cv;
         f3 acc{0.f, 0.f, 0.f};
-        auto pair = [&](float a, float4 ja, float4 jb) {
+        auto pair = [&](float2 ab, float4 ja, f3 pj) {
             const f3 xij = xi - mk(ja);
-            const f3 pij = pi - rec_p(ja, jb);
-            const float dist = len3(xij);
-            const float G = Gk<CUBIC>(dist, invHi);
-            const float cij = 0.5f * (mi + jb.z) / jb.w / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
-            const float bij = bfac * cij;
-            acc = acc + (a * pij + (bij * dot3(pij, xij)) * xij);
+            const f3 pij = pi - pj;
+            acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
         };
-        const uint32_t c = nb.cff[i];
-        uint32_t k = 0;
-        // four neighbours per trip: all index/coefficient loads, then all gathers, then the
-        // sums in list order (same result as the one-at-a-time loop)
-        constexpr int U = 4;
-        for (; k + U <= c; k += U) {
+        // batches in list order; within a batch the pairs are summed in list order too
+        for (uint32_t k = 0; k < c; k += U) {
             uint32_t j[U];
-            float a[U];
+            float2 ab[U];
 #pragma unroll
-            for (int q = 0; q < U; q++) {
-                const size_t slot = size_t(k + q) * N + i;
-                j[q] = __builtin_nontemporal_load(nb.ff + slot);
-                a[q] = __builtin_nontemporal_load(ca + slot);
+            for (uint32_t q = 0; q < U; q++) {
+                j[q] = jn[q];
+                ab[q] = abn[q];
             }
-            float4 ja[U], jb[U];
+            if (k + U < c) fetch(k + U);
+            if (staged) {
 #pragma unroll
-            for (int q = 0; q < U; q++) {
-                ja[q] = in[2 * size_t(j[q])];
-                jb[q] = in[2 * size_t(j[q]) + 1];
+                for (uint32_t q = 0; q < U; q++)
+                    if (k + q < c) pair(ab[q], la[j[q]], rec_p(la[j[q]], lb[j[q]]));
+            } else {  // halo over the cap (rare): gather from global memory
+                for (uint32_t q = 0; q < U && k + q < c; q++) {
+                    const float4 ja = in[2 * size_t(j[q])], jb = in[2 * size_t(j[q]) + 1];
+                    pair(ab[q], ja, rec_p(ja, jb));
+                }
             }
-#pragma unroll
-            for (int q = 0; q < U; q++) pair(a[q], ja[q], jb[q]);
-        }
-        for (; k < c; k++) {
-            const size_t slot = size_t(k) * N + i;
-            const uint32_t j = nb.ff[slot];
-            pair(ca[slot], in[2 * size_t(j)], in[2 * size_t(j) + 1]);
         }
         const float f = MODE == 0 ? 0.0f : P.cg_factor;
         const f3 o = pi * (1.0f + f) + acc;
@@ -574,7 +594,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, const fl
             const f3 ri{bi.x - o.x, bi.y - o.y, bi.z - o.z};
             r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
             out[2 * size_t(i)] = make_float4(xi.x, xi.y, xi.z, ri.x);
-            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, mi, di);
+            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, 0.0f, 0.0f);
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
             Ap[i] = make_float4(o.x, o.y, o.z, 0.0f);
@@ -935,20 +955,15 @@ void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, con
     if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
     else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
 }
-void cg_pack(uint32_t N, const float4* pos, const float4* x0, const float4* vel, const float* rho, float4* rec, hipStream_t s) {
-    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, vel, rho, rec);
+void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
+    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
 }
-void cg_apply(int mode, int cubic, const Scenes& sc, const Nbr& nb, const float* ca, const float4* pos, const float4* in,
-              float4* out, float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
+void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* in, float4* out,
+              float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (cubic) {
-        if (mode == 0) k_cg_apply<0, true><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
-        else k_cg_apply<1, true><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
-    } else {
-        if (mode == 0) k_cg_apply<0, false><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
-        else k_cg_apply<1, false><<<g, kBlock, 0, s>>>(sc, nb, ca, pos, in, out, Ap, b, r, st, partials);
-    }
+    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, in, out, Ap, b, r, st, partials);
+    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, in, out, Ap, b, r, st, partials);
 }
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;

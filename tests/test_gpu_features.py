@@ -187,3 +187,31 @@ def test_autotune_keeps_state_and_results():
         a.step()
         b.step()
     assert np.array_equal(a.get("FLUID_POS"), b.get("FLUID_POS"))
+
+
+@pytest.mark.parametrize("limit", [1, 300])
+def test_viscosity_halo_paths_bit_identical(limit):
+    """The viscosity operator stages each 256-slot block's neighbour records in LDS; a block
+    whose halo is over the limit gathers from global memory instead. Both paths visit the same
+    pairs in the same order: limit 1 (every block global) and 300 give the same bits as
+    the default, step after step."""
+    sc = scenes.dam_break_cpu(n_side=12)
+    a = HipSim(scenes.make_config(sc))
+    cfg = scenes.make_config(sc)
+    cfg.halo_limit = limit
+    b = HipSim(cfg)
+    scenes.load_into(a, sc)
+    scenes.load_into(b, sc)
+    spilled = 0
+    for s in range(12):
+        sa, sb = a.step(), b.step()
+        assert sa.halo_spill == 0 and 0 < sa.halo_max <= 1264 and sa.halo_blocks > 0, s
+        assert sb.halo_blocks == sa.halo_blocks and sb.halo_max <= limit, s
+        if limit == 1:
+            assert sb.halo_spill == sb.halo_blocks, s
+        spilled += sb.halo_spill
+        assert (sa.l_df, sa.l_cg, sa.l_cd) == (sb.l_df, sb.l_cg, sb.l_cd), s
+        assert np.array_equal(a.get("FLUID_POS"), b.get("FLUID_POS")), s
+        assert np.array_equal(a.get("FLUID_VEL"), b.get("FLUID_VEL")), s
+    say(f"[halo] limit {limit}: default max halo {sa.halo_max}, {spilled} block-steps on the global path")
+    assert spilled > 0
