@@ -190,6 +190,7 @@ struct mps_ctx {
     DBuf<uint16_t> nb16;
     DBuf<float4> cgrecA, cgrecB;  // 32-byte (position, vector) records for the CG gathers
     bool pairs_valid = false;
+    bool halo_valid = false;  // block halos match the fluid lists
     uint32_t nei_regrow = 0;
     // small device records + pinned host mirrors
     NbrCounters* d_ctr = nullptr;
@@ -565,6 +566,7 @@ struct mps_ctx {
         });
         h.n_valid = N;
         pairs_valid = false;
+        halo_valid = false;
     }
     // The reference's fluid hash in its own layout (keys sorted by linear cell index, ties by
     // reference id; MPSSpatialHash.cu:53-77), rebuilt on the host from the last hash.
@@ -641,6 +643,7 @@ struct mps_ctx {
                 nnz_ff = h_ctr->nnz_ff;
                 nnz_fb = h_ctr->nnz_fb;
                 pairs_valid = false;
+                halo_valid = false;
                 return;
             }
             nei_regrow++;
@@ -667,6 +670,20 @@ struct mps_ctx {
     }
     void ensure_pairs() {
         if (!pairs_valid) density_alpha(mps::launch::kDoPairs);
+    }
+    // Block halos of the fluid lists for the halo-staged kernels (pressure solves, A.p)
+    mps::Halo halo() const { return mps::Halo{halo_idx.p, halo_n.p, nb16.p}; }
+    void ensure_halo() {
+        if (halo_valid) return;
+        const size_t m = std::max<uint32_t>(N, 1);
+        const uint32_t nblk = uint32_t((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock);
+        nb16.reserve(size_t(Kff) * m);
+        halo_idx.reserve(size_t(std::max<uint32_t>(nblk, 1)) * mps::kHaloCap);
+        halo_n.reserve(std::max<uint32_t>(nblk, 1));
+        T_(K_HALO, [&] {
+            mps::launch::halo_build(scn(), nbr(), nblk, cfg.halo_limit, halo_idx.p, halo_n.p, nb16.p, d_flags + 4, stream);
+        });
+        halo_valid = true;
     }
 
     // number of scenes whose loop `idx` is still running
@@ -697,7 +714,9 @@ struct mps_ctx {
         mps::launch::solver_reset(sd, S, d_active + idx, stream);
         ensure_partials();
         ensure_pairs();
+        ensure_halo();
         const mps::Scenes P = scn();
+        const mps::Halo hl = halo();
         const double tol = df ? cfg.df_tol : cfg.cd_tol;
         const uint32_t lmin = df ? cfg.df_min_iter : cfg.cd_min_iter;
         const uint32_t cap = cfg.max_iter;
@@ -710,20 +729,20 @@ struct mps_ctx {
             for (const void* q : {(const void*)pos.p, (const void*)vel.p, (const void*)bpos.p, (const void*)bvol.p,
                                   (const void*)bvel.p, (const void*)rho.p, (const void*)alpha.p, (const void*)pairG_ff.p,
                                   (const void*)pairG_fb.p, (const void*)kappa.p, (const void*)delta.p, (const void*)sd,
-                                  (const void*)partials.p})
+                                  (const void*)partials.p, (const void*)hl.idx, (const void*)hl.n, (const void*)hl.nb16})
                 key.push_back(u(q));
             key.push_back(u(tol));
             key.push_back(lmin);
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
                     T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
-                        mps::launch::pressure_eval(df, P, nbr(), pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
+                        mps::launch::pressure_eval(df, P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
                                                    pairG_ff.p, pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, tol, lmin, stream); }, false);
                     T_(K_APPLY, [&] {
-                        mps::launch::pressure_apply(P, nbr(), pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p, sd,
-                                                    stream);
+                        mps::launch::pressure_apply(P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p,
+                                                    sd, stream);
                     }, false);
                     rec_snap();
                 }
@@ -780,16 +799,12 @@ struct mps_ctx {
         pairAB.reserve(size_t(Kff) * m);
         cgrecA.reserve(2 * m);
         cgrecB.reserve(2 * m);
-        const uint32_t nblk = uint32_t((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock);
-        nb16.reserve(size_t(Kff) * m);
-        halo_idx.reserve(size_t(std::max<uint32_t>(nblk, 1)) * mps::kHaloCap);
-        halo_n.reserve(std::max<uint32_t>(nblk, 1));
-        const mps::Halo hl{halo_idx.p, halo_n.p, nb16.p};
+        ensure_halo();
+        const mps::Halo hl = halo();
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
         T_(K_CG_SETUP, [&] {
             mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
         });
-        T_(K_HALO, [&] { mps::launch::halo_build(P, nbr(), nblk, cfg.halo_limit, halo_idx.p, halo_n.p, nb16.p, d_flags + 4, stream); });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         T_(K_CG_INIT, [&] {
@@ -956,6 +971,7 @@ struct mps_ctx {
         mps::launch::invert(perm.p, N, inv.p, stream);
         frame = c.frame;
         pairs_valid = false;
+        halo_valid = false;
         bnd_dirty = true;
     }
     // Forward n steps from the current state, keeping a checkpoint every `every` steps.
@@ -1046,6 +1062,7 @@ struct mps_ctx {
         pairG_ff.reserve(size_t(T.Kff) * m);
         pairG_fb.reserve(size_t(T.Kfb) * m);
         pairs_valid = false;
+        halo_valid = false;
         mps::launch::density_alpha(1, mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb, T.pos.p,
                                    T.snaps[0].p, T.bpos.p, T.bvol.p, rho.p, alpha.p, color.p, pairG_ff.p, pairG_fb.p, stream);
         abar.reserve(m); rbar.reserve(m); dbar.reserve(m); fbar.reserve(m); fstr.reserve(m);
@@ -1327,6 +1344,7 @@ struct mps_ctx {
         fh.cells_init = false;
         fh.n_valid = 0;
         pairs_valid = false;
+        halo_valid = false;
         frame = 0;
         pred[0] = 2; pred[1] = 3; pred[2] = 8;
         params_dirty = true;
@@ -1369,6 +1387,7 @@ struct mps_ctx {
         fh.n_valid = bh.n_valid = 0;
         bnd_dirty = true;
         pairs_valid = false;
+        halo_valid = false;
         frame = 0;
         params_dirty = true;
     }
@@ -1772,12 +1791,12 @@ struct mps_ctx {
         switch (id) {
             case MPS_ARR_FLUID_POS:
                 fl4(pos, -1);
-                if (src) pairs_valid = false;
+                if (src) pairs_valid = halo_valid = false;
                 break;
             case MPS_ARR_FLUID_VEL: fl4(vel, -1); break;
             case MPS_ARR_FLUID_RADIUS:
                 fl4(pos, 3);
-                if (src) pairs_valid = false;
+                if (src) pairs_valid = halo_valid = false;
                 break;
             case MPS_ARR_FLUID_MASS: fl4(vel, 3); break;
             case MPS_ARR_FLUID_COLOR: fl4(color, 4); break;

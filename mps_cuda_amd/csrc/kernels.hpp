@@ -130,13 +130,13 @@ void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipSt
 void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s);
-void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
 // one workgroup per scene; `active` counts the scenes whose loop has not finished
 void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
                    hipStream_t s);
-void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s);
 void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,

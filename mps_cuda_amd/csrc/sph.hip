@@ -52,6 +52,10 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
     return block_sum_d(s, lds);
 }
 
+// Fluid-fluid list entries each thread keeps in flight in the halo-staged kernels (their
+// occupancy is bounded by LDS, so registers are free for it).
+constexpr uint32_t kListBatch = 4;
+
 // ----------------------------------------------------------------- boundary volume
 // ComputeBoundaryParticleVolume_kernel (MPSSPHBaseUtil.cuh:10-41): S_b = W(0) + sum W_bk.
 // S_b depends only on boundary positions, so it is computed when they change.
@@ -188,49 +192,77 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
 // ComputeDensityDelta_kernel ff/fb (MPSSPHDFSPHUtil.cuh:139-257) fused with
 // ComputeDFStiffness_kernel / ComputeCDStiffness_kernel (:259-340); G_ij comes from the
 // alpha pass. Each workgroup publishes its sum of positive errors for k_fold_pressure.
+// The records of the block's halo (x_j, v_j) are staged in LDS (see k_cg_apply); the pairs
+// are summed in list order on both paths.
 template <bool DF>
-__global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, const float4* __restrict__ bvel,
                                                         const float* __restrict__ rho, const float* __restrict__ alpha,
                                                         const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
                                                         float* __restrict__ kappa, float* __restrict__ delta,
                                                         const SolverDev* st, double* __restrict__ partials) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
     if (st[sid].done) return;  // this scene's loop has finished
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
+    const bool live = i < P.end;
+    const uint32_t c = live ? nb.cff[i] : 0u;
+    __shared__ float4 la[kHaloCap];
+    __shared__ float2 lb[kHaloCap];
+    const uint32_t H = hl.n[blk];
+    const bool staged = H <= kHaloCap;
+    constexpr uint32_t U = kListBatch;
+    uint32_t jn[U];
+    float gn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                gn[q] = __builtin_nontemporal_load(pG_ff + slot);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
+            const uint32_t s = hs[t];
+            const float4 x = pos[s], v = vel[s];
+            la[t] = make_float4(x.x, x.y, x.z, v.x);
+            lb[t] = make_float2(v.y, v.z);
+        }
+        __syncthreads();
+    }
     double epos = 0.0;
-    if (i < P.end) {
+    if (live) {
         const f3 xi = mk(pos[i]), vi = mk(vel[i]);
         float df = 0.0f;
-        const uint32_t c = nb.cff[i];
-        uint32_t k = 0;
-        for (; k + 4 <= c; k += 4) {
-            uint32_t j[4];
-            float G[4];
+        for (uint32_t k = 0; k < c; k += U) {
+            uint32_t j[U];
+            float G[U];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const size_t slot = size_t(k + q) * N + i;
-                j[q] = nb.ff[slot];
-                G[q] = pG_ff[slot];
+            for (uint32_t q = 0; q < U; q++) {
+                j[q] = jn[q];
+                G[q] = gn[q];
             }
-            float4 xj[4], vj[4];
+            if (k + U < c) fetch(k + U);
+            if (staged) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                xj[q] = pos[j[q]];
-                vj[q] = vel[j[q]];
+                for (uint32_t q = 0; q < U; q++) {
+                    if (k + q < c) {
+                        const float4 a = la[j[q]];
+                        const float2 b = lb[j[q]];
+                        df += P.V * G[q] * dot3(xi - mk(a), vi - f3{a.w, b.x, b.y});
+                    }
+                }
+            } else {
+                for (uint32_t q = 0; q < U && k + q < c; q++) df += P.V * G[q] * dot3(xi - mk(pos[j[q]]), vi - mk(vel[j[q]]));
             }
-#pragma unroll
-            for (int q = 0; q < 4; q++) df += P.V * G[q] * dot3(xi - mk(xj[q]), vi - mk(vj[q]));
-        }
-        for (; k < c; k++) {
-            const size_t slot = size_t(k) * N + i;
-            const uint32_t j = nb.ff[slot];
-            const f3 xij = xi - mk(pos[j]);
-            const f3 vij = vi - mk(vel[j]);
-            df += P.V * pG_ff[slot] * dot3(xij, vij);
         }
         float db = 0.0f;
         const uint32_t cb = nb.cfb[i];
@@ -261,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, con
         }
         kappa[i] = e;
     }
-    block_partial(epos, partials);
+    block_partial(epos, partials, blk);
 }
 
 // Mean positive error and the loop decision of the DF/CD drivers
@@ -291,46 +323,70 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(Scenes sc, const
 }
 
 // ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
-__global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                         const float4* __restrict__ bpos, const float* __restrict__ bvol,
-                                                         const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
-                                                         const float* __restrict__ kappa, const SolverDev* st) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
+// The halo's (x_j, kappa_j) are staged in LDS (see k_cg_apply).
+__global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
+                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
+                                                         const float* __restrict__ bvol, const float* __restrict__ pG_ff,
+                                                         const float* __restrict__ pG_fb, const float* __restrict__ kappa,
+                                                         const SolverDev* st) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
     if (st[sid].done) return;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= P.end) return;
+    const bool live = i < P.end;
+    const uint32_t c = live ? nb.cff[i] : 0u;
+    __shared__ float4 la[kHaloCap];
+    const uint32_t H = hl.n[blk];
+    const bool staged = H <= kHaloCap;
+    constexpr uint32_t U = kListBatch;
+    uint32_t jn[U];
+    float gn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                gn[q] = __builtin_nontemporal_load(pG_ff + slot);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
+            const uint32_t s = hs[t];
+            const float4 x = pos[s];
+            la[t] = make_float4(x.x, x.y, x.z, kappa[s]);
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     const f3 xi = mk(pos[i]);
     const float pi = kappa[i];
     f3 ff{0.f, 0.f, 0.f};
-    const uint32_t c = nb.cff[i];
-    uint32_t k = 0;
-    for (; k + 4 <= c; k += 4) {
-        uint32_t j[4];
-        float G[4];
+    for (uint32_t k = 0; k < c; k += U) {
+        uint32_t j[U];
+        float G[U];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const size_t slot = size_t(k + q) * N + i;
-            j[q] = nb.ff[slot];
-            G[q] = pG_ff[slot];
+        for (uint32_t q = 0; q < U; q++) {
+            j[q] = jn[q];
+            G[q] = gn[q];
         }
-        float4 xj[4];
-        float kj[4];
+        if (k + U < c) fetch(k + U);
+        if (staged) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xj[q] = pos[j[q]];
-            kj[q] = kappa[j[q]];
+            for (uint32_t q = 0; q < U; q++) {
+                if (k + q < c) {
+                    const float4 a = la[j[q]];
+                    ff = ff - 0.5f * P.V * (pi + a.w) * G[q] * (xi - mk(a));
+                }
+            }
+        } else {
+            for (uint32_t q = 0; q < U && k + q < c; q++) ff = ff - 0.5f * P.V * (pi + kappa[j[q]]) * G[q] * (xi - mk(pos[j[q]]));
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) ff = ff - 0.5f * P.V * (pi + kj[q]) * G[q] * (xi - mk(xj[q]));
-    }
-    for (; k < c; k++) {
-        const size_t slot = size_t(k) * N + i;
-        const uint32_t j = nb.ff[slot];
-        const f3 xij = xi - mk(pos[j]);
-        const f3 fij = 0.5f * P.V * (pi + kappa[j]) * pG_ff[slot] * xij;
-        ff = ff - fij;
     }
     f3 fb{0.f, 0.f, 0.f};
     const uint32_t cb = nb.cfb[i];
@@ -509,7 +565,6 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 // MODE 0: A x0 with f = 0 (x0 in `in`) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
 //         r = p = b - A x0 (p written to `out`), partial sums of r.r.
 // MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
-constexpr uint32_t kApplyBatch = 4;
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
                                                    const float4* __restrict__ in, float4* __restrict__ out,
@@ -530,7 +585,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     const bool staged = H <= kHaloCap;  // same for the whole block
     // LDS bounds the occupancy of this kernel, not registers: each thread keeps a batch of
     // kApplyBatch list entries in flight, the first issued before the halo is staged
-    constexpr uint32_t U = kApplyBatch;
+    constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
     float2 abn[U];
     auto fetch = [&](uint32_t k0) {
@@ -928,22 +983,22 @@ void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const 
     if (cubic) k_density_alpha<true><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
     else k_density_alpha<false><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
-void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
-    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
 void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
                    hipStream_t s) {
     if (sc.N) k_fold_pressure<<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol, lmin);
 }
-void pressure_apply(const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel, const float4* bpos,
+void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s) {
-    if (sc.N) k_pressure_apply<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
+    if (sc.N) k_pressure_apply<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
 void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {

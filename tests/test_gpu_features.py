@@ -191,10 +191,10 @@ def test_autotune_keeps_state_and_results():
 
 @pytest.mark.parametrize("limit", [1, 300])
 def test_viscosity_halo_paths_bit_identical(limit):
-    """The viscosity operator stages each 256-slot block's neighbour records in LDS; a block
-    whose halo is over the limit gathers from global memory instead. Both paths visit the same
-    pairs in the same order: limit 1 (every block global) and 300 give the same bits as
-    the default, step after step."""
+    """The pressure solves and the viscosity operator stage each 256-slot block's neighbour
+    records in LDS; a block whose halo is over the limit gathers from global memory instead.
+    Both paths visit the same pairs in the same order: limit 1 (every block global) and 300
+    give the same bits as the default, step after step."""
     sc = scenes.dam_break_cpu(n_side=12)
     a = HipSim(scenes.make_config(sc))
     cfg = scenes.make_config(sc)
