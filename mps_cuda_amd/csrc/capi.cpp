@@ -160,7 +160,7 @@ struct mps_ctx {
     DBuf<float2> cabT;
     DBuf<double> acc_mu, acc_sig;
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
-    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
+    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgq, cgAp, jprev, jt[3], jcand, tmp4;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
     // second buffer of each array the hash reorders: the two alternate, so the arrays a solver
     // chunk reads take one of two addresses (cached solver graphs stay valid)
@@ -797,8 +797,7 @@ struct mps_ctx {
         const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
         pairAB.reserve(size_t(Kff) * m);
-        cgrecA.reserve(2 * m);
-        cgrecB.reserve(2 * m);
+        cgq.reserve(m);
         ensure_halo();
         const mps::Halo hl = halo();
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
@@ -807,9 +806,11 @@ struct mps_ctx {
         });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
+        // p of iteration k is written to pbuf(k) while pbuf(k - 1) is read; p_0 = r
+        auto pbuf = [&](uint32_t k) { return (k & 1u) ? cgq.p : cgp.p; };
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_pack(N, pos.p, cgx.p, cgrecA.p, stream);
-            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, cgrecA.p, cgrecB.p, cgAp.p, cgb.p, cgr.p, sd, partials.p, stream);
+            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, pos.p, cgx.p, nullptr, pbuf(0), nullptr, cgb.p, cgr.p, sd,
+                                  partials.p, stream);
         });
         T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
@@ -819,22 +820,24 @@ struct mps_ctx {
             uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
             chunk = std::min(chunk, cap - it + 1);
             std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
-            for (const void* q : {(const void*)pairAB.p, (const void*)cgrecB.p, (const void*)cgAp.p,
-                                  (const void*)cgb.p, (const void*)cgr.p, (const void*)cgx.p, (const void*)sd,
+            for (const void* q : {(const void*)pairAB.p, (const void*)pos.p, (const void*)pbuf(it - 1), (const void*)pbuf(it),
+                                  (const void*)cgAp.p, (const void*)cgr.p, (const void*)cgx.p, (const void*)sd,
                                   (const void*)partials.p, (const void*)act, (const void*)hl.idx, (const void*)hl.n,
                                   (const void*)hl.nb16})
                 key.push_back(u(q));
             key.push_back(u(tol));
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
+                    const uint32_t k = it + c;
                     T_(K_CG_APPLY, [&] {
-                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, cgrecB.p, nullptr, cgAp.p, cgb.p, cgr.p, sd, partials.p,
-                                              stream);
+                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, pos.p, cgr.p, pbuf(k - 1), pbuf(k), cgAp.p, nullptr,
+                                              nullptr, sd, partials.p, stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream); }, false);
-                    T_(K_CG_UPDATE, [&] { mps::launch::cg_update(P, cgx.p, cgr.p, cgrecB.p, cgAp.p, sd, partials.p, stream); }, false);
+                    T_(K_CG_UPDATE, [&] {
+                        mps::launch::cg_update_p(P, cgx.p, cgr.p, pbuf(k), cgAp.p, sd, partials.p, stream);
+                    }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(2, P, partials.p, sd, act, tol, stream); }, false);
-                    T_(K_CG_DIR, [&] { mps::launch::cg_direction(P, cgr.p, cgrecB.p, sd, stream); }, false);
                 }
             });
             it += chunk;
@@ -843,11 +846,10 @@ struct mps_ctx {
         uint32_t l = 0;
         const std::vector<uint32_t> ls = fetch_records(2, cap, &l);
         pred[2] = std::max<uint32_t>(l, 1);
-        if (prof_on) {  // A.p and the update ran for iterations 1..l-1, the direction for 1..l-2
+        if (prof_on) {  // A.p (with the direction) and the update ran for iterations 1..l-1
             kuse[K_CG_APPLY] += std::min(l - 1, cap);
             kuse[K_CG_UPDATE] += std::min(l - 1, cap);
             kuse[K_FOLD] += 2 * std::min(l - 1, cap);
-            kuse[K_CG_DIR] += done ? (l >= 2 ? l - 2 : 0) : cap;
         }
         T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(P, cgx.p, vel.p, stream); });
         for (uint32_t q = 0; q < S && q < sstats.size(); q++) {
@@ -1847,7 +1849,7 @@ struct mps_ctx {
     }
 
     ~mps_ctx() {
-        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release();
+        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release(); cgq.release();
         cgAp.release(); jprev.release(); for (auto& t : jt) t.release(); jcand.release(); tmp4.release();
         rho.release(); alpha.release(); kappa.release(); delta.release(); fst.release(); tmp1.release();
         pos_alt.release(); vel_alt.release(); color_alt.release(); cgb_alt.release(); rho_alt.release();

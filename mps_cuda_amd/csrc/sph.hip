@@ -560,14 +560,20 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 // loads (recomputing b_ij costs more ALU time than the 4 bytes it saves). The block's halo
 // records (x_j, p_j: 24 bytes) are staged in LDS first, with coalesced loads of the sorted
 // halo list, and each pair reads its neighbour there through the 16-bit list; a block whose
-// halo is over kHaloCap gathers from global memory. Both paths visit the same pairs in the
+// halo is over the limit gathers from global memory. Both paths visit the same pairs in the
 // same order, so the result does not depend on the path.
-// MODE 0: A x0 with f = 0 (x0 in `in`) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
-//         r = p = b - A x0 (p written to `out`), partial sums of r.r.
-// MODE 1: A p with f (p in `in`), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
+// The search direction of UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133),
+// p = r + beta p_prev, is formed here for every record read (own and halo) instead of in a
+// pass of its own; the block writes its own p for the update kernel (pout, while pprev is
+// read: the host alternates the two buffers). beta = 0 in the first iteration, where
+// pprev = r, so p = r.
+// MODE 0: A x0 with f = 0 (x0 in u) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
+//         r = b - A x0, pout = r, partial sums of r.r.
+// MODE 1: A p with f (r in u), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
-                                                   const float4* __restrict__ in, float4* __restrict__ out,
+                                                   const float4* __restrict__ pos, const float4* __restrict__ u,
+                                                   const float4* __restrict__ pprev, float4* __restrict__ pout,
                                                    float4* __restrict__ Ap, const float4* __restrict__ b,
                                                    float4* __restrict__ r, const SolverDev* st,
                                                    double* __restrict__ partials) {
@@ -575,16 +581,22 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     const uint32_t sid = sc.blk_scene[blk];
     if (MODE == 1 && st[sid].done) return;
     const StepParams P = sc.P[sid];
+    const float bta = MODE == 1 ? st[sid].beta : 0.0f;
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     const bool live = i < P.end;
     const uint32_t c = live ? nb.cff[i] : 0u;
+    // the vector the operator is applied to, at slot j
+    auto vec = [&](uint32_t j) -> f3 {
+        const float4 a = u[j];
+        if (MODE == 0) return mk(a);
+        const float4 q = pprev[j];
+        return {a.x + bta * q.x, a.y + bta * q.y, a.z + bta * q.z};
+    };
     __shared__ float4 la[kHaloCap];
     __shared__ float2 lb[kHaloCap];
     const uint32_t H = hl.n[blk];
     const bool staged = H <= kHaloCap;  // same for the whole block
-    // LDS bounds the occupancy of this kernel, not registers: each thread keeps a batch of
-    // kApplyBatch list entries in flight, the first issued before the halo is staged
     constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
     float2 abn[U];
@@ -603,21 +615,21 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
         for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const size_t s = hs[t];
-            const float4 ra = in[2 * s], rb = in[2 * s + 1];
-            la[t] = ra;
-            lb[t] = make_float2(rb.x, rb.y);
+            const uint32_t s = hs[t];
+            const float4 x = pos[s];
+            const f3 v = vec(s);
+            la[t] = make_float4(x.x, x.y, x.z, v.x);
+            lb[t] = make_float2(v.y, v.z);
         }
         __syncthreads();
     }
     double part = 0.0;
     if (live) {
-        const float4 ra = in[2 * size_t(i)], rb = in[2 * size_t(i) + 1];
-        const f3 xi = mk(ra);
-        const f3 pi = rec_p(ra, rb);
+        const f3 xi = mk(pos[i]);
+        const f3 pi = vec(i);
         f3 acc{0.f, 0.f, 0.f};
-        auto pair = [&](float2 ab, float4 ja, f3 pj) {
-            const f3 xij = xi - mk(ja);
+        auto pair = [&](float2 ab, f3 xj, f3 pj) {
+            const f3 xij = xi - xj;
             const f3 pij = pi - pj;
             acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
         };
@@ -633,13 +645,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             if (k + U < c) fetch(k + U);
             if (staged) {
 #pragma unroll
-                for (uint32_t q = 0; q < U; q++)
-                    if (k + q < c) pair(ab[q], la[j[q]], rec_p(la[j[q]], lb[j[q]]));
-            } else {  // halo over the cap (rare): gather from global memory
-                for (uint32_t q = 0; q < U && k + q < c; q++) {
-                    const float4 ja = in[2 * size_t(j[q])], jb = in[2 * size_t(j[q]) + 1];
-                    pair(ab[q], ja, rec_p(ja, jb));
+                for (uint32_t q = 0; q < U; q++) {
+                    if (k + q < c) {
+                        const float4 a = la[j[q]];
+                        const float2 e = lb[j[q]];
+                        pair(ab[q], mk(a), f3{a.w, e.x, e.y});
+                    }
                 }
+            } else {  // halo over the limit: gather from global memory
+                for (uint32_t q = 0; q < U && k + q < c; q++) pair(ab[q], mk(pos[j[q]]), vec(j[q]));
             }
         }
         const float f = MODE == 0 ? 0.0f : P.cg_factor;
@@ -648,10 +662,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             const float4 bi = b[i];
             const f3 ri{bi.x - o.x, bi.y - o.y, bi.z - o.z};
             r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
-            out[2 * size_t(i)] = make_float4(xi.x, xi.y, xi.z, ri.x);
-            out[2 * size_t(i) + 1] = make_float4(ri.y, ri.z, 0.0f, 0.0f);
+            pout[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
+            pout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
             Ap[i] = make_float4(o.x, o.y, o.z, 0.0f);
             part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
@@ -697,7 +711,9 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const doubl
     }
 }
 
-// UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121)
+// UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121).
+// REC: p in the 32-byte records (the adjoint's solve), else one float4 per slot.
+template <bool REC>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restrict__ x, float4* __restrict__ r,
                                                     const float4* __restrict__ prec, const float4* __restrict__ Ap,
                                                     const SolverDev* st, double* __restrict__ partials) {
@@ -708,7 +724,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restr
     const float a = st[sid].alpha;
     double part = 0.0;
     if (i < end) {
-        const f3 pi = rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]);
+        const f3 pi = REC ? rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]) : mk(prec[i]);
         const float4 api = Ap[i];
         float4 xi = x[i], ri = r[i];
         xi.x = xi.x + a * pi.x;
@@ -1013,12 +1029,13 @@ void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, con
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
     if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
 }
-void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* in, float4* out,
-              float4* Ap, const float4* b, float4* r, const SolverDev* st, double* partials, hipStream_t s) {
+void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
+              const float4* u, const float4* pprev, float4* pout, float4* Ap, const float4* b, float4* r,
+              const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, in, out, Ap, b, r, st, partials);
-    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, in, out, Ap, b, r, st, partials);
+    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, u, pprev, pout, Ap, b, r, st, partials);
+    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, u, pprev, pout, Ap, b, r, st, partials);
 }
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;
@@ -1028,7 +1045,11 @@ void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, 
 }
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s) {
-    if (sc.N) k_cg_update<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
+    if (sc.N) k_cg_update<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
+}
+void cg_update_p(const Scenes& sc, float4* x, float4* r, const float4* p, const float4* Ap, const SolverDev* st,
+                 double* partials, hipStream_t s) {
+    if (sc.N) k_cg_update<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, p, Ap, st, partials);
 }
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
     if (sc.N) k_cg_direction<<<nb_(sc.N), kBlock, 0, s>>>(sc, r, prec, st);
