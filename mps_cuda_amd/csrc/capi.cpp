@@ -188,6 +188,7 @@ struct mps_ctx {
     // workgroup, their count, and every list entry as a 16-bit position in its block's halo
     DBuf<uint32_t> halo_idx, halo_n;
     DBuf<uint16_t> nb16;
+    DBuf<uint2> nstat;  // per-workgroup neighbour-count records of the searches
     DBuf<float4> cgrecA, cgrecB;  // 32-byte (position, vector) records for the CG gathers
     bool pairs_valid = false;
     bool halo_valid = false;  // block halos match the fluid lists
@@ -610,13 +611,14 @@ struct mps_ctx {
         if (Kbb == 0) Kbb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
         for (int attempt = 0; attempt < 8; attempt++) {
             reset_counters();
+            nstat.reserve(2 * ((size_t(std::max(N, B)) + mps::dev::kBlock - 1) / mps::dev::kBlock) + 2);
             const bool do_b = bnd_dirty;
             if (do_b && B) {
                 nbb.reserve(size_t(Kbb) * B);
                 cbb.reserve(B);
                 T_(K_NEI_BND, [&] {
                     mps::launch::neighbors_boundary(bpos.p, B, bnd_scene(), uint32_t(cells()), g, view(bh, B), nbb.p, Kbb, cbb.p,
-                                                    d_ctr, stream);
+                                                    nstat.p, d_ctr, stream);
                 });
             }
             nff.reserve(size_t(Kff) * std::max<uint32_t>(N, 1));
@@ -625,7 +627,7 @@ struct mps_ctx {
             cfb.reserve(std::max<uint32_t>(N, 1));
             T_(K_NEI_FLUID, [&] {
                 mps::launch::neighbors_fluid(pos.p, scn(), uint32_t(cells()), g, fluid_view(), view(bh, B), nff.p, Kff, cff.p,
-                                             nfb.p, Kfb, cfb.p, d_ctr, stream);
+                                             nfb.p, Kfb, cfb.p, nstat.p, d_ctr, stream);
             });
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
             sync();
@@ -1858,7 +1860,7 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
-        pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); cgrecA.release(); cgrecB.release();
+        pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();

@@ -294,6 +294,20 @@ __global__ void __launch_bounds__(kBlock) k_cell_table(const uint32_t* __restric
 // The 27 (start, end) pairs are loaded up front, so their latencies overlap instead of
 // adding up cell after cell; candidates are read two at a time for the same reason. Lists
 // are written column-major (out[k*stride + i]) when COLS, else as rows of K slots.
+// Neighbour test of the reference: |x_i - x_j| < (h_i + h_j) / 2 with h = 4 r.
+// sqrt is monotone and correctly rounded: far from the cut-off the squared distance decides;
+// near it the exact test of the reference runs.
+__device__ __forceinline__ bool within(const float4& xi, float ri, const float4& q) {
+    const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
+    const float dd = (ddx * ddx + ddy * ddy) + ddz * ddz;
+    const float rj = q.w * 4.0f;
+    const float thr = (ri + rj) * 0.5f;
+    const float t2 = thr * thr;
+    if (dd < t2 * 0.9999f) return true;
+    if (dd > t2 * 1.0001f) return false;
+    return __builtin_sqrtf(dd) < thr;
+}
+
 template <bool SELF, bool COLS>
 __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const HashGeom& g, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, const uint32_t* __restrict__ ids,
@@ -310,18 +324,7 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
     }
     uint32_t k = 0u;
     auto test = [&](float4 q, uint32_t s) {
-        const float ddx = xi.x - q.x, ddy = xi.y - q.y, ddz = xi.z - q.z;
-        const float dd = (ddx * ddx + ddy * ddy) + ddz * ddz;
-        const float rj = q.w * 4.0f;
-        const float thr = (ri + rj) * 0.5f;
-        const float t2 = thr * thr;
-        // sqrt is monotone and correctly rounded: far from the cut-off the squared distance
-        // decides; near it the exact test of the reference runs.
-        bool in;
-        if (dd < t2 * 0.9999f) in = true;
-        else if (dd > t2 * 1.0001f) in = false;
-        else in = __builtin_sqrtf(dd) < thr;
-        if (in) {
+        if (within(xi, ri, q)) {
             const uint32_t j = ids ? ids[s] : s;
             if (!(SELF && j == i)) {
                 if (k < K) out[COLS ? size_t(k) * stride + i : size_t(i) * K + k] = j;
@@ -347,47 +350,271 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
     return k;
 }
 
-__device__ __forceinline__ void report_counts(uint32_t c, uint32_t* maxc, unsigned long long* nnz) {
-    // wave-level max/sum then one atomic per wave
-    uint32_t mx = c;
-    unsigned long long sm = c;
+// Per-workgroup (largest count, sum of counts) of up to two lists, written to st[blk] and
+// st[n + blk]; k_count_fold reduces them (no same-address atomics from every wave).
+template <int L>
+__device__ __forceinline__ void block_counts(const uint32_t (&c)[L], uint2* __restrict__ st, uint32_t n, uint32_t blk) {
+    __shared__ uint32_t wm[L][kBlock / 64], ws[L][kBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+        uint32_t mx = c[l], sm = c[l];
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+            sm += (uint32_t)__shfl_xor((int)sm, off, 64);
+        }
+        if (lane == 0) {
+            wm[l][wave] = mx;
+            ws[l][wave] = sm;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int l = 0; l < L; l++) {
+            uint32_t mx = 0u, sm = 0u;
+            for (uint32_t w = 0; w < kBlock / 64; w++) {
+                mx = max(mx, wm[l][w]);
+                sm += ws[l][w];
+            }
+            st[size_t(l) * n + blk] = make_uint2(mx, sm);
+        }
+    }
+}
+
+constexpr int kCountFoldThreads = 1024;
+__global__ void __launch_bounds__(kCountFoldThreads) k_count_fold(const uint2* __restrict__ st, uint32_t n, uint32_t* maxc,
+                                                                unsigned long long* nnz) {
+    __shared__ uint32_t wm[kCountFoldThreads / 64];
+    __shared__ unsigned long long ws[kCountFoldThreads / 64];
+    uint32_t mx = 0u;
+    unsigned long long sm = 0ull;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const uint2 v = st[k];
+        mx = max(mx, v.x);
+        sm += v.y;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
         sm += __shfl_xor(sm, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
-        atomicMax(maxc, mx);
-        atomicAdd(nnz, sm);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wm[wave] = mx;
+        ws[wave] = sm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) {
+            mx = max(mx, wm[w]);
+            sm += ws[w];
+        }
+        *maxc = mx;
+        *nnz = sm;
     }
 }
 
 // Scene s's cells are entries [s*C, (s+1)*C) of both tables.
+//
+// A workgroup owns 256 consecutive fluid slots (sorted by cell, so each cell's particles are
+// one run). The first particle of each run registers the 27 cells around it in an LDS hash
+// of the block's distinct neighbour cells; each distinct cell's bounds are read once, and
+// all fluid (and wall) particles of those cells are staged in LDS. Every particle then walks
+// its 27 cells in the reference order over the staged copies, so the lists are the same as
+// from the global tables. A block with more distinct cells or candidates than the LDS holds
+// searches the global tables.
+constexpr uint32_t kSrchHash = 1024;   // LDS hash slots for cell indices
+constexpr uint32_t kSrchCells = 384;   // distinct neighbour cells per block
+constexpr uint32_t kSrchCand = 1792;   // staged fluid candidates
+constexpr uint32_t kSrchBCand = 512;   // staged wall candidates
+constexpr uint32_t kNoCell = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t cell_hash(uint32_t c) { return (c * 2654435761u) >> 22; }
+__device__ __forceinline__ i3 cell_offset(i3 p, int t) { return {p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1}; }
+
 __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __restrict__ pos, Scenes sc, uint32_t C, HashGeom g,
                                                           HashView fl, HashView bd, uint32_t* __restrict__ nff, uint32_t Kff,
                                                           uint32_t* __restrict__ cff, uint32_t* __restrict__ nfb, uint32_t Kfb,
-                                                          uint32_t* __restrict__ cfb, NbrCounters* __restrict__ ctr) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          uint32_t* __restrict__ cfb, uint2* __restrict__ bstat) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = blk * kBlock + tid;
     const uint32_t N = sc.N;
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
-    uint32_t a = 0u, b = 0u;
-    if (i < N) {
-        if (i < sc.P[sid].end) {
-            const float4 xi = pos[i];
-            const size_t base = size_t(sid) * C;
-            a = search_cells<true, true>(xi, i, g, fl.start + base, fl.end + base, fl.id, fl.spos, nff, Kff, N);
-            if (bd.n) b = search_cells<false, true>(xi, i, g, bd.start + base, bd.end + base, bd.id, bd.spos, nfb, Kfb, N);
+    const uint32_t sid = sc.blk_scene[blk];
+    const uint32_t end = sc.P[sid].end;
+    const bool live = i < end;
+    const size_t base = size_t(sid) * C;
+    const uint32_t* fstart = fl.start + base;
+    const uint32_t* fend = fl.end + base;
+    const uint32_t* bstart = bd.start + base;
+    const uint32_t* bend = bd.end + base;
+    __shared__ uint32_t ckey[kSrchHash];
+    __shared__ uint16_t cent[kSrchHash];
+    __shared__ uint32_t ecell[kSrchCells], efs[kSrchCells], ebs[kSrchCells];
+    __shared__ uint16_t efo[kSrchCells + 1], ebo[kSrchCells + 1];
+    __shared__ float4 cpos[kSrchCand];
+    __shared__ float4 bcand[kSrchBCand];
+    __shared__ unsigned long long scan[kBlock];
+    __shared__ uint32_t ncell, tf, tb;
+    for (uint32_t t = tid; t < kSrchHash; t += kBlock) ckey[t] = kNoCell;
+    if (tid == 0) ncell = 0u;
+    const float4 xi = live ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
+    bool leader = live;
+    if (live && tid > 0) {
+        const float4 xp = pos[i - 1];
+        const i3 pp = grid_pos(xp.x, xp.y, xp.z, g);
+        leader = pp.x != p.x || pp.y != p.y || pp.z != p.z;
+    }
+    __syncthreads();
+    // 1. distinct neighbour cells (inserts stop once the block is known to be over the cap,
+    //    so the table never fills; every probe loop is bounded anyway)
+    if (leader) {
+        for (int t = 0; t < 27; t++) {
+            if (*(volatile uint32_t*)&ncell > kSrchCells) break;
+            const i3 q = cell_offset(p, t);
+            const uint32_t c = grid_index(q.x, q.y, q.z, g);
+            uint32_t h = cell_hash(c);
+            for (uint32_t probe = 0; probe < kSrchHash; probe++) {
+                const uint32_t prev = atomicCAS(&ckey[h], kNoCell, c);
+                if (prev == kNoCell) {
+                    const uint32_t e = atomicAdd(&ncell, 1u);
+                    if (e < kSrchCells) {
+                        cent[h] = static_cast<uint16_t>(e);
+                        ecell[e] = c;
+                    }
+                    break;
+                }
+                if (prev == c) break;
+                h = (h + 1u) & (kSrchHash - 1u);
+            }
         }
+    }
+    __syncthreads();
+    const uint32_t nc = ncell;
+    bool staged = nc <= kSrchCells;
+    // 2. bounds of each distinct cell (read once) and offsets of its staged copies
+    if (staged) {
+        uint32_t fc[2] = {0u, 0u}, bc[2] = {0u, 0u};
+#pragma unroll
+        for (uint32_t q = 0; q < 2; q++) {
+            const uint32_t e = 2u * tid + q;
+            if (e < nc) {
+                const uint32_t c = ecell[e];
+                const uint32_t s0 = fstart[c], s1 = fend[c];
+                efs[e] = s0;
+                fc[q] = s0 < s1 ? s1 - s0 : 0u;  // an empty cell has start = end = 0xFFFFFFFF
+                if (bd.n) {
+                    const uint32_t b0 = bstart[c], b1 = bend[c];
+                    ebs[e] = b0;
+                    bc[q] = b0 < b1 ? b1 - b0 : 0u;
+                }
+            }
+        }
+        const unsigned long long mine = (static_cast<unsigned long long>(bc[0] + bc[1]) << 32) | (fc[0] + fc[1]);
+        scan[tid] = mine;
+        __syncthreads();
+        for (uint32_t off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan of (wall, fluid) counts
+            const unsigned long long v = tid >= off ? scan[tid - off] : 0ull;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        const unsigned long long ex = scan[tid] - mine;
+        uint32_t fo = static_cast<uint32_t>(ex & 0xFFFFFFFFull), bo = static_cast<uint32_t>(ex >> 32);
+#pragma unroll
+        for (uint32_t q = 0; q < 2; q++) {
+            const uint32_t e = 2u * tid + q;
+            if (e < nc) {
+                efo[e] = static_cast<uint16_t>(fo);
+                ebo[e] = static_cast<uint16_t>(bo);
+            }
+            fo += fc[q];
+            bo += bc[q];
+        }
+        if (tid == kBlock - 1) {
+            const unsigned long long tot = scan[kBlock - 1];
+            tf = static_cast<uint32_t>(tot & 0xFFFFFFFFull);
+            tb = static_cast<uint32_t>(tot >> 32);
+            efo[nc] = static_cast<uint16_t>(tf);
+            ebo[nc] = static_cast<uint16_t>(tb);
+        }
+        __syncthreads();
+        staged = tf <= kSrchCand && tb <= kSrchBCand;
+    }
+    // 3. stage the candidates: copy t belongs to the last cell e with efo[e] <= t
+    if (staged) {
+        const uint32_t Tf = tf, Tb = tb;
+        for (uint32_t t = tid; t < Tf; t += kBlock) {
+            uint32_t lo = 0u, hi = nc;  // efo[lo] <= t < efo[hi]
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (efo[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            cpos[t] = pos[efs[lo] + (t - efo[lo])];
+        }
+        for (uint32_t t = tid; t < Tb; t += kBlock) {
+            uint32_t lo = 0u, hi = nc;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (ebo[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            bcand[t] = bd.spos[ebs[lo] + (t - ebo[lo])];
+        }
+        __syncthreads();
+    }
+    // 4. the search proper
+    uint32_t a = 0u, b = 0u;
+    if (live) {
+        if (staged) {
+            const float ri = xi.w * 4.0f;
+            for (int t = 0; t < 27; t++) {
+                const i3 q = cell_offset(p, t);
+                const uint32_t c = grid_index(q.x, q.y, q.z, g);
+                uint32_t h = cell_hash(c);
+                for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
+                if (ckey[h] != c) continue;  // not reached: every cell of a run was registered by its first particle
+                const uint32_t e = cent[h];
+                const uint32_t f0 = efo[e], f1 = efo[e + 1], s0 = efs[e];
+                for (uint32_t u = f0; u < f1; u++) {
+                    if (within(xi, ri, cpos[u])) {
+                        const uint32_t j = s0 + (u - f0);
+                        if (j != i) {
+                            if (a < Kff) nff[size_t(a) * N + i] = j;
+                            a++;
+                        }
+                    }
+                }
+                if (bd.n) {
+                    const uint32_t w0 = ebo[e], w1 = ebo[e + 1], z0 = ebs[e];
+                    for (uint32_t u = w0; u < w1; u++) {
+                        if (within(xi, ri, bcand[u])) {
+                            const uint32_t s = z0 + (u - w0);
+                            const uint32_t j = bd.id ? bd.id[s] : s;
+                            if (b < Kfb) nfb[size_t(b) * N + i] = j;
+                            b++;
+                        }
+                    }
+                }
+            }
+        } else {
+            a = search_cells<true, true>(xi, i, g, fstart, fend, fl.id, fl.spos, nff, Kff, N);
+            if (bd.n) b = search_cells<false, true>(xi, i, g, bstart, bend, bd.id, bd.spos, nfb, Kfb, N);
+        }
+    }
+    if (i < N) {
         cff[i] = a;
         cfb[i] = b;
     }
-    report_counts(a, &ctr->max_ff, &ctr->nnz_ff);
-    report_counts(b, &ctr->max_fb, &ctr->nnz_fb);
+    const uint32_t cnt[2] = {a, b};
+    block_counts<2>(cnt, bstat, gridDim.x, blk);
 }
 
 __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __restrict__ pos, uint32_t B,
                                                              const uint32_t* __restrict__ scene, uint32_t C, HashGeom g,
                                                              HashView bd, uint32_t* __restrict__ nbb, uint32_t Kbb,
-                                                             uint32_t* __restrict__ cbb, NbrCounters* __restrict__ ctr) {
+                                                             uint32_t* __restrict__ cbb, uint2* __restrict__ bstat) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t a = 0u;
     if (i < B) {
@@ -395,7 +622,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __r
         a = search_cells<true, false>(pos[i], i, g, bd.start + base, bd.end + base, bd.id, bd.spos, nbb, Kbb, B);
         cbb[i] = a;
     }
-    report_counts(a, &ctr->max_bb, &ctr->nnz_bb);
+    const uint32_t cnt[1] = {a};
+    block_counts<1>(cnt, bstat, gridDim.x, blockIdx.x);
 }
 
 // Z-sort payload permutation (MPSSpatialHash.cu:101-140): out[s] = in[perm[s]].
@@ -498,13 +726,20 @@ void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* s
     if (n) k_cell_table<<<nb(n), kBlock, 0, s>>>(key, id, n, start, end, pos, spos);
 }
 void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const HashGeom& g, const HashView& fl, const HashView& bd,
-                     uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, NbrCounters* ctr,
-                     hipStream_t s) {
-    if (sc.N) k_neighbors_fluid<<<nb(sc.N), kBlock, 0, s>>>(pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, ctr);
+                     uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, uint2* bstat,
+                     NbrCounters* ctr, hipStream_t s) {
+    if (!sc.N) return;
+    const uint32_t g_ = nb(sc.N);
+    k_neighbors_fluid<<<g_, kBlock, 0, s>>>(pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, bstat);
+    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat, g_, &ctr->max_ff, &ctr->nnz_ff);
+    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat + g_, g_, &ctr->max_fb, &ctr->nnz_fb);
 }
 void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const HashGeom& g, const HashView& bd,
-                        uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s) {
-    if (B) k_neighbors_boundary<<<nb(B), kBlock, 0, s>>>(pos, B, scene, C, g, bd, nbb, Kbb, cbb, ctr);
+                        uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, uint2* bstat, NbrCounters* ctr, hipStream_t s) {
+    if (!B) return;
+    const uint32_t g_ = nb(B);
+    k_neighbors_boundary<<<g_, kBlock, 0, s>>>(pos, B, scene, C, g, bd, nbb, Kbb, cbb, bstat);
+    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat, g_, &ctr->max_bb, &ctr->nnz_bb);
 }
 void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s) {
     if (n) k_gather_f4<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);

@@ -111,11 +111,14 @@ void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s);
 void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
                 float4* spos, hipStream_t s);
+// bstat: scratch of 2 * ceil(N / kBlock) (fluid) or ceil(B / kBlock) (walls) per-workgroup
+// count records, folded into ctr after the search
 void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const dev::HashGeom& g, const HashView& fl,
                      const HashView& bd, uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb,
-                     NbrCounters* ctr, hipStream_t s);
+                     uint2* bstat, NbrCounters* ctr, hipStream_t s);
 void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const dev::HashGeom& g,
-                        const HashView& bd, uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, NbrCounters* ctr, hipStream_t s);
+                        const HashView& bd, uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, uint2* bstat, NbrCounters* ctr,
+                        hipStream_t s);
 void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s);
 void gather_u32(const uint32_t* perm, uint32_t n, const uint32_t* in, uint32_t* out, hipStream_t s);
 void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, hipStream_t s);

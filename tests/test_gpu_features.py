@@ -81,6 +81,15 @@ def test_hash_smaller_than_domain():
     _phases_match(g, o, "wrap", steps=1)
 
 
+def test_search_global_fallback_coarse_cells():
+    """Cells twice the support radius put ~8x more candidates in every 27-cell neighbourhood,
+    more than the search stages in LDS, so blocks search the global cell tables: same lists."""
+    sc = scenes.dam_break_cpu(n_side=12)
+    sc.config["cell_size"] = 2.0 * sc.config["cell_size"]
+    g, o = make_pair(sc)
+    _phases_match(g, o, "coarse", steps=1)
+
+
 def test_neighbour_capacity_regrow():
     sc = scenes.dam_break_cpu(n_side=10)
     sc.config["nei_capacity"] = 8
