@@ -9,12 +9,16 @@ viscosity 0.01, surface tension 0.05, FP32.
 A "step" is one System::OnUpdate: hash, neighbour lists, boundary volume, density/alpha,
 divergence-free solve, viscosity + surface-tension CG, constant-density solve, advection.
 `value` = fluid particles x timed steps / wall time of the timed steps (inputs resident in
-HBM). The timed steps are steps 0..K-1 of the workload: the W warm-up steps run first and
-the scene is then reloaded into the same context (no device allocation inside the region).
+HBM). The timed steps are steps 0..K-1 of the workload: the W warm-up steps run first (then
+the library's autotuner picks graph replay and the host-poll interval) and the scene is
+reloaded into the same context (no device allocation inside the region). The timed region
+has no per-kernel events; the same steps then run again from the initial state with HIP
+events around every kernel on the context's stream, which gives the per-kernel times.
 
-Prints ONE JSON line. Extra objects: `roofline` (dominant kernel, algorithmic bytes from
-the measured neighbour counts and iteration counts / its live HIP-event time), `cpu_baseline`
-(the FP32 CPU oracle on a bounded sample of the same workload), per-kernel table.
+Prints ONE JSON line. Extra objects: `roofline` (dominant kernel: algorithmic bytes from the
+measured neighbour counts and iteration counts / its HIP-event time; `traffic` = its HBM bytes
+per launch from the committed rocprofv3 counter summary), `cpu_baseline` (the FP32 CPU
+oracle on a bounded sample of the same workload), per-kernel table.
 """
 from __future__ import annotations
 
@@ -57,6 +61,24 @@ def kernel_bytes(name: str, N: int, Eff: float, Efb: float) -> float | None:
         "k_advect": 48 * N,
     }
     return table.get(name)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_hbm_bytes.md")
+
+
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
+    rocprofv3 counter passes, each scaled by what the counter reads for a plain copy of known
+    size (the summary's last column, MB per dispatch, largest dispatch)."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) >= 5 and cells[0] == kernel:
+                    return float(cells[-1]) * 1.0e6
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def step_bytes(N, B, Eff, Efb, Ebb, l_df, l_cd, l_cg, cap, C):
@@ -234,11 +256,16 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
     ap.add_argument("--cpu-max-steps", type=int, default=3)
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps of the profiled replay that times every kernel (0 = as many as --steps)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip it)")
     ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched leg")
     ap.add_argument("--adjoint-particles", type=int, default=65536, help="droplet of the adjoint leg (0 = skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--graphs", default="auto", choices=("auto", "0", "1"),
+                    help="replay solver iteration chunks from HIP graphs: auto = the library's autotuner "
+                         "picks graphs on/off and the host-poll interval after the warmup")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, launch the library's plain copy kernel (k_calib_copy) 3 times on "
                          "512 MiB so hardware byte counters can be scaled against a copy of known size")
@@ -253,29 +280,44 @@ def main() -> int:
 
     scene = scenes.dam_break_1m(n_side=args.n_side, steps=args.steps)
     cfg = scenes.make_config(scene)
+    if args.graphs != "auto":
+        cfg.use_graphs = int(args.graphs)
     sim = HipSim(cfg)
     scenes.load_into(sim, scene)
     for _ in range(args.warmup):
         sim.step()
-    scenes.load_into(sim, scene)
-    sim.synchronize()
-    if not args.no_profile:
-        sim.profile(True)
+    tuned = sim.autotune(2) if args.graphs == "auto" else None
+
+    def run(steps: int, profile: bool):
+        """Steps 0..steps-1 of the workload from its initial state: (seconds, per-step stats,
+        halo stats, per-kernel HIP-event times when profiling)."""
+        scenes.load_into(sim, scene)
+        sim.synchronize()
+        sim.profile(profile)
+        stats, halo = [], []
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = sim.step()
+            stats.append((st.l_df, st.l_cd, st.l_cg, st.nnz_ff, st.nnz_fb, st.nnz_bb, st.hit_cap_cd,
+                          st.err_cd[max(0, min(st.l_cd, 128) - 1)], st.nonfinite, st.n_boundary))
+            halo.append((st.halo_max, st.halo_spill, st.halo_blocks))
+        sim.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = sim.profile_read() if profile else {}
+        sim.profile(False)
+        return elapsed, stats, halo, prof
 
     N = scene.n
-    stats, halo = [], []
-    sim.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = sim.step()
-        stats.append((st.l_df, st.l_cd, st.l_cg, st.nnz_ff, st.nnz_fb, st.nnz_bb, st.hit_cap_cd,
-                      st.err_cd[max(0, min(st.l_cd, 128) - 1)], st.nonfinite, st.n_boundary))
-        halo.append((st.halo_max, st.halo_spill, st.halo_blocks))
-    sim.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
+    # the throughput: no per-kernel events (they add ~1.5 ms per step on this workload)
+    elapsed, stats, halo, _ = run(args.steps, False)
     value = N * args.steps / elapsed
-    prof = sim.profile_read() if not args.no_profile else {}
+    # per-kernel times: the same steps again from the initial state, with HIP events around
+    # every kernel on the context's stream
+    prof_steps = 0 if args.no_profile else (args.profile_steps or args.steps)
+    prof, pstats, p_elapsed = {}, [], 0.0
+    if prof_steps:
+        p_elapsed, pstats, _, prof = run(prof_steps, True)
 
     cap = cfg.max_iter
     C = cfg.hash_size[0] * cfg.hash_size[1] * cfg.hash_size[2]
@@ -284,6 +326,7 @@ def main() -> int:
     kbytes = {}
     for (l_df, l_cd, l_cg, eff, efb, ebb, _, _, _, _) in stats:
         tot_step_bytes += step_bytes(N, B, eff, efb, ebb, l_df, l_cd, l_cg, cap, C)
+    for (l_df, l_cd, l_cg, eff, efb, ebb, _, _, _, _) in pstats:
         uses = {
             "k_cg_apply<1>": max(min(l_cg - 1, cfg.cg_max_iter), 0), "k_cg_apply<0>": 1,
             "k_pressure_eval<DF>": min(l_df, cap), "k_pressure_eval<CD>": min(l_cd, cap),
@@ -310,10 +353,13 @@ def main() -> int:
         achieved = kbytes[dom] / (p["total_ms"] * 1e-3) / 1e9
         roofline = {
             "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+            "algorithmic_bytes_per_launch": kbytes[dom] / max(p["useful"], 1),
             "avg_launch_us": 1000.0 * p["total_ms"] / max(p["launches"], 1),
             "launches": p["launches"], "useful_launches": p["useful"],
             "share_of_gpu_time": p["total_ms"] / max(sum(q["total_ms"] for q in prof.values()), 1e-9),
+            "profiled_steps": prof_steps, "profiled_ms_per_step": 1000.0 * p_elapsed / prof_steps,
             "step_algorithmic_GBps": tot_step_bytes / elapsed / 1e9,
             "step_frac": tot_step_bytes / elapsed / 1e9 / HBM_PEAK_GBS,
         }
@@ -347,6 +393,7 @@ def main() -> int:
                         f"3.0x1.5x1.5 m tank, cubic kernel h=0.02 m, dt=0.5 ms, CD tol 1e-3, DF tol 1e-2, "
                         f"viscosity 0.01 + surface tension 0.05 (CG), steps 0..{args.steps - 1}",
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
+            "solver_tuning": tuned,
         },
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,
         "adjoint": adjoint,
