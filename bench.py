@@ -66,6 +66,10 @@ def kernel_bytes(name: str, N: int, Eff: float, Efb: float) -> float | None:
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dam_break_1m_hbm_bytes.md")
 
 
+def say(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(kernel: str) -> float | None:
     """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
     rocprofv3 counter passes, each scaled by what the counter reads for a plain copy of known
@@ -312,12 +316,14 @@ def main() -> int:
     # the throughput: no per-kernel events (they add ~1.5 ms per step on this workload)
     elapsed, stats, halo, _ = run(args.steps, False)
     value = N * args.steps / elapsed
+    say(f"timed region: {args.steps} steps, {1000.0 * elapsed / args.steps:.3f} ms per step")
     # per-kernel times: the same steps again from the initial state, with HIP events around
     # every kernel on the context's stream
     prof_steps = 0 if args.no_profile else (args.profile_steps or args.steps)
     prof, pstats, p_elapsed = {}, [], 0.0
     if prof_steps:
         p_elapsed, pstats, _, prof = run(prof_steps, True)
+        say(f"profiled replay: {prof_steps} steps, {1000.0 * p_elapsed / prof_steps:.3f} ms per step")
 
     cap = cfg.max_iter
     C = cfg.hash_size[0] * cfg.hash_size[1] * cfg.hash_size[2]
@@ -367,11 +373,16 @@ def main() -> int:
     sim.profile(False)
     sim.close()
     batched = batched_leg(args.batch_scenes, args.batch_steps, args.warmup) if args.batch_scenes > 0 else None
+    if batched:
+        say(f"batched leg: {batched.get('speedup', batched)}")
     adjoint = adjoint_leg(args.adjoint_particles, args.adjoint_steps, 10) if args.adjoint_particles > 0 else None
+    if adjoint:
+        say("adjoint leg done")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
+        say(f"cpu baseline: {cpu['value']:.3e} particle-steps/s")
 
     n = len(stats)
     iters = {
