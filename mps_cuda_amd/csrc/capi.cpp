@@ -160,7 +160,9 @@ struct mps_ctx {
     DBuf<float2> cabT;
     DBuf<double> acc_mu, acc_sig;
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
-    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgq, cgAp, jprev, jt[3], jcand, tmp4;
+    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
+    // the forward CG's vectors as packed triples: x, r, A p and two p buffers
+    DBuf<mps::dev::f3> v3x, v3r, v3Ap, v3p, v3q;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
     // second buffer of each array the hash reorders: the two alternate, so the arrays a solver
     // chunk reads take one of two addresses (cached solver graphs stay valid)
@@ -799,7 +801,7 @@ struct mps_ctx {
         const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
         pairAB.reserve(size_t(Kff) * m);
-        cgq.reserve(m);
+        v3x.reserve(m); v3r.reserve(m); v3Ap.reserve(m); v3p.reserve(m); v3q.reserve(m);
         ensure_halo();
         const mps::Halo hl = halo();
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
@@ -809,10 +811,10 @@ struct mps_ctx {
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         // p of iteration k is written to pbuf(k) while pbuf(k - 1) is read; p_0 = r
-        auto pbuf = [&](uint32_t k) { return (k & 1u) ? cgq.p : cgp.p; };
+        auto pbuf = [&](uint32_t k) { return (k & 1u) ? v3q.p : v3p.p; };
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, pos.p, cgx.p, nullptr, pbuf(0), nullptr, cgb.p, cgr.p, sd,
-                                  partials.p, stream);
+            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, pos.p, cgx.p, nullptr, nullptr, pbuf(0), nullptr, cgb.p, v3r.p,
+                                  v3x.p, sd, partials.p, stream);
         });
         T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
@@ -823,7 +825,7 @@ struct mps_ctx {
             chunk = std::min(chunk, cap - it + 1);
             std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
             for (const void* q : {(const void*)pairAB.p, (const void*)pos.p, (const void*)pbuf(it - 1), (const void*)pbuf(it),
-                                  (const void*)cgAp.p, (const void*)cgr.p, (const void*)cgx.p, (const void*)sd,
+                                  (const void*)v3Ap.p, (const void*)v3r.p, (const void*)v3x.p, (const void*)sd,
                                   (const void*)partials.p, (const void*)act, (const void*)hl.idx, (const void*)hl.n,
                                   (const void*)hl.nb16})
                 key.push_back(u(q));
@@ -832,12 +834,12 @@ struct mps_ctx {
                 for (uint32_t c = 0; c < chunk; c++) {
                     const uint32_t k = it + c;
                     T_(K_CG_APPLY, [&] {
-                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, pos.p, cgr.p, pbuf(k - 1), pbuf(k), cgAp.p, nullptr,
-                                              nullptr, sd, partials.p, stream);
+                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, pos.p, nullptr, v3r.p, pbuf(k - 1), pbuf(k), v3Ap.p,
+                                              nullptr, nullptr, nullptr, sd, partials.p, stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream); }, false);
                     T_(K_CG_UPDATE, [&] {
-                        mps::launch::cg_update_p(P, cgx.p, cgr.p, pbuf(k), cgAp.p, sd, partials.p, stream);
+                        mps::launch::cg_update3(P, v3x.p, v3r.p, pbuf(k), v3Ap.p, sd, partials.p, stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(2, P, partials.p, sd, act, tol, stream); }, false);
                 }
@@ -853,7 +855,7 @@ struct mps_ctx {
             kuse[K_CG_UPDATE] += std::min(l - 1, cap);
             kuse[K_FOLD] += 2 * std::min(l - 1, cap);
         }
-        T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out(P, cgx.p, vel.p, stream); });
+        T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out3(P, v3x.p, vel.p, stream); });
         for (uint32_t q = 0; q < S && q < sstats.size(); q++) {
             mps_step_stats& o = sstats[q];
             const SolverDev& r = h_solver[2 * size_t(S) + q];
@@ -1851,7 +1853,8 @@ struct mps_ctx {
     }
 
     ~mps_ctx() {
-        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release(); cgq.release();
+        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release();
+        v3x.release(); v3r.release(); v3Ap.release(); v3p.release(); v3q.release();
         cgAp.release(); jprev.release(); for (auto& t : jt) t.release(); jcand.release(); tmp4.release();
         rho.release(); alpha.release(); kappa.release(); delta.release(); fst.release(); tmp1.release();
         pos_alt.release(); vel_alt.release(); color_alt.release(); cgb_alt.release(); rho_alt.release();

@@ -11,6 +11,7 @@
 namespace mps {
 namespace dev {
 struct HashGeom;
+struct f3;
 }
 
 // Read-only view of one set's hash (cell table + ids and positions in key order). For the
@@ -149,12 +150,13 @@ void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, con
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
 // CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
-// mode 0: A x0 with f = 0 (x0 in u), then r = b - A x0, pout = r, partial r.r;
-// mode 1: p = r + beta pprev (r in u, beta from the record), A p with f into Ap, p into pout,
+// The forward solve's vectors are packed triples (dev::f3).
+// mode 0: A x0 with f = 0 (x0 from the float4 array x0), then r = b - A x0, pout = r, x = x0, partial r.r;
+// mode 1: p = r + beta pprev (r in rr, beta from the record), A p with f into Ap, p into pout,
 // partial p.Ap. Halo records are staged in LDS (kHaloCap).
 void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
-              const float4* u, const float4* pprev, float4* pout, float4* Ap, const float4* b, float4* r,
-              const SolverDev* st, double* partials, hipStream_t s);
+              const float4* x0, const dev::f3* rr, const dev::f3* pprev, dev::f3* pout, dev::f3* Ap, const float4* b,
+              dev::f3* r, dev::f3* x, const SolverDev* st, double* partials, hipStream_t s);
 // ---- halo.hip
 // stats (3 words, accumulated): largest staged halo, blocks over kHaloCap, blocks with particles
 // limit: largest halo staged (0 or > kHaloCap: kHaloCap)
@@ -164,9 +166,10 @@ void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, 
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s);
-// the same with p as one float4 per slot
-void cg_update_p(const Scenes& sc, float4* x, float4* r, const float4* p, const float4* Ap, const SolverDev* st,
-                 double* partials, hipStream_t s);
+// the same on packed triples (the forward solve), and its copy of x into the velocities
+void cg_update3(const Scenes& sc, dev::f3* x, dev::f3* r, const dev::f3* p, const dev::f3* Ap, const SolverDev* st,
+                double* partials, hipStream_t s);
+void copy_vel_out3(const Scenes& sc, const dev::f3* x, float4* vel, hipStream_t s);
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);
 void copy_vel_out(const Scenes& sc, const float4* x, float4* vel, hipStream_t s);
 void jacobi_candidate(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,

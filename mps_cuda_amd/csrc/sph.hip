@@ -567,16 +567,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 // pass of its own; the block writes its own p for the update kernel (pout, while pprev is
 // read: the host alternates the two buffers). beta = 0 in the first iteration, where
 // pprev = r, so p = r.
-// MODE 0: A x0 with f = 0 (x0 in u) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
-//         r = b - A x0, pout = r, partial sums of r.r.
-// MODE 1: A p with f (r in u), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
+// The CG vectors are packed 12-byte triples (f3).
+// MODE 0: A x0 with f = 0 (x0 in x0) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
+//         r = b - A x0, pout = r, x = x0, partial sums of r.r.
+// MODE 1: A p with f (r in rr), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
-                                                   const float4* __restrict__ pos, const float4* __restrict__ u,
-                                                   const float4* __restrict__ pprev, float4* __restrict__ pout,
-                                                   float4* __restrict__ Ap, const float4* __restrict__ b,
-                                                   float4* __restrict__ r, const SolverDev* st,
-                                                   double* __restrict__ partials) {
+                                                   const float4* __restrict__ pos, const float4* __restrict__ x0,
+                                                   const f3* __restrict__ rr, const f3* __restrict__ pprev,
+                                                   f3* __restrict__ pout, f3* __restrict__ Ap,
+                                                   const float4* __restrict__ b, f3* __restrict__ r, f3* __restrict__ x,
+                                                   const SolverDev* st, double* __restrict__ partials) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t sid = sc.blk_scene[blk];
     if (MODE == 1 && st[sid].done) return;
@@ -588,9 +589,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     const uint32_t c = live ? nb.cff[i] : 0u;
     // the vector the operator is applied to, at slot j
     auto vec = [&](uint32_t j) -> f3 {
-        const float4 a = u[j];
-        if (MODE == 0) return mk(a);
-        const float4 q = pprev[j];
+        if (MODE == 0) return mk(x0[j]);
+        const f3 a = rr[j], q = pprev[j];
         return {a.x + bta * q.x, a.y + bta * q.y, a.z + bta * q.z};
     };
     __shared__ float4 la[kHaloCap];
@@ -661,12 +661,13 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
         if (MODE == 0) {
             const float4 bi = b[i];
             const f3 ri{bi.x - o.x, bi.y - o.y, bi.z - o.z};
-            r[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
-            pout[i] = make_float4(ri.x, ri.y, ri.z, 0.0f);
+            r[i] = ri;
+            pout[i] = ri;
+            x[i] = pi;
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
-            pout[i] = make_float4(pi.x, pi.y, pi.z, 0.0f);
-            Ap[i] = make_float4(o.x, o.y, o.z, 0.0f);
+            pout[i] = pi;
+            Ap[i] = o;
             part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
     }
@@ -712,7 +713,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const doubl
 }
 
 // UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121).
-// REC: p in the 32-byte records (the adjoint's solve), else one float4 per slot.
+// p in the 32-byte records (the adjoint's transposed solve).
 template <bool REC>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restrict__ x, float4* __restrict__ r,
                                                     const float4* __restrict__ prec, const float4* __restrict__ Ap,
@@ -727,6 +728,32 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restr
         const f3 pi = REC ? rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]) : mk(prec[i]);
         const float4 api = Ap[i];
         float4 xi = x[i], ri = r[i];
+        xi.x = xi.x + a * pi.x;
+        xi.y = xi.y + a * pi.y;
+        xi.z = xi.z + a * pi.z;
+        ri.x = ri.x - a * api.x;
+        ri.y = ri.y - a * api.y;
+        ri.z = ri.z - a * api.z;
+        x[i] = xi;
+        r[i] = ri;
+        part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
+    }
+    block_partial(part, partials);
+}
+
+// The update on packed triples (the forward solve): same expressions as k_cg_update.
+__global__ void __launch_bounds__(kBlock) k_cg_update3(Scenes sc, f3* __restrict__ x, f3* __restrict__ r,
+                                                     const f3* __restrict__ p, const f3* __restrict__ Ap,
+                                                     const SolverDev* st, double* __restrict__ partials) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    if (st[sid].done) return;
+    const uint32_t end = sc.P[sid].end;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float a = st[sid].alpha;
+    double part = 0.0;
+    if (i < end) {
+        const f3 pi = p[i], api = Ap[i];
+        f3 xi = x[i], ri = r[i];
         xi.x = xi.x + a * pi.x;
         xi.y = xi.y + a * pi.y;
         xi.z = xi.z + a * pi.z;
@@ -757,6 +784,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_direction(Scenes sc, const float4
     prec[2 * size_t(i) + 1] = b;
 }
 
+__global__ void __launch_bounds__(kBlock) k_copy_vel_out3(Scenes sc, const f3* __restrict__ x, float4* __restrict__ vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sc.P[sc.blk_scene[blockIdx.x]].end) return;
+    const f3 xi = x[i];
+    float4 v = vel[i];
+    v.x = xi.x;
+    v.y = xi.y;
+    v.z = xi.z;
+    vel[i] = v;
+}
 __global__ void __launch_bounds__(kBlock) k_copy_vel_out(Scenes sc, const float4* __restrict__ x, float4* __restrict__ vel) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= sc.P[sc.blk_scene[blockIdx.x]].end) return;
@@ -1030,12 +1067,12 @@ void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipSt
     if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
 }
 void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
-              const float4* u, const float4* pprev, float4* pout, float4* Ap, const float4* b, float4* r,
+              const float4* x0, const f3* rr, const f3* pprev, f3* pout, f3* Ap, const float4* b, f3* r, f3* x,
               const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, u, pprev, pout, Ap, b, r, st, partials);
-    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, u, pprev, pout, Ap, b, r, st, partials);
+    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
+    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
 }
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;
@@ -1047,9 +1084,12 @@ void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const
                double* partials, hipStream_t s) {
     if (sc.N) k_cg_update<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
 }
-void cg_update_p(const Scenes& sc, float4* x, float4* r, const float4* p, const float4* Ap, const SolverDev* st,
-                 double* partials, hipStream_t s) {
-    if (sc.N) k_cg_update<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, p, Ap, st, partials);
+void cg_update3(const Scenes& sc, f3* x, f3* r, const f3* p, const f3* Ap, const SolverDev* st, double* partials,
+                hipStream_t s) {
+    if (sc.N) k_cg_update3<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, p, Ap, st, partials);
+}
+void copy_vel_out3(const Scenes& sc, const f3* x, float4* vel, hipStream_t s) {
+    if (sc.N) k_copy_vel_out3<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, vel);
 }
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
     if (sc.N) k_cg_direction<<<nb_(sc.N), kBlock, 0, s>>>(sc, r, prec, st);
