@@ -660,6 +660,7 @@ struct mps_ctx {
         T_(K_BVOL, [&] { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); });
     }
     void density_alpha(int flags) {
+        ensure_halo();
         if (flags & (mps::launch::kDoAlpha | mps::launch::kDoPairs)) {
             flags |= mps::launch::kDoPairs;
             const size_t m = std::max<uint32_t>(N, 1);
@@ -667,7 +668,7 @@ struct mps_ctx {
             pairG_fb.reserve(size_t(Kfb) * m);
         }
         T_(K_DENSITY, [&] {
-            mps::launch::density_alpha(cubic(), flags, scn(), nbr(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
+            mps::launch::density_alpha(cubic(), flags, scn(), nbr(), halo(), pos.p, vel.p, bpos.p, bvol.p, rho.p, alpha.p, color.p,
                                        pairG_ff.p, pairG_fb.p, stream);
         });
         if (flags & mps::launch::kDoPairs) pairs_valid = true;
@@ -804,9 +805,9 @@ struct mps_ctx {
         v3x.reserve(m); v3r.reserve(m); v3Ap.reserve(m); v3p.reserve(m); v3q.reserve(m);
         ensure_halo();
         const mps::Halo hl = halo();
-        T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
+        T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), hl, pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
         T_(K_CG_SETUP, [&] {
-            mps::launch::cg_setup(cubic(), P, nbr(), pos.p, vel.p, rho.p, fst.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
+            mps::launch::cg_setup(cubic(), P, nbr(), hl, pos.p, vel.p, rho.p, fst.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
         });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
@@ -1069,7 +1070,7 @@ struct mps_ctx {
         pairG_fb.reserve(size_t(T.Kfb) * m);
         pairs_valid = false;
         halo_valid = false;
-        mps::launch::density_alpha(1, mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb, T.pos.p,
+        mps::launch::density_alpha(1, mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb, mps::Halo{}, T.pos.p,
                                    T.snaps[0].p, T.bpos.p, T.bvol.p, rho.p, alpha.p, color.p, pairG_ff.p, pairG_fb.p, stream);
         abar.reserve(m); rbar.reserve(m); dbar.reserve(m); fbar.reserve(m); fstr.reserve(m);
         aux4.reserve(m); gvec.reserve(m); avin.reserve(m); adx2.reserve(m); adv2.reserve(m);
@@ -1098,8 +1099,8 @@ struct mps_ctx {
             const float4* vst = T.snaps[T.cg_out].p;
             pairAB.reserve(size_t(T.Kff) * m); cabT.reserve(size_t(T.Kff) * m); rev.reserve(size_t(T.Kff) * m);
             cgrecB.reserve(2 * m);
-            mps::launch::st_factor(P, nb, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
-            mps::launch::cg_setup(1, P, nb, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
+            mps::launch::st_factor(P, nb, mps::Halo{}, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
+            mps::launch::cg_setup(1, P, nb, mps::Halo{}, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
             mps::launch::zero_words(d_flags + 2, 1, stream);
             mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
             mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);

@@ -131,7 +131,7 @@ void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb
                   hipStream_t s);
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
 // density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
-void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel,
+void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s);
 void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
@@ -143,10 +143,10 @@ void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint
 void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s);
-void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
+void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s);
 // pair coefficients of the implicit operator: a_ij into ca and/or (a_ij, b_ij) into cab (either may be null)
-void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
 // CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);

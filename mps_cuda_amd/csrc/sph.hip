@@ -56,6 +56,10 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 // occupancy is bounded by LDS, so registers are free for it).
 constexpr uint32_t kListBatch = 4;
 
+// Halo count of workgroup blk: > kHaloCap (not staged) when there are no halos (hl.n null: the
+// adjoint's kernels run on its taped lists).
+__device__ __forceinline__ uint32_t halo_count(const Halo& hl, uint32_t blk) { return hl.n ? hl.n[blk] : kHaloCap + 1u; }
+
 // ----------------------------------------------------------------- boundary volume
 // ComputeBoundaryParticleVolume_kernel (MPSSPHBaseUtil.cuh:10-41): S_b = W(0) + sum W_bk.
 // S_b depends only on boundary positions, so it is computed when they change.
@@ -92,18 +96,43 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, Nbr nb, const float4* __restrict__ pos,
+__global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
                                                         float* __restrict__ alpha, float4* __restrict__ color,
                                                         float* __restrict__ pG_ff, float* __restrict__ pG_fb) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= P.end) return;
+    const bool live = i < P.end;
     const bool dens = flags & launch::kDoDensity, alph = flags & launch::kDoAlpha;
     const bool pairs = flags & launch::kDoPairs;
+    const bool walk = dens || alph || pairs;
+    const uint32_t c = live && walk ? nb.cff[i] : 0u;
+    // the block's halo positions in LDS (see k_cg_apply)
+    __shared__ float4 la[kHaloCap];
+    const uint32_t H = halo_count(hl, blk);
+    const bool staged = walk && H <= kHaloCap;
+    constexpr uint32_t U = kListBatch;
+    uint32_t jn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) la[t] = pos[hs[t]];
+        __syncthreads();
+    }
+    if (!live) return;
     const float4 xi4 = pos[i];
     const f3 xi = mk(xi4);
     const float hi = xi4.w * 4.0f;
@@ -111,8 +140,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
     float sf = dens ? P.V * Wk<CUBIC>(0.0f, invHi) : 0.0f;
     f3 gf{0.f, 0.f, 0.f};
     float s = 0.0f;
-    if (dens || alph || pairs) {
-        const uint32_t c = nb.cff[i];
+    if (walk) {
         auto pair = [&](uint32_t k, float4 xj4) {
             const f3 xij = xi - mk(xj4);
             const float dist = len3(xij);
@@ -125,18 +153,19 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
                 s += dot3(grad, grad);
             }
         };
-        uint32_t k = 0;
-        for (; k + 4 <= c; k += 4) {
-            uint32_t j[4];
+        for (uint32_t k = 0; k < c; k += U) {
+            uint32_t j[U];
 #pragma unroll
-            for (int q = 0; q < 4; q++) j[q] = nbr_ff(nb, k + q, i, N);
-            float4 xj[4];
+            for (uint32_t q = 0; q < U; q++) j[q] = jn[q];
+            if (k + U < c) fetch(k + U);
+            if (staged) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) xj[q] = pos[j[q]];
-#pragma unroll
-            for (int q = 0; q < 4; q++) pair(k + q, xj[q]);
+                for (uint32_t q = 0; q < U; q++)
+                    if (k + q < c) pair(k + q, la[j[q]]);
+            } else {
+                for (uint32_t q = 0; q < U && k + q < c; q++) pair(k + q, pos[j[q]]);
+            }
         }
-        for (; k < c; k++) pair(k, pos[nbr_ff(nb, k, i, N)]);
         float sb = 0.0f;
         f3 gb{0.f, 0.f, 0.f};
         const uint32_t cb = nb.cfb[i];
@@ -415,45 +444,71 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
 // ----------------------------------------------------------------- viscosity + surface tension CG
 // ComputeSurfaceTensionFactor_kernel (MPSSPHNonPressureUtil.cuh:425-465) with the stored
 // G_ij; also x0 = v (the copy into pPredictVel, MPSSPHNonPressureUtil.cu:144-147).
-__global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                    const float* __restrict__ pG_ff, float* __restrict__ fst,
-                                                    float4* __restrict__ cgx) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
+__global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
+                                                    const float4* __restrict__ vel, const float* __restrict__ pG_ff,
+                                                    float* __restrict__ fst, float4* __restrict__ cgx) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= P.end) return;
+    const bool live = i < P.end;
+    const uint32_t c = live ? nb.cff[i] : 0u;
+    // the block's halo (x_j, v_j, m_j) in LDS (see k_cg_apply)
+    __shared__ float4 la[kHaloCap], lb[kHaloCap];
+    const uint32_t H = halo_count(hl, blk);
+    const bool staged = H <= kHaloCap;
+    constexpr uint32_t U = kListBatch;
+    uint32_t jn[U];
+    float gn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                gn[q] = __builtin_nontemporal_load(pG_ff + slot);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
+            const uint32_t s = hs[t];
+            const float4 x = pos[s], v = vel[s];
+            la[t] = make_float4(x.x, x.y, x.z, v.x);
+            lb[t] = make_float4(v.y, v.z, v.w, 0.0f);
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     const float4 vi4 = vel[i];
     const f3 xi = mk(pos[i]), vi = mk(vi4);
     float d = 0.0f;
-    const uint32_t c = nb.cff[i];
-    uint32_t k = 0;
-    for (; k + 4 <= c; k += 4) {
-        uint32_t j[4];
-        float G[4];
+    for (uint32_t k = 0; k < c; k += U) {
+        uint32_t j[U];
+        float G[U];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const size_t slot = size_t(k + q) * N + i;
-            j[q] = nb.ff[slot];
-            G[q] = pG_ff[slot];
+        for (uint32_t q = 0; q < U; q++) {
+            j[q] = jn[q];
+            G[q] = gn[q];
         }
-        float4 xj[4], vj[4];
+        if (k + U < c) fetch(k + U);
+        if (staged) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xj[q] = pos[j[q]];
-            vj[q] = vel[j[q]];
+            for (uint32_t q = 0; q < U; q++) {
+                if (k + q < c) {
+                    const float4 e = la[j[q]], g = lb[j[q]];
+                    d += g.z * dot3(vi - f3{e.w, g.x, g.y}, G[q] * (xi - mk(e)));
+                }
+            }
+        } else {
+            for (uint32_t q = 0; q < U && k + q < c; q++) {
+                const float4 vj = vel[j[q]];
+                d += vj.w * dot3(vi - mk(vj), G[q] * (xi - mk(pos[j[q]])));
+            }
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) d += vj[q].w * dot3(vi - mk(vj[q]), G[q] * (xi - mk(xj[q])));
-    }
-    for (; k < c; k++) {
-        const size_t slot = size_t(k) * N + i;
-        const uint32_t j = nb.ff[slot];
-        const f3 xij = xi - mk(pos[j]);
-        const float4 vj4 = vel[j];
-        const f3 vij = vi - mk(vj4);
-        const f3 grad = pG_ff[slot] * xij;
-        d += vj4.w * dot3(vij, grad);
     }
     fst[i] = d;
     cgx[i] = make_float4(vi4.x, vi4.y, vi4.z, 0.0f);
@@ -475,15 +530,48 @@ __device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float
 //   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON),
 //   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
 template <bool CUBIC>
-__global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                   const float* __restrict__ rho, const float* __restrict__ fst,
-                                                   const float* __restrict__ pG_ff, float* __restrict__ ca,
-                                                   float2* __restrict__ cab, float4* __restrict__ cgb) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
+__global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                   const float* __restrict__ fst, const float* __restrict__ pG_ff,
+                                                   float* __restrict__ ca, float2* __restrict__ cab, float4* __restrict__ cgb) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
-    if (i >= P.end) return;
+    const bool live = i < P.end;
+    const uint32_t c = live ? nb.cff[i] : 0u;
+    // the block's halo (x_j, v_j, m_j, rho_j, f_j) in LDS (see k_cg_apply)
+    __shared__ float4 la[kHaloCap], lb[kHaloCap];
+    __shared__ float lc[kHaloCap];
+    const uint32_t H = halo_count(hl, blk);
+    const bool staged = H <= kHaloCap;
+    constexpr uint32_t U = kListBatch;
+    uint32_t jn[U];
+    float gn[U];
+    auto fetch = [&](uint32_t k0) {
+#pragma unroll
+        for (uint32_t q = 0; q < U; q++) {
+            if (k0 + q < c) {
+                const size_t slot = size_t(k0 + q) * N + i;
+                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                gn[q] = __builtin_nontemporal_load(pG_ff + slot);
+            }
+        }
+    };
+    fetch(0);
+    if (staged) {
+        const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
+        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
+            const uint32_t s = hs[t];
+            const float4 x = pos[s], v = vel[s];
+            la[t] = make_float4(x.x, x.y, x.z, v.x);
+            lb[t] = make_float4(v.y, v.z, v.w, rho[s]);
+            lc[t] = fst[s];
+        }
+        __syncthreads();
+    }
+    if (!live) return;
     const float4 xi4 = pos[i];
     const float4 vi4 = vel[i];
     const f3 xi = mk(xi4), vi = mk(vi4);
@@ -495,13 +583,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const fl
     const float cx = P.xsph / P.dt;
     const float cv = 10.0f * P.rho0 * P.mu / di;
     f3 bi{0.f, 0.f, 0.f};
-    const uint32_t c = nb.cff[i];
-    auto pair = [&](size_t slot, float4 xj4, float4 vj4, float dj, float fj, float G) {
-        const f3 xij = xi - mk(xj4);
-        const float mj = vj4.w;
+    auto pair = [&](size_t slot, f3 xj, f3 vj, float mj, float dj, float fj, float G) {
+        const f3 xij = xi - xj;
         const float dist = len3(xij);
         float mij;
-        const float gij = st_g(P, vi - mk(vj4), xij, dist, invHi, mi, mj, di, dj, fi, fj, mij);
+        const float gij = st_g(P, vi - vj, xij, dist, invHi, mi, mj, di, dj, fi, fj, mij);
         const float sij = mij * gij;
         const float wij = mj / dj * Wk<CUBIC>(dist, invHi);
         const float cij = 0.5f * (mi + mj) / dj / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
@@ -510,32 +596,30 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, const fl
         if (cab) cab[slot] = make_float2(aij, -P.dt * cv * cij);
         bi = bi - sij * xij;
     };
-    uint32_t k = 0;
-    for (; k + 4 <= c; k += 4) {
-        uint32_t j[4];
-        float G[4];
+    for (uint32_t k = 0; k < c; k += U) {
+        uint32_t j[U];
+        float G[U];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const size_t slot = size_t(k + q) * N + i;
-            j[q] = nb.ff[slot];
-            G[q] = pG_ff[slot];
+        for (uint32_t q = 0; q < U; q++) {
+            j[q] = jn[q];
+            G[q] = gn[q];
         }
-        float4 xj[4], vj[4];
-        float dj[4], fj[4];
+        if (k + U < c) fetch(k + U);
+        if (staged) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            xj[q] = pos[j[q]];
-            vj[q] = vel[j[q]];
-            dj[q] = rho[j[q]];
-            fj[q] = fst[j[q]];
+            for (uint32_t q = 0; q < U; q++) {
+                if (k + q < c) {
+                    const float4 e = la[j[q]], g = lb[j[q]];
+                    pair(size_t(k + q) * N + i, mk(e), f3{e.w, g.x, g.y}, g.z, g.w, lc[j[q]], G[q]);
+                }
+            }
+        } else {
+            for (uint32_t q = 0; q < U && k + q < c; q++) {
+                const uint32_t jj = j[q];
+                const float4 vj = vel[jj];
+                pair(size_t(k + q) * N + i, mk(pos[jj]), mk(vj), vj.w, rho[jj], fst[jj], G[q]);
+            }
         }
-#pragma unroll
-        for (int q = 0; q < 4; q++) pair(size_t(k + q) * N + i, xj[q], vj[q], dj[q], fj[q], G[q]);
-    }
-    for (; k < c; k++) {
-        const size_t slot = size_t(k) * N + i;
-        const uint32_t j = nb.ff[slot];
-        pair(slot, pos[j], vel[j], rho[j], fst[j], pG_ff[slot]);
     }
     const f3 b = (P.dt * P.sigma / mi) * bi + vi;
     cgb[i] = make_float4(b.x, b.y, b.z, 0.0f);
@@ -1029,12 +1113,12 @@ void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
     if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
 }
-void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const float4* pos, float4* vel,
+void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
-    else k_density_alpha<false><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    if (cubic) k_density_alpha<true><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    else k_density_alpha<false><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
 void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
@@ -1053,15 +1137,15 @@ void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float
                     hipStream_t s) {
     if (sc.N) k_pressure_apply<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
-void st_factor(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
+void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {
-    if (sc.N) k_st_factor<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, pG_ff, fst, cgx);
+    if (sc.N) k_st_factor<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, pG_ff, fst, cgx);
 }
-void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float* rho,
+void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
-    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
 }
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
     if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
