@@ -15,13 +15,14 @@
 namespace mps {
 namespace dev {
 
-constexpr uint32_t kHaloHash = 4096;           // LDS hash slots; occupancy stays below kHaloCap + kBlock
+constexpr uint32_t kHaloHashBits = 11;
+constexpr uint32_t kHaloHash = 1u << kHaloHashBits;  // LDS hash slots; occupancy stays below kHaloCap + kBlock
 constexpr uint32_t kHaloSort = 2048;           // power of two >= kHaloCap
 constexpr uint32_t kNoKey = 0xFFFFFFFFu;
 static_assert(kHaloCap <= kHaloSort && kHaloCap + kBlock < kHaloHash, "halo table sizes");
 static_assert(kHaloCap <= 65535u, "local positions are 16-bit");
 
-__device__ __forceinline__ uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32u - 12u); }
+__device__ __forceinline__ uint32_t halo_hash(uint32_t j) { return (j * 2654435761u) >> (32u - kHaloHashBits); }
 
 // Slot of key j in the table (j must be present).
 __device__ __forceinline__ uint32_t halo_find(const uint32_t* key, uint32_t j) {

@@ -315,13 +315,6 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
                                                  uint32_t stride) {
     const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
     const float ri = xi.w * 4.0f;
-    uint32_t cs[27], ce[27];
-#pragma unroll
-    for (int t = 0; t < 27; t++) {
-        const uint32_t c = grid_index(p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1, g);
-        cs[t] = start[c];
-        ce[t] = end[c];
-    }
     uint32_t k = 0u;
     auto test = [&](float4 q, uint32_t s) {
         if (within(xi, ri, q)) {
@@ -332,10 +325,13 @@ __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const Ha
             }
         }
     };
-#pragma unroll
+    // the 27 cells in the reference order; the loop stays rolled (an unrolled walk does not fit
+    // the instruction cache)
+#pragma unroll 1
     for (int t = 0; t < 27; t++) {
-        uint32_t s = cs[t];
-        const uint32_t e = ce[t];
+        const uint32_t c = grid_index(p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1, g);
+        uint32_t s = start[c];
+        const uint32_t e = end[c];
         // an empty cell has start = end = 0xFFFFFFFF: only e - s (never s + 1) is formed,
         // and only once s < e is known
         if (s < e) {
