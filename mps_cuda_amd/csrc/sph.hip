@@ -53,7 +53,9 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 }
 
 // Fluid-fluid list entries each thread keeps in flight in the halo-staged kernels (their
-// occupancy is bounded by LDS, so registers are free for it).
+// occupancy is bounded by LDS, so registers are free for it). The 16-bit halo positions are
+// read with ordinary loads: they are reused by every solver iteration and stay in the
+// memory-side cache, while the larger per-pair coefficient streams are read non-temporally.
 constexpr uint32_t kListBatch = 4;
 
 // Halo count of workgroup blk: > kHaloCap (not staged) when there are no halos (hl.n null: the
@@ -122,7 +124,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
             }
         }
     };
@@ -251,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
                 gn[q] = __builtin_nontemporal_load(pG_ff + slot);
             }
         }
@@ -377,7 +379,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
                 gn[q] = __builtin_nontemporal_load(pG_ff + slot);
             }
         }
@@ -466,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, Halo hl
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
                 gn[q] = __builtin_nontemporal_load(pG_ff + slot);
             }
         }
@@ -554,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
                 gn[q] = __builtin_nontemporal_load(pG_ff + slot);
             }
         }
@@ -689,7 +691,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
-                jn[q] = staged ? uint32_t(__builtin_nontemporal_load(hl.nb16 + slot)) : __builtin_nontemporal_load(nb.ff + slot);
+                jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
                 const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
                 abn[q] = __builtin_bit_cast(float2, d);
             }
