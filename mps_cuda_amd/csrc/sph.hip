@@ -56,7 +56,8 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 // occupancy is bounded by LDS, so registers are free for it). The 16-bit halo positions are
 // read with ordinary loads: they are reused by every solver iteration and stay in the
 // memory-side cache, while the larger per-pair coefficient streams are read non-temporally.
-constexpr uint32_t kListBatch = 4;
+constexpr uint32_t kListBatch = 4;       // once-per-step walks (density/alpha, ST factor, CG setup)
+constexpr uint32_t kListBatchSolve = 2;  // solver iterations (DF/CD evaluate and apply, A.p): measured best
 
 // Halo count of workgroup blk: > kHaloCap (not staged) when there are no halos (hl.n null: the
 // adjoint's kernels run on its taped lists).
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
     __shared__ float2 lb[kHaloCap];
     const uint32_t H = hl.n[blk];
     const bool staged = H <= kHaloCap;
-    constexpr uint32_t U = kListBatch;
+    constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
     float gn[U];
     auto fetch = [&](uint32_t k0) {
@@ -371,7 +372,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     __shared__ float4 la[kHaloCap];
     const uint32_t H = hl.n[blk];
     const bool staged = H <= kHaloCap;
-    constexpr uint32_t U = kListBatch;
+    constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
     float gn[U];
     auto fetch = [&](uint32_t k0) {
@@ -683,7 +684,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     __shared__ float2 lb[kHaloCap];
     const uint32_t H = hl.n[blk];
     const bool staged = H <= kHaloCap;  // same for the whole block
-    constexpr uint32_t U = kListBatch;
+    constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
     float2 abn[U];
     auto fetch = [&](uint32_t k0) {
