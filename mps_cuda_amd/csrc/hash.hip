@@ -573,15 +573,20 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
                 if (ckey[h] != c) continue;  // not reached: every cell of a run was registered by its first particle
                 const uint32_t e = cent[h];
                 const uint32_t f0 = efo[e], f1 = efo[e + 1], s0 = efs[e];
-                for (uint32_t u = f0; u < f1; u++) {
-                    if (within(xi, ri, cpos[u])) {
-                        const uint32_t j = s0 + (u - f0);
-                        if (j != i) {
-                            if (a < Kff) nff[size_t(a) * N + i] = j;
-                            a++;
-                        }
+                auto take = [&](uint32_t u) {
+                    const uint32_t j = s0 + (u - f0);
+                    if (j != i) {
+                        if (a < Kff) nff[size_t(a) * N + i] = j;
+                        a++;
                     }
+                };
+                uint32_t u = f0;
+                for (; u + 1u < f1; u += 2u) {
+                    const bool in0 = within(xi, ri, cpos[u]), in1 = within(xi, ri, cpos[u + 1u]);
+                    if (in0) take(u);
+                    if (in1) take(u + 1u);
                 }
+                if (u < f1 && within(xi, ri, cpos[u])) take(u);
                 if (bd.n) {
                     const uint32_t w0 = ebo[e], w1 = ebo[e + 1], z0 = ebs[e];
                     for (uint32_t u = w0; u < w1; u++) {
