@@ -160,7 +160,7 @@ struct mps_ctx {
     DBuf<float2> cabT;
     DBuf<double> acc_mu, acc_sig;
     // fluid: pos = (x, y, z, r), vel = (vx, vy, vz, m)
-    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgp, cgAp, jprev, jt[3], jcand, tmp4;
+    DBuf<float4> pos, vel, color, cgb, cgx, cgr, cgAp, jprev, jt[3], jcand, tmp4;
     // the forward CG's vectors as packed triples: x, r, A p and two p buffers
     DBuf<mps::dev::f3> v3x, v3r, v3Ap, v3p, v3q;
     DBuf<float> rho, alpha, kappa, delta, fst, tmp1;
@@ -1324,7 +1324,7 @@ struct mps_ctx {
     // ------------------------------------------------------------------ setup
     void alloc_fluid(uint32_t n) {
         const size_t m = std::max<uint32_t>(n, 1);
-        pos.reserve(m); vel.reserve(m); color.reserve(m); cgb.reserve(m); cgx.reserve(m); cgr.reserve(m); cgp.reserve(m);
+        pos.reserve(m); vel.reserve(m); color.reserve(m); cgb.reserve(m); cgx.reserve(m); cgr.reserve(m);
         cgAp.reserve(m); rho.reserve(m); alpha.reserve(m); kappa.reserve(m); delta.reserve(m); fst.reserve(m);
         cff.reserve(m); cfb.reserve(m); perm.reserve(m); inv.reserve(m); mkey.reserve(m);
     }
@@ -1854,7 +1854,7 @@ struct mps_ctx {
     }
 
     ~mps_ctx() {
-        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release(); cgp.release();
+        pos.release(); vel.release(); color.release(); cgb.release(); cgx.release(); cgr.release();
         v3x.release(); v3r.release(); v3Ap.release(); v3p.release(); v3q.release();
         cgAp.release(); jprev.release(); for (auto& t : jt) t.release(); jcand.release(); tmp4.release();
         rho.release(); alpha.release(); kappa.release(); delta.release(); fst.release(); tmp1.release();

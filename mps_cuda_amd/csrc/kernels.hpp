@@ -148,7 +148,8 @@ void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* po
 // pair coefficients of the implicit operator: a_ij into ca and/or (a_ij, b_ij) into cab (either may be null)
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
-// CG vectors read by neighbours live in 32-byte records rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
+// The adjoint's transposed solve keeps p in 32-byte records with the position:
+// rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
 // The forward solve's vectors are packed triples (dev::f3).
 // mode 0: A x0 with f = 0 (x0 from the float4 array x0), then r = b - A x0, pout = r, x = x0, partial r.r;
@@ -164,12 +165,14 @@ void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, 
                 uint32_t* stats, hipStream_t s);
 // mode 0: initial r.r + test; 1: alpha; 2: new r.r, beta + test
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
+// x += alpha p, r -= alpha Ap, partial r.r (p from the records; the adjoint's solve)
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s);
 // the same on packed triples (the forward solve), and its copy of x into the velocities
 void cg_update3(const Scenes& sc, dev::f3* x, dev::f3* r, const dev::f3* p, const dev::f3* Ap, const SolverDev* st,
                 double* partials, hipStream_t s);
 void copy_vel_out3(const Scenes& sc, const dev::f3* x, float4* vel, hipStream_t s);
+// p = r + beta p in the records (the adjoint's solve; the forward forms p inside cg_apply)
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s);
 void copy_vel_out(const Scenes& sc, const float4* x, float4* vel, hipStream_t s);
 void jacobi_candidate(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,

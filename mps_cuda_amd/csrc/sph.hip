@@ -801,7 +801,6 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_cg(Scenes sc, const doubl
 
 // UpdateCGResidual_kernel + the r.r half of ComputeCGBetaParam_kernel (MPSCGKernel.cuh:69-121).
 // p in the 32-byte records (the adjoint's transposed solve).
-template <bool REC>
 __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restrict__ x, float4* __restrict__ r,
                                                     const float4* __restrict__ prec, const float4* __restrict__ Ap,
                                                     const SolverDev* st, double* __restrict__ partials) {
@@ -812,7 +811,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restr
     const float a = st[sid].alpha;
     double part = 0.0;
     if (i < end) {
-        const f3 pi = REC ? rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]) : mk(prec[i]);
+        const f3 pi = rec_p(prec[2 * size_t(i)], prec[2 * size_t(i) + 1]);
         const float4 api = Ap[i];
         float4 xi = x[i], ri = r[i];
         xi.x = xi.x + a * pi.x;
@@ -1169,7 +1168,7 @@ void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, 
 }
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s) {
-    if (sc.N) k_cg_update<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
+    if (sc.N) k_cg_update<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
 }
 void cg_update3(const Scenes& sc, f3* x, f3* r, const f3* p, const f3* Ap, const SolverDev* st, double* partials,
                 hipStream_t s) {
