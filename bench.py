@@ -208,6 +208,8 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
         return g.get("FLUID_POS").astype(np.float64), time.perf_counter() - t0
 
     target, _ = forward(0.02, 0.3, v0)
+    # graph replay and the poll interval for this small, many-iteration scene (state kept)
+    tuning = g.autotune(2)
     mu0, sig0 = 0.025, 0.1
     x, t_fwd = forward(mu0, sig0, v0)
 
@@ -247,6 +249,7 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
         "backward_ms_per_step": 1000 * t_back / steps, "adjoint_over_forward": t_back / t_fwd,
         "rollout_plus_backward_over_forward": (t_roll + t_back) / t_fwd,
         "l_cg_mean": float(counts[:, 0, 1].mean()), "fd_eps": eps, "gradient_checks": checks,
+        "solver_tuning": tuning,
     }
 
 

@@ -59,6 +59,25 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 constexpr uint32_t kListBatch = 4;       // once-per-step walks (density/alpha, ST factor, CG setup)
 constexpr uint32_t kListBatchSolve = 2;  // solver iterations (DF/CD evaluate and apply, A.p): measured best
 
+// Calls body(t, s) for every halo record t < H (s = its slot), with R slots per thread loaded
+// ahead so the bodies' loads can be in flight together.
+template <uint32_t R, typename F>
+__device__ __forceinline__ void for_halo(const uint32_t* __restrict__ hs, uint32_t H, F&& body) {
+    for (uint32_t t0 = threadIdx.x; t0 < H; t0 += R * blockDim.x) {
+        uint32_t s[R];
+#pragma unroll
+        for (uint32_t q = 0; q < R; q++) {
+            const uint32_t t = t0 + q * blockDim.x;
+            s[q] = t < H ? hs[t] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < R; q++) {
+            const uint32_t t = t0 + q * blockDim.x;
+            if (t < H) body(t, s[q]);
+        }
+    }
+}
+
 // Halo count of workgroup blk: > kHaloCap (not staged) when there are no halos (hl.n null: the
 // adjoint's kernels run on its taped lists).
 __device__ __forceinline__ uint32_t halo_count(const Halo& hl, uint32_t blk) { return hl.n ? hl.n[blk] : kHaloCap + 1u; }
@@ -132,7 +151,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) la[t] = pos[hs[t]];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) { la[t] = pos[s]; });
         __syncthreads();
     }
     if (!live) return;
@@ -262,12 +281,11 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const uint32_t s = hs[t];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float2(v.y, v.z);
-        }
+        });
         __syncthreads();
     }
     double epos = 0.0;
@@ -388,11 +406,10 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const uint32_t s = hs[t];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s];
             la[t] = make_float4(x.x, x.y, x.z, kappa[s]);
-        }
+        });
         __syncthreads();
     }
     if (!live) return;
@@ -477,12 +494,11 @@ __global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, Halo hl
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const uint32_t s = hs[t];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float4(v.y, v.z, v.w, 0.0f);
-        }
+        });
         __syncthreads();
     }
     if (!live) return;
@@ -565,13 +581,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const uint32_t s = hs[t];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float4(v.y, v.z, v.w, rho[s]);
             lc[t] = fst[s];
-        }
+        });
         __syncthreads();
     }
     if (!live) return;
@@ -701,13 +716,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for (uint32_t t = threadIdx.x; t < H; t += blockDim.x) {
-            const uint32_t s = hs[t];
+        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s];
             const f3 v = vec(s);
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float2(v.y, v.z);
-        }
+        });
         __syncthreads();
     }
     double part = 0.0;
