@@ -211,6 +211,15 @@ struct mps_ctx {
     DBuf<float> rb_xyz, rb_rad;
     DBuf<float4> rb_rgba;
     uint32_t pred[3] = {2, 3, 8};
+    uint32_t pred_adj = 8;  // iterations of the last transposed solve of the adjoint
+    // Iterations enqueued before the next host poll: the predicted count first (the previous
+    // solve's), then small chunks, so a solve that runs a little longer than predicted does not
+    // enqueue a whole second prediction of launches that would exit at once.
+    static constexpr uint32_t kFollowChunk = 2;
+    uint32_t chunk_len(uint32_t it, uint32_t predicted, uint32_t left) const {
+        if (cfg.host_sync_every) return std::min(cfg.host_sync_every, left);
+        return std::min(it == 1 ? std::max<uint32_t>(predicted, 1) : kFollowChunk, left);
+    }
 
     // ------------------------------------------------------------------ kernel timing
     enum KId {
@@ -728,8 +737,7 @@ struct mps_ctx {
         uint32_t it = 1;
         bool done = false;
         while (it <= cap && !done) {
-            uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[idx], lmin + 1);
-            chunk = std::min(chunk, cap - it + 1);
+            const uint32_t chunk = chunk_len(it, std::max<uint32_t>(pred[idx], lmin + 1), cap - it + 1);
             std::vector<uint64_t> key = key_base(df ? 1 : 2, chunk, P, nbr());
             for (const void* q : {(const void*)pos.p, (const void*)vel.p, (const void*)bpos.p, (const void*)bvol.p,
                                   (const void*)bvel.p, (const void*)rho.p, (const void*)alpha.p, (const void*)pairG_ff.p,
@@ -822,8 +830,7 @@ struct mps_ctx {
         uint32_t it = 1;
         bool done = false;
         while (it <= cap && !done) {
-            uint32_t chunk = cfg.host_sync_every ? cfg.host_sync_every : std::max<uint32_t>(pred[2], 1);
-            chunk = std::min(chunk, cap - it + 1);
+            const uint32_t chunk = chunk_len(it, pred[2], cap - it + 1);
             std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
             for (const void* q : {(const void*)pairAB.p, (const void*)pos.p, (const void*)pbuf(it - 1), (const void*)pbuf(it),
                                   (const void*)v3Ap.p, (const void*)v3r.p, (const void*)v3x.p, (const void*)sd,
@@ -1040,7 +1047,7 @@ struct mps_ctx {
         uint32_t it = 1;
         bool done = fetch_active(2) == 0;
         while (it <= cap && !done) {
-            const uint32_t chunk = std::min(std::max<uint32_t>(pred[2], 1), cap - it + 1);
+            const uint32_t chunk = chunk_len(it, pred_adj, cap - it + 1);
             std::vector<uint64_t> key = key_base(4, chunk, P, nb);
             for (const void* q : {(const void*)pairAB.p, (const void*)cabT.p, (const void*)cgrecB.p, (const void*)cgAp.p,
                                   (const void*)cgx.p, (const void*)cgr.p, (const void*)sd, (const void*)partials.p,
@@ -1059,6 +1066,9 @@ struct mps_ctx {
             it += chunk;
             done = fetch_active(2) == 0;
         }
+        uint32_t l = 0;
+        (void)fetch_records(2, cap, &l);
+        pred_adj = std::max<uint32_t>(l, 1);
     }
     // Reverse of one recorded step: (adx, adv) from the step's end state to its start, in the
     // slot order of the previous step.
@@ -1353,7 +1363,7 @@ struct mps_ctx {
         pairs_valid = false;
         halo_valid = false;
         frame = 0;
-        pred[0] = 2; pred[1] = 3; pred[2] = 8;
+        pred[0] = 2; pred[1] = 3; pred[2] = 8; pred_adj = 8;
         params_dirty = true;
     }
     void alloc_boundary(uint32_t n) {
