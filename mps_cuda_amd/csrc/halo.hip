@@ -35,11 +35,9 @@ __device__ __forceinline__ uint32_t halo_find(const uint32_t* key, uint32_t j) {
 }
 
 // Blocks with more than `limit` (<= kHaloCap) distinct neighbours are marked not staged
-// (n = kHaloCap + 1). stats[0] = largest halo among staged blocks, stats[1] = blocks not
-// staged, stats[2] = blocks with particles.
+// (n = kHaloCap + 1).
 __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32_t limit, uint32_t* __restrict__ hidx,
-                                                     uint32_t* __restrict__ hn, uint16_t* __restrict__ n16,
-                                                     uint32_t* __restrict__ stats) {
+                                                     uint32_t* __restrict__ hn, uint16_t* __restrict__ n16) {
     const uint32_t blk = blockIdx.x;
     const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
@@ -84,11 +82,7 @@ __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32
     __syncthreads();
     const uint32_t H = cnt;
     if (H > limit) {
-        if (tid == 0) {
-            hn[blk] = kHaloCap + 1u;
-            atomicAdd(&stats[1], 1u);
-            atomicAdd(&stats[2], 1u);
-        }
+        if (tid == 0) hn[blk] = kHaloCap + 1u;
         return;
     }
     // compact the occupied slots (each thread owns 16 consecutive slots) ...
@@ -133,13 +127,7 @@ __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32
         out[t] = j;
         at[halo_find(key, j)] = static_cast<uint16_t>(t);
     }
-    if (tid == 0) {
-        hn[blk] = H;
-        if (H) {
-            atomicMax(&stats[0], H);
-            atomicAdd(&stats[2], 1u);
-        }
-    }
+    if (tid == 0) hn[blk] = H;
     __syncthreads();
     for (uint32_t k0 = 0; k0 < c; k0 += kBatch) {
         uint32_t jb[kBatch];
@@ -152,13 +140,53 @@ __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32
     }
 }
 
+// stats[0] = largest halo staged, stats[1] = blocks not staged, stats[2] = blocks with a halo
+// or not staged: one workgroup folds the per-block counts (no same-address atomics from every
+// workgroup of the build).
+constexpr int kHaloStatThreads = 1024;
+__global__ void __launch_bounds__(kHaloStatThreads) k_halo_stats(const uint32_t* __restrict__ hn, uint32_t nblk,
+                                                                uint32_t* __restrict__ stats) {
+    __shared__ uint32_t wm[kHaloStatThreads / 64], ws[kHaloStatThreads / 64], wb[kHaloStatThreads / 64];
+    uint32_t mx = 0u, sp = 0u, bl = 0u;
+    for (uint32_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+        const uint32_t h = hn[b];
+        if (h > kHaloCap) sp++;
+        else mx = max(mx, h);
+        if (h) bl++;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+        sp += (uint32_t)__shfl_xor((int)sp, off, 64);
+        bl += (uint32_t)__shfl_xor((int)bl, off, 64);
+    }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        wm[wave] = mx;
+        ws[wave] = sp;
+        wb[wave] = bl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) {
+            mx = max(mx, wm[w]);
+            sp += ws[w];
+            bl += wb[w];
+        }
+        stats[0] = mx;
+        stats[1] = sp;
+        stats[2] = bl;
+    }
+}
+
 }  // namespace dev
 
 namespace launch {
 void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, uint32_t* hidx, uint32_t* hn, uint16_t* n16,
                 uint32_t* stats, hipStream_t s) {
     limit = limit == 0u || limit > kHaloCap ? kHaloCap : limit;
-    if (nblk) dev::k_halo_build<<<nblk, dev::kBlock, 0, s>>>(sc, nb, limit, hidx, hn, n16, stats);
+    if (!nblk) return;
+    dev::k_halo_build<<<nblk, dev::kBlock, 0, s>>>(sc, nb, limit, hidx, hn, n16);
+    dev::k_halo_stats<<<1, dev::kHaloStatThreads, 0, s>>>(hn, nblk, stats);
 }
 }  // namespace launch
 }  // namespace mps
