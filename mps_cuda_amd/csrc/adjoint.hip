@@ -683,77 +683,77 @@ using namespace dev;
 static inline unsigned nbk(size_t n) { return unsigned((n + kBlock - 1) / kBlock); }
 
 void adj_advect(const Scenes& sc, const float4* pos, const float4* vin, const float4* ax, float4* av, hipStream_t s) {
-    if (sc.N) k_adj_advect<<<nbk(sc.N), kBlock, 0, s>>>(sc, pos, vin, ax, av);
+    if (sc.N) launch_k(k_adj_advect, nbk(sc.N), kBlock, 0, s, sc, pos, vin, ax, av);
 }
 void adj_peval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                const float* pG_fb, float4* aux, hipStream_t s) {
     if (!sc.N) return;
-    if (df) k_adj_peval<true><<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
-    else k_adj_peval<false><<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
+    if (df) launch_k(k_adj_peval<true>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
+    else launch_k(k_adj_peval<false>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
 }
 void adj_kbar(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
               const float* pG_ff, const float* pG_fb, const float4* aux, const float* alpha, const float4* av,
               const uint32_t* L, uint32_t it, float* dbar, float4* ax, float* abar, float* rbar, hipStream_t s) {
     if (!sc.N) return;
     if (df)
-        k_adj_kbar<true><<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
+        launch_k(k_adj_kbar<true>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
                                                       abar, rbar);
     else
-        k_adj_kbar<false><<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
+        launch_k(k_adj_kbar<false>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
                                                        abar, rbar);
 }
 void adj_dbar(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos, const float* bvol,
               const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar, const uint32_t* L, uint32_t it,
               float4* av, float4* ax, hipStream_t s) {
-    if (sc.N) k_adj_dbar<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, bpos, bvol, bvel, pG_ff, pG_fb, dbar, L, it, av, ax);
+    if (sc.N) launch_k(k_adj_dbar, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, pG_ff, pG_fb, dbar, L, it, av, ax);
 }
 void adj_rev_index(const Scenes& sc, const Nbr& nb, uint32_t* rev, uint32_t* bad, hipStream_t s) {
-    if (sc.N) k_adj_rev_index<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, rev, bad);
+    if (sc.N) launch_k(k_adj_rev_index, nbk(sc.N), kBlock, 0, s, sc, nb, rev, bad);
 }
 void adj_transpose_coeffs(const Scenes& sc, const Nbr& nb, const uint32_t* rev, const float2* cab, float2* cabT, hipStream_t s) {
-    if (sc.N) k_adj_transpose_coeffs<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, rev, cab, cabT);
+    if (sc.N) launch_k(k_adj_transpose_coeffs, nbk(sc.N), kBlock, 0, s, sc, nb, rev, cab, cabT);
 }
 void adj_cg_init(const Scenes& sc, const float4* pos, const float4* b, float4* x, float4* r, float4* rec, double* partials,
                  hipStream_t s) {
-    if (sc.N) k_adj_cg_init<<<nbk(sc.N), kBlock, 0, s>>>(sc, pos, b, x, r, rec, partials);
+    if (sc.N) launch_k(k_adj_cg_init, nbk(sc.N), kBlock, 0, s, sc, pos, b, x, r, rec, partials);
 }
 void adj_cg_applyT(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* rec, float4* Ap,
                    const SolverDev* st, double* partials, hipStream_t s) {
-    if (sc.N) k_adj_cg_applyT<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, cab, cabT, rec, Ap, st, partials);
+    if (sc.N) launch_k(k_adj_cg_applyT, nbk(sc.N), kBlock, 0, s, sc, nb, cab, cabT, rec, Ap, st, partials);
 }
 void adj_cg_grad(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar, const float* rho,
                  const float* fst, const float4* lam, float4* avin, float4* ax, float* rbar, float* fbar, double* partials,
                  hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nbk(sc.N);
-    k_adj_cg_grad<<<g, kBlock, 0, s>>>(sc, nb, pos, vin, vstar, rho, fst, lam, avin, ax, rbar, fbar, partials, g);
+    launch_k(k_adj_cg_grad, g, kBlock, 0, s, sc, nb, pos, vin, vstar, rho, fst, lam, avin, ax, rbar, fbar, partials, g);
 }
 void adj_fst(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff, const float* fbar,
              float4* avin, float4* ax, hipStream_t s) {
-    if (sc.N) k_adj_fst<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, vin, pG_ff, fbar, avin, ax);
+    if (sc.N) launch_k(k_adj_fst, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vin, pG_ff, fbar, avin, ax);
 }
 void adj_fold2(const Scenes& sc, const double* partials, double* acc0, double* acc1, hipStream_t s) {
-    if (sc.N) k_adj_fold2<<<sc.S, kBlock, 0, s>>>(sc, partials, nbk(sc.N), acc0, acc1);
+    if (sc.N) launch_k(k_adj_fold2, sc.S, kBlock, 0, s, sc, partials, nbk(sc.N), acc0, acc1);
 }
 void adj_gvec(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
               const float* pG_fb, const float* abar, float4* gv, hipStream_t s) {
-    if (sc.N) k_adj_gvec<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, bpos, bvol, pG_ff, pG_fb, abar, gv);
+    if (sc.N) launch_k(k_adj_gvec, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, abar, gv);
 }
 void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
                  const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s) {
-    if (sc.N) k_adj_density<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, pos, bpos, bvol, pG_ff, pG_fb, rbar, gv, ax);
+    if (sc.N) launch_k(k_adj_density, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, rbar, gv, ax);
 }
 void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, const float4* al, float4* ax_out,
                    float4* av_out, float4* al_out, hipStream_t s) {
-    if (n) k_adj_unpermute<<<nbk(n), kBlock, 0, s>>>(n, src, ax, av, al, ax_out, av_out, al_out);
+    if (n) launch_k(k_adj_unpermute, nbk(n), kBlock, 0, s, n, src, ax, av, al, ax_out, av_out, al_out);
 }
 void adj_cg_init_warm(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* recA,
                       const float4* b, float4* x, float4* r, float4* recB, double* partials, hipStream_t s) {
-    if (sc.N) k_adj_cg_init_warm<<<nbk(sc.N), kBlock, 0, s>>>(sc, nb, cab, cabT, recA, b, x, r, recB, partials, nbk(sc.N));
+    if (sc.N) launch_k(k_adj_cg_init_warm, nbk(sc.N), kBlock, 0, s, sc, nb, cab, cabT, recA, b, x, r, recB, partials, nbk(sc.N));
 }
 void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
-    if (sc.N) k_adj_fold_start<<<sc.S, kBlock, 0, s>>>(sc, partials, nbk(sc.N), st, active, tol);
+    if (sc.N) launch_k(k_adj_fold_start, sc.S, kBlock, 0, s, sc, partials, nbk(sc.N), st, active, tol);
 }
 
 }  // namespace launch

@@ -23,6 +23,12 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 
+namespace mps {
+namespace launch {
+thread_local Timing* timing = nullptr;
+}
+}  // namespace mps
+
 namespace {
 
 using mps::HashView;
@@ -254,20 +260,37 @@ struct mps_ctx {
         HIPCHK(hipEventCreate(&e));
         return e;
     }
-    // Run launch(s) f; when profiling, bracket them with events on the context stream.
+    // Hands every kernel launched inside T_ its own event pair (stamped by the dispatch itself).
+    struct KTimer final : mps::launch::Timing {
+        mps_ctx* c = nullptr;
+        int k = 0;
+        uint32_t n = 0;
+        void events(hipEvent_t* a, hipEvent_t* b) override {
+            *a = c->ev_get();
+            *b = c->ev_get();
+            c->recs.push_back(Rec{k, *a, *b});
+            n++;
+        }
+    } ktimer;
+    // Run launch(s) f; when profiling, time each of their kernels (durations add up per id k).
     template <class F>
     void T_(int k, F&& f, bool always_useful = true) {
         if (!prof_on) {
             f();
             return;
         }
-        hipEvent_t a = ev_get(), b = ev_get();
-        HIPCHK(hipEventRecord(a, stream));
+        ktimer.c = this;
+        ktimer.k = k;
+        ktimer.n = 0;
+        struct Scope {
+            explicit Scope(mps::launch::Timing* t) { mps::launch::timing = t; }
+            ~Scope() { mps::launch::timing = nullptr; }
+        } scope(&ktimer);
         f();
-        HIPCHK(hipEventRecord(b, stream));
-        recs.push_back(Rec{k, a, b});
-        kn[k]++;
-        if (always_useful) kuse[k]++;
+        if (ktimer.n) {
+            kn[k]++;
+            if (always_useful) kuse[k]++;
+        }
     }
     void prof_collect() {
         for (const Rec& r : recs) {

@@ -185,8 +185,8 @@ void halo_build(const Scenes& sc, const Nbr& nb, uint32_t nblk, uint32_t limit, 
                 uint32_t* stats, hipStream_t s) {
     limit = limit == 0u || limit > kHaloCap ? kHaloCap : limit;
     if (!nblk) return;
-    dev::k_halo_build<<<nblk, dev::kBlock, 0, s>>>(sc, nb, limit, hidx, hn, n16);
-    dev::k_halo_stats<<<1, dev::kHaloStatThreads, 0, s>>>(hn, nblk, stats);
+    launch_k(dev::k_halo_build, nblk, dev::kBlock, 0, s, sc, nb, limit, hidx, hn, n16);
+    launch_k(dev::k_halo_stats, 1, dev::kHaloStatThreads, 0, s, hn, nblk, stats);
 }
 }  // namespace launch
 }  // namespace mps

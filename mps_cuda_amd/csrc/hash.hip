@@ -668,13 +668,13 @@ int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*&
     const uint32_t nblk = (n + kSortTile - 1) / kSortTile;
     const uint32_t L = 256u * nblk;
     for (int shift = 0; shift < key_bits; shift += 8) {
-        k_radix_hist<<<nblk, kSortThreads, 0, s>>>(keys, n, shift, temp, nblk);
+        launch_k(k_radix_hist, nblk, kSortThreads, 0, s, keys, n, shift, temp, nblk);
         const uint32_t nch = (L + kScanChunk - 1) / kScanChunk;
         uint32_t* sums = temp + L;
-        k_scan_sums<<<nch, kScanChunk, 0, s>>>(temp, L, sums);
-        k_scan_top<<<1, kScanChunk, 0, s>>>(sums, nch);
-        k_scan_apply<<<nch, kScanChunk, 0, s>>>(temp, L, sums);
-        k_radix_scatter<<<nblk, kSortThreads, 0, s>>>(keys, vals, keys_alt, vals_alt, n, shift, temp, nblk);
+        launch_k(k_scan_sums, nch, kScanChunk, 0, s, temp, L, sums);
+        launch_k(k_scan_top, 1, kScanChunk, 0, s, sums, nch);
+        launch_k(k_scan_apply, nch, kScanChunk, 0, s, temp, L, sums);
+        launch_k(k_radix_scatter, nblk, kSortThreads, 0, s, keys, vals, keys_alt, vals_alt, n, shift, temp, nblk);
         std::swap(keys, keys_alt);
         std::swap(vals, vals_alt);
     }
@@ -683,76 +683,76 @@ int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*&
 
 void cell_keys(const float4* pos, uint32_t n, const HashGeom& g, int morton, const uint32_t* scene, int shift, uint32_t* key,
                uint32_t* id, hipStream_t s) {
-    if (n) k_cell_keys<<<nb(n), kBlock, 0, s>>>(pos, n, g, morton, scene, shift, key, id);
+    if (n) launch_k(k_cell_keys, nb(n), kBlock, 0, s, pos, n, g, morton, scene, shift, key, id);
 }
 void cell_keys_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, int morton,
                      const uint32_t* blk_scene, int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s) {
-    if (n) k_cell_keys_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, morton, blk_scene, shift, keyref, iota);
+    if (n) launch_k(k_cell_keys_slots, nb(n), kBlock, 0, s, pos, perm, n, g, morton, blk_scene, shift, keyref, iota);
 }
 void cell_morton_slots(const float4* pos, const uint32_t* perm, uint32_t n, const HashGeom& g, const uint32_t* blk_scene,
                        int shift, uint32_t* keyref, uint32_t* iota, hipStream_t s) {
-    if (n) k_cell_morton_slots<<<nb(n), kBlock, 0, s>>>(pos, perm, n, g, blk_scene, shift, keyref, iota);
+    if (n) launch_k(k_cell_morton_slots, nb(n), kBlock, 0, s, pos, perm, n, g, blk_scene, shift, keyref, iota);
 }
 void place(const uint32_t* key, const uint32_t* id, uint32_t nref, int shift, const uint32_t* delta, const uint32_t* inv,
            uint32_t* perm, uint32_t* src, uint32_t* mkey, hipStream_t s) {
-    if (nref) k_place<<<nb(nref), kBlock, 0, s>>>(key, id, nref, shift, delta, inv, perm, src, mkey);
+    if (nref) launch_k(k_place, nb(nref), kBlock, 0, s, key, id, nref, shift, delta, inv, perm, src, mkey);
 }
 void cell_table_m(const uint32_t* mkey, const float4* pos, uint32_t n, const HashGeom& g, int shift, uint32_t C, uint32_t* start,
                   uint32_t* end, uint32_t* lkey, hipStream_t s) {
-    if (n) k_cell_table_m<<<nb(n), kBlock, 0, s>>>(mkey, pos, n, g, shift, C, start, end, lkey);
+    if (n) launch_k(k_cell_table_m, nb(n), kBlock, 0, s, mkey, pos, n, g, shift, C, start, end, lkey);
 }
 void clear_cells_m(const uint32_t* mkey, const uint32_t* lkey, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s) {
-    if (n) k_clear_cells_m<<<nb(n), kBlock, 0, s>>>(mkey, lkey, n, start, end);
+    if (n) launch_k(k_clear_cells_m, nb(n), kBlock, 0, s, mkey, lkey, n, start, end);
 }
 void invert(const uint32_t* perm, uint32_t n, uint32_t* inv, hipStream_t s) {
-    if (n) k_invert<<<nb(n), kBlock, 0, s>>>(perm, n, inv);
+    if (n) launch_k(k_invert, nb(n), kBlock, 0, s, perm, n, inv);
 }
 void compose(const uint32_t* perm_new, const uint32_t* inv, uint32_t n, uint32_t* src, hipStream_t s) {
-    if (n) k_compose<<<nb(n), kBlock, 0, s>>>(perm_new, inv, n, src);
+    if (n) launch_k(k_compose, nb(n), kBlock, 0, s, perm_new, inv, n, src);
 }
 void relabel(uint32_t* perm, const uint32_t* map, uint32_t n, hipStream_t s) {
-    if (n) k_relabel<<<nb(n), kBlock, 0, s>>>(perm, map, n);
+    if (n) launch_k(k_relabel, nb(n), kBlock, 0, s, perm, map, n);
 }
 void iota(uint32_t* p, uint32_t n, hipStream_t s) {
-    if (n) k_iota<<<nb(n), kBlock, 0, s>>>(p, n);
+    if (n) launch_k(k_iota, nb(n), kBlock, 0, s, p, n);
 }
 void fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
-    if (n) k_fill_u32<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n, v);
+    if (n) launch_k(k_fill_u32, unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s, p, n, v);
 }
 void clear_cells(const uint32_t* key, uint32_t n, uint32_t* start, uint32_t* end, hipStream_t s) {
-    if (n) k_clear_cells<<<nb(n), kBlock, 0, s>>>(key, n, start, end);
+    if (n) launch_k(k_clear_cells, nb(n), kBlock, 0, s, key, n, start, end);
 }
 void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* start, uint32_t* end, const float4* pos,
                 float4* spos, hipStream_t s) {
-    if (n) k_cell_table<<<nb(n), kBlock, 0, s>>>(key, id, n, start, end, pos, spos);
+    if (n) launch_k(k_cell_table, nb(n), kBlock, 0, s, key, id, n, start, end, pos, spos);
 }
 void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const HashGeom& g, const HashView& fl, const HashView& bd,
                      uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb, uint2* bstat,
                      NbrCounters* ctr, hipStream_t s) {
     if (!sc.N) return;
     const uint32_t g_ = nb(sc.N);
-    k_neighbors_fluid<<<g_, kBlock, 0, s>>>(pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, bstat);
-    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat, g_, &ctr->max_ff, &ctr->nnz_ff);
-    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat + g_, g_, &ctr->max_fb, &ctr->nnz_fb);
+    launch_k(k_neighbors_fluid, g_, kBlock, 0, s, pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, bstat);
+    launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat, g_, &ctr->max_ff, &ctr->nnz_ff);
+    launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat + g_, g_, &ctr->max_fb, &ctr->nnz_fb);
 }
 void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const HashGeom& g, const HashView& bd,
                         uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, uint2* bstat, NbrCounters* ctr, hipStream_t s) {
     if (!B) return;
     const uint32_t g_ = nb(B);
-    k_neighbors_boundary<<<g_, kBlock, 0, s>>>(pos, B, scene, C, g, bd, nbb, Kbb, cbb, bstat);
-    k_count_fold<<<1, kCountFoldThreads, 0, s>>>(bstat, g_, &ctr->max_bb, &ctr->nnz_bb);
+    launch_k(k_neighbors_boundary, g_, kBlock, 0, s, pos, B, scene, C, g, bd, nbb, Kbb, cbb, bstat);
+    launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat, g_, &ctr->max_bb, &ctr->nnz_bb);
 }
 void gather_f4(const uint32_t* perm, uint32_t n, const float4* in, float4* out, hipStream_t s) {
-    if (n) k_gather_f4<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+    if (n) launch_k(k_gather_f4, nb(n), kBlock, 0, s, perm, n, in, out);
 }
 void gather_u32(const uint32_t* perm, uint32_t n, const uint32_t* in, uint32_t* out, hipStream_t s) {
-    if (n) k_gather_u32<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+    if (n) launch_k(k_gather_u32, nb(n), kBlock, 0, s, perm, n, in, out);
 }
 void gather_f2(const uint32_t* perm, uint32_t n, const float2* in, float2* out, hipStream_t s) {
-    if (n) k_gather_f2<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+    if (n) launch_k(k_gather_f2, nb(n), kBlock, 0, s, perm, n, in, out);
 }
 void gather_f1(const uint32_t* perm, uint32_t n, const float* in, float* out, hipStream_t s) {
-    if (n) k_gather_f1<<<nb(n), kBlock, 0, s>>>(perm, n, in, out);
+    if (n) launch_k(k_gather_f1, nb(n), kBlock, 0, s, perm, n, in, out);
 }
 
 }  // namespace launch

@@ -6,6 +6,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <tuple>
+#include <type_traits>
+#include <utility>
+
 #include "../../include/mps_sph.h"
 
 namespace mps {
@@ -87,6 +91,37 @@ struct Halo {
 };
 
 namespace launch {
+// ---- kernel timing
+// While a context times its kernels it points `timing` (per host thread) at a Timing; every
+// launch below then asks it for an event pair and is dispatched with hipExtLaunchKernel, which
+// stamps the pair with the dispatch's own start and end. No marker packets (and none of the
+// cache write-back they carry) sit between the kernels, so the durations are those of the
+// kernels as they run untimed.
+struct Timing {
+    virtual void events(hipEvent_t* start, hipEvent_t* stop) = 0;
+
+protected:
+    ~Timing() = default;
+};
+extern thread_local Timing* timing;
+
+// Launch `kernel` on `s` with the arguments converted to its parameter types, as a call would.
+template <class... P, class... A>
+inline void launch_k(void (*kernel)(P...), dim3 grid, dim3 block, uint32_t shm, hipStream_t s, A&&... args) {
+    static_assert(sizeof...(P) == sizeof...(A), "argument count");
+    std::tuple<std::decay_t<P>...> t(std::forward<A>(args)...);
+    void* ptrs[sizeof...(P) + 1];
+    std::apply([&](auto&... a) {
+        size_t k = 0;
+        ((ptrs[k++] = static_cast<void*>(&a)), ...);
+    }, t);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timing) timing->events(&e0, &e1);
+    const hipError_t rc = e0 ? hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shm, s, e0, e1, 0)
+                             : hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shm, s);
+    (void)rc;  // launch errors surface at the next checked call, as with <<< >>>
+}
+
 // ---- hash.hip
 size_t radix_temp_words(uint32_t n);
 int radix_sort(uint32_t*& keys, uint32_t*& vals, uint32_t*& keys_alt, uint32_t*& vals_alt, uint32_t* temp, uint32_t n,

@@ -164,15 +164,15 @@ namespace launch {
 using namespace dev;
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
                   hipStream_t s) {
-    if (nt) k_sample_count<<<(nt + 63) / 64, 64, 0, s>>>(V, F, R, nt, radius, counts);
+    if (nt) launch_k(k_sample_count, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, counts);
 }
 void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
                      const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad, hipStream_t s) {
-    if (nt) k_sample_generate<<<(nt + 63) / 64, 64, 0, s>>>(V, F, R, nt, radius, offs, pos, face, bad);
+    if (nt) launch_k(k_sample_generate, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, offs, pos, face, bad);
 }
 void sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n, const double* pos,
                    const uint32_t* face, double radius, float4* bpos, float4* bvel, float2* bbcc, hipStream_t s) {
-    if (n) k_sample_finish<<<(n + 255) / 256, 256, 0, s>>>(V, F, nt, vmass, n, pos, face, radius, bpos, bvel, bbcc);
+    if (n) launch_k(k_sample_finish, (n + 255) / 256, 256, 0, s, V, F, nt, vmass, n, pos, face, radius, bpos, bvel, bbcc);
 }
 }  // namespace launch
 }  // namespace mps

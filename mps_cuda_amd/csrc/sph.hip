@@ -1123,114 +1123,114 @@ static inline unsigned nb_(size_t n) { return unsigned((n + kBlock - 1) / kBlock
 void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb, float* S,
                   hipStream_t s) {
     if (!B) return;
-    if (cubic) k_boundary_sum<true><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, Kbb, cbb, S);
-    else k_boundary_sum<false><<<nb_(B), kBlock, 0, s>>>(bpos, B, nbb, Kbb, cbb, S);
+    if (cubic) launch_k(k_boundary_sum<true>, nb_(B), kBlock, 0, s, bpos, B, nbb, Kbb, cbb, S);
+    else launch_k(k_boundary_sum<false>, nb_(B), kBlock, 0, s, bpos, B, nbb, Kbb, cbb, S);
 }
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
-    if (B) k_boundary_volume<<<nb_(B), kBlock, 0, s>>>(S, vol, B, faithful);
+    if (B) launch_k(k_boundary_volume, nb_(B), kBlock, 0, s, S, vol, B, faithful);
 }
 void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_density_alpha<true><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
-    else k_density_alpha<false><<<nb_(sc.N), kBlock, 0, s>>>(flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    if (cubic) launch_k(k_density_alpha<true>, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    else launch_k(k_density_alpha<false>, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
 void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
                    const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                    const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (df) k_pressure_eval<true><<<g, kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
-    else k_pressure_eval<false><<<g, kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    if (df) launch_k(k_pressure_eval<true>, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    else launch_k(k_pressure_eval<false>, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
 void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
                    hipStream_t s) {
-    if (sc.N) k_fold_pressure<<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol, lmin);
+    if (sc.N) launch_k(k_fold_pressure, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol, lmin);
 }
 void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s) {
-    if (sc.N) k_pressure_apply<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
+    if (sc.N) launch_k(k_pressure_apply, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
 void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {
-    if (sc.N) k_st_factor<<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, pG_ff, fst, cgx);
+    if (sc.N) launch_k(k_st_factor, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, pG_ff, fst, cgx);
 }
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
               const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_cg_setup<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
-    else k_cg_setup<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    if (cubic) launch_k(k_cg_setup<true>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    else launch_k(k_cg_setup<false>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
 }
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
-    if (N) k_cg_pack<<<nb_(N), kBlock, 0, s>>>(N, pos, x0, rec);
+    if (N) launch_k(k_cg_pack, nb_(N), kBlock, 0, s, N, pos, x0, rec);
 }
 void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
               const float4* x0, const f3* rr, const f3* pprev, f3* pout, f3* Ap, const float4* b, f3* r, f3* x,
               const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (mode == 0) k_cg_apply<0><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
-    else k_cg_apply<1><<<g, kBlock, 0, s>>>(sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
+    if (mode == 0) launch_k(k_cg_apply<0>, g, kBlock, 0, s, sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
+    else launch_k(k_cg_apply<1>, g, kBlock, 0, s, sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
 }
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;
-    if (mode == 0) k_fold_cg<0><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
-    else if (mode == 1) k_fold_cg<1><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
-    else k_fold_cg<2><<<sc.S, kFoldThreads, 0, s>>>(sc, partials, st, active, tol);
+    if (mode == 0) launch_k(k_fold_cg<0>, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol);
+    else if (mode == 1) launch_k(k_fold_cg<1>, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol);
+    else launch_k(k_fold_cg<2>, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol);
 }
 void cg_update(const Scenes& sc, float4* x, float4* r, const float4* prec, const float4* Ap, const SolverDev* st,
                double* partials, hipStream_t s) {
-    if (sc.N) k_cg_update<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, prec, Ap, st, partials);
+    if (sc.N) launch_k(k_cg_update, nb_(sc.N), kBlock, 0, s, sc, x, r, prec, Ap, st, partials);
 }
 void cg_update3(const Scenes& sc, f3* x, f3* r, const f3* p, const f3* Ap, const SolverDev* st, double* partials,
                 hipStream_t s) {
-    if (sc.N) k_cg_update3<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, r, p, Ap, st, partials);
+    if (sc.N) launch_k(k_cg_update3, nb_(sc.N), kBlock, 0, s, sc, x, r, p, Ap, st, partials);
 }
 void copy_vel_out3(const Scenes& sc, const f3* x, float4* vel, hipStream_t s) {
-    if (sc.N) k_copy_vel_out3<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, vel);
+    if (sc.N) launch_k(k_copy_vel_out3, nb_(sc.N), kBlock, 0, s, sc, x, vel);
 }
 void cg_direction(const Scenes& sc, const float4* r, float4* prec, const SolverDev* st, hipStream_t s) {
-    if (sc.N) k_cg_direction<<<nb_(sc.N), kBlock, 0, s>>>(sc, r, prec, st);
+    if (sc.N) launch_k(k_cg_direction, nb_(sc.N), kBlock, 0, s, sc, r, prec, st);
 }
 void copy_vel_out(const Scenes& sc, const float4* x, float4* vel, hipStream_t s) {
-    if (sc.N) k_copy_vel_out<<<nb_(sc.N), kBlock, 0, s>>>(sc, x, vel);
+    if (sc.N) launch_k(k_copy_vel_out, nb_(sc.N), kBlock, 0, s, sc, x, vel);
 }
 void jacobi_candidate(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                       const float* rho, const float4* u, float4* out, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) k_jacobi_candidate<true><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, u, out);
-    else k_jacobi_candidate<false><<<nb_(sc.N), kBlock, 0, s>>>(sc, nb, pos, vel, rho, u, out);
+    if (cubic) launch_k(k_jacobi_candidate<true>, nb_(sc.N), kBlock, 0, s, sc, nb, pos, vel, rho, u, out);
+    else launch_k(k_jacobi_candidate<false>, nb_(sc.N), kBlock, 0, s, sc, nb, pos, vel, rho, u, out);
 }
 void jacobi_relax(uint32_t N, const float4* cand, const float4* cur, const float4* prev, float ur, const float* w3,
                   float4* o0, float4* o1, float4* o2, hipStream_t s) {
-    if (N) k_jacobi_relax<<<nb_(N), kBlock, 0, s>>>(N, cand, cur, prev, ur, w3[0], w3[1], w3[2], o0, o1, o2);
+    if (N) launch_k(k_jacobi_relax, nb_(N), kBlock, 0, s, N, cand, cur, prev, ur, w3[0], w3[1], w3[2], o0, o1, o2);
 }
 void jacobi_error(int cubic, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel,
                   const float* rho, const float4* u0, const float4* u1, const float4* u2, int ncand, double* partials,
                   double* err3, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (cubic) k_jacobi_error<true><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
-    else k_jacobi_error<false><<<g, kBlock, 0, s>>>(sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
-    k_fold_partials<<<1, kBlock, 0, s>>>(partials, g, ncand, err3);
+    if (cubic) launch_k(k_jacobi_error<true>, g, kBlock, 0, s, sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    else launch_k(k_jacobi_error<false>, g, kBlock, 0, s, sc, nb, pos, vel, rho, u0, u1, u2, ncand, partials);
+    launch_k(k_fold_partials, 1, kBlock, 0, s, partials, g, ncand, err3);
 }
 void advect(const Scenes& sc, float4* pos, float4* vel, uint32_t* nonfinite, hipStream_t s) {
-    if (sc.N) k_advect<<<nb_(sc.N), kBlock, 0, s>>>(sc, pos, vel, nonfinite);
+    if (sc.N) launch_k(k_advect, nb_(sc.N), kBlock, 0, s, sc, pos, vel, nonfinite);
 }
 void solver_reset(SolverDev* st, uint32_t S, uint32_t* active, hipStream_t s) {
-    if (S) k_solver_reset<<<S, kBlock, 0, s>>>(st, S, active);
+    if (S) launch_k(k_solver_reset, S, kBlock, 0, s, st, S, active);
 }
 void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s) {
-    if (n) k_calib_copy<<<4096, kBlock, 0, s>>>(a, b, n);
+    if (n) launch_k(k_calib_copy, 4096, kBlock, 0, s, a, b, n);
 }
 void zero_words(uint32_t* p, size_t n, hipStream_t s) {
-    if (n) k_zero_words<<<unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n);
+    if (n) launch_k(k_zero_words, unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s, p, n);
 }
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s) {
-    if (n) k_pack_readback<<<nb_(n), kBlock, 0, s>>>(n, pos, color, perm, xyz, rad, rgba);
+    if (n) launch_k(k_pack_readback, nb_(n), kBlock, 0, s, n, pos, color, perm, xyz, rad, rgba);
 }
 
 }  // namespace launch
