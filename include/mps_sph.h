@@ -17,6 +17,8 @@
  *   mps_upload_boundary              BoundaryParticleObject::Resize        MPSBoundaryParticleObject.cu:14-24
  *   mps_sample_boundary_mesh         MeshObject::LoadMesh + kernel::ParticleSampling::ParticleSampling
  *                                    MPSMeshObject.cu:58-103, MPSParticleSamplingUtil.cu:42-133
+ *   mps_read_obj,                    the OBJ reading of MeshObject::LoadMesh  MPSMeshObject.cu:58-94
+ *   mps_sample_boundary_obj          (+ the above for the file's mesh)
  *   mps_step                         System::OnUpdate                      MPSSystem.cpp:191-291
  *   mps_run_phase                    the kernel::SPH / kernel::Advect / SpatialHash calls made by
  *                                    OnUpdate, one at a time              MPSSystem.cpp:220-278
@@ -233,6 +235,20 @@ int mps_upload_boundary(mps_ctx* ctx, size_t n, const float* pos, const float* r
 int mps_sample_boundary_mesh(mps_ctx* ctx, const double* verts, size_t nv,
                              const uint32_t* tris, size_t nt, const double* center,
                              const double* size, const mps_mesh_material* m);
+/* The file-reading half of MeshObject::LoadMesh (MPSMeshObject.cu:58-103): the vertices
+ * (x, y, z) and 0-based triangles of a Wavefront OBJ file, read token by token as the
+ * reference does (tokens longer than one character are skipped; `v` takes three reals, `f`
+ * three 1-based indices; a quad keeps its first triangle; `a/b/c` indices use `a`). Call
+ * with verts = tris = NULL for the counts, then with buffers of at least *nv x 3 doubles and
+ * *nt x 3 indices. Malformed numbers, indices of 0 or past the vertex count, an unreadable
+ * file or one without vertices or faces give MPS_ERR_INVALID (the reference loops or exits).
+ * `ctx` may be NULL (no GPU needed; no error message is kept then). */
+int mps_read_obj(mps_ctx* ctx, const char* path, double* verts, size_t* nv, uint32_t* tris,
+                 size_t* nt);
+/* MeshObject::LoadMesh + ParticleSampling from an OBJ file: mps_read_obj, then
+ * mps_sample_boundary_mesh with the same center/size/material arguments. */
+int mps_sample_boundary_obj(mps_ctx* ctx, const char* path, const double* center,
+                            const double* size, const mps_mesh_material* m);
 
 /* ---- stepping ---------------------------------------------------------------------- */
 /* One System::OnUpdate. `out` may be NULL. */

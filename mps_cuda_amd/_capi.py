@@ -119,6 +119,8 @@ _SIGS = {
     "mps_upload_fluid": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "mps_upload_boundary": (C.c_int, [_P, C.c_size_t, _P, _P, _P]),
     "mps_sample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(MeshMaterial)]),
+    "mps_read_obj": (C.c_int, [_P, C.c_char_p, _P, C.POINTER(C.c_size_t), _P, C.POINTER(C.c_size_t)]),
+    "mps_sample_boundary_obj": (C.c_int, [_P, C.c_char_p, _P, _P, C.POINTER(MeshMaterial)]),
     "mps_step": (C.c_int, [_P, C.POINTER(StepStats)]),
     "mps_run_phase": (C.c_int, [_P, C.c_int32, C.POINTER(StepStats)]),
     "mps_synchronize": (C.c_int, [_P]),
@@ -180,6 +182,23 @@ def default_config() -> Config:
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def read_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Vertices (nv, 3) float64 and 0-based triangles (nt, 3) uint32 of a Wavefront OBJ file,
+    read as MeshObject::LoadMesh reads it (MPSMeshObject.cu:58-94). Host-only: no GPU needed."""
+    lib = load()
+    nv, nt = C.c_size_t(), C.c_size_t()
+    p = os.fsencode(path)
+    rc = lib.mps_read_obj(None, p, None, C.byref(nv), None, C.byref(nt))
+    if rc:
+        raise MPSError(rc, f"cannot read OBJ file {path}")
+    v = np.zeros((nv.value, 3), np.float64)
+    t = np.zeros((nt.value, 3), np.uint32)
+    rc = lib.mps_read_obj(None, p, _ptr(v), C.byref(nv), _ptr(t), C.byref(nt))
+    if rc:
+        raise MPSError(rc, f"cannot read OBJ file {path}")
+    return v, t
 
 
 class MPSError(RuntimeError):
@@ -249,6 +268,13 @@ class HipSim:
         m = MeshMaterial(radius, density)
         self._chk(self.lib.mps_sample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
                                                     _ptr(c), _ptr(s), C.byref(m)))
+
+    def sample_boundary_obj(self, path, radius, density=1.0, center=None, size=None):
+        """MeshObject::LoadMesh(path, center, size, density) + ParticleSampling."""
+        c = None if center is None else np.ascontiguousarray(center, np.float64)
+        s = None if size is None else np.ascontiguousarray(size, np.float64)
+        m = MeshMaterial(radius, density)
+        self._chk(self.lib.mps_sample_boundary_obj(self.h, os.fsencode(path), _ptr(c), _ptr(s), C.byref(m)))
 
     # ---- batched scenes
     def batch_clear(self):

@@ -61,7 +61,29 @@ def build(verbose: bool = True, csrc: str = CSRC, lib: str = LIB, out_dir: str =
     os.replace(tmp, lib)
     if verbose:
         print(f"[mps_cuda_amd] built {lib}", file=sys.stderr)
+    if lib == LIB:
+        build_facade_driver(verbose)
     return lib
+
+
+REPO = os.path.dirname(HERE)
+FACADE_SRC = os.path.join(REPO, "tests", "cpp", "facade_scene.cpp")
+FACADE_BIN = os.path.join(OUT_DIR, "facade_scene")
+
+
+def build_facade_driver(verbose: bool = True) -> str:
+    """Host-only C++ program that drives the library through include/mps_system.hpp (the
+    reference's System/SPHMaterial/MeshObject/... names); found next to the library at run time."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(REPO, "include"),
+           FACADE_SRC, "-L", HERE, "-lmps_sph", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-o", FACADE_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade driver build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"[mps_cuda_amd] built {FACADE_BIN}", file=sys.stderr)
+    return FACADE_BIN
 
 
 if __name__ == "__main__":

@@ -11,9 +11,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -1944,6 +1946,70 @@ int guarded(mps_ctx* c, F&& f) {
         return MPS_ERR_NOMEM;
     }
 }
+
+// Wavefront OBJ reading as MeshObject::LoadMesh does it (MPSMeshObject.cu:58-103): the file is
+// a stream of whitespace-separated tokens; a token longer than one character is skipped (so
+// `vn`, `vt`, `usemtl`, ... and every word of a comment are ignored, but a one-character word
+// is not); `v` is followed by three reals, `f` by three 1-based vertex indices (a fourth index
+// of a quad is a one-character token or longer and is skipped like any other, so a quad keeps
+// its first triangle). Where the reference's stream would stop making progress or read garbage
+// (a malformed number, an index of 0 or past the vertices, a missing file, no faces or no
+// vertices), this reader reports an error instead. Indices written `a/b/c` use `a`.
+struct ObjMesh {
+    std::vector<double> v;
+    std::vector<uint32_t> f;
+};
+ObjMesh read_obj_file(const char* path) {
+    if (!path) throw ApiError{MPS_ERR_INVALID, "OBJ path is null"};
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) throw ApiError{MPS_ERR_INVALID, std::string("cannot open OBJ file ") + path};
+    std::string text;
+    char buf[1 << 16];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), fp)) > 0) text.append(buf, got);
+    std::fclose(fp);
+    size_t at = 0;
+    auto next = [&](std::string& tok) {
+        while (at < text.size() && std::isspace(static_cast<unsigned char>(text[at]))) at++;
+        const size_t b = at;
+        while (at < text.size() && !std::isspace(static_cast<unsigned char>(text[at]))) at++;
+        tok.assign(text, b, at - b);
+        return !tok.empty();
+    };
+    ObjMesh m;
+    std::string tok;
+    std::vector<uint64_t> idx;
+    while (next(tok)) {
+        if (tok.size() > 1) continue;
+        if (tok[0] == 'v') {
+            for (int k = 0; k < 3; k++) {
+                std::string t;
+                char* end = nullptr;
+                const double x = next(t) ? std::strtod(t.c_str(), &end) : 0.0;
+                if (t.empty() || !end || *end != '\0') throw ApiError{MPS_ERR_INVALID, "malformed vertex in OBJ file"};
+                m.v.push_back(x);
+            }
+        } else if (tok[0] == 'f') {
+            for (int k = 0; k < 3; k++) {
+                std::string t;
+                if (!next(t)) throw ApiError{MPS_ERR_INVALID, "malformed face in OBJ file"};
+                const size_t slash = t.find('/');
+                const std::string a = t.substr(0, slash);
+                if (a.empty() || a.size() > 10 || a.find_first_not_of("0123456789") != std::string::npos)
+                    throw ApiError{MPS_ERR_INVALID, "malformed face index in OBJ file: " + t};
+                idx.push_back(std::stoull(a));
+            }
+        }
+    }
+    const size_t nv = m.v.size() / 3;
+    if (nv == 0 || idx.empty()) throw ApiError{MPS_ERR_INVALID, "OBJ file has no vertices or no faces"};
+    m.f.reserve(idx.size());
+    for (const uint64_t q : idx) {
+        if (q == 0 || q > nv) throw ApiError{MPS_ERR_INVALID, "OBJ face index out of range"};
+        m.f.push_back(static_cast<uint32_t>(q - 1));
+    }
+    return m;
+}
 }  // namespace
 
 extern "C" {
@@ -2067,6 +2133,38 @@ int mps_sample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const u
                              const double* center, const double* size, const mps_mesh_material* m) {
     if (!c || !m) return MPS_ERR_INVALID;
     return guarded(c, [&] { c->sample_mesh(verts, nv, tris, nt, center, size, m); });
+}
+
+int mps_read_obj(mps_ctx* c, const char* path, double* verts, size_t* nv, uint32_t* tris, size_t* nt) {
+    if (!nv || !nt) return MPS_ERR_INVALID;
+    auto body = [&] {
+        const ObjMesh m = read_obj_file(path);
+        const size_t v = m.v.size() / 3, t = m.f.size() / 3;
+        if (verts || tris) {
+            if (!verts || !tris || *nv < v || *nt < t) throw ApiError{MPS_ERR_INVALID, "OBJ output buffers too small"};
+            std::memcpy(verts, m.v.data(), m.v.size() * sizeof(double));
+            std::memcpy(tris, m.f.data(), m.f.size() * sizeof(uint32_t));
+        }
+        *nv = v;
+        *nt = t;
+    };
+    if (c) return guarded(c, body);
+    try {  // no context: host work only, status code without a message
+        body();
+        return MPS_OK;
+    } catch (const ApiError& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return MPS_ERR_INVALID;
+    }
+}
+
+int mps_sample_boundary_obj(mps_ctx* c, const char* path, const double* center, const double* size, const mps_mesh_material* m) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        const ObjMesh o = read_obj_file(path);
+        c->sample_mesh(o.v.data(), o.v.size() / 3, o.f.data(), o.f.size() / 3, center, size, m);
+    });
 }
 
 int mps_step(mps_ctx* c, mps_step_stats* out) {
