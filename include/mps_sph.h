@@ -17,6 +17,8 @@
  *   mps_upload_boundary              BoundaryParticleObject::Resize        MPSBoundaryParticleObject.cu:14-24
  *   mps_sample_boundary_mesh         MeshObject::LoadMesh + kernel::ParticleSampling::ParticleSampling
  *                                    MPSMeshObject.cu:58-103, MPSParticleSamplingUtil.cu:42-133
+ *   mps_resample_boundary_mesh       ParticleSampling again on a moved mesh (IsSamplingParticleRequired,
+ *                                    pIsGenerated / prevIdx)  MPSParticleSamplingUtil.cu:14-133
  *   mps_read_obj,                    the OBJ reading of MeshObject::LoadMesh  MPSMeshObject.cu:58-94
  *   mps_sample_boundary_obj          (+ the above for the file's mesh)
  *   mps_step                         System::OnUpdate                      MPSSystem.cpp:191-291
@@ -95,6 +97,7 @@ enum mps_array_id {
     MPS_ARR_BND_MASS = 23,       /* real[B] */
     MPS_ARR_BND_VEL = 24,        /* real[B][3] */
     MPS_ARR_BND_FACE_ID = 25,    /* uint32[B] */
+    MPS_ARR_BND_BCC = 26,        /* real[B][2]  barycentric weights (w0, w1) on the owning face */
     MPS_ARR_FHASH_KEY = 30,      /* uint32[N]   sorted cell keys of the fluid hash */
     MPS_ARR_FHASH_ID = 31,       /* uint32[N]   particle ids in key order */
     MPS_ARR_FHASH_START = 32,    /* uint32[C]   first sorted slot of each cell, 0xFFFFFFFF if empty */
@@ -235,6 +238,19 @@ int mps_upload_boundary(mps_ctx* ctx, size_t n, const float* pos, const float* r
 int mps_sample_boundary_mesh(mps_ctx* ctx, const double* verts, size_t nv,
                              const uint32_t* tris, size_t nt, const double* center,
                              const double* size, const mps_mesh_material* m);
+/* ParticleSampling called again on the mesh of the last mps_sample_boundary_mesh after its
+ * vertices moved (MPSParticleSamplingUtil.cu:14-133, .cuh:69-246). `verts` are the nv new
+ * vertex positions (same count and faces; no fitting), m->radius the sampling radius. The
+ * sample count and shortest edge of every face are kept from the previous sampling; a face
+ * where either changed is generated anew, every other face keeps its previous particles (moved
+ * to the face's new offset, positions unchanged). Nothing changes when no face changed. As in
+ * the reference, the particle arrays are resized by index: with faithful_quirks a kept
+ * particle takes the mass, velocity and volume stored at its new index (new slots start at
+ * zero); without, it carries its own mass and velocity. The vertex masses stay those of the
+ * first sampling. *regenerated (may be NULL) receives the number of faces generated anew.
+ * MPS_ERR_STATE if no mesh was sampled or the walls were replaced since. */
+int mps_resample_boundary_mesh(mps_ctx* ctx, const double* verts, size_t nv,
+                               const mps_mesh_material* m, size_t* regenerated);
 /* The file-reading half of MeshObject::LoadMesh (MPSMeshObject.cu:58-103): the vertices
  * (x, y, z) and 0-based triangles of a Wavefront OBJ file, read token by token as the
  * reference does (tokens longer than one character are skipped; `v` takes three reals, `f`

@@ -15,7 +15,8 @@
  *   mps::MeshObject::LoadMesh      MPSMeshObject.cu:58-103
  *   mps::SpatialHash               MPSSpatialHash.h:20-57 (sizes; UpdateHash / ZSort /
  *                                  BuildNeighorhood run the context's phases)
- *   mps::kernel::ParticleSampling  MPSParticleSamplingUtil.h:13-14, .cu:14-133
+ *   mps::kernel::ParticleSampling  MPSParticleSamplingUtil.h:13-14, .cu:14-133 (a second call
+ *                                  after MeshObject::SetVertexPositions re-samples)
  */
 #ifndef MPS_SYSTEM_HPP
 #define MPS_SYSTEM_HPP
@@ -174,7 +175,16 @@ public:
         m_center = vCenter;
         m_size = vSize;
         m_density = density;
+        m_world.clear();
     }
+    // Moves the vertices (world coordinates, after the fit; same count): the next
+    // ParticleSampling into a boundary already sampled from this mesh re-samples only the
+    // faces whose sampling changed (MPSParticleSamplingUtil.cu:14-133).
+    void SetVertexPositions(const std::vector<double>& world) {
+        if (world.size() != m_vertices.size()) throw Exception(MPS_ERR_INVALID, "MeshObject::SetVertexPositions: vertex count");
+        m_world = world;
+    }
+    const std::vector<double>& GetVertexPositions() const { return m_world; }
     size_t GetSize() const { return m_faces.size() / 3; }
     size_t GetVertexSize() const { return m_vertices.size() / 3; }
     const std::vector<double>& GetVertices() const { return m_vertices; }
@@ -184,7 +194,7 @@ public:
     REAL GetDensity() const { return m_density; }
 
 private:
-    std::vector<double> m_vertices;
+    std::vector<double> m_vertices, m_world;
     std::vector<uint32_t> m_faces;
     REAL3 m_center, m_size;
     REAL m_density = 1.0;
@@ -201,6 +211,9 @@ public:
         return n;
     }
     mps_ctx* context() const { return m_ctx ? *m_ctx : nullptr; }
+    // the mesh and context of the last full sampling (a later call with moved vertices re-samples)
+    const void* m_sampledFrom = nullptr;
+    const mps_ctx* m_sampledIn = nullptr;
 
 private:
     mps_ctx* const* m_ctx;
@@ -263,9 +276,18 @@ namespace kernel {
 namespace ParticleSampling {
 // ParticleSampling(material, mesh, boundary) (MPSParticleSamplingUtil.cu:14-133) after
 // LoadMesh's Translate and InitFaceInfo: samples the mesh into the boundary's context.
+// Called again for the same mesh and boundary after MeshObject::SetVertexPositions, it
+// re-samples the faces whose sample count or shortest edge changed and keeps the others.
 inline void ParticleSampling(const MeshMaterialParam& material, const MeshObject& mesh, BoundaryParticleObject& boundary) {
     mps_ctx* c = boundary.context();
     if (!c) throw Exception(MPS_ERR_STATE, "ParticleSampling: the boundary object has no context");
+    if (boundary.m_sampledFrom == &mesh && boundary.m_sampledIn == c && !mesh.GetVertexPositions().empty()) {
+        const mps_mesh_material m{material.radius, mesh.GetDensity()};
+        detail::check(c,
+                      mps_resample_boundary_mesh(c, mesh.GetVertexPositions().data(), mesh.GetVertexSize(), &m, nullptr),
+                      "ParticleSampling");
+        return;
+    }
     const double center[3] = {mesh.GetCenter().x, mesh.GetCenter().y, mesh.GetCenter().z};
     const double size[3] = {mesh.GetFitSize().x, mesh.GetFitSize().y, mesh.GetFitSize().z};
     const mps_mesh_material m{material.radius, mesh.GetDensity()};
@@ -273,6 +295,8 @@ inline void ParticleSampling(const MeshMaterialParam& material, const MeshObject
                   mps_sample_boundary_mesh(c, mesh.GetVertices().data(), mesh.GetVertexSize(), mesh.GetFaces().data(),
                                            mesh.GetSize(), center, size, &m),
                   "ParticleSampling");
+    boundary.m_sampledFrom = &mesh;
+    boundary.m_sampledIn = c;
 }
 }  // namespace ParticleSampling
 }  // namespace kernel

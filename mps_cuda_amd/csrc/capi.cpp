@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -181,6 +182,14 @@ struct mps_ctx {
     DBuf<float> bvol, bS;
     DBuf<uint32_t> bface;
     DBuf<float2> bbcc;
+    // The mesh of the last mps_sample_boundary_mesh (vertices after fitting, faces, ownership
+    // bits, vertex masses) and each face's sample count and shortest edge, as MeshObject keeps
+    // them between ParticleSampling calls (m_samplingParticleSize, m_shortestEdgeID).
+    struct MeshKeep {
+        std::vector<double> V, vmass;
+        std::vector<uint32_t> F, R, count, shorts;
+        bool ok = false;
+    } mesh;
     bool bnd_dirty = true;
     HashSet fh, bh;
     DBuf<uint32_t> zkey, zid, zkey2, zid2, ztemp, tmpu;
@@ -1479,6 +1488,7 @@ struct mps_ctx {
         N = uint32_t(n);
         Nref = uint32_t(nref);
         B = uint32_t(nb);
+        mesh.ok = false;
         alloc_fluid(N);
         alloc_boundary(B);
         std::vector<float4> hp(N, make_float4(0.f, 0.f, 0.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -1531,6 +1541,7 @@ struct mps_ctx {
     void upload_boundary(size_t n, const float* p, const float* r, const float* m) {
         if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
         require_single("mps_upload_boundary");
+        mesh.ok = false;
         alloc_boundary(uint32_t(n));
         std::vector<float4> hp(n), hv(n);
         for (size_t i = 0; i < n; i++) {
@@ -1629,10 +1640,14 @@ struct mps_ctx {
         HIPCHK(hipMemcpyAsync(dmass.p, vmass.data(), nv * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dF.p, tris, 3 * nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dR.p, rtri.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-        mps::launch::sample_count(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, stream);
-        std::vector<uint32_t> cnt(nt + 1, 0u);
+        DBuf<uint32_t> dshort;
+        dshort.reserve(nt);
+        mps::launch::sample_count(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, dshort.p, stream);
+        std::vector<uint32_t> cnt(nt + 1, 0u), shorts(nt, 0u);
         HIPCHK(hipMemcpyAsync(cnt.data(), dcnt.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(shorts.data(), dshort.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
+        dshort.release();
         std::vector<uint32_t> offs(nt + 1, 0u);
         for (size_t t = 0; t < nt; t++) offs[t + 1] = offs[t] + cnt[t];
         const uint32_t n = offs[nt];
@@ -1641,19 +1656,135 @@ struct mps_ctx {
         dface.reserve(std::max<uint32_t>(n, 1));
         mps::launch::zero_words(d_flags + 1, 1, stream);
         mps::launch::fill_u32(dface.p, std::max<uint32_t>(n, 1), 0xFFFFFFFFu, stream);
-        mps::launch::sample_generate(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, dpos.p, dface.p, d_flags + 1, stream);
+        mps::launch::sample_generate(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, nullptr, dcnt.p, dpos.p, dface.p, d_flags + 1,
+                                     stream);
         alloc_boundary(n);
         B = n;
-        mps::launch::sample_finish(dV.p, dF.p, uint32_t(nt), dmass.p, n, dpos.p, dface.p, m->radius, bpos.p, bvel.p, bbcc.p, stream);
+        mps::launch::sample_finish(dV.p, dF.p, uint32_t(nt), dmass.p, n, dpos.p, dface.p, m->radius, 0, bpos.p, bvel.p, bbcc.p,
+                                   bface.p, bcolor.p, stream);
         if (n) {
-            HIPCHK(hipMemcpyAsync(bface.p, dface.p, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
-            mps::launch::zero_words(reinterpret_cast<uint32_t*>(bcolor.p), 4 * size_t(n), stream);
             mps::launch::zero_words(reinterpret_cast<uint32_t*>(bvol.p), n, stream);
             mps::launch::zero_words(cbb.p, n, stream);
         }
         HIPCHK(hipMemcpyAsync(h_flags + 1, d_flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
         dV.release(); dmass.release(); dpos.release(); dF.release(); dR.release(); dcnt.release(); dface.release();
+        bh.cells_init = false;
+        bh.n_valid = 0;
+        bnd_dirty = true;
+        if (h_flags[1]) throw ApiError{MPS_ERR_STATE, "sample count mismatch between count and generate passes"};
+        mesh.V = std::move(V);
+        mesh.F.assign(tris, tris + 3 * nt);
+        mesh.R = std::move(rtri);
+        mesh.vmass = std::move(vmass);
+        mesh.count.assign(cnt.begin(), cnt.begin() + nt);
+        mesh.shorts = std::move(shorts);
+        mesh.ok = true;
+    }
+
+    // ParticleSampling on the same mesh after its vertices moved (IsSamplingParticleRequired +
+    // GenerateBoundaryParticle_kernel + SetBarycentric_kernel, MPSParticleSamplingUtil.cu:14-133,
+    // .cuh:69-294). The count pass runs on the GPU; the host compares each face's count and
+    // shortest edge with the kept ones and scans the offsets (faces are few); faces that
+    // changed are generated anew and finished on the GPU, the other faces' particles move to
+    // their new offset in a per-slot copy. The arrays are resized by index first, as
+    // BoundaryParticleObject::Resize does: volume always, and mass/velocity with faithful
+    // quirks, stay with the index; otherwise mass and velocity move with the particle.
+    void resample_mesh(const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) {
+        require_single("mps_resample_boundary_mesh");
+        if (!mesh.ok) throw ApiError{MPS_ERR_STATE, "no mesh sampled on this context (mps_sample_boundary_mesh first)"};
+        if (!verts || 3 * nv != mesh.V.size()) throw ApiError{MPS_ERR_INVALID, "vertex count differs from the sampled mesh"};
+        const size_t nt = mesh.count.size();
+        std::vector<double> V(verts, verts + 3 * nv);
+        DBuf<double> dV, dpos, dmass;
+        DBuf<uint32_t> dF, dR, dcnt, dshort, dface, dcurr, dprev, p_face;
+        DBuf<uint8_t> dgen;
+        DBuf<float4> p_pos, p_col, p_vel;
+        DBuf<float2> p_bcc;
+        struct Release {  // scratch is freed on every way out (DBuf frees only on release())
+            std::function<void()> f;
+            ~Release() { f(); }
+        } scratch{[&] {
+            dV.release(); dpos.release(); dmass.release(); dF.release(); dR.release(); dcnt.release(); dshort.release();
+            dface.release(); dcurr.release(); dprev.release(); p_face.release(); dgen.release(); p_pos.release();
+            p_col.release(); p_vel.release(); p_bcc.release();
+        }};
+        dV.reserve(3 * nv); dF.reserve(3 * nt); dR.reserve(nt); dcnt.reserve(nt); dshort.reserve(nt);
+        HIPCHK(hipMemcpyAsync(dV.p, V.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dF.p, mesh.F.data(), 3 * nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dR.p, mesh.R.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        mps::launch::sample_count(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, dshort.p, stream);
+        std::vector<uint32_t> cnt(nt), shorts(nt);
+        HIPCHK(hipMemcpyAsync(cnt.data(), dcnt.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(shorts.data(), dshort.p, nt * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        std::vector<uint32_t> prev(nt + 1, 0u), curr(nt + 1, 0u);
+        std::vector<uint8_t> gen(nt, 0u);
+        size_t ngen = 0;
+        for (size_t t = 0; t < nt; t++) {
+            prev[t + 1] = prev[t] + mesh.count[t];
+            curr[t + 1] = curr[t] + cnt[t];
+            if (shorts[t] != mesh.shorts[t] || cnt[t] != mesh.count[t]) {
+                gen[t] = 1u;
+                ngen++;
+            }
+        }
+        mesh.V = std::move(V);
+        mesh.count = cnt;
+        mesh.shorts = shorts;
+        if (regenerated) *regenerated = ngen;
+        if (ngen == 0) return;
+        if (prev[nt] != B) throw ApiError{MPS_ERR_STATE, "boundary particles no longer match the sampled mesh"};
+        const uint32_t B0 = B, n = curr[nt];
+        params_dirty = true;
+        // the previous particle arrays
+        const size_t m0 = std::max<uint32_t>(B0, 1);
+        p_pos.reserve(m0); p_col.reserve(m0); p_vel.reserve(m0); p_bcc.reserve(m0); p_face.reserve(m0);
+        if (B0) {
+            HIPCHK(hipMemcpyAsync(p_pos.p, bpos.p, B0 * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(p_col.p, bcolor.p, B0 * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(p_vel.p, bvel.p, B0 * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(p_bcc.p, bbcc.p, B0 * sizeof(float2), hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(p_face.p, bface.p, B0 * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+        }
+        // resize by index: the leading min(B0, n) values stay, new slots are zero
+        auto resize_keep = [&](auto& buf, size_t words_per) {
+            using E = std::remove_reference_t<decltype(*buf.p)>;
+            std::remove_reference_t<decltype(buf)> nb;
+            nb.reserve(std::max<uint32_t>(n, 1));
+            const size_t keep = std::min(B0, n);
+            if (keep) HIPCHK(hipMemcpyAsync(nb.p, buf.p, keep * sizeof(E), hipMemcpyDeviceToDevice, stream));
+            if (n > keep) mps::launch::zero_words(reinterpret_cast<uint32_t*>(nb.p + keep), (n - keep) * words_per, stream);
+            sync();
+            buf.release();
+            buf = nb;
+            nb.p = nullptr;
+            nb.cap = 0;
+        };
+        resize_keep(bvel, 4);
+        resize_keep(bvol, 1);
+        bpos.reserve(std::max<uint32_t>(n, 1)); bcolor.reserve(std::max<uint32_t>(n, 1));
+        bbcc.reserve(std::max<uint32_t>(n, 1)); bface.reserve(std::max<uint32_t>(n, 1));
+        bS.reserve(std::max<uint32_t>(n, 1)); cbb.reserve(std::max<uint32_t>(n, 1));
+        if (n) mps::launch::zero_words(cbb.p, n, stream);
+        // faces that changed: generate and finish; the rest: copy
+        dpos.reserve(3 * size_t(std::max<uint32_t>(n, 1))); dface.reserve(std::max<uint32_t>(n, 1));
+        dmass.reserve(nv); dcurr.reserve(nt + 1); dprev.reserve(nt + 1); dgen.reserve(nt);
+        HIPCHK(hipMemcpyAsync(dmass.p, mesh.vmass.data(), nv * sizeof(double), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dcurr.p, curr.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dprev.p, prev.data(), (nt + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dgen.p, gen.data(), nt, hipMemcpyHostToDevice, stream));
+        mps::launch::zero_words(d_flags + 1, 1, stream);
+        mps::launch::fill_u32(dface.p, std::max<uint32_t>(n, 1), 0xFFFFFFFFu, stream);
+        mps::launch::sample_generate(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dgen.p, dcurr.p, dpos.p, dface.p, d_flags + 1,
+                                     stream);
+        mps::launch::sample_finish(dV.p, dF.p, uint32_t(nt), dmass.p, n, dpos.p, dface.p, m->radius, 1, bpos.p, bvel.p, bbcc.p,
+                                   bface.p, bcolor.p, stream);
+        mps::launch::sample_copy(uint32_t(nt), dcurr.p, dprev.p, dgen.p, n, p_pos.p, p_bcc.p, p_face.p, p_col.p,
+                                 cfg.faithful_quirks ? nullptr : p_vel.p, bpos.p, bbcc.p, bface.p, bcolor.p, bvel.p, stream);
+        HIPCHK(hipMemcpyAsync(h_flags + 1, d_flags + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        sync();
+        B = n;
         bh.cells_init = false;
         bh.n_valid = 0;
         bnd_dirty = true;
@@ -1860,6 +1991,7 @@ struct mps_ctx {
             case MPS_ARR_BND_MASS: f4w(bvel, B); break;
             case MPS_ARR_BND_VEL: f4xyz(bvel, B); break;
             case MPS_ARR_BND_FACE_ID: raw(bface.p, B, 4); break;
+            case MPS_ARR_BND_BCC: raw(bbcc.p, B, 8); break;
             case MPS_ARR_FHASH_KEY: {
                 std::vector<uint32_t> v;
                 if (dst) fluid_hash_export(&v, nullptr); else v.resize(fh.n_valid);
@@ -2133,6 +2265,11 @@ int mps_sample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const u
                              const double* center, const double* size, const mps_mesh_material* m) {
     if (!c || !m) return MPS_ERR_INVALID;
     return guarded(c, [&] { c->sample_mesh(verts, nv, tris, nt, center, size, m); });
+}
+
+int mps_resample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->resample_mesh(verts, nv, m, regenerated); });
 }
 
 int mps_read_obj(mps_ctx* c, const char* path, double* verts, size_t* nv, uint32_t* tris, size_t* nt) {

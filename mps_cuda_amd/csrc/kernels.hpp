@@ -265,10 +265,16 @@ void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uin
 
 // ---- sampling.hip
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
-                  hipStream_t s);
-void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+                  uint32_t* shorts, hipStream_t s);
+void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, const uint8_t* gen,
                      const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad, hipStream_t s);
 void sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n, const double* pos,
-                   const uint32_t* face, double radius, float4* bpos, float4* bvel, float2* bbcc, hipStream_t s);
+                   const uint32_t* face, double radius, int keep_vel, float4* bpos, float4* bvel, float2* bbcc,
+                   uint32_t* bface, float4* bcolor, hipStream_t s);
+// re-sampling: slots of faces not generated anew take their particle from the previous arrays
+void sample_copy(uint32_t nt, const uint32_t* curr, const uint32_t* prev, const uint8_t* gen, uint32_t n,
+                 const float4* prev_bpos, const float2* prev_bbcc, const uint32_t* prev_bface, const float4* prev_bcolor,
+                 const float4* prev_bvel, float4* bpos, float2* bbcc, uint32_t* bface, float4* bcolor, float4* bvel,
+                 hipStream_t s);
 }  // namespace launch
 }  // namespace mps

@@ -70,7 +70,7 @@ __device__ void gen_line(d3 a, d3 b, float d, uint32_t f, Emit* e, uint32_t* cou
 // (:146-246) otherwise. The count pass uses the double spacing, the generate pass the
 // float one, as in the reference.
 __device__ void sample_face(const double* V, const uint32_t* F, const uint32_t* R, uint32_t id, double radius, Emit* e,
-                            uint32_t* count) {
+                            uint32_t* count, uint32_t* short_edge = nullptr) {
     const uint32_t f0 = F[3 * id], f1 = F[3 * id + 1], f2 = F[3 * id + 2];
     d3 ns[3] = {{V[3 * f0], V[3 * f0 + 1], V[3 * f0 + 2]}, {V[3 * f1], V[3 * f1 + 1], V[3 * f1 + 2]},
                 {V[3 * f2], V[3 * f2 + 1], V[3 * f2 + 2]}};
@@ -79,6 +79,7 @@ __device__ void sample_face(const double* V, const uint32_t* F, const uint32_t* 
     uint32_t iShort = 0u;
     if (ls[iShort] > ls[1]) iShort = 1u;
     if (ls[iShort] > ls[2]) iShort = 2u;
+    if (short_edge) *short_edge = iShort;
     const double d = (radius + radius) * 0.25;
     const uint32_t rTri = R[id];
     uint32_t num = 0u;
@@ -110,32 +111,39 @@ __device__ void sample_face(const double* V, const uint32_t* F, const uint32_t* 
     if (count) *count = num;
 }
 
+// Per face: sample count and shortest edge (kept by the mesh between samplings).
 __global__ void k_sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
-                               uint32_t* counts) {
+                               uint32_t* counts, uint32_t* shorts) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= nt) return;
-    uint32_t c = 0;
-    sample_face(V, F, R, id, radius, nullptr, &c);
+    uint32_t c = 0, sh = 0;
+    sample_face(V, F, R, id, radius, nullptr, &c, &sh);
     counts[id] = c;
+    if (shorts) shorts[id] = sh;
 }
 
+// Faces with gen[id] == 0 (gen may be null: every face) keep their previous particles and
+// are skipped here.
 __global__ void k_sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
-                                  const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad) {
+                                  const uint8_t* gen, const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= nt) return;
+    if (id >= nt || (gen && !gen[id])) return;
     Emit e{pos, face, offs[id], offs[id + 1]};
     sample_face(V, F, R, id, radius, &e, nullptr);
     if (e.cur != e.end) atomicOr(bad, 1u);
 }
 
-// SetBarycentric_kernel (:248-294) and the float4 boundary record (x, y, z, r = R_m/4).
+// SetBarycentric_kernel (:248-294) and the float4 boundary record (x, y, z, r = R_m/4) for the
+// slots written by k_sample_generate (face[id] < nt; other slots are left alone). The mass
+// goes to bvel.w; bvel.xyz is zeroed, or kept when keep_vel (a re-sampling leaves the
+// velocity stored at each index where it is).
 __global__ void k_sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n,
-                                const double* pos, const uint32_t* face, double radius, float4* bpos, float4* bvel,
-                                float2* bbcc) {
+                                const double* pos, const uint32_t* face, double radius, int keep_vel, float4* bpos,
+                                float4* bvel, float2* bbcc, uint32_t* bface, float4* bcolor) {
     const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= n) return;
     const uint32_t fi = face[id];
-    if (fi >= nt) return;  // only after a count/generate mismatch, which the host reports
+    if (fi >= nt) return;  // a kept particle, or a count/generate mismatch (reported by the host)
     const uint32_t f0 = F[3 * fi], f1 = F[3 * fi + 1], f2 = F[3 * fi + 2];
     const d3 n0{V[3 * f0], V[3 * f0 + 1], V[3 * f0 + 2]}, n1{V[3 * f1], V[3 * f1 + 1], V[3 * f1 + 2]},
         n2{V[3 * f2], V[3 * f2 + 1], V[3 * f2 + 2]};
@@ -154,8 +162,36 @@ __global__ void k_sample_finish(const double* V, const uint32_t* F, uint32_t nt,
     w1 = fmin(fmax(w1, 0.0), 1.0);
     const double m = vmass[f0] * w0 + vmass[f1] * w1 + vmass[f2] * (1.0 - w0 - w1);
     bpos[id] = make_float4(float(x.x), float(x.y), float(x.z), float(radius * 0.25));
-    bvel[id] = make_float4(0.f, 0.f, 0.f, float(m));
+    const float4 v = keep_vel ? bvel[id] : make_float4(0.f, 0.f, 0.f, 0.f);
+    bvel[id] = make_float4(v.x, v.y, v.z, float(m));
     bbcc[id] = make_float2(float(w0), float(w1));
+    bface[id] = fi;
+    bcolor[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// The particles of faces that were not generated anew move from their previous offset to the
+// new one (GenerateBoundaryParticle_kernel's else branch, :230-245), one thread per new slot:
+// position/radius, barycentric weights, face id and colour; mass and velocity too when
+// prev_bvel is given (otherwise they stay with the index).
+__global__ void k_sample_copy(uint32_t nt, const uint32_t* curr, const uint32_t* prev, const uint8_t* gen, uint32_t n,
+                              const float4* prev_bpos, const float2* prev_bbcc, const uint32_t* prev_bface,
+                              const float4* prev_bcolor, const float4* prev_bvel, float4* bpos, float2* bbcc,
+                              uint32_t* bface, float4* bcolor, float4* bvel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = nt;  // last face t with curr[t] <= i (its range holds i: curr[nt] = n > i)
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (curr[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    if (gen[lo]) return;
+    const uint32_t j = prev[lo] + (i - curr[lo]);
+    bpos[i] = prev_bpos[j];
+    bbcc[i] = prev_bbcc[j];
+    bface[i] = prev_bface[j];
+    bcolor[i] = prev_bcolor[j];
+    if (prev_bvel) bvel[i] = prev_bvel[j];
 }
 
 }  // namespace dev
@@ -163,16 +199,27 @@ __global__ void k_sample_finish(const double* V, const uint32_t* F, uint32_t nt,
 namespace launch {
 using namespace dev;
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
-                  hipStream_t s) {
-    if (nt) launch_k(k_sample_count, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, counts);
+                  uint32_t* shorts, hipStream_t s) {
+    if (nt) launch_k(k_sample_count, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, counts, shorts);
 }
-void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius,
+void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, const uint8_t* gen,
                      const uint32_t* offs, double* pos, uint32_t* face, uint32_t* bad, hipStream_t s) {
-    if (nt) launch_k(k_sample_generate, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, offs, pos, face, bad);
+    if (nt) launch_k(k_sample_generate, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, gen, offs, pos, face, bad);
 }
 void sample_finish(const double* V, const uint32_t* F, uint32_t nt, const double* vmass, uint32_t n, const double* pos,
-                   const uint32_t* face, double radius, float4* bpos, float4* bvel, float2* bbcc, hipStream_t s) {
-    if (n) launch_k(k_sample_finish, (n + 255) / 256, 256, 0, s, V, F, nt, vmass, n, pos, face, radius, bpos, bvel, bbcc);
+                   const uint32_t* face, double radius, int keep_vel, float4* bpos, float4* bvel, float2* bbcc,
+                   uint32_t* bface, float4* bcolor, hipStream_t s) {
+    if (n)
+        launch_k(k_sample_finish, (n + 255) / 256, 256, 0, s, V, F, nt, vmass, n, pos, face, radius, keep_vel, bpos, bvel,
+                 bbcc, bface, bcolor);
+}
+void sample_copy(uint32_t nt, const uint32_t* curr, const uint32_t* prev, const uint8_t* gen, uint32_t n,
+                 const float4* prev_bpos, const float2* prev_bbcc, const uint32_t* prev_bface, const float4* prev_bcolor,
+                 const float4* prev_bvel, float4* bpos, float2* bbcc, uint32_t* bface, float4* bcolor, float4* bvel,
+                 hipStream_t s) {
+    if (n && nt)
+        launch_k(k_sample_copy, (n + 255) / 256, 256, 0, s, nt, curr, prev, gen, n, prev_bpos, prev_bbcc, prev_bface,
+                 prev_bcolor, prev_bvel, bpos, bbcc, bface, bcolor, bvel);
 }
 }  // namespace launch
 }  // namespace mps

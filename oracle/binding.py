@@ -30,6 +30,7 @@ _SIGS = {
     "orc_upload_fluid": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "orc_upload_boundary": (C.c_int, [_P, C.c_size_t, _P, _P, _P]),
     "orc_sample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(capi.MeshMaterial)]),
+    "orc_resample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(capi.MeshMaterial), C.POINTER(C.c_size_t)]),
     "orc_step": (C.c_int, [_P, C.POINTER(capi.StepStats)]),
     "orc_run_phase": (C.c_int, [_P, C.c_int32, C.POINTER(capi.StepStats)]),
     "orc_array_info": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -136,6 +137,13 @@ class OracleSim:
         m = capi.MeshMaterial(radius, density)
         self._chk(self.lib.orc_sample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
                                                     _ptr(c), _ptr(s), C.byref(m)))
+
+    def resample_boundary_mesh(self, verts, radius) -> int:
+        verts = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)
+        m = capi.MeshMaterial(radius, 1.0)
+        k = C.c_size_t()
+        self._chk(self.lib.orc_resample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], C.byref(m), C.byref(k)))
+        return int(k.value)
 
     def step(self) -> capi.StepStats:
         st = capi.StepStats()

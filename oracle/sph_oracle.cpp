@@ -312,7 +312,8 @@ inline void gen_line(V3<double> a, V3<double> b, float d, uint32_t face, std::ve
 
 // Per-face sampling: ComputeSamplingNum_kernel (counts, .cuh:69-144) and
 // GenerateBoundaryParticle_kernel (positions, .cuh:146-246).
-void sample_face(const Mesh& M, size_t id, double radius, std::vector<Sample>* out, uint32_t* count) {
+void sample_face(const Mesh& M, size_t id, double radius, std::vector<Sample>* out, uint32_t* count,
+                 uint32_t* short_edge = nullptr) {
     const uint32_t* face = &M.f[3 * id];
     V3<double> ns[3];
     for (int k = 0; k < 3; k++) ns[k] = {M.v[3 * face[k]], M.v[3 * face[k] + 1], M.v[3 * face[k] + 2]};
@@ -321,6 +322,7 @@ void sample_face(const Mesh& M, size_t id, double radius, std::vector<Sample>* o
     uint32_t iShort = 0u;
     if (ls[iShort] > ls[1]) iShort = 1u;
     if (ls[iShort] > ls[2]) iShort = 2u;
+    if (short_edge) *short_edge = iShort;
     const double d = (radius + radius) * 0.25;
     const uint32_t rTri = M.rtri[id];
     uint32_t num = 0u;
@@ -363,6 +365,7 @@ struct Base {
     virtual int upload_boundary(size_t n, const double* pos, const double* rad, const double* mass) = 0;
     virtual int sample_mesh(const double* verts, size_t nv, const uint32_t* tris, size_t nt, const double* center,
                             const double* size, const mps_mesh_material* m) = 0;
+    virtual int resample_mesh(const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) = 0;
     virtual int step(mps_step_stats* st) = 0;
     virtual int run_phase(int32_t ph, mps_step_stats* st) = 0;
     virtual int array_info(int32_t id, size_t* count, size_t* eb) = 0;
@@ -385,6 +388,11 @@ struct Ctx final : Base {
     std::vector<T> bx, br, bvol, bm, bv, bbcc;
     std::vector<uint32_t> bface;
     std::vector<float> bcolor;
+    // the mesh of the last sample_mesh and its per-face sampling state
+    // (MeshObject::m_samplingParticleSize / m_shortestEdgeID, MPSMeshObject.cu:31-45)
+    Mesh mesh;
+    std::vector<uint32_t> mcount, mshort;
+    bool mesh_ok = false;
     Hash fh, bh;
     CSR ff, fb, bb;
 
@@ -437,6 +445,7 @@ struct Ctx final : Base {
         bh = Hash{};
     }
     int upload_boundary(size_t n, const double* pos, const double* rad, const double* mass) override {
+        mesh_ok = false;
         resize_boundary(n);
         for (size_t i = 0; i < 3 * n; i++) bx[i] = T(pos[i]);
         for (size_t i = 0; i < n; i++) { br[i] = T(rad[i]); bm[i] = mass ? T(mass[i]) : T(0); }
@@ -484,38 +493,124 @@ struct Ctx final : Base {
         M.rtri.resize(nt);
         rtri_build(nv, M.f.data(), nt, M.rtri.data());
         std::vector<Sample> S;
+        std::vector<uint32_t> counts(nt), shorts(nt);
         for (size_t t = 0; t < nt; t++) {
-            uint32_t cnt = 0;
             const size_t before = S.size();
-            sample_face(M, t, mat->radius, &S, &cnt);
-            if (S.size() - before != cnt) { err = "sample count mismatch between count and generate passes"; return MPS_ERR_STATE; }
+            sample_face(M, t, mat->radius, &S, &counts[t], &shorts[t]);
+            if (S.size() - before != counts[t]) { err = "sample count mismatch between count and generate passes"; return MPS_ERR_STATE; }
         }
         resize_boundary(S.size());
         for (size_t i = 0; i < S.size(); i++) {
             bx[3 * i] = T(S[i].x.x); bx[3 * i + 1] = T(S[i].x.y); bx[3 * i + 2] = T(S[i].x.z);
             br[i] = T(mat->radius * 0.25);
             bface[i] = S[i].face;
-            // SetBarycentric_kernel (MPSParticleSamplingUtil.cuh:248-294), in double
-            const uint32_t* f = &M.f[3 * S[i].face];
-            V3<double> n0{M.v[3 * f[0]], M.v[3 * f[0] + 1], M.v[3 * f[0] + 2]};
-            V3<double> n1{M.v[3 * f[1]], M.v[3 * f[1] + 1], M.v[3 * f[1] + 2]};
-            V3<double> n2{M.v[3 * f[2]], M.v[3 * f[2] + 1], M.v[3 * f[2] + 2]};
-            const V3<double> n20 = n0 - n2, n21 = n1 - n2;
-            const double t0 = dot(n20, n20), t1 = dot(n21, n21), t2 = dot(n20, n21);
-            const double t3 = dot(n20, S[i].x - n2), t4 = dot(n21, S[i].x - n2);
-            const double det = t0 * t1 - t2 * t2;
-            double w0 = 0.0, w1 = 0.0;
-            if (std::fabs(det) > 1.0e-20) {
-                const double invDet = 1.0 / det;
-                w0 = (+t1 * t3 - t2 * t4) * invDet;
-                w1 = (-t2 * t3 + t0 * t4) * invDet;
-            }
-            w0 = std::min(std::max(w0, 0.0), 1.0);
-            w1 = std::min(std::max(w1, 0.0), 1.0);
-            bbcc[2 * i] = T(w0);
-            bbcc[2 * i + 1] = T(w1);
-            bm[i] = T(M.vmass[f[0]] * w0 + M.vmass[f[1]] * w1 + M.vmass[f[2]] * (1.0 - w0 - w1));
+            set_barycentric(M, i, S[i].x, S[i].face);
         }
+        mesh = std::move(M);
+        mcount = std::move(counts);
+        mshort = std::move(shorts);
+        mesh_ok = true;
+        return MPS_OK;
+    }
+
+    // SetBarycentric_kernel (MPSParticleSamplingUtil.cuh:248-294), in double, for a particle
+    // generated at x on face `face`: barycentric weights (clamped) and mass from the vertex masses.
+    void set_barycentric(const Mesh& M, size_t i, V3<double> x, uint32_t face) {
+        const uint32_t* f = &M.f[3 * face];
+        V3<double> n0{M.v[3 * f[0]], M.v[3 * f[0] + 1], M.v[3 * f[0] + 2]};
+        V3<double> n1{M.v[3 * f[1]], M.v[3 * f[1] + 1], M.v[3 * f[1] + 2]};
+        V3<double> n2{M.v[3 * f[2]], M.v[3 * f[2] + 1], M.v[3 * f[2] + 2]};
+        const V3<double> n20 = n0 - n2, n21 = n1 - n2;
+        const double t0 = dot(n20, n20), t1 = dot(n21, n21), t2 = dot(n20, n21);
+        const double t3 = dot(n20, x - n2), t4 = dot(n21, x - n2);
+        const double det = t0 * t1 - t2 * t2;
+        double w0 = 0.0, w1 = 0.0;
+        if (std::fabs(det) > 1.0e-20) {
+            const double invDet = 1.0 / det;
+            w0 = (+t1 * t3 - t2 * t4) * invDet;
+            w1 = (-t2 * t3 + t0 * t4) * invDet;
+        }
+        w0 = std::min(std::max(w0, 0.0), 1.0);
+        w1 = std::min(std::max(w1, 0.0), 1.0);
+        bbcc[2 * i] = T(w0);
+        bbcc[2 * i + 1] = T(w1);
+        bm[i] = T(M.vmass[f[0]] * w0 + M.vmass[f[1]] * w1 + M.vmass[f[2]] * (1.0 - w0 - w1));
+    }
+
+    // ParticleSampling called again on the same MeshObject after its vertices moved
+    // (MPSParticleSamplingUtil.cu:14-133, .cuh:69-246). Per face, the count pass compares the
+    // sample count and shortest edge with the ones kept from the previous call; a face where
+    // either changed is generated anew (position, radius, face id, barycentric weights and
+    // mass), every other face copies its previous particles, found by the exclusive scan of
+    // the previous counts, to its new place. Nothing changes when no face changed. The
+    // particle arrays are resized by index first (BoundaryParticleObject::Resize keeps the
+    // leading elements, new ones are zero), and mass, velocity and volume are not moved with
+    // the particles: a copied particle keeps the mass and velocity that were stored at its new
+    // index. Without faithful quirks the copied particles carry their mass and velocity along.
+    // The vertex masses of LoadMesh stay as they were.
+    int resample_mesh(const double* verts, size_t nv, const mps_mesh_material* mat, size_t* regenerated) override {
+        if (!mesh_ok) { err = "no mesh sampled on this context (mps_sample_boundary_mesh first)"; return MPS_ERR_STATE; }
+        if (!verts || nv * 3 != mesh.v.size()) { err = "vertex count differs from the sampled mesh"; return MPS_ERR_INVALID; }
+        const size_t nt = mesh.f.size() / 3;
+        mesh.v.assign(verts, verts + 3 * nv);
+        std::vector<uint32_t> prev_idx(nt + 1, 0u), curr_idx(nt + 1, 0u);
+        std::vector<char> gen(nt, 0);
+        size_t ngen = 0;
+        for (size_t t = 0; t < nt; t++) {
+            uint32_t c = 0, sh = 0;
+            sample_face(mesh, t, mat->radius, nullptr, &c, &sh);
+            prev_idx[t + 1] = prev_idx[t] + mcount[t];
+            curr_idx[t + 1] = curr_idx[t] + c;
+            if (sh != mshort[t] || c != mcount[t]) {
+                gen[t] = 1;
+                ngen++;
+            }
+            mshort[t] = sh;
+            mcount[t] = c;
+        }
+        if (regenerated) *regenerated = ngen;
+        if (ngen == 0) return MPS_OK;
+        if (prev_idx[nt] != B) { err = "boundary particles no longer match the sampled mesh"; return MPS_ERR_STATE; }
+        const std::vector<T> px = bx, pr = br, pbcc = bbcc, pm = bm, pv = bv;
+        const std::vector<uint32_t> pface = bface;
+        const std::vector<float> pcolor = bcolor;
+        const size_t n = curr_idx[nt];
+        B = n;
+        bx.resize(3 * n, T(0)); br.resize(n, T(0)); bvol.resize(n, T(0)); bm.resize(n, T(0));
+        bv.resize(3 * n, T(0)); bbcc.resize(2 * n, T(0)); bface.resize(n, 0u); bcolor.resize(4 * n, 0.f);
+        const bool faithful = cfg.faithful_quirks != 0;
+        for (size_t t = 0; t < nt; t++) {
+            size_t i = curr_idx[t];
+            if (gen[t]) {
+                std::vector<Sample> S;
+                sample_face(mesh, t, mat->radius, &S, nullptr);
+                if (S.size() != size_t(curr_idx[t + 1] - curr_idx[t])) { err = "sample count mismatch between count and generate passes"; return MPS_ERR_STATE; }
+                for (const Sample& q : S) {
+                    bx[3 * i] = T(q.x.x); bx[3 * i + 1] = T(q.x.y); bx[3 * i + 2] = T(q.x.z);
+                    br[i] = T(mat->radius * 0.25);
+                    bface[i] = q.face;
+                    for (int k = 0; k < 4; k++) bcolor[4 * i + k] = 0.f;
+                    set_barycentric(mesh, i, q.x, q.face);
+                    i++;
+                }
+            } else {
+                for (size_t j = prev_idx[t]; i < curr_idx[t + 1]; i++, j++) {
+                    for (int k = 0; k < 3; k++) bx[3 * i + k] = px[3 * j + k];
+                    br[i] = pr[j];
+                    bface[i] = pface[j];
+                    bbcc[2 * i] = pbcc[2 * j];
+                    bbcc[2 * i + 1] = pbcc[2 * j + 1];
+                    for (int k = 0; k < 4; k++) bcolor[4 * i + k] = pcolor[4 * j + k];
+                    if (!faithful) {
+                        bm[i] = pm[j];
+                        for (int k = 0; k < 3; k++) bv[3 * i + k] = pv[3 * j + k];
+                    }
+                }
+            }
+        }
+        bb = CSR{};
+        bb.idx.assign(n + 1, 0u);
+        bh = Hash{};
         return MPS_OK;
     }
 
@@ -1100,6 +1195,7 @@ struct Ctx final : Base {
             case MPS_ARR_BND_MASS: return R(bm, t);
             case MPS_ARR_BND_VEL: return R(bv, 3 * t);
             case MPS_ARR_BND_FACE_ID: return R(bface, 4);
+            case MPS_ARR_BND_BCC: return R(bbcc, 2 * t);
             case MPS_ARR_FHASH_KEY: return R(fh.key, 4);
             case MPS_ARR_FHASH_ID: return R(fh.id, 4);
             case MPS_ARR_FHASH_START: return R(fh.start, 4);
@@ -1196,6 +1292,11 @@ int orc_sample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const u
                              const double* center, const double* size, const mps_mesh_material* m) {
     if (!c || !m) return MPS_ERR_INVALID;
     return c->impl->sample_mesh(verts, nv, tris, nt, center, size, m);
+}
+int orc_resample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m,
+                               size_t* regenerated) {
+    if (!c || !m) return MPS_ERR_INVALID;
+    return c->impl->resample_mesh(verts, nv, m, regenerated);
 }
 int orc_step(orc_ctx* c, mps_step_stats* out) { return c ? c->impl->step(out) : MPS_ERR_INVALID; }
 int orc_run_phase(orc_ctx* c, int32_t phase, mps_step_stats* out) {

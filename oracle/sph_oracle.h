@@ -44,6 +44,10 @@ int orc_upload_boundary(orc_ctx* c, size_t n, const double* pos, const double* r
 int orc_sample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const uint32_t* tris,
                              size_t nt, const double* center, const double* size,
                              const mps_mesh_material* m);
+/* ParticleSampling again on the mesh of the last orc_sample_boundary_mesh with moved vertices
+ * (same faces): faces whose sample count or shortest edge changed are generated anew. */
+int orc_resample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m,
+                               size_t* regenerated);
 
 int orc_step(orc_ctx* c, mps_step_stats* out);
 int orc_run_phase(orc_ctx* c, int32_t phase, mps_step_stats* out);

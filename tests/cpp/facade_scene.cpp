@@ -7,7 +7,9 @@
 //       mesh LoadMesh read, one "name value..." line each
 //   facade_scene run <mesh.obj> <keys> <steps> <out.bin>
 //       Initalize, then OnKeyDown for each key character ('+' = ']', '-' = '[', 'g' = 'G'),
-//       ' ' to start, <steps> OnUpdate calls; writes <out.bin>:
+//       ' ' to start, <steps> OnUpdate calls; with 'w' among the keys the +x wall of the box
+//       moves out by 1 m after half of the steps (MeshObject::SetVertexPositions, then
+//       ParticleSampling again); writes <out.bin>:
 //       uint64 n, uint64 frame, uint64 n_boundary, uint32 l_df, l_cd, l_cg, pad,
 //       float pos[n][3]
 #include <cstdio>
@@ -61,7 +63,25 @@ static int run(const char* mesh, const char* keys, int steps, const char* out) {
         return 1;
     }
     sys.OnKeyDown(32);
-    for (int s = 0; s < steps; s++) sys.OnUpdate();
+    const bool move_wall = std::strchr(keys, 'w') != nullptr;
+    for (int s = 0; s < steps; s++) {
+        if (move_wall && s == steps / 2) {
+            // the mesh is the unit cube fitted into the collision box: x = (v - 0.5) * size + center
+            mps::MeshObject& mesh = sys.GetMeshObject();
+            const mps::REAL3 c = mesh.GetCenter(), sz = mesh.GetFitSize();
+            std::vector<double> w(mesh.GetVertices());
+            for (size_t i = 0; i < w.size() / 3; i++) {
+                const bool xmax = w[3 * i] == 1.0;
+                w[3 * i] = (w[3 * i] - 0.5) * sz.x + c.x + (xmax ? 1.0 : 0.0);
+                w[3 * i + 1] = (w[3 * i + 1] - 0.5) * sz.y + c.y;
+                w[3 * i + 2] = (w[3 * i + 2] - 0.5) * sz.z + c.z;
+            }
+            mesh.SetVertexPositions(w);
+            mps::kernel::ParticleSampling::ParticleSampling(sys.GetMeshMaterial().GetParam(), mesh,
+                                                            sys.GetBoundaryParticleObject());
+        }
+        sys.OnUpdate();
+    }
     const std::vector<float> pos = sys.GetPositions();
     const uint64_t hdr[3] = {pos.size() / 3, sys.GetFrame(), sys.GetBoundaryParticleObject().GetSize()};
     const mps_step_stats& st = sys.GetLastStats();

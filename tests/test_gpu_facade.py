@@ -29,15 +29,21 @@ def _facade(keys, steps, out):
     return int(frame), int(nb), loops[:3].tolist(), pos
 
 
-@pytest.mark.parametrize("keys,size,gravity", [("", 5, -9.8), ("+g", 10, 0.0)])
+@pytest.mark.parametrize("keys,size,gravity", [("", 5, -9.8), ("+g", 10, 0.0), ("w", 5, -9.8)])
 def test_facade_run_equals_python_path(tmp_path, keys, size, gravity):
+    """'w': the +x wall moves out by 1 m after half of the steps and is re-sampled."""
     steps = 12
     frame, nb, loops, pos = _facade(keys, steps, tmp_path / "out.bin")
     sc = scenes.surface_tension_test_scene(size)
     sc.physics["gravity"] = (0.0, gravity, 0.0)
     g = HipSim(scenes.make_config(sc))
     scenes.load_into(g, sc)
-    for _ in range(steps):
+    b = sc.boundary_mesh
+    walls = (scenes.CUBE_VERTS - 0.5) * np.asarray(b["size"]) + np.asarray(b["center"])
+    walls[scenes.CUBE_VERTS[:, 0] == 1.0, 0] += 1.0
+    for s in range(steps):
+        if "w" in keys and s == steps // 2:
+            assert g.resample_boundary_mesh(walls, b["radius"]) > 0
         st = g.step()
     ref = g.get("FLUID_POS").astype(np.float32)
     print(f"[facade] size {size}: {pos.shape[0]} particles, {nb} walls, loops {loops}", flush=True)
