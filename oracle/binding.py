@@ -30,6 +30,11 @@ _SIGS = {
     "orc_upload_fluid": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "orc_upload_boundary": (C.c_int, [_P, C.c_size_t, _P, _P, _P]),
     "orc_sample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(capi.MeshMaterial)]),
+    "orc_world_create": (C.c_int, [C.POINTER(capi.Config), C.c_int, C.c_uint32, C.POINTER(_P)]),
+    "orc_world_destroy": (None, [_P]),
+    "orc_world_phase": (_P, [_P, C.c_uint32]),
+    "orc_world_step": (C.c_int, [_P, _P]),
+    "orc_world_run_phase": (C.c_int, [_P, C.c_int32, _P]),
     "orc_resample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(capi.MeshMaterial), C.POINTER(C.c_size_t)]),
     "orc_step": (C.c_int, [_P, C.POINTER(capi.StepStats)]),
     "orc_run_phase": (C.c_int, [_P, C.c_int32, C.POINTER(capi.StepStats)]),
@@ -191,6 +196,66 @@ class OracleSim:
     @frame.setter
     def frame(self, v: int):
         self._chk(self.lib.orc_set_frame(self.h, int(v)))
+
+
+class _Phase(OracleSim):
+    """One fluid object of an OracleWorld (the world owns the context)."""
+
+    def __init__(self, world, handle, cfg, precision):  # noqa: super().__init__ not called on purpose
+        self.lib = world.lib
+        self.cfg = cfg
+        self.precision = precision
+        self.real = np.float32 if precision == 32 else np.float64
+        self.h = handle
+        self._world = world
+
+    def close(self):
+        self.h = None
+
+
+class OracleWorld:
+    """Coupled fluid objects in the oracle: phase k is loaded like an OracleSim
+    (world.phase(k).set_physics(...), ...); step() advances all of them together."""
+
+    def __init__(self, cfg: capi.Config, nphase: int, precision: int = 32):
+        self.lib = load()
+        self.n = nphase
+        h = C.c_void_p()
+        rc = self.lib.orc_world_create(C.byref(cfg), precision, nphase, C.byref(h))
+        if rc:
+            raise capi.MPSError(rc, "orc_world_create failed")
+        self.h = h
+        self.phases = [_Phase(self, self.lib.orc_world_phase(h, k), cfg, precision) for k in range(nphase)]
+
+    def phase(self, k: int) -> OracleSim:
+        return self.phases[k]
+
+    def close(self):
+        if getattr(self, "h", None):
+            for p in self.phases:
+                p.close()
+            self.lib.orc_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stats(self, fn, *args):
+        st = (capi.StepStats * self.n)()
+        rc = fn(self.h, *args, C.cast(st, C.c_void_p))
+        if rc:
+            raise capi.MPSError(rc, "oracle world call failed")
+        return list(st)
+
+    def step(self) -> list:
+        return self._stats(self.lib.orc_world_step)
+
+    def run_phase(self, phase) -> list:
+        ph = capi.PHASE[phase] if isinstance(phase, str) else int(phase)
+        return self._stats(self.lib.orc_world_run_phase, ph)
 
 
 def kernel_eval(precision: int, kernel: int, dist, invh: float):

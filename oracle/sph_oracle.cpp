@@ -388,6 +388,21 @@ struct Ctx final : Base {
     std::vector<T> bx, br, bvol, bm, bv, bbcc;
     std::vector<uint32_t> bface;
     std::vector<float> bcolor;
+    // Other fluid objects of a coupled world, in ascending phase order, with this object's
+    // ref neighbour list into each (SpatialHash::BuildNeighorhood(obj, ref, refHash),
+    // MPSSpatialHash.cu:184-218): the "ref" overloads of the density, factor, delta and apply
+    // kernels add one sum per ref object, after this object's own sum and before the walls'.
+    struct RefPhase {
+        const Ctx<T>* o;
+        CSR list;
+    };
+    std::vector<RefPhase> refs;
+    void build_ref_lists() {
+        for (RefPhase& rf : refs) {
+            build_csr(rf.list, x, r, N, rf.o->x, rf.o->r, rf.o->fh, H(), cell(), false);
+            if (rf.o->N == 0) rf.list.idx.assign(N + 1, 0u), rf.list.nei.clear();
+        }
+    }
     // the mesh of the last sample_mesh and its per-face sampling state
     // (MeshObject::m_samplingParticleSize / m_shortestEdgeID, MPSMeshObject.cu:31-45)
     Mesh mesh;
@@ -686,6 +701,11 @@ struct Ctx final : Base {
             nsearch(fb, i, [&](uint32_t j) { sb += bvol[j] * Wk(length(xi - ld3(bx, j)), invHi, cubic); });
             T d = T(0);
             d += sf;
+            for (const RefPhase& rf : refs) {  // ComputeDensity_kernel ref (MPSSPHBaseUtil.cuh:122-156)
+                T sr = T(0);
+                nsearch(rf.list, i, [&](uint32_t j) { sr += rf.o->mV * Wk(length(xi - ld3(rf.o->x, j)), invHi, cubic); });
+                d += sr;
+            }
             d += sb;
             rho[i] = d * mRho0;
             const float ratio = static_cast<float>(rho[i] / mRho0);
@@ -720,9 +740,21 @@ struct Ctx final : Base {
             });
             V3<T> g{0, 0, 0};
             g = g + gf;
-            g = g + gb;
             T a = T(0);
             a += s;
+            for (const RefPhase& rf : refs) {  // ComputeDFSPHFactor_kernel ref (MPSSPHDFSPHUtil.cuh:49-88)
+                V3<T> gr{0, 0, 0};
+                T sr = T(0);
+                nsearch(rf.list, i, [&](uint32_t j) {
+                    const V3<T> xij = xi - ld3(rf.o->x, j);
+                    const V3<T> grad = rf.o->mV * Gk(length(xij), invHi, cubic) * xij;
+                    gr = gr + grad;
+                    sr += dot(grad, grad);
+                });
+                g = g + gr;
+                a += sr;
+            }
+            g = g + gb;
             a += dot(g, g);
             alpha[i] = a > KC<T>::EPS ? T(1) / a : T(0);
         }
@@ -758,6 +790,15 @@ struct Ctx final : Base {
             });
             T d = T(0);
             d += df;
+            for (const RefPhase& rf : refs) {  // ComputeDensityDelta_kernel ref (MPSSPHDFSPHUtil.cuh:177-216)
+                T dr = T(0);
+                nsearch(rf.list, i, [&](uint32_t j) {
+                    const V3<T> xij = xi - ld3(rf.o->x, j);
+                    const V3<T> vij = vi - ld3(rf.o->v, j);
+                    dr += rf.o->mV * Gk(length(xij), invHi, cubic) * dot(xij, vij);
+                });
+                d += dr;
+            }
             d += db;
             delta[i] = d;
         }
@@ -805,6 +846,15 @@ struct Ctx final : Base {
             });
             V3<T> F{0, 0, 0};
             F = F + ffo;
+            for (const RefPhase& rf : refs) {  // ApplyDFSPH_kernel ref (MPSSPHDFSPHUtil.cuh:379-418)
+                V3<T> fro{0, 0, 0};
+                nsearch(rf.list, i, [&](uint32_t j) {
+                    const V3<T> xij = xi - ld3(rf.o->x, j);
+                    const V3<T> fij = T(0.5) * rf.o->mV * (pi + rf.o->kappa[j]) * Gk(length(xij), invHi, cubic) * xij;
+                    fro = fro - fij;
+                });
+                F = F + fro;
+            }
             F = F + fbo;
             F = m[i] * F;
             st3(force, i, F);
@@ -1242,6 +1292,112 @@ struct orc_ctx {
     std::unique_ptr<Base> impl;
 };
 
+// A coupled world of fluid objects ("phases"), each a complete context with its own material,
+// hash, lists and walls. Phase k's ref lists point into every other phase in ascending order
+// (the reference's ref overloads, which its own step never calls). The step runs OnUpdate's
+// order over all phases (MPSSystem.cpp:191-291); the pressure loops run jointly: one
+// evaluation of every phase per iteration, each phase's error E_k = sum of its positive
+// errors / its particle count as in ComputeDFSPHDivergenceFree / ConstantDensity
+// (MPSSPHDFSPHUtil.cu:270-441), the loop ending at the first iteration where every phase has
+// E_k < tol and l > l_min; then every phase applies. Viscosity and surface tension stay within
+// each phase (the reference's CG takes the object's own list only).
+template <class T>
+struct World {
+    std::vector<Ctx<T>*> P;
+    void wire() {
+        for (size_t k = 0; k < P.size(); k++) {
+            if (P[k]->refs.size() == P.size() - 1) continue;
+            P[k]->refs.clear();
+            for (size_t t = 0; t < P.size(); t++)
+                if (t != k) P[k]->refs.push_back(typename Ctx<T>::RefPhase{P[t], CSR{}});
+        }
+    }
+    void pressure_joint(bool df, mps_step_stats* st) {
+        const mps_config& cfg = P[0]->cfg;
+        const double tol = df ? cfg.df_tol : cfg.cd_tol;
+        const uint32_t lmin = df ? cfg.df_min_iter : cfg.cd_min_iter;
+        uint32_t l = 1u;
+        bool capped = true;
+        while (l <= cfg.max_iter) {
+            for (Ctx<T>* p : P) p->density_delta();
+            bool ok = true;
+            for (size_t k = 0; k < P.size(); k++) {
+                const double E = P[k]->N ? P[k]->stiffness(df) / double(P[k]->N) : 0.0;
+                double* log = df ? st[k].err_df : st[k].err_cd;
+                if (l - 1 < MPS_MAX_ITER_LOG) log[l - 1] = E;
+                ok = ok && E < tol && l > lmin;
+            }
+            if (ok) {
+                capped = false;
+                break;
+            }
+            for (Ctx<T>* p : P) p->apply_pressure();
+            l++;
+        }
+        for (size_t k = 0; k < P.size(); k++) {
+            if (df) { st[k].l_df = l; st[k].hit_cap_df = capped; }
+            else { st[k].l_cd = l; st[k].hit_cap_cd = capped; }
+        }
+    }
+    int run_phase(int32_t ph, mps_step_stats* st) {
+        wire();
+        switch (ph) {
+            case MPS_PHASE_NEIGHBORS:
+                for (Ctx<T>* p : P) p->neighbors();
+                for (Ctx<T>* p : P) p->build_ref_lists();
+                return MPS_OK;
+            case MPS_PHASE_DIVERGENCE: pressure_joint(true, st); return MPS_OK;
+            case MPS_PHASE_CONSTANT_DENSITY: pressure_joint(false, st); return MPS_OK;
+            default:
+                for (size_t k = 0; k < P.size(); k++) {
+                    const int rc = P[k]->run_phase(ph, &st[k]);
+                    if (rc) return rc;
+                }
+                return MPS_OK;
+        }
+    }
+    int step(mps_step_stats* st) {
+        const uint32_t frame = P[0]->frame;
+        const mps_config& cfg = P[0]->cfg;
+        for (size_t k = 0; k < P.size(); k++) {
+            std::memset(&st[k], 0, sizeof(st[k]));
+            st[k].frame = frame;
+            st[k].n_fluid = uint32_t(P[k]->N);
+            st[k].n_boundary = uint32_t(P[k]->B);
+        }
+        if (cfg.zsort_interval && (frame % cfg.zsort_interval) == 1u)
+            for (size_t k = 0; k < P.size(); k++) {
+                P[k]->zsort();
+                st[k].zsorted = 1;
+            }
+        for (int32_t ph = MPS_PHASE_HASH; ph <= MPS_PHASE_ADVECT; ph++) {
+            const int rc = run_phase(ph, st);
+            if (rc) return rc;
+        }
+        for (size_t k = 0; k < P.size(); k++) {
+            Ctx<T>* p = P[k];
+            st[k].nnz_ff = p->ff.nei.size();
+            st[k].nnz_fb = p->fb.nei.size();
+            st[k].nnz_bb = p->bb.nei.size();
+            st[k].nonfinite = p->finite_state() ? 0u : 1u;
+            p->frame++;
+        }
+        return MPS_OK;
+    }
+};
+
+struct orc_world {
+    int precision = 32;
+    std::vector<orc_ctx*> ph;
+    std::string err;
+    template <class T>
+    World<T> world() {
+        World<T> w;
+        for (orc_ctx* c : ph) w.P.push_back(static_cast<Ctx<T>*>(c->impl.get()));
+        return w;
+    }
+};
+
 extern "C" {
 
 int orc_create(const mps_config* cfg, int precision, orc_ctx** out) {
@@ -1264,6 +1420,48 @@ int orc_create(const mps_config* cfg, int precision, orc_ctx** out) {
     return MPS_OK;
 }
 void orc_destroy(orc_ctx* c) { delete c; }
+
+int orc_world_create(const mps_config* cfg, int precision, uint32_t nphase, orc_world** out) {
+    if (!cfg || !out || nphase == 0) return MPS_ERR_INVALID;
+    auto* w = new orc_world;
+    w->precision = precision;
+    for (uint32_t k = 0; k < nphase; k++) {
+        orc_ctx* c = nullptr;
+        const int rc = orc_create(cfg, precision, &c);
+        if (rc) {
+            for (orc_ctx* q : w->ph) delete q;
+            delete w;
+            return rc;
+        }
+        w->ph.push_back(c);
+    }
+    *out = w;
+    return MPS_OK;
+}
+void orc_world_destroy(orc_world* w) {
+    if (!w) return;
+    for (orc_ctx* c : w->ph) delete c;
+    delete w;
+}
+orc_ctx* orc_world_phase(orc_world* w, uint32_t k) { return w && k < w->ph.size() ? w->ph[k] : nullptr; }
+int orc_world_step(orc_world* w, mps_step_stats* st) {
+    if (!w || !st) return MPS_ERR_INVALID;
+    if (w->precision == 64) {
+        World<double> x = w->world<double>();
+        return x.step(st);
+    }
+    World<float> x = w->world<float>();
+    return x.step(st);
+}
+int orc_world_run_phase(orc_world* w, int32_t phase, mps_step_stats* st) {
+    if (!w || !st) return MPS_ERR_INVALID;
+    if (w->precision == 64) {
+        World<double> x = w->world<double>();
+        return x.run_phase(phase, st);
+    }
+    World<float> x = w->world<float>();
+    return x.run_phase(phase, st);
+}
 const char* orc_last_error(const orc_ctx* c) { return c ? c->impl->err.c_str() : "null context"; }
 int orc_precision(const orc_ctx* c) { return c ? c->impl->precision() : 0; }
 int orc_set_threads(int n) {

@@ -49,6 +49,16 @@ int orc_sample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const u
 int orc_resample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m,
                                size_t* regenerated);
 
+/* A coupled world of fluid objects: nphase contexts (load each with the calls above through
+ * orc_world_phase) whose density, factor, delta and pressure force include the other phases
+ * (the reference's "ref" overloads); the pressure loops run jointly over all phases. */
+typedef struct orc_world orc_world;
+int orc_world_create(const mps_config* cfg, int precision, uint32_t nphase, orc_world** out);
+void orc_world_destroy(orc_world* w);
+orc_ctx* orc_world_phase(orc_world* w, uint32_t k);
+int orc_world_step(orc_world* w, mps_step_stats* st /* [nphase] */);
+int orc_world_run_phase(orc_world* w, int32_t phase, mps_step_stats* st /* [nphase] */);
+
 int orc_step(orc_ctx* c, mps_step_stats* out);
 int orc_run_phase(orc_ctx* c, int32_t phase, mps_step_stats* out);
 
