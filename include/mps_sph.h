@@ -172,7 +172,15 @@ typedef struct mps_config {
     uint32_t halo_limit;         /* HIP library: largest block halo of the viscosity operator staged in
                                     LDS; blocks with more gather from global memory. 0 = the LDS
                                     capacity (1264); smaller values exist to test the global path */
-    uint32_t reserved[4];
+    uint32_t couple_scenes;      /* 1: the scenes of a batch are fluid objects of one world (same dt,
+                                    gravity, box; walls given per scene): density, factor, density
+                                    change and pressure force include the other scenes' particles
+                                    (the reference's "ref" overloads, MPSSPHBaseUtil.cu:73-92,
+                                    MPSSPHDFSPHUtil.cu:31-51,105-127,212-234) and the pressure loops
+                                    run jointly, ending when every scene meets its tolerance;
+                                    viscosity and surface tension stay within each scene. 0: the
+                                    scenes are independent. */
+    uint32_t reserved[3];
     double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
 } mps_config;
 

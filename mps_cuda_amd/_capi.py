@@ -51,7 +51,7 @@ class Config(C.Structure):
         ("cg_max_iter", C.c_uint32), ("cg_factor", C.c_double), ("jacobi_max_iter", C.c_uint32),
         ("jacobi_tol", C.c_double), ("zsort_interval", C.c_uint32), ("nei_capacity", C.c_uint32),
         ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
-        ("halo_limit", C.c_uint32), ("reserved", C.c_uint32 * 4),
+        ("halo_limit", C.c_uint32), ("couple_scenes", C.c_uint32), ("reserved", C.c_uint32 * 3),
         ("adj_cg_tol", C.c_double),
     ]
 

@@ -200,6 +200,10 @@ struct mps_ctx {
     // neighbour lists (column-major ELL)
     DBuf<uint32_t> nff, cff, nfb, cfb, nbb, cbb;
     uint32_t Kff = 0, Kfb = 0, Kbb = 0;
+    // coupled fluid objects: ref lists into the other scenes (column-major, Kfr slots)
+    DBuf<uint32_t> nfr, cfr;
+    uint32_t Kfr = 0;
+    bool coupled() const { return cfg.couple_scenes != 0 && S > 1; }
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
     DBuf<float2> pairAB;  // (a_ij, b_ij) of the implicit viscosity operator
@@ -359,7 +363,7 @@ struct mps_ctx {
         return b;
     }
     std::vector<uint64_t> key_base(uint64_t kind, uint32_t chunk, const mps::Scenes& P, const mps::Nbr& nb) const {
-        return {kind, chunk, u(P.P), u(P.blk_scene), P.N, P.S, u(nb.ff), u(nb.cff), u(nb.fb), u(nb.cfb)};
+        return {kind, chunk, u(P.P), u(P.blk_scene), P.N, P.S, u(nb.ff), u(nb.cff), u(nb.fb), u(nb.cfb), u(nb.fr), u(nb.cfr)};
     }
 
     // ------------------------------------------------------------------ helpers
@@ -453,7 +457,14 @@ struct mps_ctx {
     }
     int cell_bits() const { return bits_for(cells()); }
     const uint32_t* bnd_scene() const { return S > 1 ? bscene.p : nullptr; }
-    mps::Nbr nbr() const { return mps::Nbr{nff.p, cff.p, nfb.p, cfb.p}; }
+    mps::Nbr nbr() const {
+        mps::Nbr nb{nff.p, cff.p, nfb.p, cfb.p};
+        if (coupled()) {
+            nb.fr = nfr.p;
+            nb.cfr = cfr.p;
+        }
+        return nb;
+    }
     int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
     void sync() { HIPCHK(hipStreamSynchronize(stream)); }
 
@@ -654,6 +665,7 @@ struct mps_ctx {
         if (Kff == 0) Kff = (std::max<uint32_t>(cfg.nei_capacity, 8) + 3u) & ~3u;
         if (Kfb == 0) Kfb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
         if (Kbb == 0) Kbb = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
+        if (Kfr == 0) Kfr = (std::max<uint32_t>(cfg.nei_capacity / 2, 8) + 3u) & ~3u;
         for (int attempt = 0; attempt < 8; attempt++) {
             reset_counters();
             nstat.reserve(2 * ((size_t(std::max(N, B)) + mps::dev::kBlock - 1) / mps::dev::kBlock) + 2);
@@ -674,9 +686,18 @@ struct mps_ctx {
                 mps::launch::neighbors_fluid(pos.p, scn(), uint32_t(cells()), g, fluid_view(), view(bh, B), nff.p, Kff, cff.p,
                                              nfb.p, Kfb, cfb.p, nstat.p, d_ctr, stream);
             });
+            if (coupled()) {
+                nfr.reserve(size_t(Kfr) * std::max<uint32_t>(N, 1));
+                cfr.reserve(std::max<uint32_t>(N, 1));
+                T_(K_NEI_FLUID, [&] {
+                    mps::launch::neighbors_ref(pos.p, scn(), uint32_t(cells()), g, fluid_view(), nfr.p, Kfr, cfr.p, nstat.p, d_ctr,
+                                               stream);
+                });
+            }
             HIPCHK(hipMemcpyAsync(h_ctr, d_ctr, sizeof(NbrCounters), hipMemcpyDeviceToHost, stream));
             sync();
             bool grow = false;
+            if (coupled() && h_ctr->max_fr > Kfr) { Kfr = (h_ctr->max_fr + 15u) & ~15u; grow = true; }
             if (h_ctr->max_ff > Kff) { Kff = (h_ctr->max_ff + 15u) & ~15u; grow = true; }
             if (h_ctr->max_fb > Kfb) { Kfb = (h_ctr->max_fb + 15u) & ~15u; grow = true; }
             if (do_b && h_ctr->max_bb > Kbb) { Kbb = (h_ctr->max_bb + 15u) & ~15u; grow = true; }
@@ -783,12 +804,14 @@ struct mps_ctx {
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
                     T_(df ? K_EVAL_DF : K_EVAL_CD, [&] {
-                        mps::launch::pressure_eval(df, P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
+                        mps::launch::pressure_eval(cubic(), df, P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, bvel.p, rho.p, alpha.p,
                                                    pairG_ff.p, pairG_fb.p, kappa.p, delta.p, sd, partials.p, stream);
                     }, false);
-                    T_(K_FOLD, [&] { mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, tol, lmin, stream); }, false);
+                    T_(K_FOLD, [&] {
+                        mps::launch::fold_pressure(P, partials.p, sd, d_active + idx, tol, lmin, coupled(), stream);
+                    }, false);
                     T_(K_APPLY, [&] {
-                        mps::launch::pressure_apply(P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p,
+                        mps::launch::pressure_apply(cubic(), P, nbr(), hl, pos.p, vel.p, bpos.p, bvol.p, pairG_ff.p, pairG_fb.p, kappa.p,
                                                     sd, stream);
                     }, false);
                     rec_snap();
@@ -1024,6 +1047,7 @@ struct mps_ctx {
     }
     // Forward n steps from the current state, keeping a checkpoint every `every` steps.
     void adj_rollout(uint32_t n, uint32_t every, uint32_t* counts) {
+        if (coupled()) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover coupled scenes"};
         if (cfg.kernel != MPS_KERNEL_CUBIC) throw ApiError{MPS_ERR_STATE, "the adjoint needs the cubic kernel"};
         if (cfg.visc_solver != MPS_VISC_CG) throw ApiError{MPS_ERR_STATE, "the adjoint needs the CG viscosity solve"};
         if (N == 0 || n == 0) throw ApiError{MPS_ERR_INVALID, "nothing to roll out"};
@@ -1482,6 +1506,10 @@ struct mps_ctx {
             lay.push_back(h);
         }
         const uint32_t nsc = uint32_t(lay.size());
+        if (cfg.couple_scenes)  // fluid objects of one world: one time step, gravity and box
+            for (const SceneHost& h : lay)
+                if (std::memcmp(&h.phys, &lay[0].phys, sizeof(mps_physics)) != 0)
+                    throw ApiError{MPS_ERR_INVALID, "coupled scenes must share dt, gravity and the box"};
         if (morton_bits() + bits_for(nsc) > 31 || cell_bits() + bits_for(nsc) > 31 ||
             uint64_t(nsc) * cells() > 0xFFFFFFFFull)
             throw ApiError{MPS_ERR_INVALID, "too many scenes for the hash size"};
@@ -2030,7 +2058,8 @@ struct mps_ctx {
         bpos.release(); bvel.release(); bcolor.release(); bvol.release(); bS.release(); bface.release(); bbcc.release();
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
-        nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); partials.release();
+        nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); nfr.release(); cfr.release();
+        partials.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();

@@ -627,6 +627,32 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_boundary(const float4* __r
     block_counts<1>(cnt, bstat, gridDim.x, blockIdx.x);
 }
 
+// Ref lists of coupled fluid objects (SpatialHash::BuildNeighorhood(obj, ref, refHash),
+// MPSSpatialHash.cu:184-218, .cuh:180-271): for each fluid slot, the particles of every other
+// scene within (h_i + h_j)/2, scene after scene in ascending order, each scene's part in the
+// 27-cell order of that scene's own table (the walk of the wall lists, on another table).
+__global__ void __launch_bounds__(kBlock) k_neighbors_ref(const float4* __restrict__ pos, Scenes sc, uint32_t C, HashGeom g,
+                                                        HashView fl, uint32_t* __restrict__ nfr, uint32_t Kfr,
+                                                        uint32_t* __restrict__ cfr, uint2* __restrict__ bstat) {
+    const uint32_t blk = blockIdx.x;
+    const uint32_t i = blk * kBlock + threadIdx.x;
+    const uint32_t N = sc.N;
+    const uint32_t sid = sc.blk_scene[blk];
+    uint32_t k = 0u;
+    if (i < sc.P[sid].end) {
+        const float4 xi = pos[i];
+        for (uint32_t t = 0; t < sc.S; t++) {
+            if (t == sid) continue;
+            const uint32_t at = k < Kfr ? k : Kfr;  // entries past the capacity are counted, not written
+            k += search_cells<false, true>(xi, i, g, fl.start + size_t(t) * C, fl.end + size_t(t) * C, nullptr, fl.spos,
+                                           nfr + size_t(at) * N, Kfr - at, N);
+        }
+    }
+    if (i < N) cfr[i] = k;
+    const uint32_t cnt[1] = {k};
+    block_counts<1>(cnt, bstat, gridDim.x, blk);
+}
+
 // Z-sort payload permutation (MPSSpatialHash.cu:101-140): out[s] = in[perm[s]].
 __global__ void __launch_bounds__(kBlock) k_gather_f4(const uint32_t* __restrict__ perm, uint32_t n, const float4* __restrict__ in,
                                                     float4* __restrict__ out) {
@@ -734,6 +760,13 @@ void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const Hash
     launch_k(k_neighbors_fluid, g_, kBlock, 0, s, pos, sc, C, g, fl, bd, nff, Kff, cff, nfb, Kfb, cfb, bstat);
     launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat, g_, &ctr->max_ff, &ctr->nnz_ff);
     launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat + g_, g_, &ctr->max_fb, &ctr->nnz_fb);
+}
+void neighbors_ref(const float4* pos, const Scenes& sc, uint32_t C, const HashGeom& g, const HashView& fl, uint32_t* nfr,
+                   uint32_t Kfr, uint32_t* cfr, uint2* bstat, NbrCounters* ctr, hipStream_t s) {
+    if (!sc.N) return;
+    const uint32_t g_ = nb(sc.N);
+    launch_k(k_neighbors_ref, g_, kBlock, 0, s, pos, sc, C, g, fl, nfr, Kfr, cfr, bstat);
+    launch_k(k_count_fold, 1, kCountFoldThreads, 0, s, bstat, g_, &ctr->max_fr, &ctr->nnz_fr);
 }
 void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const HashGeom& g, const HashView& bd,
                         uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, uint2* bstat, NbrCounters* ctr, hipStream_t s) {

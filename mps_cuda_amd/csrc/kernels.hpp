@@ -30,8 +30,8 @@ struct HashView {
 
 // Neighbour statistics written by the search kernels.
 struct NbrCounters {
-    uint32_t max_ff, max_fb, max_bb, pad;
-    unsigned long long nnz_ff, nnz_fb, nnz_bb;
+    uint32_t max_ff, max_fb, max_bb, max_fr;
+    unsigned long long nnz_ff, nnz_fb, nnz_bb, nnz_fr;
 };
 
 // Device-resident state of one iterative solve. The fold kernels finalise the sums here,
@@ -72,12 +72,15 @@ struct Scenes {
     uint32_t S;                 // scenes
 };
 
-// Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i].
+// Column-major ELL neighbour lists: nbr[k*N + i], k < cnt[i]. fr/cfr: the ref lists into the
+// other scenes when the scenes are coupled fluid objects (null otherwise).
 struct Nbr {
     const uint32_t* ff;
     const uint32_t* cff;
     const uint32_t* fb;
     const uint32_t* cfb;
+    const uint32_t* fr = nullptr;
+    const uint32_t* cfr = nullptr;
 };
 
 // Block halos of the fluid-fluid lists (halo.hip): for workgroup b of slots, idx[b*kHaloCap + t]
@@ -152,6 +155,9 @@ void cell_table(const uint32_t* key, const uint32_t* id, uint32_t n, uint32_t* s
 void neighbors_fluid(const float4* pos, const Scenes& sc, uint32_t C, const dev::HashGeom& g, const HashView& fl,
                      const HashView& bd, uint32_t* nff, uint32_t Kff, uint32_t* cff, uint32_t* nfb, uint32_t Kfb, uint32_t* cfb,
                      uint2* bstat, NbrCounters* ctr, hipStream_t s);
+// ref lists of coupled scenes (other scenes' fluid, ascending scene order)
+void neighbors_ref(const float4* pos, const Scenes& sc, uint32_t C, const dev::HashGeom& g, const HashView& fl, uint32_t* nfr,
+                   uint32_t Kfr, uint32_t* cfr, uint2* bstat, NbrCounters* ctr, hipStream_t s);
 void neighbors_boundary(const float4* pos, uint32_t B, const uint32_t* scene, uint32_t C, const dev::HashGeom& g,
                         const HashView& bd, uint32_t* nbb, uint32_t Kbb, uint32_t* cbb, uint2* bstat, NbrCounters* ctr,
                         hipStream_t s);
@@ -169,13 +175,15 @@ void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipSt
 void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s);
-void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
-                   const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
-                   const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s);
-// one workgroup per scene; `active` counts the scenes whose loop has not finished
-void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
+// nb.fr set: the coupled-scene variants (ref sums, G_ij evaluated for them with `cubic`)
+void pressure_eval(int cubic, int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel,
+                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
+                   const float* pG_ff, const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials,
                    hipStream_t s);
-void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
+// joint: one decision for all scenes (coupled fluid objects)
+void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
+                   int joint, hipStream_t s);
+void pressure_apply(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s);
 void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
 // ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
-template <bool CUBIC>
+template <bool CUBIC, bool REF>
 __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
@@ -203,9 +203,52 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
                 gb = gb + (vb * G) * xij;
             }
         }
+        // ((own + each ref object's sum, in scene order) + walls), as the reference's Sub
+        // kernels accumulate them one after the other
+        float d = 0.0f;
+        d += sf;
+        f3 g{0.f, 0.f, 0.f};
+        g = g + gf;
+        float a = 0.0f;
+        a += s;
+        if constexpr (REF) {  // ComputeDensity / ComputeDFSPHFactor ref (MPSSPHBaseUtil.cuh:122-156, MPSSPHDFSPHUtil.cuh:49-88)
+            if (dens || alph) {
+                const uint32_t cr = nb.cfr[i];
+                uint32_t cur = kEmpty;
+                float Vt = 0.0f, sr = 0.0f, ssr = 0.0f;
+                f3 gr{0.f, 0.f, 0.f};
+                for (uint32_t k = 0; k < cr; k++) {
+                    const uint32_t j = nb.fr[size_t(k) * N + i];
+                    const uint32_t t = sc.blk_scene[j / kBlock];
+                    if (t != cur) {
+                        if (cur != kEmpty) {
+                            d += sr;
+                            g = g + gr;
+                            a += ssr;
+                        }
+                        cur = t;
+                        Vt = sc.P[t].V;
+                        sr = 0.0f;
+                        ssr = 0.0f;
+                        gr = f3{0.f, 0.f, 0.f};
+                    }
+                    const f3 xij = xi - mk(pos[j]);
+                    const float dist = len3(xij);
+                    if (dens) sr += Vt * Wk<CUBIC>(dist, invHi);
+                    if (alph) {
+                        const f3 grad = (Vt * Gk<CUBIC>(dist, invHi)) * xij;
+                        gr = gr + grad;
+                        ssr += dot3(grad, grad);
+                    }
+                }
+                if (cur != kEmpty) {
+                    d += sr;
+                    g = g + gr;
+                    a += ssr;
+                }
+            }
+        }
         if (dens) {
-            float d = 0.0f;
-            d += sf;
             d += sb;
             const float r = d * P.rho0;
             rho[i] = r;
@@ -216,11 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
             color[i] = make_float4(red, green, blue, 1.0f);
         }
         if (alph) {
-            f3 g{0.f, 0.f, 0.f};
-            g = g + gf;
             g = g + gb;
-            float a = 0.0f;
-            a += s;
             a += dot3(g, g);
             alpha[i] = a > kEps ? 1.0f / a : 0.0f;
         }
@@ -245,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
 // alpha pass. Each workgroup publishes its sum of positive errors for k_fold_pressure.
 // The records of the block's halo (x_j, v_j) are staged in LDS (see k_cg_apply); the pairs
 // are summed in list order on both paths.
-template <bool DF>
+template <bool DF, bool REF, bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, const float4* __restrict__ bvel,
@@ -325,6 +364,26 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
         }
         float d = 0.0f;
         d += df;
+        if constexpr (REF) {  // ComputeDensityDelta ref (MPSSPHDFSPHUtil.cuh:177-216): one sum per ref object
+            const float invHi = 1.0f / (pos[i].w * 4.0f);
+            const uint32_t cr = nb.cfr[i];
+            uint32_t cur = kEmpty;
+            float Vt = 0.0f, dr = 0.0f;
+            for (uint32_t k = 0; k < cr; k++) {
+                const uint32_t j = nb.fr[size_t(k) * N + i];
+                const uint32_t t = sc.blk_scene[j / kBlock];
+                if (t != cur) {
+                    if (cur != kEmpty) d += dr;
+                    cur = t;
+                    Vt = sc.P[t].V;
+                    dr = 0.0f;
+                }
+                const f3 xij = xi - mk(pos[j]);
+                const f3 vij = vi - mk(vel[j]);
+                dr += Vt * Gk<CUBIC>(len3(xij), invHi) * dot3(xij, vij);
+            }
+            if (cur != kEmpty) d += dr;
+        }
         d += db;
         delta[i] = d;
         float e;
@@ -372,8 +431,39 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_pressure(Scenes sc, const
     }
 }
 
+// Coupled fluid objects run one pressure loop: a single workgroup folds every scene's errors
+// and the loop ends at the first iteration where every scene has E < tol and l > lmin (all
+// scenes then finish together, with the same l).
+__global__ void __launch_bounds__(kFoldThreads) k_fold_pressure_joint(Scenes sc, const double* __restrict__ partials,
+                                                              SolverDev* st, uint32_t* active, double tol, uint32_t lmin) {
+    if (st[0].done) return;
+    const uint32_t it = st[0].l;
+    __shared__ uint32_t all_ok;
+    if (threadIdx.x == 0) all_ok = 1u;
+    __syncthreads();
+    for (uint32_t sid = 0; sid < sc.S; sid++) {
+        const StepParams P = sc.P[sid];
+        const double s = fold(partials + P.blk0, P.nblk);
+        if (threadIdx.x == 0) {
+            const double E = P.count ? s / double(P.count) : 0.0;
+            if (it - 1 < MPS_MAX_ITER_LOG) st[sid].log[it - 1] = E;
+            if (!(E < tol && it > lmin)) all_ok = 0u;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool ok = all_ok != 0u;
+        for (uint32_t sid = 0; sid < sc.S; sid++) {
+            if (ok) st[sid].done = 1u;
+            st[sid].l = ok ? it : it + 1;
+        }
+        if (ok) *active = 0u;
+    }
+}
+
 // ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
 // The halo's (x_j, kappa_j) are staged in LDS (see k_cg_apply).
+template <bool REF, bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                          float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                          const float* __restrict__ bvol, const float* __restrict__ pG_ff,
@@ -448,6 +538,26 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     }
     f3 F{0.f, 0.f, 0.f};
     F = F + ff;
+    if constexpr (REF) {  // ApplyDFSPH ref (MPSSPHDFSPHUtil.cuh:379-418): one sum per ref object
+        const float invHi = 1.0f / (pos[i].w * 4.0f);
+        const uint32_t cr = nb.cfr[i];
+        uint32_t cur = kEmpty;
+        float Vt = 0.0f;
+        f3 fr{0.f, 0.f, 0.f};
+        for (uint32_t k = 0; k < cr; k++) {
+            const uint32_t j = nb.fr[size_t(k) * N + i];
+            const uint32_t t = sc.blk_scene[j / kBlock];
+            if (t != cur) {
+                if (cur != kEmpty) F = F + fr;
+                cur = t;
+                Vt = sc.P[t].V;
+                fr = f3{0.f, 0.f, 0.f};
+            }
+            const f3 xij = xi - mk(pos[j]);
+            fr = fr - 0.5f * Vt * (pi + kappa[j]) * Gk<CUBIC>(len3(xij), invHi) * xij;
+        }
+        if (cur != kEmpty) F = F + fr;
+    }
     F = F + fb;
     float4 v4 = vel[i];
     const float m = v4.w;
@@ -1133,25 +1243,35 @@ void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const 
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) launch_k(k_density_alpha<true>, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
-    else launch_k(k_density_alpha<false>, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
+    const bool ref = nb.fr != nullptr;
+    auto k = cubic ? (ref ? k_density_alpha<true, true> : k_density_alpha<true, false>)
+                   : (ref ? k_density_alpha<false, true> : k_density_alpha<false, false>);
+    launch_k(k, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
-void pressure_eval(int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float4* bpos,
-                   const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
-                   const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials, hipStream_t s) {
+void pressure_eval(int cubic, int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel,
+                   const float4* bpos, const float* bvol, const float4* bvel, const float* rho, const float* alpha,
+                   const float* pG_ff, const float* pG_fb, float* kappa, float* delta, const SolverDev* st, double* partials,
+                   hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (df) launch_k(k_pressure_eval<true>, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
-    else launch_k(k_pressure_eval<false>, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
+    // the ref sums evaluate G_ij themselves (the stored G covers the own and wall lists only)
+    auto k = nb.fr ? (df ? (cubic ? k_pressure_eval<true, true, true> : k_pressure_eval<true, true, false>)
+                         : (cubic ? k_pressure_eval<false, true, true> : k_pressure_eval<false, true, false>))
+                   : (df ? k_pressure_eval<true, false, false> : k_pressure_eval<false, false, false>);
+    launch_k(k, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
 void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
-                   hipStream_t s) {
-    if (sc.N) launch_k(k_fold_pressure, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol, lmin);
+                   int joint, hipStream_t s) {
+    if (!sc.N) return;
+    if (joint) launch_k(k_fold_pressure_joint, 1, kFoldThreads, 0, s, sc, partials, st, active, tol, lmin);
+    else launch_k(k_fold_pressure, sc.S, kFoldThreads, 0, s, sc, partials, st, active, tol, lmin);
 }
-void pressure_apply(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
+void pressure_apply(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel, const float4* bpos,
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s) {
-    if (sc.N) launch_k(k_pressure_apply, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
+    if (!sc.N) return;
+    auto k = nb.fr ? (cubic ? k_pressure_apply<true, true> : k_pressure_apply<true, false>) : k_pressure_apply<false, false>;
+    launch_k(k, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
 void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,
                float4* cgx, hipStream_t s) {
