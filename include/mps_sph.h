@@ -17,6 +17,8 @@
  *   mps_upload_boundary              BoundaryParticleObject::Resize        MPSBoundaryParticleObject.cu:14-24
  *   mps_sample_boundary_mesh         MeshObject::LoadMesh + kernel::ParticleSampling::ParticleSampling
  *                                    MPSMeshObject.cu:58-103, MPSParticleSamplingUtil.cu:42-133
+ *   mps_set_boundary_objects         several BoundaryParticleObjects; boundary "ref" volume overload
+ *                                    MPSSPHBaseUtil.cu:25-42
  *   mps_resample_boundary_mesh       ParticleSampling again on a moved mesh (IsSamplingParticleRequired,
  *                                    pIsGenerated / prevIdx)  MPSParticleSamplingUtil.cu:14-133
  *   mps_read_obj,                    the OBJ reading of MeshObject::LoadMesh  MPSMeshObject.cu:58-94
@@ -98,6 +100,7 @@ enum mps_array_id {
     MPS_ARR_BND_VEL = 24,        /* real[B][3] */
     MPS_ARR_BND_FACE_ID = 25,    /* uint32[B] */
     MPS_ARR_BND_BCC = 26,        /* real[B][2]  barycentric weights (w0, w1) on the owning face */
+    MPS_ARR_BND_OBJECT = 27,     /* uint32[B]   wall object of each wall particle (mps_set_boundary_objects) */
     MPS_ARR_FHASH_KEY = 30,      /* uint32[N]   sorted cell keys of the fluid hash */
     MPS_ARR_FHASH_ID = 31,       /* uint32[N]   particle ids in key order */
     MPS_ARR_FHASH_START = 32,    /* uint32[C]   first sorted slot of each cell, 0xFFFFFFFF if empty */
@@ -246,6 +249,16 @@ int mps_upload_boundary(mps_ctx* ctx, size_t n, const float* pos, const float* r
 int mps_sample_boundary_mesh(mps_ctx* ctx, const double* verts, size_t nv,
                              const uint32_t* tris, size_t nt, const double* center,
                              const double* size, const mps_mesh_material* m);
+/* The wall particles as k <= 4 boundary objects (several obstacles, each its own
+ * BoundaryParticleObject in the reference): counts[o] particles each, contiguous, in the
+ * current order (e.g. upload the objects' walls concatenated). A wall's volume then sums its
+ * own object first and every other object after it, in object order (the boundary "ref"
+ * overload, ComputeBoundaryParticleVolumeSub(boundary, nei, refBoundary),
+ * MPSSPHBaseUtil.cu:25-42); the fluid's wall sums in density, factor, density change and
+ * pressure force are taken one object after the other; each object is Z-sorted within its own
+ * range. Call after the walls are uploaded or sampled; replacing the walls makes them one
+ * object again. Single-scene contexts only; the adjoint refuses several objects. */
+int mps_set_boundary_objects(mps_ctx* ctx, uint32_t k, const uint32_t* counts);
 /* ParticleSampling called again on the mesh of the last mps_sample_boundary_mesh after its
  * vertices moved (MPSParticleSamplingUtil.cu:14-133, .cuh:69-246). `verts` are the nv new
  * vertex positions (same count and faces; no fitting), m->radius the sampling radius. The

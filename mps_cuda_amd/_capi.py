@@ -33,7 +33,7 @@ ARRAYS = {
     "FLUID_ALPHA": (7, "real", 1), "FLUID_KAPPA": (8, "real", 1), "FLUID_DELTA": (9, "real", 1),
     "FLUID_FACTOR_ST": (10, "real", 1), "FLUID_CG_B": (11, "real", 3),
     "BND_POS": (20, "real", 3), "BND_RADIUS": (21, "real", 1), "BND_VOLUME": (22, "real", 1),
-    "BND_MASS": (23, "real", 1), "BND_VEL": (24, "real", 3), "BND_FACE_ID": (25, "u32", 1), "BND_BCC": (26, "real", 2),
+    "BND_MASS": (23, "real", 1), "BND_VEL": (24, "real", 3), "BND_FACE_ID": (25, "u32", 1), "BND_BCC": (26, "real", 2), "BND_OBJECT": (27, "u32", 1),
     "FHASH_KEY": (30, "u32", 1), "FHASH_ID": (31, "u32", 1), "FHASH_START": (32, "u32", 1),
     "FHASH_END": (33, "u32", 1), "BHASH_KEY": (34, "u32", 1), "BHASH_ID": (35, "u32", 1),
     "BHASH_START": (36, "u32", 1), "BHASH_END": (37, "u32", 1),
@@ -119,6 +119,7 @@ _SIGS = {
     "mps_upload_fluid": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "mps_upload_boundary": (C.c_int, [_P, C.c_size_t, _P, _P, _P]),
     "mps_sample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(MeshMaterial)]),
+    "mps_set_boundary_objects": (C.c_int, [_P, C.c_uint32, _P]),
     "mps_resample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(MeshMaterial), C.POINTER(C.c_size_t)]),
     "mps_read_obj": (C.c_int, [_P, C.c_char_p, _P, C.POINTER(C.c_size_t), _P, C.POINTER(C.c_size_t)]),
     "mps_sample_boundary_obj": (C.c_int, [_P, C.c_char_p, _P, _P, C.POINTER(MeshMaterial)]),
@@ -269,6 +270,11 @@ class HipSim:
         m = MeshMaterial(radius, density)
         self._chk(self.lib.mps_sample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
                                                     _ptr(c), _ptr(s), C.byref(m)))
+
+    def set_boundary_objects(self, counts):
+        """The walls as len(counts) objects of these sizes (contiguous, current order)."""
+        c = np.ascontiguousarray(counts, np.uint32)
+        self._chk(self.lib.mps_set_boundary_objects(self.h, len(c), _ptr(c)))
 
     def resample_boundary_mesh(self, verts, radius) -> int:
         """ParticleSampling again after the sampled mesh's vertices moved; returns the number

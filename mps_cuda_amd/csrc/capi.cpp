@@ -203,6 +203,10 @@ struct mps_ctx {
     // coupled fluid objects: ref lists into the other scenes (column-major, Kfr slots)
     DBuf<uint32_t> nfr, cfr;
     uint32_t Kfr = 0;
+    // wall objects (mps_set_boundary_objects): object id per wall particle, per-object sums
+    uint32_t nbobj = 1;
+    DBuf<uint32_t> bobj;
+    DBuf<float4> bS4;
     bool coupled() const { return cfg.couple_scenes != 0 && S > 1; }
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
@@ -363,7 +367,7 @@ struct mps_ctx {
         return b;
     }
     std::vector<uint64_t> key_base(uint64_t kind, uint32_t chunk, const mps::Scenes& P, const mps::Nbr& nb) const {
-        return {kind, chunk, u(P.P), u(P.blk_scene), P.N, P.S, u(nb.ff), u(nb.cff), u(nb.fb), u(nb.cfb), u(nb.fr), u(nb.cfr)};
+        return {kind, chunk, u(P.P), u(P.blk_scene), P.N, P.S, u(nb.ff), u(nb.cff), u(nb.fb), u(nb.cfb), u(nb.fr), u(nb.cfr), u(nb.fbo), nb.nbo};
     }
 
     // ------------------------------------------------------------------ helpers
@@ -463,6 +467,10 @@ struct mps_ctx {
             nb.fr = nfr.p;
             nb.cfr = cfr.p;
         }
+        if (nbobj > 1) {
+            nb.fbo = bobj.p;
+            nb.nbo = nbobj;
+        }
         return nb;
     }
     int cubic() const { return cfg.kernel == MPS_KERNEL_CUBIC; }
@@ -481,8 +489,12 @@ struct mps_ctx {
         zkey.reserve(n); zid.reserve(n); zkey2.reserve(n); zid2.reserve(n);
         ztemp.reserve(mps::launch::radix_temp_words(n));
         const int zb = morton_bits();
-        T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, bnd_scene(), zb, zkey.p, zid.p, stream); });
-        const int bits = zb + scene_bits();
+        // several wall objects: each is Z-sorted within its own range (the object id above the
+        // Morton key), as each reference object Z-sorts with its own hash
+        const bool objs = boundary && nbobj > 1;
+        const uint32_t* zscene = objs ? bobj.p : bnd_scene();
+        T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, zscene, zb, zkey.p, zid.p, stream); });
+        const int bits = zb + (objs ? bits_for(nbobj) : scene_bits());
         uint32_t *k = zkey.p, *v = zid.p, *k2 = zkey2.p, *v2 = zid2.p;
         T_(K_ZSORT, [&] { mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream); });
         const uint32_t* perm = v;
@@ -704,8 +716,15 @@ struct mps_ctx {
             if (!grow) {
                 if (do_b) {
                     nnz_bb = h_ctr->nnz_bb;
-                    bS.reserve(std::max<uint32_t>(B, 1));
-                    T_(K_BSUM, [&] { mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, Kbb, cbb.p, bS.p, stream); });
+                    if (nbobj > 1) {
+                        bS4.reserve(std::max<uint32_t>(B, 1));
+                        T_(K_BSUM, [&] {
+                            mps::launch::boundary_sum_obj(cubic(), bpos.p, B, nbb.p, Kbb, cbb.p, bobj.p, bS4.p, stream);
+                        });
+                    } else {
+                        bS.reserve(std::max<uint32_t>(B, 1));
+                        T_(K_BSUM, [&] { mps::launch::boundary_sum(cubic(), bpos.p, B, nbb.p, Kbb, cbb.p, bS.p, stream); });
+                    }
                     bnd_dirty = false;
                 }
                 nnz_ff = h_ctr->nnz_ff;
@@ -721,7 +740,31 @@ struct mps_ctx {
     unsigned long long nnz_ff = 0, nnz_fb = 0, nnz_bb = 0;
 
     void boundary_volume() {
-        T_(K_BVOL, [&] { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); });
+        if (nbobj > 1)
+            T_(K_BVOL, [&] { mps::launch::boundary_volume_obj(bS4.p, bobj.p, nbobj, bvol.p, B, cfg.faithful_quirks, stream); });
+        else
+            T_(K_BVOL, [&] { mps::launch::boundary_volume(bS.p, bvol.p, B, cfg.faithful_quirks, stream); });
+    }
+    // The walls as k <= 4 objects of the given sizes (contiguous ranges in the current order).
+    void set_boundary_objects(uint32_t k, const uint32_t* counts) {
+        require_single("mps_set_boundary_objects");
+        if (k == 0 || k > mps::kMaxWallObjects || !counts) throw ApiError{MPS_ERR_INVALID, "1 to 4 wall objects"};
+        uint64_t total = 0;
+        for (uint32_t o = 0; o < k; o++) total += counts[o];
+        if (total != B) throw ApiError{MPS_ERR_INVALID, "object sizes do not add up to the wall particles"};
+        if (morton_bits() + bits_for(k) > 31) throw ApiError{MPS_ERR_INVALID, "too many wall objects for the hash size"};
+        std::vector<uint32_t> h(std::max<uint32_t>(B, 1), 0u);
+        size_t at = 0;
+        for (uint32_t o = 0; o < k; o++)
+            for (uint32_t q = 0; q < counts[o]; q++) h[at++] = o;
+        bobj.reserve(h.size());
+        up(bobj.p, h);
+        nbobj = k;
+        mesh.ok = false;
+        bh.cells_init = false;
+        bh.n_valid = 0;
+        bnd_dirty = true;
+        params_dirty = true;
     }
     void density_alpha(int flags) {
         ensure_halo();
@@ -1048,6 +1091,7 @@ struct mps_ctx {
     // Forward n steps from the current state, keeping a checkpoint every `every` steps.
     void adj_rollout(uint32_t n, uint32_t every, uint32_t* counts) {
         if (coupled()) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover coupled scenes"};
+        if (nbobj > 1) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover several wall objects"};
         if (cfg.kernel != MPS_KERNEL_CUBIC) throw ApiError{MPS_ERR_STATE, "the adjoint needs the cubic kernel"};
         if (cfg.visc_solver != MPS_VISC_CG) throw ApiError{MPS_ERR_STATE, "the adjoint needs the CG viscosity solve"};
         if (N == 0 || n == 0) throw ApiError{MPS_ERR_INVALID, "nothing to roll out"};
@@ -1517,6 +1561,7 @@ struct mps_ctx {
         Nref = uint32_t(nref);
         B = uint32_t(nb);
         mesh.ok = false;
+        nbobj = 1;
         alloc_fluid(N);
         alloc_boundary(B);
         std::vector<float4> hp(N, make_float4(0.f, 0.f, 0.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
@@ -1570,6 +1615,7 @@ struct mps_ctx {
         if (n > 0x7FFFFFFFu) throw ApiError{MPS_ERR_INVALID, "too many particles"};
         require_single("mps_upload_boundary");
         mesh.ok = false;
+        nbobj = 1;
         alloc_boundary(uint32_t(n));
         std::vector<float4> hp(n), hv(n);
         for (size_t i = 0; i < n; i++) {
@@ -1632,6 +1678,7 @@ struct mps_ctx {
         if (!verts || !tris || nv == 0 || nt == 0) throw ApiError{MPS_ERR_INVALID, "empty mesh"};
         require_single("mps_sample_boundary_mesh");
         params_dirty = true;
+        nbobj = 1;
         for (size_t i = 0; i < 3 * nt; i++)
             if (tris[i] >= nv) throw ApiError{MPS_ERR_INVALID, "triangle index out of range"};
         std::vector<double> V(verts, verts + 3 * nv);
@@ -1720,7 +1767,7 @@ struct mps_ctx {
     // quirks, stay with the index; otherwise mass and velocity move with the particle.
     void resample_mesh(const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) {
         require_single("mps_resample_boundary_mesh");
-        if (!mesh.ok) throw ApiError{MPS_ERR_STATE, "no mesh sampled on this context (mps_sample_boundary_mesh first)"};
+        if (!mesh.ok || nbobj != 1) throw ApiError{MPS_ERR_STATE, "no mesh sampled on this context (mps_sample_boundary_mesh first)"};
         if (!verts || 3 * nv != mesh.V.size()) throw ApiError{MPS_ERR_INVALID, "vertex count differs from the sampled mesh"};
         const size_t nt = mesh.count.size();
         std::vector<double> V(verts, verts + 3 * nv);
@@ -2020,6 +2067,10 @@ struct mps_ctx {
             case MPS_ARR_BND_VEL: f4xyz(bvel, B); break;
             case MPS_ARR_BND_FACE_ID: raw(bface.p, B, 4); break;
             case MPS_ARR_BND_BCC: raw(bbcc.p, B, 8); break;
+            case MPS_ARR_BND_OBJECT:
+                if (nbobj > 1) ro_raw(bobj.p, B);
+                else ro(std::vector<uint32_t>(B, 0u));
+                break;
             case MPS_ARR_FHASH_KEY: {
                 std::vector<uint32_t> v;
                 if (dst) fluid_hash_export(&v, nullptr); else v.resize(fh.n_valid);
@@ -2059,6 +2110,7 @@ struct mps_ctx {
         fh.release(); bh.release();
         zkey.release(); zid.release(); zkey2.release(); zid2.release(); ztemp.release(); tmpu.release(); tmp2.release();
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); nfr.release(); cfr.release();
+        bobj.release(); bS4.release();
         partials.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
         rb_xyz.release(); rb_rad.release(); rb_rgba.release();
@@ -2294,6 +2346,11 @@ int mps_sample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const u
                              const double* center, const double* size, const mps_mesh_material* m) {
     if (!c || !m) return MPS_ERR_INVALID;
     return guarded(c, [&] { c->sample_mesh(verts, nv, tris, nt, center, size, m); });
+}
+
+int mps_set_boundary_objects(mps_ctx* c, uint32_t k, const uint32_t* counts) {
+    if (!c) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->set_boundary_objects(k, counts); });
 }
 
 int mps_resample_boundary_mesh(mps_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) {

@@ -81,7 +81,11 @@ struct Nbr {
     const uint32_t* cfb;
     const uint32_t* fr = nullptr;
     const uint32_t* cfr = nullptr;
+    // wall objects (null: one object): object of each wall particle, and their number (<= 4)
+    const uint32_t* fbo = nullptr;
+    uint32_t nbo = 1;
 };
+constexpr uint32_t kMaxWallObjects = 4;
 
 // Block halos of the fluid-fluid lists (halo.hip): for workgroup b of slots, idx[b*kHaloCap + t]
 // (t < n[b]) are the distinct neighbour slots in ascending order and nb16[k*N + i] is the
@@ -171,6 +175,12 @@ enum DensityFlags { kDoDensity = 1, kDoAlpha = 2, kDoGravity = 4, kDoPairs = 8 }
 void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb, float* S,
                   hipStream_t s);
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s);
+// several wall objects: per-object sums S4[i] = (sum over object 0, 1, 2, 3), W(0) in the own
+// object's; volume = 1/(((V_prev + own) + other objects in order))
+void boundary_sum_obj(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb,
+                      const uint32_t* obj, float4* S4, hipStream_t s);
+void boundary_volume_obj(const float4* S4, const uint32_t* obj, uint32_t nobj, float* vol, uint32_t B, int faithful,
+                         hipStream_t s);
 // density, alpha, gravity; with kDoPairs also G_ij per neighbour slot
 void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, float4* vel,
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,

@@ -112,12 +112,57 @@ __global__ void __launch_bounds__(kBlock) k_boundary_volume(const float* __restr
     vol[i] = 1.0f / v;
 }
 
+// Several wall objects (a wall particle's object in obj[]): sums kept per object, walls of the
+// own object first (ComputeBoundaryParticleVolume_kernel self + ref, MPSSPHBaseUtil.cuh:10-76).
+// Per-object accumulators stay in registers: updates and reads select the element with
+// unrolled compares instead of a dynamic index.
+template <class T, class F>
+__device__ __forceinline__ void at_obj(T (&a)[kMaxWallObjects], uint32_t o, F&& f) {
+#pragma unroll
+    for (uint32_t q = 0; q < kMaxWallObjects; q++)
+        if (q == o) f(a[q]);
+}
+template <bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_boundary_sum_obj(const float4* __restrict__ bpos, uint32_t B,
+                                                           const uint32_t* __restrict__ nbb, uint32_t Kbb,
+                                                           const uint32_t* __restrict__ cbb, const uint32_t* __restrict__ obj,
+                                                           float4* __restrict__ S4) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const float4 xi4 = bpos[i];
+    const f3 xi = mk(xi4);
+    const float invHi = 1.0f / (xi4.w * 4.0f);
+    float v[kMaxWallObjects] = {0.f, 0.f, 0.f, 0.f};
+    at_obj(v, obj[i], [&](float& a) { a = Wk<CUBIC>(0.0f, invHi); });
+    const uint32_t c = cbb[i];
+    for (uint32_t k = 0; k < c; k++) {
+        const uint32_t j = nbb[size_t(i) * Kbb + k];
+        const float w = Wk<CUBIC>(len3(xi - mk(bpos[j])), invHi);
+        at_obj(v, obj[j], [&](float& a) { a += w; });
+    }
+    S4[i] = make_float4(v[0], v[1], v[2], v[3]);
+}
+__global__ void __launch_bounds__(kBlock) k_boundary_volume_obj(const float4* __restrict__ S4, const uint32_t* __restrict__ obj,
+                                                              uint32_t nobj, float* __restrict__ vol, uint32_t B, int faithful) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    const float4 s4 = S4[i];
+    const float s[kMaxWallObjects] = {s4.x, s4.y, s4.z, s4.w};
+    const uint32_t own = obj[i];
+    float v = faithful ? vol[i] : 0.0f;
+    at_obj(s, own, [&](const float& a) { v += a; });
+#pragma unroll
+    for (uint32_t q = 0; q < kMaxWallObjects; q++)
+        if (q < nobj && q != own) v += s[q];
+    vol[i] = 1.0f / v;
+}
+
 // ----------------------------------------------------------------- density, alpha, gravity
 // ComputeDensity_kernel ff/fb + Final + DensityColorTest (MPSSPHBaseUtil.cuh:88-221),
 // ComputeDFSPHFactor_kernel ff/fb + Final (MPSSPHDFSPHUtil.cuh:10-137),
 // ApplyGravity + UpdateVelocity (MPSAdvectUtil.cuh:8-34).
 // Also stores G_ij per neighbour slot (pG_ff, pG_fb) when kDoPairs is set.
-template <bool CUBIC, bool REF>
+template <bool CUBIC, bool REF, bool BOBJ>
 __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, float* __restrict__ rho,
@@ -190,17 +235,33 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
         }
         float sb = 0.0f;
         f3 gb{0.f, 0.f, 0.f};
+        float sbo[kMaxWallObjects] = {0.f, 0.f, 0.f, 0.f};  // per wall object (BOBJ)
+        f3 gbo[kMaxWallObjects] = {};
         const uint32_t cb = nb.cfb[i];
         for (uint32_t k = 0; k < cb; k++) {
             const uint32_t j = nbr_fb(nb, k, i, N);
             const f3 xij = xi - mk(bpos[j]);
             const float dist = len3(xij);
             const float vb = bvol[j];
-            if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
-            if (alph || pairs) {
-                const float G = Gk<CUBIC>(dist, invHi);
-                if (pairs) pG_fb[size_t(k) * N + i] = G;
-                gb = gb + (vb * G) * xij;
+            if constexpr (BOBJ) {
+                const uint32_t o = nb.fbo[j];
+                if (dens) {
+                    const float w = vb * Wk<CUBIC>(dist, invHi);
+                    at_obj(sbo, o, [&](float& a) { a += w; });
+                }
+                if (alph || pairs) {
+                    const float G = Gk<CUBIC>(dist, invHi);
+                    if (pairs) pG_fb[size_t(k) * N + i] = G;
+                    const f3 gg = (vb * G) * xij;
+                    at_obj(gbo, o, [&](f3& a) { a = a + gg; });
+                }
+            } else {
+                if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
+                if (alph || pairs) {
+                    const float G = Gk<CUBIC>(dist, invHi);
+                    if (pairs) pG_fb[size_t(k) * N + i] = G;
+                    gb = gb + (vb * G) * xij;
+                }
             }
         }
         // ((own + each ref object's sum, in scene order) + walls), as the reference's Sub
@@ -249,7 +310,13 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
             }
         }
         if (dens) {
-            d += sb;
+            if constexpr (BOBJ) {
+#pragma unroll
+                for (uint32_t q = 0; q < kMaxWallObjects; q++)
+                    if (q < nb.nbo) d += sbo[q];
+            } else {
+                d += sb;
+            }
             const float r = d * P.rho0;
             rho[i] = r;
             const float ratio = r / P.rho0;
@@ -259,7 +326,13 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
             color[i] = make_float4(red, green, blue, 1.0f);
         }
         if (alph) {
-            g = g + gb;
+            if constexpr (BOBJ) {
+#pragma unroll
+                for (uint32_t q = 0; q < kMaxWallObjects; q++)
+                    if (q < nb.nbo) g = g + gbo[q];
+            } else {
+                g = g + gb;
+            }
             a += dot3(g, g);
             alpha[i] = a > kEps ? 1.0f / a : 0.0f;
         }
@@ -284,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
 // alpha pass. Each workgroup publishes its sum of positive errors for k_fold_pressure.
 // The records of the block's halo (x_j, v_j) are staged in LDS (see k_cg_apply); the pairs
 // are summed in list order on both paths.
-template <bool DF, bool REF, bool CUBIC>
+template <bool DF, bool REF, bool CUBIC, bool BOBJ>
 __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                         const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                         const float* __restrict__ bvol, const float4* __restrict__ bvel,
@@ -354,13 +427,19 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
             }
         }
         float db = 0.0f;
+        float dbo[kMaxWallObjects] = {0.f, 0.f, 0.f, 0.f};
         const uint32_t cb = nb.cfb[i];
         for (uint32_t k = 0; k < cb; k++) {
             const size_t slot = size_t(k) * N + i;
             const uint32_t j = nb.fb[slot];
             const f3 xij = xi - mk(bpos[j]);
             const f3 vij = vi - mk(bvel[j]);
-            db += bvol[j] * pG_fb[slot] * dot3(xij, vij);
+            if constexpr (BOBJ) {
+                const float t = bvol[j] * pG_fb[slot] * dot3(xij, vij);
+                at_obj(dbo, nb.fbo[j], [&](float& a) { a += t; });
+            } else {
+                db += bvol[j] * pG_fb[slot] * dot3(xij, vij);
+            }
         }
         float d = 0.0f;
         d += df;
@@ -384,7 +463,13 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
             }
             if (cur != kEmpty) d += dr;
         }
-        d += db;
+        if constexpr (BOBJ) {
+#pragma unroll
+            for (uint32_t q = 0; q < kMaxWallObjects; q++)
+                if (q < nb.nbo) d += dbo[q];
+        } else {
+            d += db;
+        }
         delta[i] = d;
         float e;
         if (DF) {
@@ -463,7 +548,7 @@ __global__ void __launch_bounds__(kFoldThreads) k_fold_pressure_joint(Scenes sc,
 
 // ApplyDFSPH_kernel ff/fb + ApplyDFSPHFinal_kernel (MPSSPHDFSPHUtil.cuh:342-468) + UpdateVelocity.
 // The halo's (x_j, kappa_j) are staged in LDS (see k_cg_apply).
-template <bool REF, bool CUBIC>
+template <bool REF, bool CUBIC, bool BOBJ>
 __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                          float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                          const float* __restrict__ bvol, const float* __restrict__ pG_ff,
@@ -528,13 +613,15 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
         }
     }
     f3 fb{0.f, 0.f, 0.f};
+    f3 fbo[kMaxWallObjects] = {};
     const uint32_t cb = nb.cfb[i];
     for (uint32_t k = 0; k < cb; k++) {
         const size_t slot = size_t(k) * N + i;
         const uint32_t j = nb.fb[slot];
         const f3 xij = xi - mk(bpos[j]);
         const f3 fij = 0.5f * bvol[j] * pi * pG_fb[slot] * xij;
-        fb = fb - fij;
+        if constexpr (BOBJ) at_obj(fbo, nb.fbo[j], [&](f3& a) { a = a - fij; });
+        else fb = fb - fij;
     }
     f3 F{0.f, 0.f, 0.f};
     F = F + ff;
@@ -558,7 +645,13 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
         }
         if (cur != kEmpty) F = F + fr;
     }
-    F = F + fb;
+    if constexpr (BOBJ) {
+#pragma unroll
+        for (uint32_t q = 0; q < kMaxWallObjects; q++)
+            if (q < nb.nbo) F = F + fbo[q];
+    } else {
+        F = F + fb;
+    }
     float4 v4 = vel[i];
     const float m = v4.w;
     F = m * F;
@@ -1236,6 +1329,15 @@ void boundary_sum(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb
     if (cubic) launch_k(k_boundary_sum<true>, nb_(B), kBlock, 0, s, bpos, B, nbb, Kbb, cbb, S);
     else launch_k(k_boundary_sum<false>, nb_(B), kBlock, 0, s, bpos, B, nbb, Kbb, cbb, S);
 }
+void boundary_sum_obj(int cubic, const float4* bpos, uint32_t B, const uint32_t* nbb, uint32_t Kbb, const uint32_t* cbb,
+                      const uint32_t* obj, float4* S4, hipStream_t s) {
+    if (!B) return;
+    launch_k(cubic ? k_boundary_sum_obj<true> : k_boundary_sum_obj<false>, nb_(B), kBlock, 0, s, bpos, B, nbb, Kbb, cbb, obj, S4);
+}
+void boundary_volume_obj(const float4* S4, const uint32_t* obj, uint32_t nobj, float* vol, uint32_t B, int faithful,
+                         hipStream_t s) {
+    if (B) launch_k(k_boundary_volume_obj, nb_(B), kBlock, 0, s, S4, obj, nobj, vol, B, faithful);
+}
 void boundary_volume(const float* S, float* vol, uint32_t B, int faithful, hipStream_t s) {
     if (B) launch_k(k_boundary_volume, nb_(B), kBlock, 0, s, S, vol, B, faithful);
 }
@@ -1243,9 +1345,12 @@ void density_alpha(int cubic, int flags, const Scenes& sc, const Nbr& nb, const 
                    const float4* bpos, const float* bvol, float* rho, float* alpha, float4* color, float* pG_ff,
                    float* pG_fb, hipStream_t s) {
     if (!sc.N) return;
-    const bool ref = nb.fr != nullptr;
-    auto k = cubic ? (ref ? k_density_alpha<true, true> : k_density_alpha<true, false>)
-                   : (ref ? k_density_alpha<false, true> : k_density_alpha<false, false>);
+    using K = decltype(&k_density_alpha<false, false, false>);
+    static constexpr K tab[8] = {k_density_alpha<false, false, false>, k_density_alpha<false, false, true>,
+                                 k_density_alpha<false, true, false>,  k_density_alpha<false, true, true>,
+                                 k_density_alpha<true, false, false>,  k_density_alpha<true, false, true>,
+                                 k_density_alpha<true, true, false>,   k_density_alpha<true, true, true>};
+    const K k = tab[(cubic ? 4 : 0) | (nb.fr ? 2 : 0) | (nb.fbo ? 1 : 0)];
     launch_k(k, nb_(sc.N), kBlock, 0, s, flags, sc, nb, hl, pos, vel, bpos, bvol, rho, alpha, color, pG_ff, pG_fb);
 }
 void pressure_eval(int cubic, int df, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel,
@@ -1254,10 +1359,19 @@ void pressure_eval(int cubic, int df, const Scenes& sc, const Nbr& nb, const Hal
                    hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    // the ref sums evaluate G_ij themselves (the stored G covers the own and wall lists only)
-    auto k = nb.fr ? (df ? (cubic ? k_pressure_eval<true, true, true> : k_pressure_eval<true, true, false>)
-                         : (cubic ? k_pressure_eval<false, true, true> : k_pressure_eval<false, true, false>))
-                   : (df ? k_pressure_eval<true, false, false> : k_pressure_eval<false, false, false>);
+    // the ref sums evaluate G_ij themselves (the stored G covers the own and wall lists only);
+    // `cubic` matters for the ref instances only
+    using K = decltype(&k_pressure_eval<true, false, false, false>);
+    static constexpr K tab[16] = {
+        k_pressure_eval<false, false, false, false>, k_pressure_eval<false, false, false, true>,
+        k_pressure_eval<false, false, false, false>, k_pressure_eval<false, false, false, true>,
+        k_pressure_eval<false, true, false, false>,  k_pressure_eval<false, true, false, true>,
+        k_pressure_eval<false, true, true, false>,   k_pressure_eval<false, true, true, true>,
+        k_pressure_eval<true, false, false, false>,  k_pressure_eval<true, false, false, true>,
+        k_pressure_eval<true, false, false, false>,  k_pressure_eval<true, false, false, true>,
+        k_pressure_eval<true, true, false, false>,   k_pressure_eval<true, true, false, true>,
+        k_pressure_eval<true, true, true, false>,    k_pressure_eval<true, true, true, true>};
+    const K k = tab[(df ? 8 : 0) | (nb.fr ? 4 : 0) | (cubic ? 2 : 0) | (nb.fbo ? 1 : 0)];
     launch_k(k, g, kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, kappa, delta, st, partials);
 }
 void fold_pressure(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, uint32_t lmin,
@@ -1270,7 +1384,12 @@ void pressure_apply(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, 
                     const float* bvol, const float* pG_ff, const float* pG_fb, const float* kappa, const SolverDev* st,
                     hipStream_t s) {
     if (!sc.N) return;
-    auto k = nb.fr ? (cubic ? k_pressure_apply<true, true> : k_pressure_apply<true, false>) : k_pressure_apply<false, false>;
+    using K = decltype(&k_pressure_apply<false, false, false>);
+    static constexpr K tab[8] = {k_pressure_apply<false, false, false>, k_pressure_apply<false, false, true>,
+                                 k_pressure_apply<false, false, false>, k_pressure_apply<false, false, true>,
+                                 k_pressure_apply<true, false, false>,  k_pressure_apply<true, false, true>,
+                                 k_pressure_apply<true, true, false>,   k_pressure_apply<true, true, true>};
+    const K k = tab[(nb.fr ? 4 : 0) | (cubic ? 2 : 0) | (nb.fbo ? 1 : 0)];
     launch_k(k, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, bpos, bvol, pG_ff, pG_fb, kappa, st);
 }
 void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* pG_ff, float* fst,

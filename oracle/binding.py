@@ -30,6 +30,7 @@ _SIGS = {
     "orc_upload_fluid": (C.c_int, [_P, C.c_size_t, _P, _P, _P, _P]),
     "orc_upload_boundary": (C.c_int, [_P, C.c_size_t, _P, _P, _P]),
     "orc_sample_boundary_mesh": (C.c_int, [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(capi.MeshMaterial)]),
+    "orc_set_boundary_objects": (C.c_int, [_P, C.c_uint32, _P]),
     "orc_world_create": (C.c_int, [C.POINTER(capi.Config), C.c_int, C.c_uint32, C.POINTER(_P)]),
     "orc_world_destroy": (None, [_P]),
     "orc_world_phase": (_P, [_P, C.c_uint32]),
@@ -142,6 +143,10 @@ class OracleSim:
         m = capi.MeshMaterial(radius, density)
         self._chk(self.lib.orc_sample_boundary_mesh(self.h, _ptr(verts), verts.shape[0], _ptr(tris), tris.shape[0],
                                                     _ptr(c), _ptr(s), C.byref(m)))
+
+    def set_boundary_objects(self, counts):
+        c = np.ascontiguousarray(counts, np.uint32)
+        self._chk(self.lib.orc_set_boundary_objects(self.h, len(c), _ptr(c)))
 
     def resample_boundary_mesh(self, verts, radius) -> int:
         verts = np.ascontiguousarray(verts, np.float64).reshape(-1, 3)

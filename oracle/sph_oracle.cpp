@@ -366,6 +366,7 @@ struct Base {
     virtual int sample_mesh(const double* verts, size_t nv, const uint32_t* tris, size_t nt, const double* center,
                             const double* size, const mps_mesh_material* m) = 0;
     virtual int resample_mesh(const double* verts, size_t nv, const mps_mesh_material* m, size_t* regenerated) = 0;
+    virtual int boundary_objects(uint32_t k, const uint32_t* counts) = 0;
     virtual int step(mps_step_stats* st) = 0;
     virtual int run_phase(int32_t ph, mps_step_stats* st) = 0;
     virtual int array_info(int32_t id, size_t* count, size_t* eb) = 0;
@@ -402,6 +403,29 @@ struct Ctx final : Base {
             build_csr(rf.list, x, r, N, rf.o->x, rf.o->r, rf.o->fh, H(), cell(), false);
             if (rf.o->N == 0) rf.list.idx.assign(N + 1, 0u), rf.list.nei.clear();
         }
+    }
+    // Wall objects: the walls may be several boundary objects (contiguous ranges, object ids in
+    // bobj). Each object keeps its own Z-sort range; a wall's volume sums its own object first
+    // and then every other object (the boundary "ref" overload, MPSSPHBaseUtil.cu:25-42,
+    // .cuh:43-76), and the fluid sums over the walls one object after the other
+    // (ComputeDensitySub(sph, nei, boundary) and the like, called once per object).
+    static constexpr uint32_t kMaxObjects = 4;
+    std::vector<uint32_t> bobj;
+    uint32_t nobj = 1;
+    int set_boundary_objects(uint32_t k, const uint32_t* counts) {
+        if (k == 0 || k > kMaxObjects || !counts) { err = "1 to 4 wall objects"; return MPS_ERR_INVALID; }
+        size_t total = 0;
+        for (uint32_t o = 0; o < k; o++) total += counts[o];
+        if (total != B) { err = "object sizes do not add up to the wall particles"; return MPS_ERR_INVALID; }
+        size_t at = 0;
+        for (uint32_t o = 0; o < k; o++)
+            for (uint32_t q = 0; q < counts[o]; q++) bobj[at++] = o;
+        nobj = k;
+        mesh_ok = false;
+        bb = CSR{};
+        bb.idx.assign(B + 1, 0u);
+        bh = Hash{};
+        return MPS_OK;
     }
     // the mesh of the last sample_mesh and its per-face sampling state
     // (MeshObject::m_samplingParticleSize / m_shortestEdgeID, MPSMeshObject.cu:31-45)
@@ -456,6 +480,8 @@ struct Ctx final : Base {
         B = n;
         bx.assign(3 * n, T(0)); br.assign(n, T(0)); bvol.assign(n, T(0)); bm.assign(n, T(0));
         bv.assign(3 * n, T(0)); bbcc.assign(2 * n, T(0)); bface.assign(n, 0u); bcolor.assign(4 * n, 0.f);
+        bobj.assign(n, 0u);
+        nobj = 1;
         bb = CSR{}; bb.idx.assign(n + 1, 0u);
         bh = Hash{};
     }
@@ -563,8 +589,9 @@ struct Ctx final : Base {
     // the particles: a copied particle keeps the mass and velocity that were stored at its new
     // index. Without faithful quirks the copied particles carry their mass and velocity along.
     // The vertex masses of LoadMesh stay as they were.
+    int boundary_objects(uint32_t k, const uint32_t* counts) override { return set_boundary_objects(k, counts); }
     int resample_mesh(const double* verts, size_t nv, const mps_mesh_material* mat, size_t* regenerated) override {
-        if (!mesh_ok) { err = "no mesh sampled on this context (mps_sample_boundary_mesh first)"; return MPS_ERR_STATE; }
+        if (!mesh_ok || nobj != 1) { err = "no mesh sampled on this context (mps_sample_boundary_mesh first)"; return MPS_ERR_STATE; }
         if (!verts || nv * 3 != mesh.v.size()) { err = "vertex count differs from the sampled mesh"; return MPS_ERR_INVALID; }
         const size_t nt = mesh.f.size() / 3;
         mesh.v.assign(verts, verts + 3 * nv);
@@ -593,6 +620,7 @@ struct Ctx final : Base {
         B = n;
         bx.resize(3 * n, T(0)); br.resize(n, T(0)); bvol.resize(n, T(0)); bm.resize(n, T(0));
         bv.resize(3 * n, T(0)); bbcc.resize(2 * n, T(0)); bface.resize(n, 0u); bcolor.resize(4 * n, 0.f);
+        bobj.assign(n, 0u);
         const bool faithful = cfg.faithful_quirks != 0;
         for (size_t t = 0; t < nt; t++) {
             size_t i = curr_idx[t];
@@ -648,7 +676,9 @@ struct Ctx final : Base {
                 for (size_t k = 0; k < w; k++) arr[w * s + k] = old[w * p[s] + k];
         };
         if (B > 0) {
-            const auto p = perm_of(bx, B);
+            auto p = perm_of(bx, B);
+            if (nobj > 1)  // each object is Z-sorted within its own range (its own hash, MPSSystem.cpp:220-224)
+                std::stable_sort(p.begin(), p.end(), [&](uint32_t a, uint32_t b) { return bobj[a] < bobj[b]; });
             apply(bx, p, 3); apply(bm, p, 1); apply(bv, p, 3); apply(bcolor, p, 4); apply(br, p, 1);
             apply(bface, p, 1); apply(bbcc, p, 2);
             if (!cfg.faithful_quirks) apply(bvol, p, 1);
@@ -681,10 +711,15 @@ struct Ctx final : Base {
             const T hi = br[i] * T(4);
             const T invHi = T(1) / hi;
             const V3<T> xi = ld3(bx, i);
-            T vol = Wk(T(0), invHi, cubic);
-            nsearch(bb, i, [&](uint32_t j) { vol += Wk(length(xi - ld3(bx, j)), invHi, cubic); });
-            const T prev = cfg.faithful_quirks ? bvol[i] : T(0);
-            bvol[i] = T(1) / (prev + vol);
+            const uint32_t own = bobj[i];
+            T vo[kMaxObjects] = {};
+            vo[own] = Wk(T(0), invHi, cubic);
+            nsearch(bb, i, [&](uint32_t j) { vo[bobj[j]] += Wk(length(xi - ld3(bx, j)), invHi, cubic); });
+            T vsum = cfg.faithful_quirks ? bvol[i] : T(0);
+            vsum += vo[own];
+            for (uint32_t o = 0; o < nobj; o++)
+                if (o != own) vsum += vo[o];
+            bvol[i] = T(1) / vsum;
         }
     }
     // ComputeDensity_kernel ff/fb + Final + DensityColorTest (MPSSPHBaseUtil.cuh:88-221)
@@ -697,8 +732,8 @@ struct Ctx final : Base {
             const V3<T> xi = ld3(x, i);
             T sf = mV * Wk(T(0), invHi, cubic);
             nsearch(ff, i, [&](uint32_t j) { sf += mV * Wk(length(xi - ld3(x, j)), invHi, cubic); });
-            T sb = T(0);
-            nsearch(fb, i, [&](uint32_t j) { sb += bvol[j] * Wk(length(xi - ld3(bx, j)), invHi, cubic); });
+            T sb[kMaxObjects] = {};
+            nsearch(fb, i, [&](uint32_t j) { sb[bobj[j]] += bvol[j] * Wk(length(xi - ld3(bx, j)), invHi, cubic); });
             T d = T(0);
             d += sf;
             for (const RefPhase& rf : refs) {  // ComputeDensity_kernel ref (MPSSPHBaseUtil.cuh:122-156)
@@ -706,7 +741,7 @@ struct Ctx final : Base {
                 nsearch(rf.list, i, [&](uint32_t j) { sr += rf.o->mV * Wk(length(xi - ld3(rf.o->x, j)), invHi, cubic); });
                 d += sr;
             }
-            d += sb;
+            for (uint32_t o = 0; o < nobj; o++) d += sb[o];
             rho[i] = d * mRho0;
             const float ratio = static_cast<float>(rho[i] / mRho0);
             auto clampf = [](float a) { return std::min(std::max(a, 0.f), 1.f); };
@@ -733,10 +768,10 @@ struct Ctx final : Base {
                 gf = gf + grad;
                 s += dot(grad, grad);
             });
-            V3<T> gb{0, 0, 0};
+            V3<T> gb[kMaxObjects] = {};
             nsearch(fb, i, [&](uint32_t j) {
                 const V3<T> xij = xi - ld3(bx, j);
-                gb = gb + bvol[j] * Gk(length(xij), invHi, cubic) * xij;
+                gb[bobj[j]] = gb[bobj[j]] + bvol[j] * Gk(length(xij), invHi, cubic) * xij;
             });
             V3<T> g{0, 0, 0};
             g = g + gf;
@@ -754,7 +789,7 @@ struct Ctx final : Base {
                 g = g + gr;
                 a += sr;
             }
-            g = g + gb;
+            for (uint32_t o = 0; o < nobj; o++) g = g + gb[o];
             a += dot(g, g);
             alpha[i] = a > KC<T>::EPS ? T(1) / a : T(0);
         }
@@ -782,11 +817,11 @@ struct Ctx final : Base {
                 const V3<T> vij = vi - ld3(v, j);
                 df += mV * Gk(length(xij), invHi, cubic) * dot(xij, vij);
             });
-            T db = T(0);
+            T db[kMaxObjects] = {};
             nsearch(fb, i, [&](uint32_t j) {
                 const V3<T> xij = xi - ld3(bx, j);
                 const V3<T> vij = vi - ld3(bv, j);
-                db += bvol[j] * Gk(length(xij), invHi, cubic) * dot(xij, vij);
+                db[bobj[j]] += bvol[j] * Gk(length(xij), invHi, cubic) * dot(xij, vij);
             });
             T d = T(0);
             d += df;
@@ -799,7 +834,7 @@ struct Ctx final : Base {
                 });
                 d += dr;
             }
-            d += db;
+            for (uint32_t o = 0; o < nobj; o++) d += db[o];
             delta[i] = d;
         }
     }
@@ -838,11 +873,11 @@ struct Ctx final : Base {
                 const V3<T> fij = T(0.5) * mV * (pi + kappa[j]) * Gk(length(xij), invHi, cubic) * xij;
                 ffo = ffo - fij;
             });
-            V3<T> fbo{0, 0, 0};
+            V3<T> fbo[kMaxObjects] = {};
             nsearch(fb, i, [&](uint32_t j) {
                 const V3<T> xij = xi - ld3(bx, j);
                 const V3<T> fij = T(0.5) * bvol[j] * pi * Gk(length(xij), invHi, cubic) * xij;
-                fbo = fbo - fij;
+                fbo[bobj[j]] = fbo[bobj[j]] - fij;
             });
             V3<T> F{0, 0, 0};
             F = F + ffo;
@@ -855,7 +890,7 @@ struct Ctx final : Base {
                 });
                 F = F + fro;
             }
-            F = F + fbo;
+            for (uint32_t o = 0; o < nobj; o++) F = F + fbo[o];
             F = m[i] * F;
             st3(force, i, F);
             if (m[i] > T(1.0e-10)) st3(v, i, ld3(v, i) + (dtv / m[i]) * F);
@@ -1246,6 +1281,7 @@ struct Ctx final : Base {
             case MPS_ARR_BND_VEL: return R(bv, 3 * t);
             case MPS_ARR_BND_FACE_ID: return R(bface, 4);
             case MPS_ARR_BND_BCC: return R(bbcc, 2 * t);
+            case MPS_ARR_BND_OBJECT: return R(bobj, 4);
             case MPS_ARR_FHASH_KEY: return R(fh.key, 4);
             case MPS_ARR_FHASH_ID: return R(fh.id, 4);
             case MPS_ARR_FHASH_START: return R(fh.start, 4);
@@ -1420,6 +1456,10 @@ int orc_create(const mps_config* cfg, int precision, orc_ctx** out) {
     return MPS_OK;
 }
 void orc_destroy(orc_ctx* c) { delete c; }
+
+int orc_set_boundary_objects(orc_ctx* c, uint32_t k, const uint32_t* counts) {
+    return c ? c->impl->boundary_objects(k, counts) : MPS_ERR_INVALID;
+}
 
 int orc_world_create(const mps_config* cfg, int precision, uint32_t nphase, orc_world** out) {
     if (!cfg || !out || nphase == 0) return MPS_ERR_INVALID;

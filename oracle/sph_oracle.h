@@ -49,6 +49,9 @@ int orc_sample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const u
 int orc_resample_boundary_mesh(orc_ctx* c, const double* verts, size_t nv, const mps_mesh_material* m,
                                size_t* regenerated);
 
+/* The walls as k <= 4 boundary objects of the given sizes (contiguous, in order). */
+int orc_set_boundary_objects(orc_ctx* c, uint32_t k, const uint32_t* counts);
+
 /* A coupled world of fluid objects: nphase contexts (load each with the calls above through
  * orc_world_phase) whose density, factor, delta and pressure force include the other phases
  * (the reference's "ref" overloads); the pressure loops run jointly over all phases. */
