@@ -33,7 +33,8 @@ ARRAYS = {
     "FLUID_ALPHA": (7, "real", 1), "FLUID_KAPPA": (8, "real", 1), "FLUID_DELTA": (9, "real", 1),
     "FLUID_FACTOR_ST": (10, "real", 1), "FLUID_CG_B": (11, "real", 3),
     "BND_POS": (20, "real", 3), "BND_RADIUS": (21, "real", 1), "BND_VOLUME": (22, "real", 1),
-    "BND_MASS": (23, "real", 1), "BND_VEL": (24, "real", 3), "BND_FACE_ID": (25, "u32", 1), "BND_BCC": (26, "real", 2), "BND_OBJECT": (27, "u32", 1),
+    "BND_MASS": (23, "real", 1), "BND_VEL": (24, "real", 3), "BND_FACE_ID": (25, "u32", 1),
+    "BND_BCC": (26, "real", 2), "BND_OBJECT": (27, "u32", 1),
     "FHASH_KEY": (30, "u32", 1), "FHASH_ID": (31, "u32", 1), "FHASH_START": (32, "u32", 1),
     "FHASH_END": (33, "u32", 1), "BHASH_KEY": (34, "u32", 1), "BHASH_ID": (35, "u32", 1),
     "BHASH_START": (36, "u32", 1), "BHASH_END": (37, "u32", 1),

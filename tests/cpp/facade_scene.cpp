@@ -5,6 +5,9 @@
 //   facade_scene host <mesh.obj>
 //       no GPU: prints the SPHMaterial / MeshMaterial defaults and derived values and the
 //       mesh LoadMesh read, one "name value..." line each
+//   facade_scene lists <mesh.obj> <out.bin>
+//       Initalize, then the SpatialHash calls (UpdateHash, BuildNeighorhood, GetNeighborhood
+//       for the fluid, fluid->wall and wall lists); writes them (format at lists())
 //   facade_scene run <mesh.obj> <keys> <steps> <out.bin>
 //       Initalize, then OnKeyDown for each key character ('+' = ']', '-' = '[', 'g' = 'G'),
 //       ' ' to start, <steps> OnUpdate calls; with 'w' among the keys the +x wall of the box
@@ -97,14 +100,42 @@ static int run(const char* mesh, const char* keys, int steps, const char* out) {
     return 0;
 }
 
+// SpatialHash::UpdateHash / BuildNeighorhood / GetNeighborhood and the hash getters on the
+// Initalize scene: writes uint64 counts (keys, ff offsets, ff ids, fb offsets, fb ids, bb
+// offsets, bb ids), then the uint32 arrays in that order.
+static int lists(const char* mesh, const char* out) {
+    mps::System sys;
+    sys.SetDevice(0);
+    sys.Initalize(mesh);
+    mps::SpatialHash& h = sys.GetSPHHash();
+    h.UpdateHash();
+    h.BuildNeighorhood();
+    const std::vector<uint32_t> ids = h.GetHashIDs();
+    const auto ff = h.GetNeighborhood();
+    const auto fb = h.GetNeighborhood(true);
+    const auto bb = sys.GetBoundaryParticleHash().GetNeighborhood();
+    const std::vector<uint32_t>* arrs[7] = {&ids, &ff.first, &ff.second, &fb.first, &fb.second, &bb.first, &bb.second};
+    FILE* f = std::fopen(out, "wb");
+    if (!f) return 1;
+    for (const auto* a : arrs) {
+        const uint64_t n = a->size();
+        std::fwrite(&n, sizeof(n), 1, f);
+    }
+    for (const auto* a : arrs) std::fwrite(a->data(), sizeof(uint32_t), a->size(), f);
+    std::fclose(f);
+    std::printf("ids %zu ff %zu fb %zu bb %zu\n", ids.size(), ff.second.size(), fb.second.size(), bb.second.size());
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
         if (argc == 3 && !std::strcmp(argv[1], "host")) return host_checks(argv[2]);
+        if (argc == 4 && !std::strcmp(argv[1], "lists")) return lists(argv[2], argv[3]);
         if (argc == 6 && !std::strcmp(argv[1], "run")) return run(argv[2], argv[3], std::atoi(argv[4]), argv[5]);
     } catch (const mps::Exception& e) {
         std::fprintf(stderr, "mps::Exception %d: %s\n", e.code(), e.what());
         return 2;
     }
-    std::fprintf(stderr, "usage: facade_scene host <mesh.obj> | run <mesh.obj> <keys> <steps> <out.bin>\n");
+    std::fprintf(stderr, "usage: facade_scene host <mesh.obj> | lists <mesh.obj> <out.bin> | run <mesh.obj> <keys> <steps> <out.bin>\n");
     return 64;
 }

@@ -52,3 +52,28 @@ def test_facade_run_equals_python_path(tmp_path, keys, size, gravity):
     assert nb == g.get("BND_POS").shape[0]
     assert loops == [st.l_df, st.l_cd, st.l_cg]
     assert np.array_equal(pos, ref)
+
+
+def test_facade_spatial_hash_lists_equal_python_path(tmp_path):
+    """SpatialHash::UpdateHash / BuildNeighorhood / GetNeighborhood through the C++ classes give
+    the same ids and CSR lists (reference order) as the HASH and NEIGHBORS phases run from Python."""
+    exe = build.FACADE_BIN
+    out = tmp_path / "lists.bin"
+    print("[facade] lists", flush=True)
+    r = subprocess.run([exe, "lists", CUBE, str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    b = out.read_bytes()
+    counts = np.frombuffer(b[:56], np.uint64).astype(int)
+    arrs, at = [], 56
+    for n in counts:
+        arrs.append(np.frombuffer(b[at:at + 4 * n], np.uint32))
+        at += 4 * n
+    sc = scenes.surface_tension_test_scene(5)
+    g = HipSim(scenes.make_config(sc))
+    scenes.load_into(g, sc)
+    g.run_phase("HASH")
+    g.run_phase("NEIGHBORS")
+    names = ["FHASH_ID", "NEI_FF_IDX", "NEI_FF", "NEI_FB_IDX", "NEI_FB", "NEI_BB_IDX", "NEI_BB"]
+    for name, a in zip(names, arrs):
+        assert np.array_equal(a, g.get(name)), name
+    assert counts[2] > 0 and counts[6] > 0    # (the fluid starts 3.5 m from the walls: no fluid->wall pairs yet)
