@@ -237,3 +237,22 @@ def test_viscosity_halo_paths_bit_identical(limit):
         assert np.array_equal(a.get("FLUID_VEL"), b.get("FLUID_VEL")), s
     say(f"[halo] limit {limit}: default max halo {sa.halo_max}, {spilled} block-steps on the global path")
     assert spilled > 0
+
+
+def test_checkpoint_resume_is_bit_identical():
+    """Snapshot at frame 30 (after the Z-sort of frame 1) through the C ABI, restore into a fresh
+    context and run across the Z-sort of frame 51: the same positions and loop counts as the
+    uninterrupted run."""
+    from tests.resume import run_resume
+    sc = scenes.dam_break_cpu(n_side=10)
+
+    def make(load=True):
+        g = HipSim(scenes.make_config(sc))
+        if load:
+            scenes.load_into(g, sc)
+        return g
+
+    say("[features] checkpoint/resume")
+    xa, xb, la, lb = run_resume(make, sc, 30, 56)
+    assert la == lb and any(z for *_, z in la)
+    assert np.array_equal(xa, xb)

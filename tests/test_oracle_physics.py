@@ -319,3 +319,22 @@ def test_fp32_semantics_track_fp64_reference():
         err = np.linalg.norm(xa - xb) / np.linalg.norm(xa)
         # FP32 rounding drifts slowly through the nonlinear step: 1e-5 over the first 5 steps
         assert err < (1e-5 if s < 5 else 1e-4), (s, err)
+
+
+def test_checkpoint_resume_is_bit_identical():
+    """Snapshot at frame 30 (after the Z-sort of frame 1), restore into a fresh context and run
+    across the Z-sort of frame 51: the same positions and loop counts as without the break."""
+    from mps_cuda_amd import scenes
+    from oracle.binding import OracleSim
+    from tests.resume import run_resume
+    sc = scenes.dam_break_cpu(n_side=10)    # the fluid sits against the walls: their volumes matter
+
+    def make(load=True):
+        o = OracleSim(scenes.make_config(sc), 32)
+        if load:
+            scenes.load_into(o, sc)
+        return o
+
+    xa, xb, la, lb = run_resume(make, sc, 30, 56)
+    assert la == lb and any(z for *_, z in la)
+    assert np.array_equal(xa, xb)
