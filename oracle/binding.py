@@ -16,7 +16,7 @@ import numpy as np
 from mps_cuda_amd import _capi as capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libmps_oracle.so")
+LIB_PATH = os.environ.get("MPS_ORACLE_LIB") or os.path.join(HERE, "_build", "libmps_oracle.so")
 
 _P = C.c_void_p
 _SIGS = {
