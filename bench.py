@@ -72,8 +72,8 @@ def say(msg: str) -> None:
 
 def measured_traffic(kernel: str) -> float | None:
     """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
-    rocprofv3 counter passes, each scaled by what the counter reads for a plain copy of known
-    size (the summary's last column, MB per dispatch, largest dispatch)."""
+    rocprofv3 counter passes over the same 1000 steps, each scaled by what the counter reads for
+    a plain copy of known size (the summary's last column: MB per working dispatch, mean)."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
