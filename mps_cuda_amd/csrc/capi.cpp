@@ -353,7 +353,14 @@ struct mps_ctx {
             if (graphs.size() >= 64) clear_graphs();
             GraphEntry e;
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-            f();
+            try {
+                f();
+            } catch (...) {  // leave the stream out of capture mode before reporting
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
             HIPCHK(hipStreamEndCapture(stream, &e.graph));
             HIPCHK(hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0));
             it = graphs.emplace(std::move(key), e).first;
@@ -2150,6 +2157,9 @@ int guarded(mps_ctx* c, F&& f) {
         return MPS_OK;
     } catch (const HipError& e) {
         c->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return MPS_ERR_HIP;
+    } catch (const mps::launch::LaunchError& e) {
+        c->err = std::string("kernel launch: ") + hipGetErrorString(e.e);
         return MPS_ERR_HIP;
     } catch (const ApiError& e) {
         c->err = e.msg;

@@ -111,6 +111,11 @@ protected:
     ~Timing() = default;
 };
 extern thread_local Timing* timing;
+// A launch the runtime refused (bad configuration, missing code object): thrown by launch_k,
+// turned into MPS_ERR_HIP with the message by the C ABI.
+struct LaunchError {
+    hipError_t e;
+};
 
 // Launch `kernel` on `s` with the arguments converted to its parameter types, as a call would.
 template <class... P, class... A>
@@ -126,7 +131,7 @@ inline void launch_k(void (*kernel)(P...), dim3 grid, dim3 block, uint32_t shm, 
     if (timing) timing->events(&e0, &e1);
     const hipError_t rc = e0 ? hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shm, s, e0, e1, 0)
                              : hipLaunchKernel(reinterpret_cast<const void*>(kernel), grid, block, ptrs, shm, s);
-    (void)rc;  // launch errors surface at the next checked call, as with <<< >>>
+    if (rc != hipSuccess) throw LaunchError{rc};  // faults while running surface at the next synchronisation
 }
 
 // ---- hash.hip
