@@ -75,7 +75,8 @@ def build_facade_driver(verbose: bool = True) -> str:
     """Host-only C++ program that drives the library through include/mps_system.hpp (the
     reference's System/SPHMaterial/MeshObject/... names); found next to the library at run time."""
     os.makedirs(OUT_DIR, exist_ok=True)
-    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(REPO, "include"),
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-Wpedantic", "-Wshadow", "-Werror",
+           "-I", os.path.join(REPO, "include"),
            FACADE_SRC, "-L", HERE, "-lmps_sph", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib",
            "-o", FACADE_BIN]
     r = subprocess.run(cmd, capture_output=True, text=True)

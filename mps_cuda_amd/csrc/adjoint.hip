@@ -433,7 +433,6 @@ __device__ __forceinline__ PairVjp cg_pair_vjp(const StepParams& P, f3 x, f3 p, 
     const float inner = STW + P.dt * (dot3(u, stG) - K);
     const float g = inner * invD;
     const float s = mij * g;
-    const float w = mj / rj * W;
     const float dd = d * d + 0.01f * hi * hi;
     const float c4 = 1.0f / (d + kFltEps);
     const float c1 = 0.5f * (mi + mj) / rj;
@@ -441,8 +440,7 @@ __device__ __forceinline__ PairVjp cg_pair_vjp(const StepParams& P, f3 x, f3 p, 
     const float cs = P.dt * P.sigma / mi;
     const float cx = P.xsph / P.dt;
     const float cv = 10.0f * P.rho0 * P.mu / ri;
-    const float a = P.dt * (cs * s + cx * w);
-    const float b = -P.dt * cv * c;
+    const float b = -P.dt * cv * c;  // a_ij itself is not needed by the reverse pass
     // reverse
     PairVjp o;
     const float Lx = dot3(lam, x), Lp = dot3(lam, p), px = dot3(p, x);

@@ -70,17 +70,17 @@ static int run(const char* mesh, const char* keys, int steps, const char* out) {
     for (int s = 0; s < steps; s++) {
         if (move_wall && s == steps / 2) {
             // the mesh is the unit cube fitted into the collision box: x = (v - 0.5) * size + center
-            mps::MeshObject& mesh = sys.GetMeshObject();
-            const mps::REAL3 c = mesh.GetCenter(), sz = mesh.GetFitSize();
-            std::vector<double> w(mesh.GetVertices());
+            mps::MeshObject& walls = sys.GetMeshObject();
+            const mps::REAL3 c = walls.GetCenter(), sz = walls.GetFitSize();
+            std::vector<double> w(walls.GetVertices());
             for (size_t i = 0; i < w.size() / 3; i++) {
                 const bool xmax = w[3 * i] == 1.0;
                 w[3 * i] = (w[3 * i] - 0.5) * sz.x + c.x + (xmax ? 1.0 : 0.0);
                 w[3 * i + 1] = (w[3 * i + 1] - 0.5) * sz.y + c.y;
                 w[3 * i + 2] = (w[3 * i + 2] - 0.5) * sz.z + c.z;
             }
-            mesh.SetVertexPositions(w);
-            mps::kernel::ParticleSampling::ParticleSampling(sys.GetMeshMaterial().GetParam(), mesh,
+            walls.SetVertexPositions(w);
+            mps::kernel::ParticleSampling::ParticleSampling(sys.GetMeshMaterial().GetParam(), walls,
                                                             sys.GetBoundaryParticleObject());
         }
         sys.OnUpdate();
