@@ -36,6 +36,7 @@ if ROOT not in sys.path:
 
 METRIC = "particle-steps/s at 0.1% density err: batched vs plain; adjoint/forward cost"
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 512 * 1024 * 1024  # per buffer of the in-run copy measurement (read + written)
 
 # Algorithmic bytes per launch, per fluid particle N and per neighbour entry E (FP32 SoA,
 # 4-byte indices, compulsory traffic only): the per-pass figures of the measurement plan.
@@ -324,9 +325,19 @@ def main() -> int:
     # every kernel on the context's stream
     prof_steps = 0 if args.no_profile else (args.profile_steps or args.steps)
     prof, pstats, p_elapsed = {}, [], 0.0
+    copy_gbps = None
     if prof_steps:
         p_elapsed, pstats, _, prof = run(prof_steps, True)
         say(f"profiled replay: {prof_steps} steps, {1000.0 * p_elapsed / prof_steps:.3f} ms per step")
+        # a STREAM-style copy in the same run (the library's k_stream_copy: one 16-byte vector
+        # per thread, non-temporal): what this device's memory system delivers to a copy
+        sim.profile(True)
+        sim.calibration_copy(COPY_BYTES, 5)
+        cp = sim.profile_read().get("k_stream_copy")
+        sim.profile(False)
+        if cp and cp["total_ms"] > 0:
+            copy_gbps = 2.0 * COPY_BYTES / (cp["total_ms"] * 1e-3 / cp["launches"]) / 1e9
+            say(f"copy kernel: {copy_gbps:.0f} GB/s")
 
     cap = cfg.max_iter
     C = cfg.hash_size[0] * cfg.hash_size[1] * cfg.hash_size[2]
@@ -371,6 +382,8 @@ def main() -> int:
             "profiled_steps": prof_steps, "profiled_ms_per_step": 1000.0 * p_elapsed / prof_steps,
             "step_algorithmic_GBps": tot_step_bytes / elapsed / 1e9,
             "step_frac": tot_step_bytes / elapsed / 1e9 / HBM_PEAK_GBS,
+            "measured_copy_GBps": copy_gbps,
+            "frac_of_measured_copy": achieved / copy_gbps if copy_gbps else None,
         }
 
     sim.profile(False)

@@ -389,8 +389,10 @@ typedef struct mps_kernel_stat {
     double total_ms;
 } mps_kernel_stat;
 int mps_profile_enable(mps_ctx* ctx, int32_t on);
-/* Calibration for hardware byte counters: `reps` launches of a plain streaming copy kernel
- * (k_calib_copy) moving `bytes` from one device buffer to another on the context's stream. */
+/* Copies of `bytes` from one device buffer to another on the context's stream, `reps` times
+ * each: first k_calib_copy (a plain grid-stride copy, the reference for scaling hardware byte
+ * counters), then k_stream_copy (one 16-byte vector per thread, non-temporal: the copy rate of
+ * the memory system). With kernel timing on, both appear in mps_profile_read. */
 int mps_calibration_copy(mps_ctx* ctx, size_t bytes, int32_t reps);
 int mps_profile_read(mps_ctx* ctx, mps_kernel_stat* out, size_t capacity, size_t* count);
 

@@ -250,14 +250,15 @@ struct mps_ctx {
     enum KId {
         K_KEYS, K_SORT, K_CELLS, K_NEI_FLUID, K_NEI_BND, K_BSUM, K_BVOL, K_DENSITY, K_EVAL_DF, K_EVAL_CD,
         K_APPLY, K_ST_FACTOR, K_CG_SETUP, K_CG_INIT, K_CG_APPLY, K_CG_UPDATE, K_CG_DIR, K_COPY_OUT, K_ADVECT,
-        K_ZSORT, K_JACOBI, K_FOLD, K_REORDER, K_HALO, K_COUNT
+        K_ZSORT, K_JACOBI, K_FOLD, K_REORDER, K_HALO, K_CALIB, K_STREAM, K_COUNT
     };
     static const char* kname(int k) {
         static const char* names[K_COUNT] = {
             "k_cell_keys", "radix_sort", "k_cell_table", "k_neighbors_fluid", "k_neighbors_boundary",
             "k_boundary_sum", "k_boundary_volume", "k_density_alpha", "k_pressure_eval<DF>", "k_pressure_eval<CD>",
             "k_pressure_apply", "k_st_factor", "k_cg_setup", "k_cg_apply<0>", "k_cg_apply<1>", "k_cg_update",
-            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi", "k_fold", "reorder", "k_halo_build"};
+            "k_cg_direction", "k_copy_vel_out", "k_advect", "zsort", "jacobi", "k_fold", "reorder", "k_halo_build",
+            "k_calib_copy", "k_stream_copy"};
         return names[k];
     }
     struct Rec {
@@ -2522,8 +2523,10 @@ int mps_calibration_copy(mps_ctx* c, size_t bytes, int32_t reps) {
         a.reserve(std::max<size_t>(n, 1));
         b.reserve(std::max<size_t>(n, 1));
         mps::launch::zero_words(reinterpret_cast<uint32_t*>(a.p), 4 * n, c->stream);
-        for (int32_t r = 0; r < reps; r++) mps::launch::calib_copy(a.p, b.p, n, c->stream);
+        for (int32_t r = 0; r < reps; r++) c->T_(mps_ctx::K_CALIB, [&] { mps::launch::calib_copy(a.p, b.p, n, c->stream); });
+        for (int32_t r = 0; r < reps; r++) c->T_(mps_ctx::K_STREAM, [&] { mps::launch::stream_copy(a.p, b.p, n, c->stream); });
         c->sync();
+        if (c->prof_on) c->prof_collect();
         a.release();
         b.release();
     });

@@ -245,6 +245,8 @@ void advect(const Scenes& sc, float4* pos, float4* vel, uint32_t* nonfinite, hip
 // resets the S solver records st[0..S) and sets *active = S
 void solver_reset(SolverDev* st, uint32_t S, uint32_t* active, hipStream_t s);
 void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s);
+// STREAM-style copy (one 16-byte vector per thread, non-temporal): the in-run copy rate
+void stream_copy(const float4* a, float4* b, size_t n, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s);

@@ -1293,6 +1293,16 @@ __global__ void k_solver_reset(SolverDev* stv, uint32_t S, uint32_t* active) {
 __global__ void __launch_bounds__(kBlock) k_calib_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) b[i] = a[i];
 }
+// STREAM-style copy for measuring what the memory system gives a copy: one 16-byte vector per
+// thread (no grid-stride loop), non-temporal loads and stores. Measured on MI355X: 6.7 TB/s
+// against 4.8 TB/s for the grid-stride k_calib_copy above, which stays as the byte-counter
+// reference (its loads and stores are ordinary ones, counted like the kernels' own).
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(kBlock) k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const v4f*>(a) + i), reinterpret_cast<v4f*>(b) + i);
+}
 __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -1463,6 +1473,9 @@ void solver_reset(SolverDev* st, uint32_t S, uint32_t* active, hipStream_t s) {
 }
 void calib_copy(const float4* a, float4* b, size_t n, hipStream_t s) {
     if (n) launch_k(k_calib_copy, 4096, kBlock, 0, s, a, b, n);
+}
+void stream_copy(const float4* a, float4* b, size_t n, hipStream_t s) {
+    if (n) launch_k(k_stream_copy, unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s, a, b, n);
 }
 void zero_words(uint32_t* p, size_t n, hipStream_t s) {
     if (n) launch_k(k_zero_words, unsigned((n + kBlock - 1) / kBlock), kBlock, 0, s, p, n);
