@@ -48,7 +48,18 @@ constexpr int kFoldThreads = 1024;
 __device__ __forceinline__ double fold(const double* __restrict__ partials, uint32_t n) {
     __shared__ double lds[kFoldThreads / 64];
     double s = 0.0;
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s += partials[k];
+    constexpr uint32_t R = 8;  // loads in flight per thread; the adds keep the index order
+    for (uint32_t k0 = threadIdx.x; k0 < n; k0 += R * blockDim.x) {
+        double v[R];
+#pragma unroll
+        for (uint32_t q = 0; q < R; q++) {
+            const uint32_t k = k0 + q * blockDim.x;
+            v[q] = k < n ? partials[k] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < R; q++)
+            if (k0 + q * blockDim.x < n) s += v[q];
+    }
     return block_sum_d(s, lds);
 }
 
