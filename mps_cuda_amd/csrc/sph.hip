@@ -33,6 +33,28 @@ __device__ __forceinline__ uint32_t nbr_fb(const Nbr& nb, uint32_t k, uint32_t i
     return nb.fb[size_t(k) * N + i];
 }
 
+// Walks the wall list of slot i in list order, kWallBatch entries at a time: the batch's
+// indices are loaded together, then gather(j, slot) fetches each entry's wall record, and only
+// then does use(k, j, slot, record) consume them in order. A batch costs about one dependent
+// memory latency instead of two per entry.
+constexpr uint32_t kWallBatch = 4;
+template <class Gather, class Use>
+__device__ __forceinline__ void walk_walls(const Nbr& nb, uint32_t i, uint32_t N, uint32_t cb, Gather&& gather, Use&& use) {
+    using Rec = decltype(gather(0u, size_t(0)));
+    for (uint32_t k0 = 0; k0 < cb; k0 += kWallBatch) {
+        uint32_t j[kWallBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kWallBatch; q++) j[q] = k0 + q < cb ? nbr_fb(nb, k0 + q, i, N) : 0u;
+        Rec w[kWallBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kWallBatch; q++)
+            if (k0 + q < cb) w[q] = gather(j[q], size_t(k0 + q) * N + i);
+#pragma unroll
+        for (uint32_t q = 0; q < kWallBatch; q++)
+            if (k0 + q < cb) use(k0 + q, j[q], size_t(k0 + q) * N + i, w[q]);
+    }
+}
+
 // Block-reduce `v` (double) in a fixed order and publish the workgroup's partial.
 __device__ __forceinline__ void block_partial(double v, double* __restrict__ partials, uint32_t blk) {
     __shared__ double lds[kBlock / 64];
@@ -248,33 +270,37 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
         f3 gb{0.f, 0.f, 0.f};
         float sbo[kMaxWallObjects] = {0.f, 0.f, 0.f, 0.f};  // per wall object (BOBJ)
         f3 gbo[kMaxWallObjects] = {};
-        const uint32_t cb = nb.cfb[i];
-        for (uint32_t k = 0; k < cb; k++) {
-            const uint32_t j = nbr_fb(nb, k, i, N);
-            const f3 xij = xi - mk(bpos[j]);
+        struct WallRec {
+            float4 x;
+            float vb;
+            uint32_t o;
+        };
+        walk_walls(nb, i, N, nb.cfb[i],
+                   [&](uint32_t j, size_t) { return WallRec{bpos[j], bvol[j], BOBJ ? nb.fbo[j] : 0u}; },
+                   [&](uint32_t, uint32_t, size_t slot, const WallRec& w) {
+            const f3 xij = xi - mk(w.x);
             const float dist = len3(xij);
-            const float vb = bvol[j];
+            const float vb = w.vb;
             if constexpr (BOBJ) {
-                const uint32_t o = nb.fbo[j];
                 if (dens) {
-                    const float w = vb * Wk<CUBIC>(dist, invHi);
-                    at_obj(sbo, o, [&](float& a) { a += w; });
+                    const float wk = vb * Wk<CUBIC>(dist, invHi);
+                    at_obj(sbo, w.o, [&](float& a) { a += wk; });
                 }
                 if (alph || pairs) {
                     const float G = Gk<CUBIC>(dist, invHi);
-                    if (pairs) pG_fb[size_t(k) * N + i] = G;
+                    if (pairs) pG_fb[slot] = G;
                     const f3 gg = (vb * G) * xij;
-                    at_obj(gbo, o, [&](f3& a) { a = a + gg; });
+                    at_obj(gbo, w.o, [&](f3& a) { a = a + gg; });
                 }
             } else {
                 if (dens) sb += vb * Wk<CUBIC>(dist, invHi);
                 if (alph || pairs) {
                     const float G = Gk<CUBIC>(dist, invHi);
-                    if (pairs) pG_fb[size_t(k) * N + i] = G;
+                    if (pairs) pG_fb[slot] = G;
                     gb = gb + (vb * G) * xij;
                 }
             }
-        }
+        });
         // ((own + each ref object's sum, in scene order) + walls), as the reference's Sub
         // kernels accumulate them one after the other
         float d = 0.0f;
@@ -439,19 +465,25 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
         }
         float db = 0.0f;
         float dbo[kMaxWallObjects] = {0.f, 0.f, 0.f, 0.f};
-        const uint32_t cb = nb.cfb[i];
-        for (uint32_t k = 0; k < cb; k++) {
-            const size_t slot = size_t(k) * N + i;
-            const uint32_t j = nb.fb[slot];
-            const f3 xij = xi - mk(bpos[j]);
-            const f3 vij = vi - mk(bvel[j]);
+        struct WallRec {
+            float4 x, v;
+            float vb, G;
+            uint32_t o;
+        };
+        walk_walls(nb, i, N, nb.cfb[i],
+                   [&](uint32_t j, size_t slot) {
+                       return WallRec{bpos[j], bvel[j], bvol[j], pG_fb[slot], BOBJ ? nb.fbo[j] : 0u};
+                   },
+                   [&](uint32_t, uint32_t, size_t, const WallRec& w) {
+            const f3 xij = xi - mk(w.x);
+            const f3 vij = vi - mk(w.v);
             if constexpr (BOBJ) {
-                const float t = bvol[j] * pG_fb[slot] * dot3(xij, vij);
-                at_obj(dbo, nb.fbo[j], [&](float& a) { a += t; });
+                const float t = w.vb * w.G * dot3(xij, vij);
+                at_obj(dbo, w.o, [&](float& a) { a += t; });
             } else {
-                db += bvol[j] * pG_fb[slot] * dot3(xij, vij);
+                db += w.vb * w.G * dot3(xij, vij);
             }
-        }
+        });
         float d = 0.0f;
         d += df;
         if constexpr (REF) {  // ComputeDensityDelta ref (MPSSPHDFSPHUtil.cuh:177-216): one sum per ref object
@@ -625,15 +657,19 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     }
     f3 fb{0.f, 0.f, 0.f};
     f3 fbo[kMaxWallObjects] = {};
-    const uint32_t cb = nb.cfb[i];
-    for (uint32_t k = 0; k < cb; k++) {
-        const size_t slot = size_t(k) * N + i;
-        const uint32_t j = nb.fb[slot];
-        const f3 xij = xi - mk(bpos[j]);
-        const f3 fij = 0.5f * bvol[j] * pi * pG_fb[slot] * xij;
-        if constexpr (BOBJ) at_obj(fbo, nb.fbo[j], [&](f3& a) { a = a - fij; });
+    struct WallRec {
+        float4 x;
+        float vb, G;
+        uint32_t o;
+    };
+    walk_walls(nb, i, N, nb.cfb[i],
+               [&](uint32_t j, size_t slot) { return WallRec{bpos[j], bvol[j], pG_fb[slot], BOBJ ? nb.fbo[j] : 0u}; },
+               [&](uint32_t, uint32_t, size_t, const WallRec& w) {
+        const f3 xij = xi - mk(w.x);
+        const f3 fij = 0.5f * w.vb * pi * w.G * xij;
+        if constexpr (BOBJ) at_obj(fbo, w.o, [&](f3& a) { a = a - fij; });
         else fb = fb - fij;
-    }
+    });
     f3 F{0.f, 0.f, 0.f};
     F = F + ff;
     if constexpr (REF) {  // ApplyDFSPH ref (MPSSPHDFSPHUtil.cuh:379-418): one sum per ref object
