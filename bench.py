@@ -104,9 +104,22 @@ def step_bytes(N, B, Eff, Efb, Ebb, l_df, l_cd, l_cg, cap, C):
     return b
 
 
+def _host_cpu() -> dict:
+    """What the CPU baseline ran on: logical CPUs of the host, those this process may use, and
+    the model name from /proc/cpuinfo."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), None)
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"logical_cpus": os.cpu_count(), "usable_cpus": usable, "model": model}
+
+
 def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
-    """FP32 CPU oracle on the same workload: one untimed step, then timed steps until the
-    budget is used (at least one)."""
+    """FP32 CPU oracle on the same workload: one untimed step, then timed steps with every
+    OpenMP thread until the budget is used (at least one), then one more step on one thread."""
     from mps_cuda_amd import scenes
     from oracle import binding
 
@@ -121,10 +134,18 @@ def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
         t_used += time.perf_counter() - t0
         n_done += 1
     rate = scene.n * n_done / t_used
+    binding.set_threads(1)
+    t0 = time.perf_counter()
+    o.step()
+    t_one = time.perf_counter() - t0
+    binding.set_threads(threads)
     return {
         "value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
         "sample": f"FP32 oracle (oracle/sph_oracle.cpp, OpenMP {threads} threads) on the same "
-                  f"{scene.n}-particle workload: steps 1..{n_done} after one untimed step, {t_used:.1f} s",
+                  f"{scene.n}-particle workload: steps 1..{n_done} after one untimed step, {t_used:.1f} s; "
+                  f"then step {n_done + 1} on one thread, {t_one:.1f} s",
+        "single_thread_value": scene.n / t_one,
+        "host": _host_cpu(),
     }
 
 
@@ -261,8 +282,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-side", type=int, default=100, help="fluid block edge in particles (default 100 -> 1M)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU oracle work")
-    ap.add_argument("--cpu-max-steps", type=int, default=3)
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (all threads)")
+    ap.add_argument("--cpu-max-steps", type=int, default=60)
     ap.add_argument("--no-profile", action="store_true", help="do not time kernels with HIP events")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="steps of the profiled replay that times every kernel (0 = as many as --steps)")
