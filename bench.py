@@ -289,7 +289,7 @@ def main() -> int:
                     help="steps of the profiled replay that times every kernel (0 = as many as --steps)")
     ap.add_argument("--kernels", action="store_true", help="print the per-kernel table to stderr")
     ap.add_argument("--batch-scenes", type=int, default=64, help="scenes of the batched leg (0 = skip it)")
-    ap.add_argument("--batch-steps", type=int, default=20, help="timed steps of the batched leg")
+    ap.add_argument("--batch-steps", type=int, default=300, help="timed steps of the batched leg")
     ap.add_argument("--adjoint-particles", type=int, default=65536, help="droplet of the adjoint leg (0 = skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--graphs", default="auto", choices=("auto", "0", "1"),
