@@ -152,22 +152,23 @@ _SIGS = {
     "mps_adj_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
 }
 
-_lib = None
+_libs: dict[str, C.CDLL] = {}
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:
-    """Load libmps_sph.so; raises if it has not been built (no fallback path exists)."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(path):
+    """Load libmps_sph.so (once per path); raises if it has not been built (no fallback path
+    exists)."""
+    key = os.path.abspath(path)
+    if key in _libs:
+        return _libs[key]
+    if not os.path.exists(key):
         raise RuntimeError(f"{path} is missing: build it with `python -m mps_cuda_amd.build`")
-    lib = C.CDLL(path)
+    lib = C.CDLL(key)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    _libs[key] = lib
     return lib
 
 

@@ -51,3 +51,10 @@ def test_invalid_config_rejected():
     c.hash_size[0] = 48  # not a power of two
     with pytest.raises(MPSError):
         HipSim(c)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No fallback path: a missing libmps_sph.so is an error at load time."""
+    from mps_cuda_amd import _capi
+    with pytest.raises(RuntimeError, match="is missing"):
+        _capi.load(str(tmp_path / "libmps_sph.so"))
