@@ -139,12 +139,30 @@ def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
     o.step()
     t_one = time.perf_counter() - t0
     binding.set_threads(threads)
+    del o
+    # the reference's CPU-runnable workload (32^3 dam break): steady state over steps 50..499,
+    # or as many of them as fit the budget
+    small = scenes.dam_break_cpu()
+    q = binding.OracleSim(scenes.make_config(small), 32)
+    scenes.load_into(q, small)
+    for _ in range(50):
+        q.step()
+    n_small, t_small = 0, 0.0
+    while 50 + n_small < small.steps and t_small < budget_s:
+        t0 = time.perf_counter()
+        q.step()
+        t_small += time.perf_counter() - t0
+        n_small += 1
     return {
         "value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
         "sample": f"FP32 oracle (oracle/sph_oracle.cpp, OpenMP {threads} threads) on the same "
                   f"{scene.n}-particle workload: steps 1..{n_done} after one untimed step, {t_used:.1f} s; "
                   f"then step {n_done + 1} on one thread, {t_one:.1f} s",
         "single_thread_value": scene.n / t_one,
+        "small_workload": {
+            "value": small.n * n_small / t_small, "unit": "particle-steps/s", "cores": threads,
+            "sample": f"{small.name}, {small.n} particles: steps 50..{49 + n_small} after 50 untimed steps, {t_small:.1f} s",
+        },
         "host": _host_cpu(),
     }
 
