@@ -1,7 +1,8 @@
 """The C++ classes of include/mps_system.hpp driven like the reference's application
 (System::Initalize, the simulation keys, OnUpdate per frame; MPSSystem.cpp:19-291) give the
 same state, bit for bit, as the same scene loaded and stepped through the Python handle: the
-header adds no arithmetic of its own. The driver runs as a child process."""
+header adds no arithmetic of its own. The C++ program (tests/cpp/facade_scene.cpp) runs as a
+child process."""
 import os
 import subprocess
 import sys
