@@ -16,6 +16,12 @@
  *                  and rounded to T. Boundary sampling always runs in double and is stored as T.
  * Arrays exchanged through orc_get_array/orc_set_array are in the working precision
  * (float or double); colours are always float; ids/keys are uint32.
+ *
+ * Parity unpinned: the order of equal keys in the reference's thrust::sort_by_key
+ * (MPSSpatialHash.cu:69,94,116,132; the Thrust of CUDA 12.6, not vendored with the reference).
+ * It decides the ids within a cell and the Z-sorted payload order. This oracle defines it as
+ * stable (ties by ascending prior index), checked against a brute-force stable sort; no
+ * reference test or fixture holds the reference's own order.
  */
 #ifndef MPS_SPH_ORACLE_H
 #define MPS_SPH_ORACLE_H
