@@ -82,19 +82,22 @@ def test_step_parity_reference_scene_multi_step():
     _say(f"[parity] 60 steps max pos err ok")
 
 
-@pytest.mark.parametrize("solver", ["cg", "jacobi"])
+@pytest.mark.parametrize("solver", ["cg", "jacobi", "cg_poly6"])
 def test_cpu_workload_free_run(solver):
     """The reference's CPU-runnable workload at its full size and length: the 32^3 = 32,768
     particle dam break (0.02 m lattice, +-0.002 m jitter, 1 m box, h = 0.04 m, dt = 1 ms) for
     500 steps, free-running on the GPU and in the FP32 oracle from the same state, with the
     reference's live CG viscosity solve and with the Jacobi solve the workload's description
-    names. Every step the DF, CD and viscosity loops end at the same iteration and both sides
-    Z-sort at the same step; every 10 steps positions and velocities agree within 1e-5."""
+    names, and with the reference's default poly6/spiky kernels instead of the cubic spline.
+    Every step the DF, CD and viscosity loops end at the same iteration and both sides Z-sort at
+    the same step; every 10 steps positions and velocities agree within 1e-5."""
     import time
     from mps_cuda_amd import _capi as capi
     sc = scenes.dam_break_cpu()
     if solver == "jacobi":
         sc.config["visc_solver"] = capi.VISC_JACOBI
+    if solver == "cg_poly6":
+        sc.config["kernel"] = capi.KERNEL_POLY6_SPIKY
     g, o = make_pair(sc)
     t0 = time.perf_counter()
     first_diff, worst = None, 0.0
