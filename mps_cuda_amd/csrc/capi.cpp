@@ -2168,6 +2168,12 @@ int guarded(mps_ctx* c, F&& f) {
     } catch (const std::bad_alloc&) {
         c->err = "host allocation failed";
         return MPS_ERR_NOMEM;
+    } catch (const std::exception& e) {  // nothing may cross the C ABI
+        c->err = std::string("internal error: ") + e.what();
+        return MPS_ERR_STATE;
+    } catch (...) {
+        c->err = "internal error";
+        return MPS_ERR_STATE;
     }
 }
 
