@@ -91,6 +91,10 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 // memory-side cache, while the larger per-pair coefficient streams are read non-temporally.
 constexpr uint32_t kListBatch = 4;       // once-per-step walks (density/alpha, ST factor, CG setup)
 constexpr uint32_t kListBatchSolve = 2;  // solver iterations (DF/CD evaluate and apply, A.p): measured best
+// A.p loads its first entries before staging the halo (a multiple of kListBatchSolve; 16 keeps
+// the kernel at five workgroups per CU: VGPRs stay under what the LDS already allows)
+constexpr uint32_t kListPrefetchApply = 16;
+static_assert(kListPrefetchApply % kListBatchSolve == 0, "the batches after the prefetch stay aligned");
 
 // Calls body(t, s) for every halo record t < H (s = its slot), with R slots per thread loaded
 // ahead so the bodies' loads can be in flight together.
@@ -963,7 +967,20 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             }
         }
     };
-    fetch(0);
+    // The first kListPrefetchApply entries are loaded before the halo staging: their stream
+    // overlaps the staging's gathers instead of following them.
+    constexpr uint32_t kPre = kListPrefetchApply;
+    uint32_t jp[kPre];
+    float2 abp[kPre];
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; q++) {
+        if (q < c) {
+            const size_t slot = size_t(q) * N + i;
+            jp[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
+            const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
+            abp[q] = __builtin_bit_cast(float2, d);
+        }
+    }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
         for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
@@ -984,8 +1001,21 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             const f3 pij = pi - pj;
             acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
         };
-        // batches in list order; within a batch the pairs are summed in list order too
-        for (uint32_t k = 0; k < c; k += U) {
+        if (kPre < c) fetch(kPre);
+        // the prefetched entries, then batches, all in list order
+#pragma unroll
+        for (uint32_t q = 0; q < kPre; q++) {
+            if (q < c) {
+                if (staged) {
+                    const float4 a = la[jp[q]];
+                    const float2 e = lb[jp[q]];
+                    pair(abp[q], mk(a), f3{a.w, e.x, e.y});
+                } else {
+                    pair(abp[q], mk(pos[jp[q]]), vec(jp[q]));
+                }
+            }
+        }
+        for (uint32_t k = kPre; k < c; k += U) {
             uint32_t j[U];
             float2 ab[U];
 #pragma unroll
