@@ -9,11 +9,14 @@ viscosity 0.01, surface tension 0.05, FP32.
 A "step" is one System::OnUpdate: hash, neighbour lists, boundary volume, density/alpha,
 divergence-free solve, viscosity + surface-tension CG, constant-density solve, advection.
 `value` = fluid particles x timed steps / wall time of the timed steps (inputs resident in
-HBM). The timed steps are steps 0..K-1 of the workload: the W warm-up steps run first (then
-the library's autotuner picks graph replay and the host-poll interval) and the scene is
-reloaded into the same context (no device allocation inside the region). The timed region
-has no per-kernel events; the same steps then run again from the initial state with HIP
-events around every kernel on the context's stream, which gives the per-kernel times.
+HBM). The W warm-up steps run first (then the library's autotuner picks graph replay and the
+host-poll interval); the scene is then reloaded and advanced --settle steps untimed (default
+500: past the start-up transient, where the lattice collapses from 0.52 rho0 towards rest
+density, the neighbour counts rise from ~28 to ~40 per particle and the CG iteration count from
+~13 to ~50), and that state is kept. The timed steps are steps settle..settle+K-1 from it (no
+device allocation inside the region, no per-kernel events); the same steps then run again from
+the kept state with HIP events around every kernel on the context's stream, which gives the
+per-kernel times. The CPU baseline steps from the same kept state.
 
 Prints ONE JSON line. Extra objects: `roofline` (dominant kernel: algorithmic bytes from the
 measured neighbour counts and iteration counts / its HIP-event time; `traffic` = its HBM bytes
@@ -117,15 +120,42 @@ def _host_cpu() -> dict:
     return {"logical_cpus": os.cpu_count(), "usable_cpus": usable, "model": model}
 
 
-def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
-    """FP32 CPU oracle on the same workload: one untimed step, then timed steps with every
-    OpenMP thread until the budget is used (at least one), then one more step on one thread."""
+SNAP_KEYS = ("FLUID_POS", "FLUID_VEL", "FLUID_RADIUS", "FLUID_MASS", "BND_POS", "BND_RADIUS", "BND_MASS", "BND_VEL",
+             "BND_VOLUME")
+
+
+def snapshot(sim) -> dict:
+    """Everything a step reads that earlier steps wrote, in reference order: the fluid, the
+    walls in their current order with their never-cleared volumes, and the frame counter."""
+    snap = {k: sim.get(k).copy() for k in SNAP_KEYS}
+    snap["frame"] = sim.frame
+    return snap
+
+
+def restore(sim, scene, snap) -> None:
+    """Put a snapshot back (same sizes: no device allocation); stepping on from it gives the
+    same results bit for bit as stepping on from the state it was taken of."""
+    p, m = scene.physics, scene.material
+    sim.set_physics(p["dt"], p["gravity"], p["box_min"], p["box_max"])
+    sim.set_fluid_material(m["radius"], m["density"], m["viscosity"], m["xsph"], m["surface_tension"])
+    sim.upload_boundary(snap["BND_POS"], snap["BND_RADIUS"], snap["BND_MASS"])
+    sim.set("BND_VEL", snap["BND_VEL"])
+    sim.set("BND_VOLUME", snap["BND_VOLUME"])
+    sim.upload_fluid(snap["FLUID_POS"], snap["FLUID_RADIUS"], snap["FLUID_VEL"], snap["FLUID_MASS"])
+    sim.frame = snap["frame"]
+
+
+def cpu_baseline(scene, snap, budget_s: float, max_steps: int) -> dict:
+    """FP32 CPU oracle on the same workload and window: from the kept state `snap`, one untimed
+    step, then timed steps with every OpenMP thread until the budget is used (at least one),
+    then one more step on one thread."""
     from mps_cuda_amd import scenes
     from oracle import binding
 
     threads = binding.set_threads(0)
     o = binding.OracleSim(scenes.make_config(scene), 32)
-    scenes.load_into(o, scene)
+    restore(o, scene, snap)
+    f0 = snap["frame"]
     o.step()
     n_done, t_used = 0, 0.0
     while n_done < max_steps and (n_done == 0 or t_used < budget_s):
@@ -156,8 +186,8 @@ def cpu_baseline(scene, budget_s: float, max_steps: int) -> dict:
     return {
         "value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
         "sample": f"FP32 oracle (oracle/sph_oracle.cpp, OpenMP {threads} threads) on the same "
-                  f"{scene.n}-particle workload: steps 1..{n_done} after one untimed step, {t_used:.1f} s; "
-                  f"then step {n_done + 1} on one thread, {t_one:.1f} s",
+                  f"{scene.n}-particle workload from the same kept state: steps {f0 + 1}..{f0 + n_done} after "
+                  f"one untimed step, {t_used:.1f} s; then step {f0 + n_done + 1} on one thread, {t_one:.1f} s",
         "single_thread_value": scene.n / t_one,
         "small_workload": {
             "value": small.n * n_small / t_small, "unit": "particle-steps/s", "cores": threads,
@@ -185,6 +215,9 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
     scenes.load_batch(bat, sc, walls)
     for _ in range(warmup):
         bat.step()
+    # launch mode (graph replay of solver chunks or direct launches, host-poll interval) picked
+    # by the library's autotuner on this batch; the timed steps use it
+    tuning = bat.autotune(2)
     bat.synchronize()
     l_cg = []
     t0 = time.perf_counter()
@@ -217,6 +250,8 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
         "value": v_bat, "ms_per_step": 1000.0 * t_bat / steps,
         "plain_value": v_plain, "plain_ms_per_step": 1000.0 * t_plain / steps,
         "speedup": v_bat / v_plain, "l_cg_max_mean": sum(l_cg) / len(l_cg),
+        "timed_launch_mode": {"use_graphs": tuning["use_graphs"], "host_sync_every": tuning["host_sync_every"]},
+        "solver_tuning": tuning,
     }
 
 
@@ -224,9 +259,9 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
     """The differentiable droplet: a sphere of n particles falling for `steps` steps; target
     from viscosity 0.02, surface tension 0.3 and a smooth seeded initial velocity; loss = MSE of
     the final positions. At the guess (0.025, 0.1, same initial velocity) time one plain forward,
-    the checkpointed rollout and the backward pass, and compare the gradients with central
-    finite differences of the FP32 forward itself (whose rounding noise is about 1e-3 of these
-    small loss differences; the FP64 check is a GPU test)."""
+    the checkpointed rollout and the backward pass. The gradients are checked against central
+    differences of the FP64 oracle's forward by the GPU tests (tests/test_gpu_adjoint.py), not
+    here: finite differences of this FP32 forward are limited by its rounding."""
     from mps_cuda_amd import HipSim, scenes
 
     sc = scenes.diff_droplet(n=n, steps=steps, seed=seed)
@@ -268,27 +303,15 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
     t_back = time.perf_counter() - t0
     print(f"adjoint leg: forward {1000 * t_fwd / steps:.2f} ms/step, rollout {1000 * t_roll / steps:.2f}, "
           f"backward {1000 * t_back / steps:.2f}", file=sys.stderr, flush=True)
-    # central differences of the FP32 forward: eps = 1e-3 relative for the coefficients, 1e-3 m/s
-    # along random unit vectors (one per particle) for the initial velocities
-    eps = 1.0e-3
-    dv = np.random.default_rng(seed + 7).normal(size=v0.shape)
-    dv /= np.linalg.norm(dv, axis=1, keepdims=True)
-    em, es = eps * mu0, eps * sig0
-    fd = {
-        "viscosity": (loss(forward(mu0 + em, sig0, v0)[0]) - loss(forward(mu0 - em, sig0, v0)[0])) / (2 * em),
-        "surface_tension": (loss(forward(mu0, sig0 + es, v0)[0]) - loss(forward(mu0, sig0 - es, v0)[0])) / (2 * es),
-        "v0_direction": (loss(forward(mu0, sig0, (v0 + eps * dv).astype(np.float32))[0])
-                         - loss(forward(mu0, sig0, (v0 - eps * dv).astype(np.float32))[0])) / (2 * eps),
-    }
-    adj = {"viscosity": float(gmu[0]), "surface_tension": float(gsig[0]),
-           "v0_direction": float(np.sum(gv0.astype(np.float64) * dv))}
-    checks = {k: {"adjoint": adj[k], "fd": fd[k], "rel_err": abs(adj[k] - fd[k]) / max(abs(fd[k]), 1e-30)} for k in fd}
     return {
         "particles": n, "steps": steps, "checkpoint_every": every, "loss": loss(xr),
         "forward_ms_per_step": 1000 * t_fwd / steps, "rollout_ms_per_step": 1000 * t_roll / steps,
         "backward_ms_per_step": 1000 * t_back / steps, "adjoint_over_forward": t_back / t_fwd,
         "rollout_plus_backward_over_forward": (t_roll + t_back) / t_fwd,
-        "l_cg_mean": float(counts[:, 0, 1].mean()), "fd_eps": eps, "gradient_checks": checks,
+        "l_cg_mean": float(counts[:, 0, 1].mean()),
+        "gradients": {"viscosity": float(gmu[0]), "surface_tension": float(gsig[0]),
+                      "v0_norm": float(np.linalg.norm(gv0))},
+        "gradient_check": "tests/test_gpu_adjoint.py: central differences (eps 1e-3) of the FP64 oracle forward",
         "solver_tuning": tuning,
     }
 
@@ -298,6 +321,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--settle", type=int, default=500,
+                    help="untimed steps from the initial state before the timed window (past the start-up transient)")
     ap.add_argument("--n-side", type=int, default=100, help="fluid block edge in particles (default 100 -> 1M)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU oracle work (all threads)")
@@ -314,8 +339,9 @@ def main() -> int:
                     help="replay solver iteration chunks from HIP graphs: auto = the library's autotuner "
                          "picks graphs on/off and the host-poll interval after the warmup")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="after the run, launch the library's plain copy kernel (k_calib_copy) 3 times on "
-                         "512 MiB so hardware byte counters can be scaled against a copy of known size")
+                    help="launch the library's plain copy kernel (k_calib_copy) 3 times on 512 MiB right before "
+                         "and right after the timed window: hardware byte counters are scaled against a copy of "
+                         "known size, and the copies mark the window in a kernel trace")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -334,11 +360,18 @@ def main() -> int:
     for _ in range(args.warmup):
         sim.step()
     tuned = sim.autotune(2) if args.graphs == "auto" else None
+    scenes.load_into(sim, scene)
+    for _ in range(args.settle):
+        sim.step()
+    kept = snapshot(sim)
+    say(f"settled: {args.settle} untimed steps, frame {kept['frame']}")
 
-    def run(steps: int, profile: bool):
-        """Steps 0..steps-1 of the workload from its initial state: (seconds, per-step stats,
-        halo stats, per-kernel HIP-event times when profiling)."""
-        scenes.load_into(sim, scene)
+    def run(steps: int, profile: bool, mark: bool = False):
+        """Steps settle..settle+steps-1 of the workload from the kept state: (seconds, per-step
+        stats, halo stats, per-kernel HIP-event times when profiling)."""
+        restore(sim, scene, kept)
+        if mark:
+            sim.calibration_copy(COPY_BYTES, 3)
         sim.synchronize()
         sim.profile(profile)
         stats, halo = [], []
@@ -351,13 +384,16 @@ def main() -> int:
             halo.append((st.halo_max, st.halo_spill, st.halo_blocks))
         sim.synchronize()
         elapsed = time.perf_counter() - t0
+        if mark:
+            sim.calibration_copy(COPY_BYTES, 3)
+            sim.synchronize()
         prof = sim.profile_read() if profile else {}
         sim.profile(False)
         return elapsed, stats, halo, prof
 
     N = scene.n
     # the throughput: no per-kernel events (they add ~1.5 ms per step on this workload)
-    elapsed, stats, halo, _ = run(args.steps, False)
+    elapsed, stats, halo, _ = run(args.steps, False, args.copy_calibration)
     value = N * args.steps / elapsed
     say(f"timed region: {args.steps} steps, {1000.0 * elapsed / args.steps:.3f} ms per step")
     # per-kernel times: the same steps again from the initial state, with HIP events around
@@ -436,7 +472,7 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(scene, args.cpu_budget, args.cpu_max_steps)
+        cpu = cpu_baseline(scene, kept, args.cpu_budget, args.cpu_max_steps)
         say(f"cpu baseline: {cpu['value']:.3e} particle-steps/s")
 
     n = len(stats)
@@ -457,18 +493,15 @@ def main() -> int:
         "config": {
             "workload": f"plain DFSPH dam break, {N} fluid + {B} wall particles, 0.01 m lattice, "
                         f"3.0x1.5x1.5 m tank, cubic kernel h=0.02 m, dt=0.5 ms, CD tol 1e-3, DF tol 1e-2, "
-                        f"viscosity 0.01 + surface tension 0.05 (CG), steps 0..{args.steps - 1}",
+                        f"viscosity 0.01 + surface tension 0.05 (CG), steps {args.settle}..{args.settle + args.steps - 1} "
+                        f"(timed after {args.settle} untimed steps from the initial lattice)",
+            "window": [args.settle, args.settle + args.steps - 1],
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
             "solver_tuning": tuned,
         },
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,
         "adjoint": adjoint,
     }
-    if args.copy_calibration:
-        sim = HipSim(cfg)
-        sim.calibration_copy(512 * 1024 * 1024, 3)
-        sim.close()
-        print("copy calibration: 3 x k_calib_copy of 536870912 bytes", file=sys.stderr)
     if args.kernels:
         for name, k in sorted(kernels.items(), key=lambda kv: -kv[1]["total_ms"]):
             print(f"{name:24s} launches={k['launches']:6d} useful={k['useful']:6d} total={k['total_ms']:9.2f} ms "

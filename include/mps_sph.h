@@ -358,12 +358,26 @@ int mps_scene_set_material(mps_ctx* ctx, uint32_t scene, const mps_sph_material*
 /* Step n_steps times from the current state, keeping a checkpoint every checkpoint_every
  * steps. loop_counts (may be NULL): n_steps * scenes * 3 values (l_df, l_cg, l_cd per scene). */
 int mps_adj_rollout(mps_ctx* ctx, uint32_t n_steps, uint32_t checkpoint_every, uint32_t* loop_counts);
+/* mps_adj_rollout that also returns the states after steps obs_step[0..n_obs) (each in
+ * 1..n_steps): positions into pos_out and velocities into vel_out (may be NULL), n_obs blocks of
+ * n*3 floats in the reference order of each state. */
+int mps_adj_rollout_traj(mps_ctx* ctx, uint32_t n_steps, uint32_t checkpoint_every, uint32_t n_obs,
+                         const uint32_t* obs_step, float* pos_out, float* vel_out, uint32_t* loop_counts);
 /* Gradients of a loss L(final positions, final velocities) given dL/dpos (n*3, reference
  * order) and dL/dvel (n*3 or NULL): dL/d(velocities at the rollout's start) into dloss_dvel0
  * (n*3), and per scene dL/dviscosity and dL/dsurface_tension (may be NULL). Steps are
  * recomputed from the checkpoints; the context is left at the rollout's final state. */
 int mps_adj_backward(mps_ctx* ctx, const float* dloss_dpos, const float* dloss_dvel, float* dloss_dvel0,
                      double* dloss_dviscosity, double* dloss_dsurface_tension);
+/* The same for a loss of the whole trajectory: the states after steps obs_step[0..n_obs) of the
+ * rollout (each in 1..n_steps; any order; a repeated step adds) with cotangents dloss_dpos
+ * (n_obs blocks of n*3, observation k at offset k*n*3, each in the reference order of its
+ * observed state) and dloss_dvel (same layout, or NULL). mps_adj_backward is the case n_obs = 1,
+ * obs_step = {n_steps}. MPS_ERR_STATE when there is no rollout, or when particles, walls,
+ * scenes, physics or materials were replaced since it. */
+int mps_adj_backward_traj(mps_ctx* ctx, uint32_t n_obs, const uint32_t* obs_step, const float* dloss_dpos,
+                          const float* dloss_dvel, float* dloss_dvel0, double* dloss_dviscosity,
+                          double* dloss_dsurface_tension);
 
 /* ---- autotuning ----------------------------------------------------------------------------
  * Times `steps` steps of the current state under each launch setting that leaves results

@@ -150,6 +150,8 @@ _SIGS = {
     "mps_adj_rollout": (C.c_int, [_P, C.c_uint32, C.c_uint32, _P]),
     "mps_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(TuneResult)]),
     "mps_adj_backward": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "mps_adj_backward_traj": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P, _P, _P]),
+    "mps_adj_rollout_traj": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P, _P, _P, _P]),
 }
 
 _libs: dict[str, C.CDLL] = {}
@@ -224,6 +226,7 @@ class HipSim:
         if rc:
             raise MPSError(rc, "mps_create failed")
         self.h = h
+        self.rollout_generation = 0  # bumped by every adj_rollout
 
     def close(self):
         if getattr(self, "h", None):
@@ -357,7 +360,21 @@ class HipSim:
         (l_df, l_cg, l_cd) of every step."""
         counts = np.zeros((n_steps, self.scene_count, 3), np.uint32)
         self._chk(self.lib.mps_adj_rollout(self.h, int(n_steps), int(checkpoint_every), _ptr(counts)))
+        self.rollout_generation += 1
         return counts
+
+    def adj_rollout_traj(self, n_steps: int, obs_steps, checkpoint_every: int = 10, velocities: bool = False):
+        """adj_rollout that also returns the positions (and velocities) after each step in
+        obs_steps (1..n_steps): (pos (k, n, 3), vel (k, n, 3) or None, loop counts)."""
+        n = self.get("FLUID_RADIUS").shape[0]
+        obs = np.ascontiguousarray(obs_steps, np.uint32).reshape(-1)
+        pos = np.zeros((obs.shape[0], n, 3), np.float32)
+        vel = np.zeros((obs.shape[0], n, 3), np.float32) if velocities else None
+        counts = np.zeros((n_steps, self.scene_count, 3), np.uint32)
+        self._chk(self.lib.mps_adj_rollout_traj(self.h, int(n_steps), int(checkpoint_every), obs.shape[0], _ptr(obs),
+                                                _ptr(pos), _ptr(vel), _ptr(counts)))
+        self.rollout_generation += 1
+        return pos, vel, counts
 
     def adj_backward(self, dloss_dpos, dloss_dvel=None):
         """(dL/dv0 (n, 3) float32, dL/dviscosity (scenes,), dL/dsurface_tension (scenes,))."""
@@ -368,6 +385,21 @@ class HipSim:
         gmu = np.zeros(self.scene_count, np.float64)
         gsig = np.zeros(self.scene_count, np.float64)
         self._chk(self.lib.mps_adj_backward(self.h, _ptr(gx), _ptr(gv), _ptr(out), _ptr(gmu), _ptr(gsig)))
+        return out, gmu, gsig
+
+    def adj_backward_traj(self, obs_steps, dloss_dpos, dloss_dvel=None):
+        """Trajectory loss: obs_steps (k,) in 1..n_steps, dloss_dpos / dloss_dvel (k, n, 3) at
+        those states. Returns (dL/dv0 (n, 3), dL/dviscosity (scenes,), dL/dsurface_tension)."""
+        n = self.get("FLUID_RADIUS").shape[0]
+        obs = np.ascontiguousarray(obs_steps, np.uint32).reshape(-1)
+        k = obs.shape[0]
+        gx = np.ascontiguousarray(dloss_dpos, np.float32).reshape(k, n, 3)
+        gv = None if dloss_dvel is None else np.ascontiguousarray(dloss_dvel, np.float32).reshape(k, n, 3)
+        out = np.zeros((n, 3), np.float32)
+        gmu = np.zeros(self.scene_count, np.float64)
+        gsig = np.zeros(self.scene_count, np.float64)
+        self._chk(self.lib.mps_adj_backward_traj(self.h, k, _ptr(obs), _ptr(gx), _ptr(gv), _ptr(out), _ptr(gmu),
+                                                 _ptr(gsig)))
         return out, gmu, gsig
 
     # ---- stepping

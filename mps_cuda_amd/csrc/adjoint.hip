@@ -673,6 +673,28 @@ __global__ void __launch_bounds__(kBlock) k_adj_unpermute(uint32_t n, const uint
     if (al) al_out[o] = al[s];
 }
 
+__global__ void __launch_bounds__(kBlock) k_adj_add_cotangent(uint32_t n, const uint32_t* __restrict__ perm,
+                                                            const float* __restrict__ gx, const float* __restrict__ gv,
+                                                            float4* __restrict__ adx, float4* __restrict__ adv) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t r = perm[s];
+    if (r == kEmpty) return;
+    const size_t o = 3 * size_t(r);
+    float4 a = adx[s];
+    a.x += gx[o];
+    a.y += gx[o + 1];
+    a.z += gx[o + 2];
+    adx[s] = a;
+    if (gv) {
+        float4 b = adv[s];
+        b.x += gv[o];
+        b.y += gv[o + 1];
+        b.z += gv[o + 2];
+        adv[s] = b;
+    }
+}
+
 }  // namespace dev
 
 // ===================================================================== host launchers
@@ -752,6 +774,11 @@ void adj_cg_init_warm(const Scenes& sc, const Nbr& nb, const float2* cab, const 
 }
 void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (sc.N) launch_k(k_adj_fold_start, sc.S, kBlock, 0, s, sc, partials, nbk(sc.N), st, active, tol);
+}
+
+void adj_add_cotangent(uint32_t n, const uint32_t* perm, const float* gx, const float* gv, float4* adx, float4* adv,
+                       hipStream_t s) {
+    if (n) launch_k(k_adj_add_cotangent, nbk(n), kBlock, 0, s, n, perm, gx, gv, adx, adv);
 }
 
 }  // namespace launch

@@ -135,13 +135,13 @@ struct mps_ctx {
         uint32_t Kff = 0, Kfb = 0;
         DBuf<float4> pos, bpos;
         DBuf<float> bvol;
-        DBuf<uint32_t> src, nff, cff, nfb, cfb;
+        DBuf<uint32_t> src, nff, cff, nfb, cfb, perm_end;
         std::vector<DBuf<float4>> snaps;
         uint32_t nsnap = 0, df_last = 0, cg_out = 0;
         std::vector<uint32_t> ldf, lcd;
         void release() {
             pos.release(); bpos.release(); bvol.release(); src.release(); nff.release(); cff.release();
-            nfb.release(); cfb.release();
+            nfb.release(); cfb.release(); perm_end.release();
             for (auto& b : snaps) b.release();
         }
     };
@@ -162,7 +162,16 @@ struct mps_ctx {
     std::vector<Checkpoint> ckpts;
     Checkpoint ck_end;
     uint32_t adj_n = 0, adj_every = 0;
+    // layout the rollout was recorded on: a backward on any other layout is refused
+    uint32_t adj_N = 0, adj_Nref = 0, adj_B = 0, adj_S = 0;
+    // Forget the recorded rollout: called by everything that replaces particles, walls, scenes
+    // or the parameters the recomputation would read.
+    void drop_rollout() {
+        adj_n = 0;
+        lam_valid = false;
+    }
     DBuf<uint32_t> perm0, d_L, rev;
+    DBuf<float> gobs;  // one observation's cotangents (Nref x 3 position, then velocity)
     DBuf<float4> adx, adv, adx2, adv2, aux4, gvec, avin, lam, lam2;
     bool lam_valid = false;  // lam holds the later step's transposed solution (warm start)
     DBuf<float> abar, rbar, dbar, fbar, fstr;
@@ -501,12 +510,18 @@ struct mps_ctx {
         // Morton key), as each reference object Z-sorts with its own hash
         const bool objs = boundary && nbobj > 1;
         const uint32_t* zscene = objs ? bobj.p : bnd_scene();
+        // every temporary first, so nothing is permuted unless the whole sort can run
+        tmp4.reserve(n);
+        if (boundary) {
+            tmpu.reserve(n);
+            tmp2.reserve(n);
+            if (!cfg.faithful_quirks) tmp1.reserve(n);
+        }
         T_(K_ZSORT, [&] { mps::launch::cell_keys(p4.p, n, g, 1, zscene, zb, zkey.p, zid.p, stream); });
         const int bits = zb + (objs ? bits_for(nbobj) : scene_bits());
         uint32_t *k = zkey.p, *v = zid.p, *k2 = zkey2.p, *v2 = zid2.p;
         T_(K_ZSORT, [&] { mps::launch::radix_sort(k, v, k2, v2, ztemp.p, n, bits, stream); });
         const uint32_t* perm = v;
-        tmp4.reserve(n);
         auto perm4 = [&](DBuf<float4>& a) {
             T_(K_ZSORT, [&] { mps::launch::gather_f4(perm, n, a.p, tmp4.p, stream); });
             std::swap(a.p, tmp4.p);
@@ -516,14 +531,11 @@ struct mps_ctx {
         if (boundary) {
             perm4(bvel);
             perm4(bcolor);
-            tmpu.reserve(n);
             mps::launch::gather_u32(perm, n, bface.p, tmpu.p, stream);
             std::swap(bface.p, tmpu.p); std::swap(bface.cap, tmpu.cap);
-            tmp2.reserve(n);
             mps::launch::gather_f2(perm, n, bbcc.p, tmp2.p, stream);
             std::swap(bbcc.p, tmp2.p); std::swap(bbcc.cap, tmp2.cap);
             if (!cfg.faithful_quirks) {
-                tmp1.reserve(n);
                 mps::launch::gather_f1(perm, n, bvol.p, tmp1.p, stream);
                 std::swap(bvol.p, tmp1.p); std::swap(bvol.cap, tmp1.cap);
             }
@@ -765,6 +777,7 @@ struct mps_ctx {
         size_t at = 0;
         for (uint32_t o = 0; o < k; o++)
             for (uint32_t q = 0; q < counts[o]; q++) h[at++] = o;
+        drop_rollout();
         bobj.reserve(h.size());
         up(bobj.p, h);
         nbobj = k;
@@ -1096,8 +1109,14 @@ struct mps_ctx {
         halo_valid = false;
         bnd_dirty = true;
     }
-    // Forward n steps from the current state, keeping a checkpoint every `every` steps.
-    void adj_rollout(uint32_t n, uint32_t every, uint32_t* counts) {
+    // Forward n steps from the current state, keeping a checkpoint every `every` steps; the
+    // positions (and velocities) after steps obs[0..n_obs) go to pos_out (vel_out), n_obs blocks
+    // of Nref x 3 in reference order.
+    void adj_rollout(uint32_t n, uint32_t every, uint32_t* counts, uint32_t n_obs = 0, const uint32_t* obs = nullptr,
+                     float* pos_out = nullptr, float* vel_out = nullptr) {
+        for (uint32_t k = 0; k < n_obs; k++)
+            if (!obs || obs[k] < 1 || obs[k] > n) throw ApiError{MPS_ERR_INVALID, "observed step outside the rollout"};
+        if (n_obs && !pos_out) throw ApiError{MPS_ERR_INVALID, "no output for the observed positions"};
         if (coupled()) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover coupled scenes"};
         if (nbobj > 1) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover several wall objects"};
         if (cfg.kernel != MPS_KERNEL_CUBIC) throw ApiError{MPS_ERR_STATE, "the adjoint needs the cubic kernel"};
@@ -1122,6 +1141,13 @@ struct mps_ctx {
             if (t % every == 0) save_ck(ckpts[t / every]);
             step(&st);
             if (st.nonfinite) throw ApiError{MPS_ERR_NONFINITE, "non-finite state during the rollout"};
+            for (uint32_t k = 0; k < n_obs; k++) {
+                if (obs[k] != t + 1) continue;
+                size_t cnt = 0, eb = 0;
+                const size_t len = 3 * size_t(Nref);
+                array_io(MPS_ARR_FLUID_POS, &cnt, &eb, pos_out + k * len, nullptr, len * sizeof(float));
+                if (vel_out) array_io(MPS_ARR_FLUID_VEL, &cnt, &eb, vel_out + k * len, nullptr, len * sizeof(float));
+            }
             if (counts)
                 for (uint32_t q = 0; q < S; q++) {
                     uint32_t* o = counts + 3 * (size_t(t) * S + q);
@@ -1134,6 +1160,10 @@ struct mps_ctx {
         sync();
         adj_n = n;
         adj_every = every;
+        adj_N = N;
+        adj_Nref = Nref;
+        adj_B = B;
+        adj_S = S;
     }
     // Transposed solve A_f^T lambda = adv (lambda in cgx), same chunked loop as the forward CG,
     // started from the later step's solution when there is one.
@@ -1243,27 +1273,30 @@ struct mps_ctx {
         std::swap(adv, adv2);
         std::swap(lam, lam2);
     }
-    // Gradients of a loss of the rollout's final positions (and velocities) with respect to the
-    // velocities at its start and each scene's viscosity and surface-tension coefficient.
-    void adj_backward(const float* gx, const float* gv, float* gv0, double* gmu, double* gsig) {
+    // Gradients of a loss of the states observed after steps obs[0..n_obs) of the rollout
+    // (cotangents gx, gv: n_obs blocks of Nref x 3 in the reference order of each observed state;
+    // gv may be null) with respect to the velocities at its start and each scene's viscosity and
+    // surface-tension coefficient. Each observation's cotangent is added to the adjoint state
+    // just before the step that produced that state is reversed.
+    void adj_backward(uint32_t n_obs, const uint32_t* obs, const float* gx, const float* gv, float* gv0, double* gmu,
+                      double* gsig) {
         if (!adj_n) throw ApiError{MPS_ERR_STATE, "no rollout to differentiate"};
+        if (N != adj_N || Nref != adj_Nref || B != adj_B || S != adj_S || ckpts.size() * adj_every < adj_n)
+            throw ApiError{MPS_ERR_STATE, "the particles changed since the rollout"};
+        if (!gx || !gv0 || n_obs == 0 || !obs) throw ApiError{MPS_ERR_INVALID, "no cotangents"};
+        std::vector<std::vector<uint32_t>> at(adj_n + 1);  // observation indices per observed step
+        for (uint32_t k = 0; k < n_obs; k++) {
+            if (obs[k] < 1 || obs[k] > adj_n) throw ApiError{MPS_ERR_INVALID, "observed step outside the rollout"};
+            at[obs[k]].push_back(k);
+        }
         sync_params();
-        restore_ck(ck_end);
         ensure_partials();
         const size_t m = std::max<uint32_t>(N, 1);
         adx.reserve(m); adv.reserve(m);
-        {
-            std::vector<uint32_t> hp = down(perm.p, N);
-            std::vector<float4> hx(N, make_float4(0.f, 0.f, 0.f, 0.f)), hv(N, make_float4(0.f, 0.f, 0.f, 0.f));
-            for (uint32_t q = 0; q < N; q++) {
-                const uint32_t r = hp[q];
-                if (r == mps::dev::kEmpty) continue;
-                hx[q] = make_float4(gx[3 * size_t(r)], gx[3 * size_t(r) + 1], gx[3 * size_t(r) + 2], 0.f);
-                if (gv) hv[q] = make_float4(gv[3 * size_t(r)], gv[3 * size_t(r) + 1], gv[3 * size_t(r) + 2], 0.f);
-            }
-            up(adx.p, hx);
-            up(adv.p, hv);
-        }
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(adx.p), 4 * size_t(N), stream);
+        mps::launch::zero_words(reinterpret_cast<uint32_t*>(adv.p), 4 * size_t(N), stream);
+        const size_t obs_len = 3 * size_t(Nref);
+        gobs.reserve(2 * std::max<size_t>(obs_len, 1));
         lam_valid = false;
         acc_mu.reserve(S); acc_sig.reserve(S);
         mps::launch::zero_words(reinterpret_cast<uint32_t*>(acc_mu.p), 2 * size_t(S), stream);
@@ -1285,7 +1318,18 @@ struct mps_ctx {
                 }
                 rec = nullptr;
             }
-            for (uint32_t t = t1; t-- > t0;) reverse_step(tape[t - t0]);
+            for (uint32_t t = t1; t-- > t0;) {
+                TapeStep& T = tape[t - t0];
+                for (uint32_t k : at[t + 1]) {  // the state after step t, in that step's slot order
+                    HIPCHK(hipMemcpyAsync(gobs.p, gx + k * obs_len, obs_len * sizeof(float), hipMemcpyHostToDevice, stream));
+                    if (gv)
+                        HIPCHK(hipMemcpyAsync(gobs.p + obs_len, gv + k * obs_len, obs_len * sizeof(float), hipMemcpyHostToDevice,
+                                              stream));
+                    mps::launch::adj_add_cotangent(N, T.perm_end.p, gobs.p, gv ? gobs.p + obs_len : nullptr, adx.p, adv.p, stream);
+                    sync();  // the host buffers may be reused by the next observation's copy
+                }
+                reverse_step(T);
+            }
         }
         HIPCHK(hipMemcpyAsync(h_flags + 2, d_flags + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
@@ -1417,6 +1461,7 @@ struct mps_ctx {
         }
         pressure_solve(false, st);
         advect();
+        if (rec) dcopy(rec->perm_end, perm.p, N);
         HIPCHK(hipMemcpyAsync(h_flags, d_flags, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
         sync();
         st->nonfinite = h_flags[0] ? 1u : 0u;
@@ -1472,11 +1517,13 @@ struct mps_ctx {
         fh.n_valid = 0;
         pairs_valid = false;
         halo_valid = false;
+        drop_rollout();
         frame = 0;
         pred[0] = 2; pred[1] = 3; pred[2] = 8; pred_adj = 8;
         params_dirty = true;
     }
     void alloc_boundary(uint32_t n) {
+        drop_rollout();
         const size_t m = std::max<uint32_t>(n, 1);
         bpos.reserve(m); bvel.reserve(m); bcolor.reserve(m); bvol.reserve(m); bS.reserve(m); bface.reserve(m); bbcc.reserve(m);
         cbb.reserve(m);
@@ -1505,6 +1552,7 @@ struct mps_ctx {
     // ------------------------------------------------------------------ batched scenes
     void batch_clear() {
         sync();
+        drop_rollout();
         staged.clear();
         scenes.clear();
         sstats.clear();
@@ -1646,41 +1694,6 @@ struct mps_ctx {
         params_dirty = true;
         sync();
     }
-    // Face ownership bits: RTriangleBuild_kernel over per-vertex face lists sorted by face id
-    // (MPSMeshObject.cu:178-237, MPSMeshObject.cuh:41-101). Integer setup work on the host.
-    static void rtri_build(size_t nv, const uint32_t* F, size_t nt, std::vector<uint32_t>& rtri) {
-        std::vector<uint32_t> idx(nv + 1, 0u);
-        for (size_t t = 0; t < 3 * nt; t++) idx[F[t] + 1]++;
-        for (size_t v = 0; v < nv; v++) idx[v + 1] += idx[v];
-        std::vector<uint32_t> nb(idx[nv]), fill(idx.begin(), idx.end() - 1);
-        for (size_t t = 0; t < nt; t++)
-            for (int k = 0; k < 3; k++) nb[fill[F[3 * t + k]]++] = uint32_t(t);
-        // faces were appended in increasing id, so each vertex list is already sorted
-        rtri.assign(nt, 0u);
-        for (size_t id = 0; id < nt; id++) {
-            const uint32_t* face = F + 3 * id;
-            uint32_t info = 0u;
-            uint32_t js = idx[face[0]], je = idx[face[0] + 1];
-            for (uint32_t i = 0, j = 1; i < 3u; i++, j = (j == 2u ? 0u : j + 1u)) {
-                uint32_t ino = js;
-                const uint32_t ie = je;
-                uint32_t itri = nb[ino++];
-                if (itri == id) info |= 1u << i;
-                js = idx[face[j]];
-                je = idx[face[j] + 1];
-                uint32_t of = mps::dev::kEmpty;
-                for (uint32_t q = js; q < je; q++)
-                    if (nb[q] == itri && id != itri) { of = itri; break; }
-                for (; ino < ie && of == mps::dev::kEmpty; ino++) {
-                    itri = nb[ino];
-                    for (uint32_t q = js; q < je; q++)
-                        if (nb[q] == itri && id != itri) { of = itri; break; }
-                }
-                if (id < of) info |= 1u << (i + 3u);
-            }
-            rtri[id] = info;
-        }
-    }
     void sample_mesh(const double* verts, size_t nv, const uint32_t* tris, size_t nt, const double* center,
                      const double* size, const mps_mesh_material* m) {
         if (!verts || !tris || nv == 0 || nt == 0) throw ApiError{MPS_ERR_INVALID, "empty mesh"};
@@ -1714,15 +1727,22 @@ struct mps_ctx {
             area += 0.5 * std::sqrt((cx * cx + cy * cy) + cz * cz);
         }
         std::vector<double> vmass(nv, area * m->density / double(nv));
-        std::vector<uint32_t> rtri;
-        rtri_build(nv, tris, nt, rtri);
         DBuf<double> dV, dmass, dpos;
         DBuf<uint32_t> dF, dR, dcnt, dface;
         dV.reserve(3 * nv); dmass.reserve(nv); dF.reserve(3 * nt); dR.reserve(nt); dcnt.reserve(nt + 1);
         HIPCHK(hipMemcpyAsync(dV.p, V.data(), 3 * nv * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dmass.p, vmass.data(), nv * sizeof(double), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dF.p, tris, 3 * nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(dR.p, rtri.data(), nt * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+        {  // which face samples each shared vertex and edge (buildAdjacency + RTriangleBuild)
+            const size_t slots = mps::launch::owner_table_slots(uint32_t(nt));
+            DBuf<uint32_t> vmin, emin;
+            DBuf<unsigned long long> ekey;
+            vmin.reserve(nv); emin.reserve(slots); ekey.reserve(slots);
+            mps::launch::owner_build(dF.p, uint32_t(nt), uint32_t(nv), vmin.p, ekey.p, emin.p, slots, dR.p, stream);
+            sync();
+            vmin.release(); emin.release(); ekey.release();
+        }
+        std::vector<uint32_t> rtri = down(dR.p, nt);
         DBuf<uint32_t> dshort;
         dshort.reserve(nt);
         mps::launch::sample_count(dV.p, dF.p, dR.p, uint32_t(nt), m->radius, dcnt.p, dshort.p, stream);
@@ -1817,6 +1837,7 @@ struct mps_ctx {
         mesh.shorts = shorts;
         if (regenerated) *regenerated = ngen;
         if (ngen == 0) return;
+        drop_rollout();
         if (prev[nt] != B) throw ApiError{MPS_ERR_STATE, "boundary particles no longer match the sampled mesh"};
         const uint32_t B0 = B, n = curr[nt];
         params_dirty = true;
@@ -2128,7 +2149,7 @@ struct mps_ctx {
         for (auto& t : tape) t.release();
         for (auto& c : ckpts) c.release();
         ck_end.release();
-        perm0.release(); d_L.release(); rev.release(); adx.release(); adv.release(); adx2.release(); adv2.release();
+        perm0.release(); d_L.release(); rev.release(); gobs.release(); adx.release(); adv.release(); adx2.release(); adv2.release();
         aux4.release(); gvec.release(); avin.release(); abar.release(); rbar.release(); dbar.release(); fbar.release();
         fstr.release(); cabT.release(); acc_mu.release(); acc_sig.release(); lam.release(); lam2.release();
         if (d_active) (void)hipFree(d_active);
@@ -2327,6 +2348,7 @@ int mps_set_physics(mps_ctx* c, const mps_physics* p) {
     c->phys = *p;
     for (auto& h : c->scenes) h.phys = *p;
     c->params_dirty = true;
+    c->drop_rollout();
     return MPS_OK;
 }
 
@@ -2346,6 +2368,7 @@ int mps_set_fluid_material(mps_ctx* c, const mps_sph_material* m) {
         h.mass = c->mMass;
     }
     c->params_dirty = true;
+    c->drop_rollout();
     return MPS_OK;
 }
 
@@ -2613,6 +2636,7 @@ int mps_scene_set_material(mps_ctx* c, uint32_t scene, const mps_sph_material* m
     h.V = m->radius * m->radius * m->radius / 48.0 * 3.141592;
     h.mass = m->density * h.V;
     c->params_dirty = true;
+    c->drop_rollout();
     return MPS_OK;
 }
 
@@ -2624,10 +2648,27 @@ int mps_adj_rollout(mps_ctx* c, uint32_t n_steps, uint32_t checkpoint_every, uin
     return guarded(c, [&] { c->adj_rollout(n_steps, checkpoint_every, loop_counts); });
 }
 
+int mps_adj_rollout_traj(mps_ctx* c, uint32_t n_steps, uint32_t checkpoint_every, uint32_t n_obs, const uint32_t* obs_step,
+                         float* pos_out, float* vel_out, uint32_t* loop_counts) {
+    return guarded(c, [&] { c->adj_rollout(n_steps, checkpoint_every, loop_counts, n_obs, obs_step, pos_out, vel_out); });
+}
+
 int mps_adj_backward(mps_ctx* c, const float* dloss_dpos, const float* dloss_dvel, float* dloss_dvel0, double* dloss_dviscosity,
                      double* dloss_dsurface_tension) {
     if (!c || !dloss_dpos || !dloss_dvel0) return MPS_ERR_INVALID;
-    return guarded(c, [&] { c->adj_backward(dloss_dpos, dloss_dvel, dloss_dvel0, dloss_dviscosity, dloss_dsurface_tension); });
+    return guarded(c, [&] {
+        const uint32_t last = c->adj_n;
+        c->adj_backward(1, &last, dloss_dpos, dloss_dvel, dloss_dvel0, dloss_dviscosity, dloss_dsurface_tension);
+    });
+}
+
+int mps_adj_backward_traj(mps_ctx* c, uint32_t n_obs, const uint32_t* obs_step, const float* dloss_dpos,
+                          const float* dloss_dvel, float* dloss_dvel0, double* dloss_dviscosity,
+                          double* dloss_dsurface_tension) {
+    if (!c || !obs_step || !dloss_dpos || !dloss_dvel0) return MPS_ERR_INVALID;
+    return guarded(c, [&] {
+        c->adj_backward(n_obs, obs_step, dloss_dpos, dloss_dvel, dloss_dvel0, dloss_dviscosity, dloss_dsurface_tension);
+    });
 }
 
 int mps_host_alloc(mps_ctx* c, size_t bytes, void** out) {

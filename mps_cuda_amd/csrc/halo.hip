@@ -53,20 +53,25 @@ __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32
     if (tid == 0) cnt = 0;
     __syncthreads();
     const uint32_t c = i < P.end ? nb.cff[i] : 0u;
-    // insert; stop as soon as the block is known to exceed the limit (each thread adds at most
-    // one key after the count passes it, so the table never fills and every probe ends).
+    // insert; stop as soon as the block is known to exceed the limit. The count is checked
+    // before every insert, so each thread adds at most one key after the count passes the
+    // limit: the table holds at most limit + kBlock keys, never fills, and every probe ends.
     // List entries are loaded kBatch at a time so their latency overlaps.
     constexpr uint32_t kBatch = 8;
-    for (uint32_t k0 = 0; k0 < c; k0 += kBatch) {
+    bool over = false;
+    for (uint32_t k0 = 0; k0 < c && !over; k0 += kBatch) {
         uint32_t jb[kBatch];
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; q++)
             jb[q] = k0 + q < c ? nb.ff[size_t(k0 + q) * N + i] : kNoKey;
-        if (*(volatile uint32_t*)&cnt > limit) break;
 #pragma unroll
         for (uint32_t q = 0; q < kBatch; q++) {
             const uint32_t j = jb[q];
             if (j == kNoKey) break;
+            if (*(volatile uint32_t*)&cnt > limit) {
+                over = true;
+                break;
+            }
             uint32_t s = halo_hash(j);
             for (uint32_t probe = 0; probe < kHaloHash; probe++) {
                 const uint32_t prev = atomicCAS(&key[s], kNoKey, j);

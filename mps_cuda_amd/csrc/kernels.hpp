@@ -287,8 +287,17 @@ void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const floa
 void adj_cg_init_warm(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* recA,
                       const float4* b, float4* x, float4* r, float4* recB, double* partials, hipStream_t s);
 void adj_fold_start(const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s);
+// adx[s] += gx[perm[s]], adv[s] += gv[perm[s]] (gx, gv: n_ref x 3 floats in reference order; gv may be
+// null): the cotangent of an observed state, added in the slot order that state was stored in
+void adj_add_cotangent(uint32_t n, const uint32_t* perm, const float* gx, const float* gv, float4* adx, float4* adv,
+                       hipStream_t s);
 
 // ---- sampling.hip
+// Face ownership bits of a triangle mesh (bits 0-2 owned vertices, 3-5 owned edges) on the GPU;
+// scratch: vmin (nv), ekey/emin (owner_table_slots(nt) entries each)
+size_t owner_table_slots(uint32_t nt);
+void owner_build(const uint32_t* F, uint32_t nt, uint32_t nv, uint32_t* vmin, unsigned long long* ekey, uint32_t* emin,
+                 size_t slots, uint32_t* R, hipStream_t s);
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
                   uint32_t* shorts, hipStream_t s);
 void sample_generate(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, const uint8_t* gen,

@@ -4,7 +4,7 @@
 // per face, owned vertices, owned edges (interior points at spacing l/(n+1) with
 // n = ceil((l - 1.01 d)/d), d = R_m/2) and scan lines parallel to the shortest edge. One
 // thread per face counts, the host scans the counts, one thread per face writes. Face
-// ownership bits come from the host (integer adjacency, see context.cpp).
+// ownership bits (which face samples each shared vertex and edge) are built here too.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -194,10 +194,83 @@ __global__ void k_sample_copy(uint32_t nt, const uint32_t* curr, const uint32_t*
     if (prev_bvel) bvel[i] = prev_bvel[j];
 }
 
+// ---- face ownership (the result of RTriangleBuild_kernel, MPSMeshObject.cuh:41-101)
+// Every vertex and every edge shared by several faces is sampled once, by its owner:
+// - a face owns vertex v when no face containing v has a smaller id;
+// - a face owns edge {a, b} when no face containing both a and b has a smaller id.
+// Bits 0-2 of a face's word: owned vertices F[3t+i]; bits 3-5: owned edges
+// (F[3t+i], F[3t+(i+1)%3]). A degenerate edge (a == b) is shared by every face containing a.
+// The smallest face per vertex is an atomicMin; the smallest face per edge is an atomicMin in an
+// open-addressed table keyed by the sorted vertex pair (load factor <= 1/2, so every probe ends).
+constexpr unsigned long long kNoEdge = 0xFFFFFFFFFFFFFFFFull;
+__device__ __forceinline__ uint32_t edge_slot(unsigned long long key, uint32_t mask) {
+    return uint32_t((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+}
+__device__ __forceinline__ unsigned long long edge_key(uint32_t a, uint32_t b) {
+    return a < b ? ((unsigned long long)a << 32) | b : ((unsigned long long)b << 32) | a;
+}
+
+__global__ void k_owner_min(const uint32_t* __restrict__ F, uint32_t nt, uint32_t* __restrict__ vmin,
+                            unsigned long long* __restrict__ ekey, uint32_t* __restrict__ emin, uint32_t mask) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (uint32_t i = 0; i < 3u; i++) {
+        const uint32_t a = F[3 * size_t(t) + i], b = F[3 * size_t(t) + (i == 2u ? 0u : i + 1u)];
+        atomicMin(&vmin[a], t);
+        if (a == b) continue;
+        const unsigned long long key = edge_key(a, b);
+        uint32_t sl = edge_slot(key, mask);
+        for (uint32_t probe = 0; probe <= mask; probe++) {
+            const unsigned long long prev = atomicCAS(&ekey[sl], kNoEdge, key);
+            if (prev == kNoEdge || prev == key) {
+                atomicMin(&emin[sl], t);
+                break;
+            }
+            sl = (sl + 1u) & mask;
+        }
+    }
+}
+
+__global__ void k_owner_bits(const uint32_t* __restrict__ F, uint32_t nt, const uint32_t* __restrict__ vmin,
+                             const unsigned long long* __restrict__ ekey, const uint32_t* __restrict__ emin, uint32_t mask,
+                             uint32_t* __restrict__ R) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    uint32_t bits = 0u;
+    for (uint32_t i = 0; i < 3u; i++) {
+        const uint32_t a = F[3 * size_t(t) + i], b = F[3 * size_t(t) + (i == 2u ? 0u : i + 1u)];
+        if (vmin[a] == t) bits |= 1u << i;
+        uint32_t owner = vmin[a];  // a degenerate edge: every face containing a shares it
+        if (a != b) {
+            const unsigned long long key = edge_key(a, b);
+            uint32_t sl = edge_slot(key, mask);
+            for (uint32_t probe = 0; probe <= mask && ekey[sl] != key; probe++) sl = (sl + 1u) & mask;
+            owner = emin[sl];
+        }
+        if (owner == t) bits |= 1u << (i + 3u);
+    }
+    R[t] = bits;
+}
+
 }  // namespace dev
 
 namespace launch {
 using namespace dev;
+size_t owner_table_slots(uint32_t nt) {
+    size_t slots = 1;
+    while (slots < 6 * size_t(nt)) slots <<= 1;
+    return slots;
+}
+void owner_build(const uint32_t* F, uint32_t nt, uint32_t nv, uint32_t* vmin, unsigned long long* ekey, uint32_t* emin,
+                 size_t slots, uint32_t* R, hipStream_t s) {
+    if (!nt) return;
+    fill_u32(vmin, nv, 0xFFFFFFFFu, s);
+    fill_u32(reinterpret_cast<uint32_t*>(ekey), 2 * slots, 0xFFFFFFFFu, s);
+    fill_u32(emin, slots, 0xFFFFFFFFu, s);
+    const uint32_t mask = uint32_t(slots - 1);
+    launch_k(k_owner_min, (nt + 255) / 256, 256, 0, s, F, nt, vmin, ekey, emin, mask);
+    launch_k(k_owner_bits, (nt + 255) / 256, 256, 0, s, F, nt, vmin, ekey, emin, mask, R);
+}
 void sample_count(const double* V, const uint32_t* F, const uint32_t* R, uint32_t nt, double radius, uint32_t* counts,
                   uint32_t* shorts, hipStream_t s) {
     if (nt) launch_k(k_sample_count, (nt + 63) / 64, 64, 0, s, V, F, R, nt, radius, counts, shorts);

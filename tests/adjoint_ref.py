@@ -163,10 +163,15 @@ class Replay:
         x = x + dt * v
         return x, v, Vb
 
-    def rollout(self, v0, mu, sigma, counts):
-        """counts: (steps, 3) = (l_df, l_cg, l_cd) per step. Returns final (x, v)."""
+    def rollout(self, v0, mu, sigma, counts, observe=None):
+        """counts: (steps, 3) = (l_df, l_cg, l_cd) per step. Returns final (x, v), or with
+        `observe` (steps in 1..len(counts)) the positions after each of those steps (k, N, 3)."""
         x, v = self.x0, v0
         Vb = torch.zeros(self.xb.shape[0], dtype=T)
-        for ldf, _, lcd in counts:
+        seen = {}
+        for t, (ldf, _, lcd) in enumerate(counts):
             x, v, Vb = self.step(x, v, Vb, mu, sigma, int(ldf), int(lcd))
+            seen[t + 1] = x
+        if observe is not None:
+            return torch.stack([seen[int(t)] for t in observe])
         return x, v

@@ -119,6 +119,96 @@ def test_autograd_function_drives_the_adjoint():
     assert np.isfinite(loss.item()) and loss.item() > 0
 
 
+def test_trajectory_cotangents_match_fp64_replay():
+    """A loss over the whole trajectory, L = sum_k w_k . x(t_k) at steps 2, 4 (twice: a repeated
+    step adds) and 6: the observed states and the gradients of the trajectory path
+    (mps_adj_rollout_traj / mps_adj_backward_traj) against the FP64 replay with torch autograd;
+    the single final observation equals mps_adj_backward bit for bit."""
+    sc = scenes.diff_droplet(n=300, radius=0.08, seed=6, steps=STEPS, h=0.04)
+    obs = [2, 4, 4, STEPS]
+    g = _load(sc)
+    say(f"[adjoint-traj] N={sc.n} steps={STEPS} obs={obs}")
+    xs, _, counts = g.adj_rollout_traj(STEPS, obs, 4)
+    rng = np.random.default_rng(7)
+    w = rng.normal(size=(len(obs), sc.n, 3))
+    gv0, gmu, gsig = g.adj_backward_traj(obs, w.astype(np.float32))
+    rep = Replay(sc.pos, sc.radius[0], g.get("FLUID_MASS")[0], g.get("BND_POS"), g.get("BND_RADIUS")[0], sc.physics,
+                 sc.material)
+    v0 = torch.as_tensor(sc.extra["v0"]).clone().requires_grad_(True)
+    mu = torch.tensor(sc.material["viscosity"], dtype=torch.float64, requires_grad=True)
+    sig = torch.tensor(sc.material["surface_tension"], dtype=torch.float64, requires_grad=True)
+    xr = rep.rollout(v0, mu, sig, counts[:, 0, :], observe=obs)
+    assert _rel(xr.detach().numpy(), xs) < 1e-5
+    L = (torch.as_tensor(w) * xr).sum()
+    rv, rm, rs = torch.autograd.grad(L, (v0, mu, sig))
+    e_v, e_mu, e_sig = _rel(gv0, rv.numpy()), _rel([gmu[0]], [rm.item()]), _rel([gsig[0]], [rs.item()])
+    say(f"[adjoint-traj] rel err vs FP64: v0 {e_v:.2e} mu {e_mu:.2e} sigma {e_sig:.2e}")
+    assert e_v < 1e-3 and e_mu < 1e-3 and e_sig < 1e-3
+    # the final state alone through both entry points
+    g2 = _load(sc)
+    g2.adj_rollout(STEPS, 4)
+    wf = w[-1].astype(np.float32)
+    a = g2.adj_backward(wf)
+    b = g2.adj_backward_traj([STEPS], wf[None])
+    assert all(np.array_equal(p, q) for p, q in zip(a, b))
+
+
+def test_stale_rollout_is_refused():
+    """A backward after the particles, walls or parameters were replaced raises before any
+    copy (the rollout's checkpoints no longer describe the context)."""
+    from mps_cuda_amd._capi import MPSError
+    sc = scenes.diff_droplet(n=300, radius=0.08, seed=3, steps=STEPS, h=0.04)
+    small = scenes.diff_droplet(n=120, radius=0.06, seed=4, steps=STEPS, h=0.04)
+    w = np.ones((sc.n, 3), np.float32)
+    for change in ("smaller fluid", "material", "walls"):
+        g = _load(sc)
+        g.adj_rollout(STEPS, 3)
+        say(f"[adjoint-stale] {change}")
+        if change == "smaller fluid":
+            scenes.load_into(g, small)
+        elif change == "material":
+            g.scene_set_material(0, **dict(sc.material, viscosity=0.05))
+        else:
+            g.upload_boundary(g.get("BND_POS"), g.get("BND_RADIUS"))
+        with pytest.raises(MPSError) as ei:
+            g.adj_backward(w[:g.get("FLUID_RADIUS").shape[0]])
+        assert ei.value.code == 4
+
+
+def test_autograd_refuses_a_replaced_rollout():
+    from mps_cuda_amd.autograd import DifferentiableRollout
+    sc = scenes.diff_droplet(n=300, radius=0.08, seed=3, steps=STEPS, h=0.04)
+    roll = DifferentiableRollout([sc], STEPS, checkpoint_every=3)
+    v0 = torch.as_tensor(sc.extra["v0"], dtype=torch.float32).clone().requires_grad_(True)
+    mu = torch.tensor([0.02], dtype=torch.float64, requires_grad=True)
+    sig = torch.tensor([0.3], dtype=torch.float64, requires_grad=True)
+    x1 = roll(v0, mu, sig)
+    x2 = roll(v0, mu * 1.5, sig)
+    with pytest.raises(RuntimeError, match="replaced"):
+        x1.sum().backward()
+    x2.sum().backward()      # the latest rollout still differentiates
+    assert mu.grad is not None and np.isfinite(mu.grad.numpy()).all()
+    assert v0.grad.dtype == v0.dtype and mu.grad.dtype == mu.dtype
+
+
+def test_autograd_trajectory_matches_the_c_abi():
+    from mps_cuda_amd.autograd import DifferentiableRollout
+    sc = scenes.diff_droplet(n=300, radius=0.08, seed=8, steps=STEPS, h=0.04)
+    obs = [3, STEPS]
+    roll = DifferentiableRollout([sc], STEPS, checkpoint_every=2)
+    v0 = torch.as_tensor(sc.extra["v0"], dtype=torch.float32).clone().requires_grad_(True)
+    mu = torch.tensor([sc.material["viscosity"]], dtype=torch.float64, requires_grad=True)
+    sig = torch.tensor([sc.material["surface_tension"]], dtype=torch.float64, requires_grad=True)
+    xs = roll.trajectory(v0, mu, sig, obs)
+    assert tuple(xs.shape) == (len(obs), sc.n, 3)
+    w = torch.as_tensor(np.random.default_rng(9).normal(size=xs.shape), dtype=torch.float32)
+    (w * xs).sum().backward()
+    g = _load(sc)
+    g.adj_rollout_traj(STEPS, obs, 2)
+    gv0, gmu, gsig = g.adj_backward_traj(obs, w.numpy())
+    assert _rel(v0.grad.numpy(), gv0) < 1e-6 and _rel(mu.grad.numpy(), gmu) < 1e-6 and _rel(sig.grad.numpy(), gsig) < 1e-6
+
+
 def test_adjoint_matches_fp64_finite_differences():
     """A 4096-particle droplet over 30 steps at (viscosity 0.025, surface tension 0.1): the GPU
     adjoint's gradients of L = sum w . x_T (fixed random w, so both sides use the same cotangent)
