@@ -304,11 +304,22 @@ int mps_get_frame(mps_ctx* ctx, uint32_t* frame);
 int mps_set_frame(mps_ctx* ctx, uint32_t frame);
 
 /* ---- render feed (replaces the CUDA-GL interop buffers) -------------------------------- */
-/* Enqueue device->host copies of positions (n*3 float), radii (n) and colours (n*4) on the
- * context's stream; any pointer may be NULL. Host buffers should be pinned
- * (mps_host_alloc) for the copy to be asynchronous. */
+/* Device->host copies of positions (n*3 float), radii (n) and colours (n*4) in reference
+ * order; any pointer may be NULL. The state is packed on the context's stream into one of two
+ * device staging slots (alternating per call) and copied on a second stream of the context
+ * behind an event, so the next mps_step's kernels run while the copy drains (step N+1 overlaps
+ * the copy of frame N). Host buffers must be pinned (mps_host_alloc) for the copy to be
+ * asynchronous, and must not be reused until mps_fence_wait on the fence returns; use two sets
+ * of host buffers to keep two frames in flight. Replaces mapping the GL buffers to CUDA
+ * (MCUDAGLBuffer.inl:49-79); a caller uploads the host buffers to its renderer. */
 int mps_readback_async(mps_ctx* ctx, float* host_pos, float* host_radius, float* host_rgba,
                        mps_fence* fence);
+/* The same in the layout SPHRenderer's shader reads (src/shader/particleShader.vert:3-5):
+ * positions (n*3) and radii (n) as double, colours (n*4) as float. */
+int mps_readback_async_f64(mps_ctx* ctx, double* host_pos, double* host_radius, float* host_rgba,
+                           mps_fence* fence);
+/* Wait until the fence's copies are in the host buffers. A fence stays valid after later
+ * readbacks (waiting then also covers any later copy of the same staging slot). */
 int mps_fence_wait(mps_ctx* ctx, mps_fence* fence);
 int mps_host_alloc(mps_ctx* ctx, size_t bytes, void** out);
 int mps_host_free(mps_ctx* ctx, void* p);

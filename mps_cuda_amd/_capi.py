@@ -133,6 +133,7 @@ _SIGS = {
     "mps_get_frame": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "mps_set_frame": (C.c_int, [_P, C.c_uint32]),
     "mps_readback_async": (C.c_int, [_P, _P, _P, _P, C.POINTER(Fence)]),
+    "mps_readback_async_f64": (C.c_int, [_P, _P, _P, _P, C.POINTER(Fence)]),
     "mps_fence_wait": (C.c_int, [_P, C.POINTER(Fence)]),
     "mps_host_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "mps_host_free": (C.c_int, [_P, _P]),
@@ -227,9 +228,14 @@ class HipSim:
             raise MPSError(rc, "mps_create failed")
         self.h = h
         self.rollout_generation = 0  # bumped by every adj_rollout
+        self._pinned = []            # pinned host buffers of host_array, freed in close()
 
     def close(self):
         if getattr(self, "h", None):
+            self.lib.mps_synchronize(self.h)
+            for p in getattr(self, "_pinned", []):
+                self.lib.mps_host_free(self.h, p)
+            self._pinned = []
             self.lib.mps_destroy(self.h)
             self.h = None
 
@@ -451,6 +457,22 @@ class HipSim:
         f = Fence()
         self._chk(self.lib.mps_readback_async(self.h, _ptr(host_pos), _ptr(host_radius), _ptr(host_rgba), C.byref(f)))
         return f
+
+    def readback_async_f64(self, host_pos=None, host_radius=None, host_rgba=None) -> Fence:
+        """The renderer's layout: positions and radii as float64 arrays, colours as float32."""
+        f = Fence()
+        self._chk(self.lib.mps_readback_async_f64(self.h, _ptr(host_pos), _ptr(host_radius), _ptr(host_rgba),
+                                                  C.byref(f)))
+        return f
+
+    def host_array(self, shape, dtype):
+        """A numpy view of pinned host memory (mps_host_alloc), freed with the context."""
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        p = C.c_void_p()
+        self._chk(self.lib.mps_host_alloc(self.h, max(n, 1), C.byref(p)))
+        self._pinned.append(p)
+        buf = (C.c_char * max(n, 1)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
     def fence_wait(self, f: Fence):
         self._chk(self.lib.mps_fence_wait(self.h, C.byref(f)))

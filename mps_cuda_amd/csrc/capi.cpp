@@ -85,6 +85,8 @@ struct HashSet {
     }
 };
 
+constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
+
 int bits_for(size_t count) {  // number of bits needed for values < count
     int b = 0;
     while ((size_t(1) << b) < count) b++;
@@ -242,8 +244,51 @@ struct mps_ctx {
     double* d_err3 = nullptr;
     double* h_err3 = nullptr;
     DBuf<double> partials;
-    DBuf<float> rb_xyz, rb_rad;
-    DBuf<float4> rb_rgba;
+    // ---- render feed: the state is packed on the compute stream into one of two staging slots
+    // and copied to the host on a second stream behind an event, so the next step's kernels run
+    // while the copy drains; a slot is packed again only after its previous copy finished.
+    struct ReadbackSlot {
+        DBuf<unsigned char> buf;
+        hipEvent_t packed = nullptr, copied = nullptr;
+        bool used = false;
+    } rb[2];
+    hipStream_t copy_stream = nullptr;
+    uint64_t rb_seq = 0;
+    void readback(void* host_pos, void* host_radius, float* host_rgba, bool f64, mps_fence* fence) {
+        if (!copy_stream) HIPCHK(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+        ReadbackSlot& sl = rb[rb_seq & 1u];
+        if (!sl.packed) HIPCHK(hipEventCreateWithFlags(&sl.packed, hipEventDisableTiming));
+        if (!sl.copied) HIPCHK(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+        const size_t n = Nref, m = std::max<size_t>(n, 1);
+        const size_t real = f64 ? sizeof(double) : sizeof(float);
+        const size_t o_rad = al256(3 * m * real), o_rgba = o_rad + al256(m * real);
+        const size_t bytes = o_rgba + m * sizeof(float4);
+        if (sl.used && bytes > sl.buf.cap) HIPCHK(hipEventSynchronize(sl.copied));  // no free under a copy
+        sl.buf.reserve(bytes);
+        if (sl.used) HIPCHK(hipStreamWaitEvent(stream, sl.copied, 0));  // the slot's last copy has drained
+        unsigned char* b = sl.buf.p;
+        if (f64)
+            mps::launch::pack_readback_f64(N, pos.p, color.p, perm.p, host_pos ? reinterpret_cast<double*>(b) : nullptr,
+                                           host_radius ? reinterpret_cast<double*>(b + o_rad) : nullptr,
+                                           host_rgba ? reinterpret_cast<float*>(b + o_rgba) : nullptr, stream);
+        else
+            mps::launch::pack_readback(N, pos.p, color.p, perm.p, host_pos ? reinterpret_cast<float*>(b) : nullptr,
+                                       host_radius ? reinterpret_cast<float*>(b + o_rad) : nullptr,
+                                       host_rgba ? reinterpret_cast<float*>(b + o_rgba) : nullptr, stream);
+        HIPCHK(hipEventRecord(sl.packed, stream));
+        HIPCHK(hipStreamWaitEvent(copy_stream, sl.packed, 0));
+        if (n) {
+            if (host_pos) HIPCHK(hipMemcpyAsync(host_pos, b, 3 * n * real, hipMemcpyDeviceToHost, copy_stream));
+            if (host_radius) HIPCHK(hipMemcpyAsync(host_radius, b + o_rad, n * real, hipMemcpyDeviceToHost, copy_stream));
+            if (host_rgba)
+                HIPCHK(hipMemcpyAsync(host_rgba, b + o_rgba, n * sizeof(float4), hipMemcpyDeviceToHost, copy_stream));
+        }
+        HIPCHK(hipEventRecord(sl.copied, copy_stream));
+        sl.used = true;
+        fence->event = sl.copied;
+        fence->frame = frame;
+        rb_seq++;
+    }
     uint32_t pred[3] = {2, 3, 8};
     uint32_t pred_adj = 8;  // iterations of the last transposed solve of the adjoint
     // Iterations enqueued before the next host poll: the predicted count first (the previous
@@ -2142,7 +2187,13 @@ struct mps_ctx {
         bobj.release(); bS4.release();
         partials.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
-        rb_xyz.release(); rb_rad.release(); rb_rgba.release();
+        if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+        for (ReadbackSlot& sl : rb) {
+            sl.buf.release();
+            if (sl.packed) (void)hipEventDestroy(sl.packed);
+            if (sl.copied) (void)hipEventDestroy(sl.copied);
+        }
+        if (copy_stream) (void)hipStreamDestroy(copy_stream);
         perm.release(); inv.release(); keyref.release(); vals.release(); src.release(); lkey.release(); hash_perm.release();
         d_params.release(); blk_scene.release(); bscene.release(); d_delta.release(); mkey.release();
         clear_graphs();
@@ -2450,7 +2501,10 @@ int mps_run_phase(mps_ctx* c, int32_t phase, mps_step_stats* out) {
 }
 
 int mps_synchronize(mps_ctx* c) {
-    return guarded(c, [&] { c->sync(); });
+    return guarded(c, [&] {
+        c->sync();
+        if (c->copy_stream) HIPCHK(hipStreamSynchronize(c->copy_stream));  // readbacks in flight too
+    });
 }
 
 int mps_array_info(mps_ctx* c, int32_t id, size_t* count, size_t* elem_bytes) {
@@ -2501,37 +2555,20 @@ int mps_set_frame(mps_ctx* c, uint32_t f) {
 
 int mps_readback_async(mps_ctx* c, float* host_pos, float* host_radius, float* host_rgba, mps_fence* fence) {
     if (!c || !fence) return MPS_ERR_INVALID;
-    return guarded(c, [&] {
-        const uint32_t n = c->Nref;
-        if (host_pos) c->rb_xyz.reserve(3 * size_t(std::max<uint32_t>(n, 1)));
-        if (host_radius) c->rb_rad.reserve(std::max<uint32_t>(n, 1));
-        if (host_rgba) c->rb_rgba.reserve(std::max<uint32_t>(n, 1));
-        mps::launch::pack_readback(c->N, c->pos.p, c->color.p, c->perm.p, host_pos ? c->rb_xyz.p : nullptr,
-                                   host_radius ? c->rb_rad.p : nullptr,
-                                   host_rgba ? reinterpret_cast<float*>(c->rb_rgba.p) : nullptr, c->stream);
-        if (n) {
-            if (host_pos)
-                HIPCHK(hipMemcpyAsync(host_pos, c->rb_xyz.p, 3 * sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
-            if (host_radius)
-                HIPCHK(hipMemcpyAsync(host_radius, c->rb_rad.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
-            if (host_rgba)
-                HIPCHK(hipMemcpyAsync(host_rgba, c->rb_rgba.p, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
-        }
-        hipEvent_t ev;
-        HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        HIPCHK(hipEventRecord(ev, c->stream));
-        fence->event = ev;
-        fence->frame = c->frame;
-    });
+    return guarded(c, [&] { c->readback(host_pos, host_radius, host_rgba, false, fence); });
+}
+
+int mps_readback_async_f64(mps_ctx* c, double* host_pos, double* host_radius, float* host_rgba, mps_fence* fence) {
+    if (!c || !fence) return MPS_ERR_INVALID;
+    return guarded(c, [&] { c->readback(host_pos, host_radius, host_rgba, true, fence); });
 }
 
 int mps_fence_wait(mps_ctx* c, mps_fence* fence) {
     if (!c || !fence) return MPS_ERR_INVALID;
     return guarded(c, [&] {
         if (!fence->event) return;
-        hipEvent_t ev = static_cast<hipEvent_t>(fence->event);
-        HIPCHK(hipEventSynchronize(ev));
-        HIPCHK(hipEventDestroy(ev));
+        // the slot's event: this copy's, or a later one on the same in-order copy stream
+        HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(fence->event)));
         fence->event = nullptr;
     });
 }

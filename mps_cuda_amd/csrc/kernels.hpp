@@ -250,6 +250,8 @@ void stream_copy(const float4* a, float4* b, size_t n, hipStream_t s);
 void zero_words(uint32_t* p, size_t nwords, hipStream_t s);
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s);
+void pack_readback_f64(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, double* xyz, double* rad,
+                       float* rgba, hipStream_t s);
 
 // ---- adjoint.hip (reverse mode of one step with frozen control flow; cubic kernel)
 void adj_advect(const Scenes& sc, const float4* pos, const float4* vin, const float4* ax, float4* av, hipStream_t s);

@@ -1403,6 +1403,26 @@ __global__ void __launch_bounds__(kBlock) k_pack_readback(uint32_t n, const floa
     if (rgba) reinterpret_cast<float4*>(rgba)[i] = color[s];
 }
 
+// The same for a renderer that reads double positions and radii (SPHRenderer's shader,
+// src/shader/particleShader.vert:3-4): the float values widened exactly.
+__global__ void __launch_bounds__(kBlock) k_pack_readback_f64(uint32_t n, const float4* __restrict__ pos,
+                                                            const float4* __restrict__ color, const uint32_t* __restrict__ perm,
+                                                            double* __restrict__ xyz, double* __restrict__ rad,
+                                                            float* __restrict__ rgba) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t i = perm[s];
+    if (i == kEmpty) return;
+    const float4 p = pos[s];
+    if (xyz) {
+        xyz[3 * size_t(i)] = double(p.x);
+        xyz[3 * size_t(i) + 1] = double(p.y);
+        xyz[3 * size_t(i) + 2] = double(p.z);
+    }
+    if (rad) rad[i] = double(p.w);
+    if (rgba) reinterpret_cast<float4*>(rgba)[i] = color[s];
+}
+
 }  // namespace dev
 
 // ===================================================================== host launchers
@@ -1560,6 +1580,10 @@ void zero_words(uint32_t* p, size_t n, hipStream_t s) {
 void pack_readback(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, float* xyz, float* rad,
                    float* rgba, hipStream_t s) {
     if (n) launch_k(k_pack_readback, nb_(n), kBlock, 0, s, n, pos, color, perm, xyz, rad, rgba);
+}
+void pack_readback_f64(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, double* xyz, double* rad,
+                       float* rgba, hipStream_t s) {
+    if (n) launch_k(k_pack_readback_f64, nb_(n), kBlock, 0, s, n, pos, color, perm, xyz, rad, rgba);
 }
 
 }  // namespace launch
