@@ -290,6 +290,19 @@ int mps_sample_boundary_obj(mps_ctx* ctx, const char* path, const double* center
 /* ---- stepping ---------------------------------------------------------------------- */
 /* One System::OnUpdate. `out` may be NULL. */
 int mps_step(mps_ctx* ctx, mps_step_stats* out);
+/* One mps_step that allocates no device memory (no counterpart in the reference; for callers
+ * that cannot allocate: tight serving loops, harnesses that forbid device allocation). The
+ * step's reduction scratch (per-workgroup FP64 partial sums, neighbour-count records, solver
+ * records, loop counters) lives in a library-wide device array of 1.5 MiB (held on each GPU the
+ * library runs on until the process ends), sized for at most 8,388,608 fluid slots or wall
+ * particles and 64 scenes (MPS_ERR_INVALID beyond). Every other buffer is the context's and must
+ * already be large enough: after one mps_step on a scene at least this large (and one Z-sort
+ * step, if this step Z-sorts); otherwise, or when the neighbour lists outgrow their capacity,
+ * the call fails with MPS_ERR_STATE before any physics is applied, and an mps_step continues
+ * from the unchanged state. Solver chunks are launched directly (no graph capture) and kernel
+ * timing must be off. Calls on one GPU share its scratch array: calls that overlap on one GPU
+ * (several threads, several contexts) race. Results equal mps_step's bit for bit. */
+int mps_step_noalloc(mps_ctx* ctx, mps_step_stats* out);
 /* Run one phase in isolation on the current state (teacher-forced parity checks).
  * Solver phases fill the matching fields of `out` (may be NULL). */
 int mps_run_phase(mps_ctx* ctx, int32_t phase, mps_step_stats* out);

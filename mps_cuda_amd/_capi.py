@@ -125,6 +125,7 @@ _SIGS = {
     "mps_read_obj": (C.c_int, [_P, C.c_char_p, _P, C.POINTER(C.c_size_t), _P, C.POINTER(C.c_size_t)]),
     "mps_sample_boundary_obj": (C.c_int, [_P, C.c_char_p, _P, _P, C.POINTER(MeshMaterial)]),
     "mps_step": (C.c_int, [_P, C.POINTER(StepStats)]),
+    "mps_step_noalloc": (C.c_int, [_P, C.POINTER(StepStats)]),
     "mps_run_phase": (C.c_int, [_P, C.c_int32, C.POINTER(StepStats)]),
     "mps_synchronize": (C.c_int, [_P]),
     "mps_array_info": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
@@ -412,6 +413,13 @@ class HipSim:
     def step(self) -> StepStats:
         st = StepStats()
         self._chk(self.lib.mps_step(self.h, C.byref(st)))
+        return st
+
+    def step_noalloc(self) -> StepStats:
+        """One step that allocates no device memory (mps_step_noalloc); the context must have
+        been sized by an earlier mps_step on a scene at least this large."""
+        st = StepStats()
+        self._chk(self.lib.mps_step_noalloc(self.h, C.byref(st)))
         return st
 
     def run_phase(self, phase) -> StepStats:

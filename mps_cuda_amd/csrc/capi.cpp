@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,12 +56,23 @@ struct ApiError {
     std::string msg;
 };
 
+// Set while mps_step_noalloc runs on this thread: any buffer that would have to grow throws
+// instead of allocating.
+thread_local bool g_no_alloc = false;
+[[noreturn]] void refuse_alloc() {
+    throw ApiError{MPS_ERR_STATE,
+                   "mps_step_noalloc: the context's buffers are smaller than this step needs (its neighbour lists grew, "
+                   "or no mps_step has sized it yet); the state is the one before the step, and one mps_step continues "
+                   "from it"};
+}
+
 template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t cap = 0;
     void reserve(size_t n) {
         if (n <= cap) return;
+        if (g_no_alloc) refuse_alloc();
         if (p) HIPCHK(hipFree(p));
         p = nullptr;
         cap = 0;
@@ -87,6 +99,44 @@ struct HashSet {
 
 constexpr size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// ---- the allocation-free step's scratch
+// mps_step_noalloc keeps the step's reduction scratch in this array instead of the context's
+// buffers: per-workgroup FP64 partial sums, per-workgroup neighbour-count records, the
+// neighbour counters, the three solver records of every scene and the running-scene counters.
+// Sized for max(fluid slots, wall particles) <= kNoAllocSlots and S <= kNoAllocScenes (1.5 MiB);
+// each GPU the library runs on holds its own copy until the process ends. Every call on a GPU
+// uses that GPU's copy: calls that overlap on one GPU race.
+constexpr size_t kNoAllocSlots = size_t(1) << 23;
+constexpr uint32_t kNoAllocScenes = 64;
+constexpr size_t kNoAllocBlocks = kNoAllocSlots / mps::dev::kBlock;
+constexpr size_t kScrPartials = 3 * kNoAllocBlocks + 8;  // doubles (ensure_partials)
+constexpr size_t kScrNstat = 2 * kNoAllocBlocks + 2;     // uint2 (neighbors)
+constexpr size_t kScrOffNstat = al256(kScrPartials * sizeof(double));
+constexpr size_t kScrOffCtr = kScrOffNstat + al256(kScrNstat * sizeof(uint2));
+constexpr size_t kScrOffSolver = kScrOffCtr + al256(sizeof(NbrCounters));
+constexpr size_t kScrOffActive = kScrOffSolver + al256(3 * size_t(kNoAllocScenes) * sizeof(SolverDev));
+constexpr size_t kScrBytes = kScrOffActive + al256(4 * sizeof(uint32_t));
+static_assert(kScrBytes <= (size_t(3) << 19), "scratch stays modest");
+
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+// The address of `scratch` on `device` (which must be current), looked up once per device.
+// Host code uses only this address, never the array's name, which on the host is a stand-in.
+unsigned char* scratch_address(int device) {
+    static std::mutex mu;
+    static std::vector<unsigned char*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) throw ApiError{MPS_ERR_INVALID, "bad device"};
+    if (addr.size() <= size_t(device)) addr.resize(size_t(device) + 1, nullptr);
+    if (!addr[device]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw ApiError{MPS_ERR_HIP, "no device address for the no-alloc scratch"};
+        addr[device] = static_cast<unsigned char*>(ptr);
+    }
+    return addr[device];
+}
+
 int bits_for(size_t count) {  // number of bits needed for values < count
     int b = 0;
     while ((size_t(1) << b) < count) b++;
@@ -107,6 +157,9 @@ struct mps_ctx {
     // N = fluid slots (scenes back to back, each starting on a kBlock boundary), Nref = fluid
     // particles (reference ids 0..Nref), B = boundary particles. With one scene N == Nref.
     uint32_t N = 0, Nref = 0, B = 0, frame = 0;
+    // largest fluid slots, wall particles and scenes a completed step has sized the buffers for
+    uint32_t sized_N = 0, sized_B = 0, sized_S = 0;
+    uint32_t zsorted_N = 0, zsorted_B = 0;  // the same for the Z-sort's temporaries
 
     // ------------------------------------------------------------------ scenes
     struct SceneHost {
@@ -389,7 +442,7 @@ struct mps_ctx {
         hipGraphExec_t exec = nullptr;
     };
     std::map<std::vector<uint64_t>, GraphEntry> graphs;
-    bool graphs_on() const { return cfg.use_graphs && !prof_on && !rec; }
+    bool graphs_on() const { return cfg.use_graphs && !prof_on && !rec && !g_no_alloc; }
     void clear_graphs() {
         for (auto& kv : graphs) {
             if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -504,6 +557,7 @@ struct mps_ctx {
     }
     void ensure_solvers() {
         if (solver_cap >= S) return;
+        if (g_no_alloc) refuse_alloc();
         if (d_solver) HIPCHK(hipFree(d_solver));
         if (h_solver) HIPCHK(hipHostFree(h_solver));
         d_solver = nullptr;
@@ -1528,7 +1582,61 @@ struct mps_ctx {
             sstats[0].nnz_bb = nnz_bb;
         }
         frame++;
+        sized_N = std::max(sized_N, N);
+        sized_B = std::max(sized_B, B);
+        sized_S = std::max(sized_S, S);
+        if (st->zsorted) {
+            zsorted_N = std::max(zsorted_N, N);
+            zsorted_B = std::max(zsorted_B, B);
+        }
         if (prof_on) prof_collect();
+    }
+
+    // One step that allocates nothing: the step's reduction scratch comes from the device's
+    // `scratch` array, and any context buffer that would have to grow makes the call fail
+    // (before any physics is applied) instead of allocating. Graph capture and kernel timing
+    // allocate, so the step launches directly and timing must be off.
+    void step_noalloc(mps_step_stats* st) {
+        if (prof_on) throw ApiError{MPS_ERR_STATE, "mps_step_noalloc: kernel timing allocates events; disable it"};
+        if (std::max(N, B) > kNoAllocSlots || S > kNoAllocScenes)
+            throw ApiError{MPS_ERR_INVALID, "mps_step_noalloc: more than 8388608 particles per set or 64 scenes "
+                                            "(the size of its scratch); use mps_step"};
+        if (N > sized_N || B > sized_B || S > sized_S ||
+            (cfg.zsort_interval && (frame % cfg.zsort_interval) == 1u && (N > zsorted_N || B > zsorted_B)))
+            refuse_alloc();
+        unsigned char* base = scratch_address(device);
+        struct Swap {
+            mps_ctx* c;
+            double* partials;
+            size_t partials_cap;
+            uint2* nstat;
+            size_t nstat_cap;
+            NbrCounters* ctr;
+            SolverDev* solver;
+            uint32_t solver_cap;
+            uint32_t* active;
+            ~Swap() {
+                g_no_alloc = false;
+                if (!c) return;
+                c->partials.p = partials; c->partials.cap = partials_cap;
+                c->nstat.p = nstat; c->nstat.cap = nstat_cap;
+                c->d_ctr = ctr; c->d_solver = solver; c->solver_cap = solver_cap; c->d_active = active;
+            }
+        } sw{nullptr, partials.p, partials.cap, nstat.p, nstat.cap, d_ctr, d_solver, solver_cap, d_active};
+        g_no_alloc = true;
+        sync_params();  // parameter upload; the solver records below replace the context's
+        sw.c = this;
+        partials.p = reinterpret_cast<double*>(base);
+        partials.cap = kScrPartials;
+        nstat.p = reinterpret_cast<uint2*>(base + kScrOffNstat);
+        nstat.cap = kScrNstat;
+        d_ctr = reinterpret_cast<NbrCounters*>(base + kScrOffCtr);
+        d_solver = reinterpret_cast<SolverDev*>(base + kScrOffSolver);
+        solver_cap = kNoAllocScenes;
+        d_active = reinterpret_cast<uint32_t*>(base + kScrOffActive);
+        // the records keep the last call's contents: start them as a fresh context would
+        for (int k = 0; k < 3; k++) mps::launch::solver_reset(d_solver + size_t(k) * S, S, d_active + k, stream);
+        step(st);
     }
 
     // ------------------------------------------------------------------ setup
@@ -2371,6 +2479,7 @@ int mps_create(const mps_config* cfg, mps_ctx** out) {
         std::memset(c->h_flags, 0, 8 * sizeof(uint32_t));
         mps::launch::zero_words(c->d_flags, 8, c->stream);
         c->ensure_solvers();
+        (void)scratch_address(c->device);  // once per device, with the device current
         c->sync();
     });
     if (rc != MPS_OK) {
@@ -2484,6 +2593,16 @@ int mps_sample_boundary_obj(mps_ctx* c, const char* path, const double* center, 
 int mps_step(mps_ctx* c, mps_step_stats* out) {
     if (!c) return MPS_ERR_INVALID;
     int rc = guarded(c, [&] { c->step(out); });
+    if (rc == MPS_OK && out && out->nonfinite) {
+        c->err = "non-finite position or velocity after the step";
+        rc = MPS_ERR_NONFINITE;
+    }
+    return rc;
+}
+
+int mps_step_noalloc(mps_ctx* c, mps_step_stats* out) {
+    if (!c) return MPS_ERR_INVALID;
+    int rc = guarded(c, [&] { c->step_noalloc(out); });
     if (rc == MPS_OK && out && out->nonfinite) {
         c->err = "non-finite position or velocity after the step";
         rc = MPS_ERR_NONFINITE;
