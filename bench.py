@@ -338,6 +338,9 @@ def main() -> int:
     ap.add_argument("--graphs", default="auto", choices=("auto", "0", "1"),
                     help="replay solver iteration chunks from HIP graphs: auto = the library's autotuner "
                          "picks graphs on/off and the host-poll interval after the warmup")
+    ap.add_argument("--cg-stream-coeffs", type=int, default=None, choices=(0, 1),
+                    help="viscosity operator: 1 streams both pair coefficients, 0 streams a_ij and recomputes "
+                         "b_ij (default: the library's configuration default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="launch the library's plain copy kernel (k_calib_copy) 3 times on 512 MiB right before "
                          "and right after the timed window: hardware byte counters are scaled against a copy of "
@@ -355,6 +358,8 @@ def main() -> int:
     cfg = scenes.make_config(scene)
     if args.graphs != "auto":
         cfg.use_graphs = int(args.graphs)
+    if args.cg_stream_coeffs is not None:
+        cfg.cg_stream_coeffs = args.cg_stream_coeffs
     sim = HipSim(cfg)
     scenes.load_into(sim, scene)
     for _ in range(args.warmup):
@@ -497,6 +502,7 @@ def main() -> int:
                         f"(timed after {args.settle} untimed steps from the initial lattice)",
             "window": [args.settle, args.settle + args.steps - 1],
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
+            "cg_stream_coeffs": int(cfg.cg_stream_coeffs),
             "solver_tuning": tuned,
         },
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,

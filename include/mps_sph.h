@@ -183,7 +183,22 @@ typedef struct mps_config {
                                     run jointly, ending when every scene meets its tolerance;
                                     viscosity and surface tension stay within each scene. 0: the
                                     scenes are independent. */
-    uint32_t reserved[3];
+    uint32_t cg_stream_coeffs;   /* HIP library: 1 = the viscosity operator streams both pair
+                                    coefficients (a_ij, b_ij) per iteration (8 bytes per pair);
+                                    0 = it streams a_ij and recomputes b_ij from the positions
+                                    (4 bytes per pair, one square root and one division). Same
+                                    results; mps_autotune may change it. */
+    uint32_t persistent_solver;  /* HIP library: nonzero = the viscosity CG runs all its iterations in
+                                    one cooperative launch (grid-wide barriers, every workgroup folds
+                                    the sums itself) when every workgroup of the context fits the
+                                    device at once; otherwise, and under kernel timing, the chunked
+                                    launches. 1: barriers with agent-scope fences (each workgroup
+                                    writes back and invalidates its L2); 2: the vectors and sums that
+                                    cross workgroups are read and written device-coherently, and the
+                                    barrier only counts arrivals. Same results; mps_autotune may
+                                    change it (it tries 0 and 2). */
+    uint32_t persistent_spin;    /* polls before a barrier wait of the persistent solver gives up
+                                    (the solve then reruns with chunked launches); 0 = 4194304 */
     double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
 } mps_config;
 
@@ -214,7 +229,9 @@ typedef struct mps_step_stats {
     uint32_t halo_max;
     uint32_t halo_spill;
     uint32_t halo_blocks;
-    uint32_t halo_pad;
+    /* how the viscosity CG ran (HIP library): 0 chunked launches, 1 one persistent cooperative
+     * launch, 2 the persistent launch gave up at a barrier and the solve reran chunked */
+    uint32_t cg_persistent;
 } mps_step_stats;
 
 typedef struct mps_ctx mps_ctx;
@@ -405,14 +422,21 @@ int mps_adj_backward_traj(mps_ctx* ctx, uint32_t n_obs, const uint32_t* obs_step
 
 /* ---- autotuning ----------------------------------------------------------------------------
  * Times `steps` steps of the current state under each launch setting that leaves results
- * unchanged (graph replay of solver chunks on/off; host polling every 1, 4 or 16 iterations or
- * adaptive), keeps the fastest in the context's configuration and restores the state. */
+ * unchanged and keeps the fastest in the context's configuration, one dimension after the
+ * other: graph replay of solver chunks on/off with host polling every 1, 4 or 16 iterations or
+ * adaptive; then the viscosity operator's coefficient stream (both coefficients, or a_ij with
+ * b_ij recomputed: 24- or 32-byte LDS records); then the persistent CG solver where the grid
+ * fits the device. The state is restored after every candidate. */
 typedef struct mps_tune_result {
     uint32_t use_graphs;        /* chosen */
     uint32_t host_sync_every;   /* chosen (0 = adaptive) */
     uint32_t n_tried;
     uint32_t pad;
     double ms_per_step[8];      /* per candidate: graphs off with polling {0,1,4,16}, then on */
+    uint32_t cg_stream_coeffs;  /* chosen with the launch setting above */
+    uint32_t persistent_solver; /* chosen last (1 only where the grid fits the device) */
+    double ms_coeffs[2];        /* cg_stream_coeffs 1, 0 */
+    double ms_persistent[2];    /* persistent_solver 0, 2 (2: 0.0 when not applicable) */
 } mps_tune_result;
 int mps_autotune(mps_ctx* ctx, uint32_t steps, mps_tune_result* out);
 

@@ -52,7 +52,8 @@ class Config(C.Structure):
         ("cg_max_iter", C.c_uint32), ("cg_factor", C.c_double), ("jacobi_max_iter", C.c_uint32),
         ("jacobi_tol", C.c_double), ("zsort_interval", C.c_uint32), ("nei_capacity", C.c_uint32),
         ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
-        ("halo_limit", C.c_uint32), ("couple_scenes", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("halo_limit", C.c_uint32), ("couple_scenes", C.c_uint32), ("cg_stream_coeffs", C.c_uint32),
+        ("persistent_solver", C.c_uint32), ("persistent_spin", C.c_uint32),
         ("adj_cg_tol", C.c_double),
     ]
 
@@ -79,7 +80,8 @@ class StepStats(C.Structure):
         ("n_boundary", C.c_uint32), ("nei_regrow", C.c_uint32), ("nnz_ff", C.c_uint64),
         ("nnz_fb", C.c_uint64), ("nnz_bb", C.c_uint64), ("err_df", C.c_double * MAX_ITER_LOG),
         ("err_cd", C.c_double * MAX_ITER_LOG), ("rr_cg", C.c_double * MAX_ITER_LOG),
-        ("halo_max", C.c_uint32), ("halo_spill", C.c_uint32), ("halo_blocks", C.c_uint32), ("halo_pad", C.c_uint32),
+        ("halo_max", C.c_uint32), ("halo_spill", C.c_uint32), ("halo_blocks", C.c_uint32),
+        ("cg_persistent", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:
@@ -100,7 +102,8 @@ class Fence(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("use_graphs", C.c_uint32), ("host_sync_every", C.c_uint32), ("n_tried", C.c_uint32),
-                ("pad", C.c_uint32), ("ms_per_step", C.c_double * 8)]
+                ("pad", C.c_uint32), ("ms_per_step", C.c_double * 8), ("cg_stream_coeffs", C.c_uint32),
+                ("persistent_solver", C.c_uint32), ("ms_coeffs", C.c_double * 2), ("ms_persistent", C.c_double * 2)]
 
 
 class SceneRange(C.Structure):
@@ -359,7 +362,9 @@ class HipSim:
         r = TuneResult()
         self._chk(self.lib.mps_autotune(self.h, int(steps), C.byref(r)))
         return dict(use_graphs=r.use_graphs, host_sync_every=r.host_sync_every,
-                    ms_per_step=list(r.ms_per_step[:r.n_tried]))
+                    ms_per_step=list(r.ms_per_step[:8]), cg_stream_coeffs=r.cg_stream_coeffs,
+                    ms_coeffs=list(r.ms_coeffs), persistent_solver=r.persistent_solver,
+                    ms_persistent=list(r.ms_persistent))
 
     # ---- adjoint
     def adj_rollout(self, n_steps: int, checkpoint_every: int = 10) -> np.ndarray:

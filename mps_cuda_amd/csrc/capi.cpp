@@ -275,6 +275,8 @@ struct mps_ctx {
     // per neighbour slot: G_ij (ff, fb) for the pressure solves, a_ij / b_ij for the CG operator
     DBuf<float> pairG_ff, pairG_fb;
     DBuf<float2> pairAB;  // (a_ij, b_ij) of the implicit viscosity operator
+    DBuf<float> pairA;    // a_ij alone, when A.p recomputes b_ij (cfg.cg_stream_coeffs == 0)
+    DBuf<float> irho;     // 1/rho per slot for that recomputation
     // block halos of the fluid lists for A.p (halo.hip): sorted distinct neighbour slots per
     // workgroup, their count, and every list entry as a 16-bit position in its block's halo
     DBuf<uint32_t> halo_idx, halo_n;
@@ -292,6 +294,9 @@ struct mps_ctx {
     uint32_t solver_cap = 0;        // scenes the two arrays above hold
     uint32_t* d_active = nullptr;   // [3]: scenes whose DF / CD / CG loop is still running
     uint32_t* h_active = nullptr;
+    mps::GridSync* d_grid = nullptr;  // barrier words of the persistent solver (allocated at creation)
+    uint32_t persist_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident workgroups per kernel variant (0 = not asked yet)
+    uint32_t persist_fallbacks = 0;       // persistent solves that gave up and reran chunked
     uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch, [4..6] halo statistics of the step
     uint32_t* h_flags = nullptr;
     double* d_err3 = nullptr;
@@ -1029,30 +1034,74 @@ struct mps_ctx {
         ensure_pairs();
         const mps::Scenes P = scn();
         const size_t m = std::max<uint32_t>(N, 1);
-        pairAB.reserve(size_t(Kff) * m);
+        mps::launch::CgOperator op;
+        op.recompute = cfg.cg_stream_coeffs ? 0 : 1;
+        op.cubic = cubic();
+        if (op.recompute) {
+            pairA.reserve(size_t(Kff) * m);
+            irho.reserve(m);
+            op.ca = pairA.p;
+            op.vel = vel.p;
+            op.rho = rho.p;
+            op.irho = irho.p;
+        } else {
+            pairAB.reserve(size_t(Kff) * m);
+            op.cab = pairAB.p;
+        }
         v3x.reserve(m); v3r.reserve(m); v3Ap.reserve(m); v3p.reserve(m); v3q.reserve(m);
         ensure_halo();
         const mps::Halo hl = halo();
         T_(K_ST_FACTOR, [&] { mps::launch::st_factor(P, nbr(), hl, pos.p, vel.p, pairG_ff.p, fst.p, cgx.p, stream); });
         T_(K_CG_SETUP, [&] {
-            mps::launch::cg_setup(cubic(), P, nbr(), hl, pos.p, vel.p, rho.p, fst.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
+            mps::launch::cg_setup(cubic(), P, nbr(), hl, pos.p, vel.p, rho.p, fst.p, op.recompute ? pairA.p : nullptr,
+                                  op.recompute ? nullptr : pairAB.p, cgb.p, op.recompute ? irho.p : nullptr, stream);
         });
         mps::launch::solver_reset(sd, S, act, stream);
         const double tol = cfg.cg_tol;
         // p of iteration k is written to pbuf(k) while pbuf(k - 1) is read; p_0 = r
         auto pbuf = [&](uint32_t k) { return (k & 1u) ? v3q.p : v3p.p; };
         T_(K_CG_INIT, [&] {
-            mps::launch::cg_apply(0, P, nbr(), hl, pairAB.p, pos.p, cgx.p, nullptr, nullptr, pbuf(0), nullptr, cgb.p, v3r.p,
+            mps::launch::cg_apply(0, op, P, nbr(), hl, pos.p, cgx.p, nullptr, nullptr, pbuf(0), nullptr, cgb.p, v3r.p,
                                   v3x.p, sd, partials.p, stream);
         });
         T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
         const uint32_t cap = cfg.cg_max_iter;
         uint32_t it = 1;
         bool done = false;
+        // all iterations in one cooperative launch when every workgroup fits the device at once
+        const uint32_t nblk = uint32_t((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock);
+        const int coh = cfg.persistent_solver == 2 ? 1 : 0;
+        const int variant = (op.recompute ? (op.cubic ? 1 : 2) : 0) + 3 * coh;
+        if (cfg.persistent_solver && !prof_on && nblk > 0) {
+            if (!persist_cap[variant]) persist_cap[variant] = std::max<uint32_t>(mps::launch::cg_persistent_capacity(op, coh), 1u);
+            if (nblk <= persist_cap[variant] && nblk <= mps::kGridGroup * mps::kGridGroups) {
+                mps::launch::zero_words(reinterpret_cast<uint32_t*>(d_grid), sizeof(mps::GridSync) / 4, stream);
+                const uint32_t spin = cfg.persistent_spin ? cfg.persistent_spin : (1u << 22);
+                mps::launch::cg_persistent(op, coh, P, nbr(), hl, pos.p, v3r.p, pbuf(0), pbuf(1), v3Ap.p, v3x.p, sd, act,
+                                           partials.p, partials.p + nblk, d_grid, cap, tol, spin, stream);
+                mps::GridSync g{};
+                HIPCHK(hipMemcpyAsync(&g, d_grid, sizeof(g), hipMemcpyDeviceToHost, stream));
+                sync();
+                if (!g.gave_up) {
+                    it = cap + 1;  // the launch ran every iteration the loop needed
+                    if (st) st->cg_persistent = 1;
+                } else {  // a barrier wait gave up: the whole solve again with chunked launches
+                    persist_fallbacks++;
+                    if (st) st->cg_persistent = 2;
+                    mps::launch::solver_reset(sd, S, act, stream);
+                    T_(K_CG_INIT, [&] {
+                        mps::launch::cg_apply(0, op, P, nbr(), hl, pos.p, cgx.p, nullptr, nullptr, pbuf(0), nullptr, cgb.p,
+                                              v3r.p, v3x.p, sd, partials.p, stream);
+                    });
+                    T_(K_FOLD, [&] { mps::launch::fold_cg(0, P, partials.p, sd, act, tol, stream); });
+                }
+            }
+        }
         while (it <= cap && !done) {
             const uint32_t chunk = chunk_len(it, pred[2], cap - it + 1);
             std::vector<uint64_t> key = key_base(3, chunk, P, nbr());
-            for (const void* q : {(const void*)pairAB.p, (const void*)pos.p, (const void*)pbuf(it - 1), (const void*)pbuf(it),
+            for (const void* q : {(const void*)op.cab, (const void*)op.ca, (const void*)op.vel, (const void*)op.rho,
+                                  (const void*)op.irho, (const void*)pos.p, (const void*)pbuf(it - 1), (const void*)pbuf(it),
                                   (const void*)v3Ap.p, (const void*)v3r.p, (const void*)v3x.p, (const void*)sd,
                                   (const void*)partials.p, (const void*)act, (const void*)hl.idx, (const void*)hl.n,
                                   (const void*)hl.nb16})
@@ -1062,7 +1111,7 @@ struct mps_ctx {
                 for (uint32_t c = 0; c < chunk; c++) {
                     const uint32_t k = it + c;
                     T_(K_CG_APPLY, [&] {
-                        mps::launch::cg_apply(1, P, nbr(), hl, pairAB.p, pos.p, nullptr, v3r.p, pbuf(k - 1), pbuf(k), v3Ap.p,
+                        mps::launch::cg_apply(1, op, P, nbr(), hl, pos.p, nullptr, v3r.p, pbuf(k - 1), pbuf(k), v3Ap.p,
                                               nullptr, nullptr, nullptr, sd, partials.p, stream);
                     }, false);
                     T_(K_FOLD, [&] { mps::launch::fold_cg(1, P, partials.p, sd, act, tol, stream); }, false);
@@ -1349,7 +1398,7 @@ struct mps_ctx {
             pairAB.reserve(size_t(T.Kff) * m); cabT.reserve(size_t(T.Kff) * m); rev.reserve(size_t(T.Kff) * m);
             cgrecB.reserve(2 * m);
             mps::launch::st_factor(P, nb, mps::Halo{}, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
-            mps::launch::cg_setup(1, P, nb, mps::Halo{}, T.pos.p, vin, rho.p, fstr.p, pairG_ff.p, nullptr, pairAB.p, cgb.p, stream);
+            mps::launch::cg_setup(1, P, nb, mps::Halo{}, T.pos.p, vin, rho.p, fstr.p, nullptr, pairAB.p, cgb.p, nullptr, stream);
             mps::launch::zero_words(d_flags + 2, 1, stream);
             mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
             mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);
@@ -1457,24 +1506,29 @@ struct mps_ctx {
         sync_params();
         Checkpoint c;
         save_ck(c);
+        mps_step_stats st;
+        // ms per step of `steps` steps from the saved state under the current configuration
+        auto time_steps = [&]() {
+            restore_ck(c);
+            step(&st);  // captures graphs, sizes buffers
+            restore_ck(c);
+            sync();
+            const auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t q = 0; q < steps; q++) step(&st);
+            sync();
+            return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        };
+        mps_tune_result r{};
+        // 1. graph replay and the host poll interval
         const uint32_t polls[4] = {0u, 1u, 4u, 16u};
         double best = 1e300;
         uint32_t bg = cfg.use_graphs, bh = cfg.host_sync_every, n = 0;
-        mps_step_stats st;
         for (uint32_t g = 0; g < 2; g++)
             for (uint32_t k = 0; k < 4; k++) {
                 cfg.use_graphs = g;
                 cfg.host_sync_every = polls[k];
-                restore_ck(c);
-                step(&st);  // captures graphs, sizes buffers
-                restore_ck(c);
-                sync();
-                const auto t0 = std::chrono::steady_clock::now();
-                for (uint32_t q = 0; q < steps; q++) step(&st);
-                sync();
-                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-                if (out) out->ms_per_step[n] = ms;
-                n++;
+                const double ms = time_steps();
+                r.ms_per_step[n++] = ms;
                 if (ms < best) {
                     best = ms;
                     bg = g;
@@ -1483,12 +1537,51 @@ struct mps_ctx {
             }
         cfg.use_graphs = bg;
         cfg.host_sync_every = bh;
+        // 2. the viscosity operator's coefficient stream (only where the CG runs)
+        uint32_t bc = cfg.cg_stream_coeffs;
+        if (cfg.visc_solver == MPS_VISC_CG) {
+            best = 1e300;
+            for (uint32_t v = 0; v < 2; v++) {
+                cfg.cg_stream_coeffs = 1u - v;
+                const double ms = time_steps();
+                r.ms_coeffs[v] = ms;
+                n++;
+                if (ms < best) {
+                    best = ms;
+                    bc = 1u - v;
+                }
+            }
+            cfg.cg_stream_coeffs = bc;
+        }
+        // 3. the persistent CG solver, where every workgroup fits the device at once
+        uint32_t bp = 0;
+        if (cfg.visc_solver == MPS_VISC_CG) {
+            cfg.persistent_solver = 0;
+            r.ms_persistent[0] = time_steps();
+            n++;
+            mps::launch::CgOperator op;
+            op.recompute = cfg.cg_stream_coeffs ? 0 : 1;
+            op.cubic = cubic();
+            const int variant = (op.recompute ? (op.cubic ? 1 : 2) : 0) + 3;
+            if (!persist_cap[variant]) persist_cap[variant] = std::max<uint32_t>(mps::launch::cg_persistent_capacity(op, 1), 1u);
+            const uint32_t nblk = uint32_t((size_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock);
+            if (nblk <= persist_cap[variant] && nblk <= mps::kGridGroup * mps::kGridGroups) {
+                cfg.persistent_solver = 2;
+                r.ms_persistent[1] = time_steps();
+                n++;
+                if (r.ms_persistent[1] < r.ms_persistent[0]) bp = 2;
+            }
+        }
+        cfg.persistent_solver = bp;
         restore_ck(c);
         sync();
         c.release();
         if (out) {
+            *out = r;
             out->use_graphs = bg;
             out->host_sync_every = bh;
+            out->cg_stream_coeffs = bc;
+            out->persistent_solver = bp;
             out->n_tried = n;
         }
     }
@@ -1575,6 +1668,7 @@ struct mps_ctx {
             o.zsorted = st->zsorted;
             o.nonfinite = st->nonfinite;
             o.nei_regrow = st->nei_regrow;
+            o.cg_persistent = st->cg_persistent;
         }
         if (S == 1) {  // totals of the one scene
             sstats[0].nnz_ff = nnz_ff;
@@ -2294,7 +2388,7 @@ struct mps_ctx {
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); nfr.release(); cfr.release();
         bobj.release(); bS4.release();
         partials.release();
-        pairG_ff.release(); pairG_fb.release(); pairAB.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
+        pairG_ff.release(); pairG_fb.release(); pairAB.release(); pairA.release(); irho.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
         if (copy_stream) (void)hipStreamSynchronize(copy_stream);
         for (ReadbackSlot& sl : rb) {
             sl.buf.release();
@@ -2318,6 +2412,7 @@ struct mps_ctx {
         if (d_ctr) (void)hipFree(d_ctr);
         if (d_solver) (void)hipFree(d_solver);
         if (d_flags) (void)hipFree(d_flags);
+        if (d_grid) (void)hipFree(d_grid);
         if (d_err3) (void)hipFree(d_err3);
         if (h_ctr) (void)hipHostFree(h_ctr);
         if (h_solver) (void)hipHostFree(h_solver);
@@ -2450,6 +2545,7 @@ int mps_config_init(mps_config* cfg) {
     cfg->host_sync_every = 0;
     cfg->adj_cg_max_iter = 300;
     cfg->adj_cg_tol = -1.0e-12;
+    cfg->cg_stream_coeffs = 1;  // measured faster on the 1M dam break (recomputing b_ij is VALU-bound)
     return MPS_OK;
 }
 
@@ -2472,6 +2568,7 @@ int mps_create(const mps_config* cfg, mps_ctx** out) {
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_active), 4 * sizeof(uint32_t)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_active), 4 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_flags), 8 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_grid), sizeof(mps::GridSync)));
         HIPCHK(hipMalloc(reinterpret_cast<void**>(&c->d_err3), 4 * sizeof(double)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_ctr), sizeof(NbrCounters)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_flags), 8 * sizeof(uint32_t)));

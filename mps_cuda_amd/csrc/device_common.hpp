@@ -70,6 +70,29 @@ __device__ __forceinline__ float Gk(float dist, float invh) {
     const float tmp = (1.0f - ratio) * invh2;
     return -kG / dist * tmp * tmp;
 }
+// Pair factor of the viscosity operator, G(d) / ((d^2 + h2) (d + FLT_EPSILON))
+// (MPSSPHNonPressureUtil.cuh:617-618), regrouped to one division: the 1/d inside G joins the
+// denominators, Phi = (G d) / ((d (d^2 + h2)) (d + FLT_EPSILON)). The oracle evaluates the
+// same expression in the same order (its Phik).
+template <bool CUBIC>
+__device__ __forceinline__ float Phik(float dist, float invh, float h2) {
+    const float ratio = dist * invh;
+    if (ratio < kEps || ratio >= 1.0f) return 0.0f;
+    const float invh2 = invh * invh;
+    float num;
+    if (CUBIC) {
+        if (ratio > 0.5f) {
+            const float temp = (1.0f - ratio) * invh2;
+            num = -kCubicG * temp * temp;
+        } else {
+            num = kCubicG * invh2 * invh2 * (3.0f * ratio - 2.0f) * ratio;
+        }
+    } else {
+        const float tmp = (1.0f - ratio) * invh2;
+        num = -kG * tmp * tmp;
+    }
+    return num / ((dist * (dist * dist + h2)) * (dist + kFltEps));
+}
 // surface-tension kernels (MPSSPHKernel.cuh:67-84)
 __device__ __forceinline__ float STWk(float dist, float invh) {
     float ratio = dist * invh;

@@ -34,6 +34,14 @@ struct NbrCounters {
     unsigned long long nnz_ff, nnz_fb, nnz_bb, nnz_fr;
 };
 
+// Barrier words of the persistent solver kernels (zeroed before each launch): arrivals per group
+// of kGridGroup workgroups, arrivals of whole groups, the barrier generation, the give-up flag.
+constexpr uint32_t kGridGroup = 16;
+constexpr uint32_t kGridGroups = 128;  // up to 2048 workgroups
+struct GridSync {
+    uint32_t group[kGridGroups];
+    uint32_t count, gen, gave_up, pad;
+};
 // Device-resident state of one iterative solve. The fold kernels finalise the sums here,
 // so the host only needs to look at `done`/`l`.
 struct SolverDev {
@@ -205,7 +213,7 @@ void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* po
                float4* cgx, hipStream_t s);
 // pair coefficients of the implicit operator: a_ij into ca and/or (a_ij, b_ij) into cab (either may be null)
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
-              const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s);
+              const float* fst, float* ca, float2* cab, float4* cgb, float* irho, hipStream_t s);
 // The adjoint's transposed solve keeps p in 32-byte records with the position:
 // rec[2i] = (x, y, z, p.x), rec[2i+1] = (p.y, p.z, 0, 0)
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s);
@@ -213,9 +221,31 @@ void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipSt
 // mode 0: A x0 with f = 0 (x0 from the float4 array x0), then r = b - A x0, pout = r, x = x0, partial r.r;
 // mode 1: p = r + beta pprev (r in rr, beta from the record), A p with f into Ap, p into pout,
 // partial p.Ap. Halo records are staged in LDS (kHaloCap).
-void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
+// The implicit operator's coefficients: streamed (a_ij, b_ij) pairs (cab), or a_ij (ca) with
+// b_ij recomputed from positions, masses (vel.w) and 1/rho (irho) of the CG set-up.
+struct CgOperator {
+    int recompute = 0;
+    int cubic = 1;
+    const float2* cab = nullptr;
+    const float* ca = nullptr;
+    const float4* vel = nullptr;
+    const float* rho = nullptr;
+    const float* irho = nullptr;
+};
+void cg_apply(int mode, const CgOperator& op, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos,
               const float4* x0, const dev::f3* rr, const dev::f3* pprev, dev::f3* pout, dev::f3* Ap, const float4* b,
               dev::f3* r, dev::f3* x, const SolverDev* st, double* partials, hipStream_t s);
+// Workgroups of the persistent CG kernel that fit the device at once (0 if unknown).
+// coherent: the barrier variant (0: agent-scope fences; 1: device-coherent accesses of the data
+// that crosses workgroups, no cache write-back or invalidation)
+uint32_t cg_persistent_capacity(const CgOperator& op, int coherent);
+// CG iterations 1..cap in one cooperative launch of one workgroup per 256-slot block (all must be
+// co-resident: nblk <= cg_persistent_capacity). r, the two p buffers (p_0 = r in pbuf0), A p and
+// x as in cg_apply / cg_update3; partA/partB: two partial-sum buffers of nblk doubles; gs: the
+// barrier words; spin: polls before a barrier wait gives up (gs->gave_up is then set).
+void cg_persistent(const CgOperator& op, int coherent, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, dev::f3* r,
+                   dev::f3* pbuf0, dev::f3* pbuf1, dev::f3* Ap, dev::f3* x, SolverDev* st, uint32_t* active, double* partA,
+                   double* partB, GridSync* gs, uint32_t cap, double tol, uint32_t spin, hipStream_t s);
 // ---- halo.hip
 // stats (3 words, accumulated): largest staged halo, blocks over kHaloCap, blocks with particles
 // limit: largest halo staged (0 or > kHaloCap: kHaloCap)

@@ -56,10 +56,40 @@ __device__ __forceinline__ void walk_walls(const Nbr& nb, uint32_t i, uint32_t N
 }
 
 // Block-reduce `v` (double) in a fixed order and publish the workgroup's partial.
+// COH: device-coherent store (agent scope), for the persistent solver's barrier without fences.
+template <bool COH = false>
 __device__ __forceinline__ void block_partial(double v, double* __restrict__ partials, uint32_t blk) {
     __shared__ double lds[kBlock / 64];
     const double bs = block_sum_d(v, lds);
-    if (threadIdx.x == 0) partials[blk] = bs;
+    if (threadIdx.x == 0) {
+        if constexpr (COH) __hip_atomic_store(partials + blk, bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else partials[blk] = bs;
+    }
+}
+// Loads and stores of the CG vectors that other workgroups read within one launch of the
+// persistent solver: device-coherent (agent scope) when COH, so that no cache write-back or
+// invalidation is needed at its barriers; plain otherwise.
+template <bool COH>
+__device__ __forceinline__ f3 ld3c(const f3* p) {
+    if constexpr (COH) {
+        const float* q = reinterpret_cast<const float*>(p);
+        return {__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+    } else {
+        return *p;
+    }
+}
+template <bool COH>
+__device__ __forceinline__ void st3c(f3* p, f3 v) {
+    if constexpr (COH) {
+        float* q = reinterpret_cast<float*>(p);
+        __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(q + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
 }
 __device__ __forceinline__ void block_partial(double v, double* __restrict__ partials) {
     block_partial(v, partials, blockIdx.x);
@@ -800,13 +830,16 @@ __device__ __forceinline__ float st_g(const StepParams& P, f3 vij, f3 xij, float
 // Once per solve: right-hand side (ComputeSurfaceTensionCGb_kernel, :467-521) and the pair
 // coefficients of the implicit operator (ST :523-585, viscosity :587-635, final :637-648):
 //   a_ij = dt (cs_i m_ij g_ij + cx (m_j/rho_j) W_ij),   b_ij = -dt cv_i c_ij,
-//   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON),
+//   c_ij = (0.5(m_i+m_j) (1/rho_j)) Phi(d_ij)   (the reference's (0.5(m_i+m_j)/rho_j) /
+//          (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON) regrouped, see Phik),
 //   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
+// Writes a_ij (ca) and/or (a_ij, b_ij) (cab), and 1/rho_i per slot (irho) for the A.p that
+// recomputes b_ij.
 template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float* __restrict__ rho,
-                                                   const float* __restrict__ fst, const float* __restrict__ pG_ff,
-                                                   float* __restrict__ ca, float2* __restrict__ cab, float4* __restrict__ cgb) {
+                                                   const float* __restrict__ fst, float* __restrict__ ca, float2* __restrict__ cab, float4* __restrict__ cgb,
+                                                   float* __restrict__ irho) {
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t sid = sc.blk_scene[blk];
     const StepParams P = sc.P[sid];
@@ -821,14 +854,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
     const bool staged = H <= kHaloCap;
     constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
-    float gn[U];
     auto fetch = [&](uint32_t k0) {
 #pragma unroll
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
                 jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
-                gn[q] = __builtin_nontemporal_load(pG_ff + slot);
             }
         }
     };
@@ -854,42 +885,42 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
     const float cs = P.dt * P.sigma / mi;
     const float cx = P.xsph / P.dt;
     const float cv = 10.0f * P.rho0 * P.mu / di;
+    if (irho) irho[i] = 1.0f / di;
     f3 bi{0.f, 0.f, 0.f};
-    auto pair = [&](size_t slot, f3 xj, f3 vj, float mj, float dj, float fj, float G) {
+    auto pair = [&](size_t slot, f3 xj, f3 vj, float mj, float dj, float fj) {
         const f3 xij = xi - xj;
         const float dist = len3(xij);
         float mij;
         const float gij = st_g(P, vi - vj, xij, dist, invHi, mi, mj, di, dj, fi, fj, mij);
         const float sij = mij * gij;
         const float wij = mj / dj * Wk<CUBIC>(dist, invHi);
-        const float cij = 0.5f * (mi + mj) / dj / (dist * dist + onePercentHSq) * G / (dist + kFltEps);
         const float aij = P.dt * (cs * sij + cx * wij);
         if (ca) ca[slot] = aij;
-        if (cab) cab[slot] = make_float2(aij, -P.dt * cv * cij);
+        if (cab) {
+            const float irhoj = 1.0f / dj;
+            const float cij = 0.5f * (mi + mj) * irhoj * Phik<CUBIC>(dist, invHi, onePercentHSq);
+            cab[slot] = make_float2(aij, -P.dt * cv * cij);
+        }
         bi = bi - sij * xij;
     };
     for (uint32_t k = 0; k < c; k += U) {
         uint32_t j[U];
-        float G[U];
 #pragma unroll
-        for (uint32_t q = 0; q < U; q++) {
-            j[q] = jn[q];
-            G[q] = gn[q];
-        }
+        for (uint32_t q = 0; q < U; q++) j[q] = jn[q];
         if (k + U < c) fetch(k + U);
         if (staged) {
 #pragma unroll
             for (uint32_t q = 0; q < U; q++) {
                 if (k + q < c) {
                     const float4 e = la[j[q]], g = lb[j[q]];
-                    pair(size_t(k + q) * N + i, mk(e), f3{e.w, g.x, g.y}, g.z, g.w, lc[j[q]], G[q]);
+                    pair(size_t(k + q) * N + i, mk(e), f3{e.w, g.x, g.y}, g.z, g.w, lc[j[q]]);
                 }
             }
         } else {
             for (uint32_t q = 0; q < U && k + q < c; q++) {
                 const uint32_t jj = j[q];
                 const float4 vj = vel[jj];
-                pair(size_t(k + q) * N + i, mk(pos[jj]), mk(vj), vj.w, rho[jj], fst[jj], G[q]);
+                pair(size_t(k + q) * N + i, mk(pos[jj]), mk(vj), vj.w, rho[jj], fst[jj]);
             }
         }
     }
@@ -912,12 +943,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 }
 
 // (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ].
-// (a_ij, b_ij) are written once per solve by k_cg_setup and streamed here with non-temporal
-// loads (recomputing b_ij costs more ALU time than the 4 bytes it saves). The block's halo
-// records (x_j, p_j: 24 bytes) are staged in LDS first, with coalesced loads of the sorted
-// halo list, and each pair reads its neighbour there through the 16-bit list; a block whose
-// halo is over the limit gathers from global memory. Both paths visit the same pairs in the
-// same order, so the result does not depend on the path.
+// a_ij is written once per solve by k_cg_setup and streamed here with non-temporal loads. b_ij
+// is either streamed with it (RECOMP false: 8 bytes per pair) or recomputed here from the
+// geometry (RECOMP true: 4 bytes per pair streamed, one square root and one division per pair:
+// b_ij = (-dt cv_i) ((0.5 (m_i + m_j) (1/rho_j)) Phi(d_ij)), the expression k_cg_setup
+// evaluates). The block's halo records (x_j, p_j: 24 bytes; with m_j and 1/rho_j when b_ij is
+// recomputed: 32 bytes) are staged in LDS first, with coalesced loads of the sorted halo list,
+// and each pair reads its neighbour there through the 16-bit list; a block whose halo is over
+// the limit gathers from global memory. Both paths visit the same pairs in the same order, so
+// the result depends on neither the path nor RECOMP.
 // The search direction of UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133),
 // p = r + beta p_prev, is formed here for every record read (own and halo) instead of in a
 // pass of its own; the block writes its own p for the update kernel (pout, while pprev is
@@ -927,18 +961,17 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 // MODE 0: A x0 with f = 0 (x0 in x0) fused with InitCGResidual_kernel (MPSCGKernel.cuh:5-19):
 //         r = b - A x0, pout = r, x = x0, partial sums of r.r.
 // MODE 1: A p with f (r in rr), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
-template <int MODE>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
-                                                   const float4* __restrict__ pos, const float4* __restrict__ x0,
-                                                   const f3* __restrict__ rr, const f3* __restrict__ pprev,
-                                                   f3* __restrict__ pout, f3* __restrict__ Ap,
-                                                   const float4* __restrict__ b, f3* __restrict__ r, f3* __restrict__ x,
-                                                   const SolverDev* st, double* __restrict__ partials) {
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t sid = sc.blk_scene[blk];
-    if (MODE == 1 && st[sid].done) return;
+// One workgroup's share of A.p: block `blk` of scene `sid`, search-direction factor bta.
+template <int MODE, bool RECOMP, bool CUBIC, bool COH = false>
+__device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float bta, const Scenes& sc, const Nbr& nb,
+                                               const Halo& hl, const float2* __restrict__ cab, const float* __restrict__ ca,
+                                               const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                               const float* __restrict__ rho, const float* __restrict__ irho,
+                                               const float4* __restrict__ x0, const f3* __restrict__ rr,
+                                               const f3* __restrict__ pprev, f3* __restrict__ pout, f3* __restrict__ Ap,
+                                               const float4* __restrict__ b, f3* __restrict__ r, f3* __restrict__ x,
+                                               double* __restrict__ partials) {
     const StepParams P = sc.P[sid];
-    const float bta = MODE == 1 ? st[sid].beta : 0.0f;
     const uint32_t i = blk * blockDim.x + threadIdx.x;
     const uint32_t N = P.N;
     const bool live = i < P.end;
@@ -946,24 +979,33 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     // the vector the operator is applied to, at slot j
     auto vec = [&](uint32_t j) -> f3 {
         if (MODE == 0) return mk(x0[j]);
-        const f3 a = rr[j], q = pprev[j];
+        const f3 a = ld3c<COH>(rr + j), q = ld3c<COH>(pprev + j);
         return {a.x + bta * q.x, a.y + bta * q.y, a.z + bta * q.z};
     };
+    using LB = typename std::conditional<RECOMP, float4, float2>::type;
     __shared__ float4 la[kHaloCap];
-    __shared__ float2 lb[kHaloCap];
+    __shared__ LB lb[kHaloCap];
     const uint32_t H = hl.n[blk];
     const bool staged = H <= kHaloCap;  // same for the whole block
+    using Coef = typename std::conditional<RECOMP, float, float2>::type;
+    auto coef = [&](size_t slot) -> Coef {
+        if constexpr (RECOMP) {
+            return __builtin_nontemporal_load(ca + slot);
+        } else {
+            const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
+            return __builtin_bit_cast(float2, d);
+        }
+    };
     constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
-    float2 abn[U];
+    Coef abn[U];
     auto fetch = [&](uint32_t k0) {
 #pragma unroll
         for (uint32_t q = 0; q < U; q++) {
             if (k0 + q < c) {
                 const size_t slot = size_t(k0 + q) * N + i;
                 jn[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
-                const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
-                abn[q] = __builtin_bit_cast(float2, d);
+                abn[q] = coef(slot);
             }
         }
     };
@@ -971,53 +1013,79 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     // overlaps the staging's gathers instead of following them.
     constexpr uint32_t kPre = kListPrefetchApply;
     uint32_t jp[kPre];
-    float2 abp[kPre];
+    Coef abp[kPre];
 #pragma unroll
     for (uint32_t q = 0; q < kPre; q++) {
         if (q < c) {
             const size_t slot = size_t(q) * N + i;
             jp[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
-            const double d = __builtin_nontemporal_load(reinterpret_cast<const double*>(cab + slot));
-            abp[q] = __builtin_bit_cast(float2, d);
+            abp[q] = coef(slot);
         }
     }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
         for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
-            const float4 x = pos[s];
+            const float4 xs = pos[s];
             const f3 v = vec(s);
-            la[t] = make_float4(x.x, x.y, x.z, v.x);
-            lb[t] = make_float2(v.y, v.z);
+            la[t] = make_float4(xs.x, xs.y, xs.z, v.x);
+            if constexpr (RECOMP) lb[t] = make_float4(v.y, v.z, vel[s].w, irho[s]);
+            else lb[t] = make_float2(v.y, v.z);
         });
         __syncthreads();
     }
     double part = 0.0;
     if (live) {
-        const f3 xi = mk(pos[i]);
+        const float4 xi4 = pos[i];
+        const f3 xi = mk(xi4);
         const f3 pi = vec(i);
+        // b_ij's per-particle factors, as k_cg_setup forms them
+        float mi = 0.0f, ncv = 0.0f, invHi = 0.0f, h2 = 0.0f;
+        if constexpr (RECOMP) {
+            mi = vel[i].w;
+            const float cv = 10.0f * P.rho0 * P.mu / rho[i];
+            ncv = -P.dt * cv;
+            const float hi = xi4.w * 4.0f;
+            invHi = 1.0f / hi;
+            h2 = 0.01f * hi * hi;
+        }
         f3 acc{0.f, 0.f, 0.f};
-        auto pair = [&](float2 ab, f3 xj, f3 pj) {
+        // one pair: record (xj, pj) and, for RECOMP, (mj, 1/rho_j)
+        auto pair = [&](Coef ab, f3 xj, f3 pj, float mj, float irhoj) {
             const f3 xij = xi - xj;
             const f3 pij = pi - pj;
-            acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
+            if constexpr (RECOMP) {
+                const float dist = len3(xij);
+                const float cij = 0.5f * (mi + mj) * irhoj * Phik<CUBIC>(dist, invHi, h2);
+                const float bij = ncv * cij;
+                acc = acc + (ab * pij + (bij * dot3(pij, xij)) * xij);
+            } else {
+                (void)mj;
+                (void)irhoj;
+                acc = acc + (ab.x * pij + (ab.y * dot3(pij, xij)) * xij);
+            }
+        };
+        auto pair_lds = [&](Coef ab, uint32_t t) {
+            const float4 a = la[t];
+            const LB e = lb[t];
+            if constexpr (RECOMP) pair(ab, mk(a), f3{a.w, e.x, e.y}, e.z, e.w);
+            else pair(ab, mk(a), f3{a.w, e.x, e.y}, 0.0f, 0.0f);
+        };
+        auto pair_glb = [&](Coef ab, uint32_t j) {
+            if constexpr (RECOMP) pair(ab, mk(pos[j]), vec(j), vel[j].w, irho[j]);
+            else pair(ab, mk(pos[j]), vec(j), 0.0f, 0.0f);
         };
         if (kPre < c) fetch(kPre);
         // the prefetched entries, then batches, all in list order
 #pragma unroll
         for (uint32_t q = 0; q < kPre; q++) {
             if (q < c) {
-                if (staged) {
-                    const float4 a = la[jp[q]];
-                    const float2 e = lb[jp[q]];
-                    pair(abp[q], mk(a), f3{a.w, e.x, e.y});
-                } else {
-                    pair(abp[q], mk(pos[jp[q]]), vec(jp[q]));
-                }
+                if (staged) pair_lds(abp[q], jp[q]);
+                else pair_glb(abp[q], jp[q]);
             }
         }
         for (uint32_t k = kPre; k < c; k += U) {
             uint32_t j[U];
-            float2 ab[U];
+            Coef ab[U];
 #pragma unroll
             for (uint32_t q = 0; q < U; q++) {
                 j[q] = jn[q];
@@ -1026,15 +1094,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             if (k + U < c) fetch(k + U);
             if (staged) {
 #pragma unroll
-                for (uint32_t q = 0; q < U; q++) {
-                    if (k + q < c) {
-                        const float4 a = la[j[q]];
-                        const float2 e = lb[j[q]];
-                        pair(ab[q], mk(a), f3{a.w, e.x, e.y});
-                    }
-                }
+                for (uint32_t q = 0; q < U; q++)
+                    if (k + q < c) pair_lds(ab[q], j[q]);
             } else {  // halo over the limit: gather from global memory
-                for (uint32_t q = 0; q < U && k + q < c; q++) pair(ab[q], mk(pos[j[q]]), vec(j[q]));
+                for (uint32_t q = 0; q < U && k + q < c; q++) pair_glb(ab[q], j[q]);
             }
         }
         const float f = MODE == 0 ? 0.0f : P.cg_factor;
@@ -1047,12 +1110,29 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
             x[i] = pi;
             part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
         } else {
-            pout[i] = pi;
+            st3c<COH>(pout + i, pi);
             Ap[i] = o;
             part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
     }
-    block_partial(part, partials, blk);
+    block_partial<COH>(part, partials, blk);
+}
+
+template <int MODE, bool RECOMP, bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
+                                                   const float* __restrict__ ca, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                   const float* __restrict__ irho, const float4* __restrict__ x0,
+                                                   const f3* __restrict__ rr, const f3* __restrict__ pprev,
+                                                   f3* __restrict__ pout, f3* __restrict__ Ap,
+                                                   const float4* __restrict__ b, f3* __restrict__ r, f3* __restrict__ x,
+                                                   const SolverDev* st, double* __restrict__ partials) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
+    if (MODE == 1 && st[sid].done) return;
+    const float bta = MODE == 1 ? st[sid].beta : 0.0f;
+    cg_apply_block<MODE, RECOMP, CUBIC>(blk, sid, bta, sc, nb, hl, cab, ca, pos, vel, rho, irho, x0, rr, pprev, pout, Ap, b,
+                                        r, x, partials);
 }
 
 // Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
@@ -1122,18 +1202,15 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restr
 }
 
 // The update on packed triples (the forward solve): same expressions as k_cg_update.
-__global__ void __launch_bounds__(kBlock) k_cg_update3(Scenes sc, f3* __restrict__ x, f3* __restrict__ r,
-                                                     const f3* __restrict__ p, const f3* __restrict__ Ap,
-                                                     const SolverDev* st, double* __restrict__ partials) {
-    const uint32_t sid = sc.blk_scene[blockIdx.x];
-    if (st[sid].done) return;
-    const uint32_t end = sc.P[sid].end;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float a = st[sid].alpha;
+template <bool COH = false>
+__device__ __forceinline__ void cg_update3_block(uint32_t blk, uint32_t end, float a, f3* __restrict__ x,
+                                                 f3* __restrict__ r, const f3* __restrict__ p, const f3* __restrict__ Ap,
+                                                 double* __restrict__ partials) {
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     double part = 0.0;
     if (i < end) {
-        const f3 pi = p[i], api = Ap[i];
-        f3 xi = x[i], ri = r[i];
+        const f3 pi = ld3c<COH>(p + i), api = Ap[i];
+        f3 xi = x[i], ri = ld3c<COH>(r + i);
         xi.x = xi.x + a * pi.x;
         xi.y = xi.y + a * pi.y;
         xi.z = xi.z + a * pi.z;
@@ -1141,10 +1218,175 @@ __global__ void __launch_bounds__(kBlock) k_cg_update3(Scenes sc, f3* __restrict
         ri.y = ri.y - a * api.y;
         ri.z = ri.z - a * api.z;
         x[i] = xi;
-        r[i] = ri;
+        st3c<COH>(r + i, ri);
         part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
     }
-    block_partial(part, partials);
+    block_partial<COH>(part, partials, blk);
+}
+__global__ void __launch_bounds__(kBlock) k_cg_update3(Scenes sc, f3* __restrict__ x, f3* __restrict__ r,
+                                                     const f3* __restrict__ p, const f3* __restrict__ Ap,
+                                                     const SolverDev* st, double* __restrict__ partials) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    if (st[sid].done) return;
+    cg_update3_block(blockIdx.x, sc.P[sid].end, st[sid].alpha, x, r, p, Ap, partials);
+}
+
+// ----------------------------------------------------------------- persistent CG
+// The fold of the kFoldThreads-thread fold kernels, computed by one kBlock-thread workgroup in
+// exactly their order: thread t plays their threads t + kBlock v (v < kFoldThreads / kBlock);
+// each sums its strided partials left to right, each of their waves is reduced by the same
+// butterfly (the played threads of one of their waves all sit in one wave here, in the same
+// lanes), and thread 0 adds the wave totals in wave order. Valid in every thread.
+template <bool COH>
+__device__ __forceinline__ double fold_as_fold_kernel(const double* __restrict__ partials, uint32_t n) {
+    constexpr uint32_t V = kFoldThreads / kBlock;
+    constexpr uint32_t kWaves = kFoldThreads / 64;
+    __shared__ double wsum[kWaves];
+    __shared__ double total;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t v = 0; v < V; v++) {
+        double s = 0.0;
+        for (uint32_t k = threadIdx.x + v * kBlock; k < n; k += kFoldThreads) {
+            if constexpr (COH) s += __hip_atomic_load(partials + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else s += partials[k];
+        }
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) wsum[wave + v * (kBlock / 64)] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t w = 0; w < kWaves; w++) t += wsum[w];
+        total = t;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Grid-wide barrier of a co-resident grid (cooperative launch). The wait is bounded: after
+// `spin` polls, or when another workgroup gave up, it sets gs->gave_up and returns false, and
+// the caller leaves the kernel; every workgroup's wait then ends within `spin` polls.
+// COH false: agent-scope fences publish the workgroup's writes before it arrives and make the
+// others' visible after the barrier (each workgroup writes back and invalidates its L2).
+// COH true: the data that crosses workgroups is written and read device-coherently (ld3c,
+// st3c, coherent partials), so each wave only waits until its own stores are performed before
+// the workgroup arrives, and the barrier orders nothing else.
+template <bool COH>
+__device__ __forceinline__ bool grid_barrier(GridSync* gs, uint32_t nwg, uint32_t spin) {
+    __shared__ uint32_t ok_s;
+    if constexpr (COH) __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1u;
+        const uint32_t g = __hip_atomic_load(&gs->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (!COH) __threadfence();
+        // arrivals are counted per group of kGridGroup workgroups, and the last of each group
+        // counts the group: 16 + nwg/16 serialized atomics instead of nwg
+        const uint32_t grp = blockIdx.x / kGridGroup;
+        const uint32_t gsize = min(kGridGroup, nwg - grp * kGridGroup);
+        const uint32_t ngrp = (nwg + kGridGroup - 1u) / kGridGroup;
+        bool last = false;
+        if (__hip_atomic_fetch_add(&gs->group[grp], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == gsize) {
+            __hip_atomic_store(&gs->group[grp], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if constexpr (COH) __builtin_amdgcn_s_waitcnt(0);
+            if (__hip_atomic_fetch_add(&gs->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == ngrp) {
+                last = true;
+                __hip_atomic_store(&gs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if constexpr (COH) {
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+        if (!last) {
+            uint32_t polls = 0;
+            while (__hip_atomic_load(&gs->gen, COH ? __ATOMIC_RELAXED : __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(&gs->gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++polls > spin) {
+                    ok = 0u;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        if (!ok) __hip_atomic_store(&gs->gave_up, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if constexpr (!COH) __threadfence();
+        ok_s = ok && __hip_atomic_load(&gs->gave_up, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+    }
+    __syncthreads();
+    return ok_s != 0u;
+}
+
+// The CG iterations of CGSolver (MPSSolver.cuh:31-65) in one launch of a co-resident grid (one
+// workgroup per 256-slot block of every scene), for iterations 1..cap after the initial
+// residual (k_cg_apply<0> + k_fold_cg<0> ran before). Per iteration: A.p with the direction
+// update (cg_apply_block) -> barrier -> every workgroup folds p.Ap itself (the fold kernels'
+// order, so alpha has the same bits) -> x, r update (cg_update3_block) -> barrier -> r.r fold,
+// beta and the test. The scene's first workgroup keeps the solver record as the fold kernels
+// do. A scene that converged sits out; the loop ends for everyone when no scene is running
+// (the running count is read right after the first barrier, where no workgroup changes it) or
+// after iteration cap. Partial sums alternate between two buffers, so a fold never reads a
+// buffer that the next phase writes. Results are those of the chunked launches bit for bit.
+template <bool RECOMP, bool CUBIC, bool COH>
+__global__ void __launch_bounds__(kBlock) k_cg_persistent(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
+                                                        const float* __restrict__ ca, const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                        const float* __restrict__ irho, f3* __restrict__ r,
+                                                        f3* __restrict__ pbuf0, f3* __restrict__ pbuf1, f3* __restrict__ Ap,
+                                                        f3* __restrict__ x, SolverDev* stv, uint32_t* active,
+                                                        double* __restrict__ partA, double* __restrict__ partB,
+                                                        GridSync* gs, uint32_t cap, double tol, uint32_t spin) {
+    const uint32_t nwg = gridDim.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
+    const StepParams P = sc.P[sid];
+    SolverDev* st = stv + sid;
+    const bool writer = blk == P.blk0 && threadIdx.x == 0;
+    __shared__ uint32_t running_s;
+    bool done = st->done != 0u;
+    double rr = st->rr;
+    const double thr = st->thr;
+    float beta = st->beta;
+    for (uint32_t k = 1; k <= cap; k++) {
+        f3* pprev = (k & 1u) ? pbuf0 : pbuf1;  // p of iteration k - 1 (p_0 = r in pbuf0)
+        f3* pout = (k & 1u) ? pbuf1 : pbuf0;
+        if (!done)
+            cg_apply_block<1, RECOMP, CUBIC, COH>(blk, sid, beta, sc, nb, hl, cab, ca, pos, vel, rho, irho, nullptr, r, pprev,
+                                                  pout, Ap, nullptr, nullptr, nullptr, partA);
+        if (!grid_barrier<COH>(gs, nwg, spin)) return;
+        if (threadIdx.x == 0) running_s = __hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (running_s == 0u) return;
+        float alpha = 0.0f;
+        if (!done) {
+            const double pAp = fold_as_fold_kernel<COH>(partA + P.blk0, P.nblk);
+            alpha = float(rr / pAp);
+            if (writer) {
+                st->pAp = pAp;
+                st->alpha = alpha;
+            }
+            cg_update3_block<COH>(blk, P.end, alpha, x, r, pout, Ap, partB);
+        }
+        if (!grid_barrier<COH>(gs, nwg, spin)) return;
+        if (!done) {
+            const double s = fold_as_fold_kernel<COH>(partB + P.blk0, P.nblk);
+            beta = float(s / rr);
+            rr = s;
+            done = s < thr || (tol < 0.0 && s == 0.0);
+            if (writer) {
+                st->beta = beta;
+                st->rr = s;
+                if (k < MPS_MAX_ITER_LOG) st->log[k] = s;
+                st->l = k + 1u;
+                if (done) {
+                    st->done = 1u;
+                    atomicSub(active, 1u);
+                }
+            }
+        }
+    }
 }
 
 // UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133), p kept in the position records
@@ -1504,21 +1746,62 @@ void st_factor(const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* po
     if (sc.N) launch_k(k_st_factor, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, pG_ff, fst, cgx);
 }
 void cg_setup(int cubic, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const float4* vel, const float* rho,
-              const float* fst, const float* pG_ff, float* ca, float2* cab, float4* cgb, hipStream_t s) {
+              const float* fst, float* ca, float2* cab, float4* cgb, float* irho, hipStream_t s) {
     if (!sc.N) return;
-    if (cubic) launch_k(k_cg_setup<true>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
-    else launch_k(k_cg_setup<false>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, pG_ff, ca, cab, cgb);
+    if (cubic) launch_k(k_cg_setup<true>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, ca, cab, cgb, irho);
+    else launch_k(k_cg_setup<false>, nb_(sc.N), kBlock, 0, s, sc, nb, hl, pos, vel, rho, fst, ca, cab, cgb, irho);
 }
 void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipStream_t s) {
     if (N) launch_k(k_cg_pack, nb_(N), kBlock, 0, s, N, pos, x0, rec);
 }
-void cg_apply(int mode, const Scenes& sc, const Nbr& nb, const Halo& hl, const float2* cab, const float4* pos,
+void cg_apply(int mode, const CgOperator& op, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos,
               const float4* x0, const f3* rr, const f3* pprev, f3* pout, f3* Ap, const float4* b, f3* r, f3* x,
               const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-    if (mode == 0) launch_k(k_cg_apply<0>, g, kBlock, 0, s, sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
-    else launch_k(k_cg_apply<1>, g, kBlock, 0, s, sc, nb, hl, cab, pos, x0, rr, pprev, pout, Ap, b, r, x, st, partials);
+#define MPS_CG_APPLY(M, R, C)                                                                                             \
+    launch_k(k_cg_apply<M, R, C>, g, kBlock, 0, s, sc, nb, hl, op.cab, op.ca, pos, op.vel, op.rho, op.irho, x0, rr, pprev, \
+             pout, Ap, b, r, x, st, partials)
+    if (!op.recompute) {
+        if (mode == 0) MPS_CG_APPLY(0, false, true);
+        else MPS_CG_APPLY(1, false, true);
+    } else if (op.cubic) {
+        if (mode == 0) MPS_CG_APPLY(0, true, true);
+        else MPS_CG_APPLY(1, true, true);
+    } else {
+        if (mode == 0) MPS_CG_APPLY(0, true, false);
+        else MPS_CG_APPLY(1, true, false);
+    }
+#undef MPS_CG_APPLY
+}
+static const void* persistent_fn(const CgOperator& op, int coherent) {
+    if (coherent) {
+        if (!op.recompute) return reinterpret_cast<const void*>(k_cg_persistent<false, true, true>);
+        return op.cubic ? reinterpret_cast<const void*>(k_cg_persistent<true, true, true>)
+                        : reinterpret_cast<const void*>(k_cg_persistent<true, false, true>);
+    }
+    if (!op.recompute) return reinterpret_cast<const void*>(k_cg_persistent<false, true, false>);
+    return op.cubic ? reinterpret_cast<const void*>(k_cg_persistent<true, true, false>)
+                    : reinterpret_cast<const void*>(k_cg_persistent<true, false, false>);
+}
+uint32_t cg_persistent_capacity(const CgOperator& op, int coherent) {
+    int dev = 0, per_cu = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent_fn(op, coherent), kBlock, 0) != hipSuccess) return 0;
+    return uint32_t(per_cu) * uint32_t(prop.multiProcessorCount);
+}
+void cg_persistent(const CgOperator& op, int coherent, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos,
+                   f3* r, f3* pbuf0, f3* pbuf1, f3* Ap, f3* x, SolverDev* st, uint32_t* active, double* partA,
+                   double* partB, GridSync* gs, uint32_t cap, double tol, uint32_t spin, hipStream_t s) {
+    if (!sc.N) return;
+    const uint32_t nblk = nb_(sc.N);
+    void* args[] = {const_cast<Scenes*>(&sc), const_cast<Nbr*>(&nb), const_cast<Halo*>(&hl), const_cast<float2**>(&op.cab),
+                    const_cast<float**>(&op.ca), const_cast<float4**>(&pos), const_cast<float4**>(&op.vel),
+                    const_cast<float**>(&op.rho), const_cast<float**>(&op.irho), &r, &pbuf0, &pbuf1, &Ap, &x, &st, &active,
+                    &partA, &partB, &gs, &cap, &tol, &spin};
+    const hipError_t rc = hipLaunchCooperativeKernel(persistent_fn(op, coherent), dim3(nblk), dim3(kBlock), args, 0, s);
+    if (rc != hipSuccess) throw LaunchError{rc};
 }
 void fold_cg(int mode, const Scenes& sc, const double* partials, SolverDev* st, uint32_t* active, double tol, hipStream_t s) {
     if (!sc.N) return;

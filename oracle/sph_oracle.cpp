@@ -102,6 +102,31 @@ inline T Gk(T dist, T invh, int cubic) {
     const T tmp = (T(1) - ratio) * invh2;
     return -KC<T>::G / dist * tmp * tmp;
 }
+// The viscosity operator's pair factor G(d) / ((d^2 + h2) (d + FLT_EPSILON)) of
+// ComputeCGViscosityAp_kernel (MPSSPHNonPressureUtil.cuh:617-618), regrouped so that it costs one
+// division: the 1/d inside G (MPSSPHKernel.cuh:46-65) joins the two denominators,
+//   Phi = (G d) / ((d (d^2 + h2)) (d + FLT_EPSILON)).
+// It differs from the reference's left-to-right evaluation by FP32 rounding only; the GPU
+// evaluates the same expression in the same order.
+template <class T>
+inline T Phik(T dist, T invh, T h2, int cubic) {
+    const T ratio = dist * invh;
+    if (ratio < KC<T>::EPS || ratio >= T(1)) return T(0);
+    const T invh2 = invh * invh;
+    T num;
+    if (cubic) {
+        if (ratio > T(0.5)) {
+            const T temp = (T(1) - ratio) * invh2;
+            num = -KC<T>::CUBICG * temp * temp;
+        } else {
+            num = KC<T>::CUBICG * invh2 * invh2 * (T(3) * ratio - T(2)) * ratio;
+        }
+    } else {
+        const T tmp = (T(1) - ratio) * invh2;
+        num = -KC<T>::G * tmp * tmp;
+    }
+    return num / ((dist * (dist * dist + h2)) * (dist + T(FLT_EPSILON)));
+}
 // STWKernel — MPSSPHKernel.cuh:67-75
 template <class T>
 inline T STWk(T dist, T invh) {
@@ -945,18 +970,21 @@ struct Ctx final : Base {
     //   (A p)_i = (1+f) p_i + sum_j [ a_ij p_ij + (b_ij (p_ij . x_ij)) x_ij ]
     //   a_ij = dt (cs_i s_ij + cx w_ij),  b_ij = -dt cv_i c_ij,
     //   s_ij = m_ij g_ij (surface tension), w_ij = (m_j/rho_j) W_ij (XSPH),
-    //   c_ij = (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON) (viscosity),
+    //   c_ij = (0.5(m_i+m_j) (1/rho_j)) Phi(d_ij) (viscosity; the reference's
+    //          (0.5(m_i+m_j)/rho_j) / (d^2 + 0.01 h_i^2) G_ij / (d + FLT_EPSILON), regrouped: Phik),
     //   cs_i = dt sigma/m_i, cx = XSPH/dt, cv_i = 10 rho0 mu/rho_i.
     // This is the operator of ComputeSurfaceTensionCGAp_kernel + ComputeCGViscosityAp_kernel
     // + ComputeFinalCGAp_kernel (MPSSPHNonPressureUtil.cuh:523-648) with the per-particle
     // scale factors distributed over the pair sums. The same pass forms the right-hand side
     // of ComputeSurfaceTensionCGb_kernel (:467-521): b_i = (dt sigma/m_i)(-sum s_ij x_ij) + v_i.
-    std::vector<T> ca, cb;
+    std::vector<T> ca, cb, irho;
     void cg_setup() {
         const T dtv = dt();
         const T cx = mXsph / dtv;
         ca.assign(ff.nei.size(), T(0));
         cb.assign(ff.nei.size(), T(0));
+        irho.assign(N, T(0));
+        for (size_t i = 0; i < N; i++) irho[i] = T(1) / rho[i];
 #pragma omp parallel for
         for (long long ii = 0; ii < (long long)N; ii++) {
             const size_t i = size_t(ii);
@@ -976,8 +1004,7 @@ struct Ctx final : Base {
                 const T gij = st_g(i, j, invHi, xij, dist, mij);
                 const T sij = mij * gij;
                 const T wij = m[j] / rho[j] * Wk(dist, invHi, cubic);
-                const T cij = T(0.5) * (mi + m[j]) / rho[j] / (dist * dist + onePercentHSq) * Gk(dist, invHi, cubic) /
-                              (dist + T(FLT_EPSILON));
+                const T cij = T(0.5) * (mi + m[j]) * irho[j] * Phik(dist, invHi, onePercentHSq, cubic);
                 ca[k] = dtv * (cs * sij + cx * wij);
                 cb[k] = -dtv * cv * cij;
                 bi = bi - sij * xij;
