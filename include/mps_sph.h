@@ -393,8 +393,9 @@ int mps_scene_set_material(mps_ctx* ctx, uint32_t scene, const mps_sph_material*
  * No counterpart in the reference. Reverse mode through mps_step with each step's control
  * flow frozen: its neighbour lists, the DF/CD iteration counts (differentiated through their
  * iterations), the per-particle branches; the viscosity / surface-tension solve differentiated
- * implicitly (transposed solve with the same operator). Needs the cubic kernel, the CG solve
- * and equal fluid radii within a scene; a relative CG tolerance (cg_tol < 0) makes the forward
+ * implicitly (transposed solve with the same operator). Either kernel set (cubic spline, or
+ * poly6/spiky) and any fluid radii; needs the CG viscosity solve (MPS_ERR_STATE otherwise, and for
+ * coupled scenes or several wall objects). A relative CG tolerance (cg_tol < 0) makes the forward
  * solve match the implicit derivative. */
 /* Step n_steps times from the current state, keeping a checkpoint every checkpoint_every
  * steps. loop_counts (may be NULL): n_steps * scenes * 3 values (l_df, l_cg, l_cd per scene). */

@@ -9,10 +9,15 @@
 // (its residual starts from A with f = 0 and iterates with A_f), so with A_f^T lambda = vbar*
 // every input theta gets thetabar = -lambda . dR/dtheta.
 //
-// Kernel derivatives are those of the cubic spline (W' = G d, STW' = STG d, closed-form G' and
-// STG'); J_ij = d(G_ij x_ij)/dx_i = G I + (G'/d) x_ij x_ij^T. Fluid radii are equal within a
-// scene, so the lists are symmetric and G_ij = G_ji: every pair term is gathered at both of its
-// ends and no kernel needs atomics.
+// Kernel derivatives in closed form for both kernel sets (cubic spline; poly6 density with the
+// spiky gradient): W' = WG d (WG = G for the cubic spline), STW' = STG d, G' and STG';
+// J_ij = d(G_ij x_ij)/dx_i = G I + (G'/d) x_ij x_ij^T. The neighbour test |x_ij| < (h_i + h_j)/2
+// is symmetric, so the lists are symmetric and every pair term is gathered at both of its ends
+// (no kernel needs atomics). Each end evaluates the kernel with its own support h = 4 r: with
+// equal radii in every scene (EQ) G_ji = G_ij and the stored value serves both ends; otherwise the
+// neighbour's G_ji and G'_ji are recomputed from h_j.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -25,16 +30,34 @@ __device__ __forceinline__ f3 f3z() { return {0.f, 0.f, 0.f}; }
 __device__ __forceinline__ f3 neg(f3 a) { return {-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ float4 f4(f3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
 
-// dG/dd of the cubic spline's G (G = kCG invh^5 (3q - 2) for q <= 1/2, -kCG invh^4 (1-q)^2 / d above)
-__device__ __forceinline__ float Gp_cubic(float d, float invh) {
+// dG/dd: cubic spline (G = kCG invh^5 (3q - 2) for q <= 1/2, -kCG invh^4 (1-q)^2 / d above) or
+// spiky (G = -kG invh^4 (1-q)^2 / d on (0, 1))
+template <bool CUBIC>
+__device__ __forceinline__ float Gpk(float d, float invh) {
     const float q = d * invh;
     if (q < kEps || q >= 1.0f) return 0.0f;
     const float h2 = invh * invh, h4 = h2 * h2;
-    if (q > 0.5f) {
-        const float t = 1.0f - q;
-        return kCubicG * h4 * t * (2.0f * invh / d + t / (d * d));
-    }
+    const float t = 1.0f - q;
+    if (!CUBIC) return kG * h4 * t * (2.0f * invh / d + t / (d * d));
+    if (q > 0.5f) return kCubicG * h4 * t * (2.0f * invh / d + t / (d * d));
     return 3.0f * kCubicG * h4 * h2;
+}
+// (dW/dd) / d of the density kernel: the cubic spline's G, or for poly6
+// (W = kW ((1 - q^2) invh)^3) -6 kW invh^3 ((1 - q^2) invh)^2
+template <bool CUBIC>
+__device__ __forceinline__ float WGk(float d, float invh) {
+    if (CUBIC) return Gk<true>(d, invh);
+    const float q = d * invh;
+    if (q >= 1.0f) return 0.0f;
+    const float tmp = (1.0f - q * q) * invh;
+    return -6.0f * kW * (invh * invh * invh) * (tmp * tmp);
+}
+// G and G' of the pair seen from the neighbour j: the values at i when the radii are equal
+template <bool CUBIC, bool EQ>
+__device__ __forceinline__ float2 other_end(float G, float Gp, float d, float4 xj4) {
+    if (EQ) return make_float2(G, Gp);
+    const float invHj = 1.0f / (xj4.w * 4.0f);
+    return make_float2(Gk<CUBIC>(d, invHj), Gpk<CUBIC>(d, invHj));
 }
 // dSTG/dd (STG = -kSTG invh^4 (1-q)^2 / d on (1/2, 1))
 __device__ __forceinline__ float STGp(float d, float invh) {
@@ -142,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_peval(Scenes sc, Nbr nb, const f
 // kappabar from vbar of the velocity after the force (v_i += -dt/2 [sum_j V (k_i+k_j) G x +
 // sum_b V_b k_i G x]), then deltabar, alphabar, rhobar, and the position terms of the force.
 // Runs for scenes whose applied iterations include `it` (it < L[scene]).
-template <bool DF>
+template <bool DF, bool CUBIC, bool EQ>
 __global__ void __launch_bounds__(kBlock) k_adj_kbar(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                    const float4* __restrict__ bpos, const float* __restrict__ bvol,
                                                    const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
@@ -167,12 +190,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_kbar(Scenes sc, Nbr nb, const fl
     for (uint32_t k = 0; k < c; k++) {
         const size_t slot = size_t(k) * N + i;
         const uint32_t j = nb.ff[slot];
-        const f3 xij = xi - mk(pos[j]);
+        const float4 xj4 = pos[j];
+        const f3 xij = xi - mk(xj4);
         const float G = pG_ff[slot];
-        const f3 w = vbi - mk(av[j]);
-        kb += G * dot3(xij, w);
         const float d = len3(xij);
-        gx = gx + (ai.x + aux[j].x) * Jmul(G, Gp_cubic(d, invH), d, xij, w);
+        const float Gp = Gpk<CUBIC>(d, invH);
+        const float kk = ai.x + aux[j].x;
+        if (EQ) {
+            const f3 w = vbi - mk(av[j]);
+            kb += G * dot3(xij, w);
+            gx = gx + kk * Jmul(G, Gp, d, xij, w);
+        } else {
+            const float2 o = other_end<CUBIC, EQ>(G, Gp, d, xj4);
+            const f3 vbj = mk(av[j]);
+            kb += G * dot3(xij, vbi) - o.x * dot3(xij, vbj);
+            gx = gx + kk * (Jmul(G, Gp, d, xij, vbi) - Jmul(o.x, o.y, d, xij, vbj));
+        }
     }
     float kbb = 0.0f;
     f3 gb = f3z();
@@ -185,7 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_kbar(Scenes sc, Nbr nb, const fl
         const float vb = bvol[j];
         kbb += vb * G * dot3(xib, vbi);
         const float d = len3(xib);
-        gb = gb + vb * Jmul(G, Gp_cubic(d, invH), d, xib, vbi);
+        gb = gb + vb * Jmul(G, Gpk<CUBIC>(d, invH), d, xib, vbi);
     }
     const float hdt = -0.5f * P.dt;
     const float kbar = hdt * (P.V * kb + kbb);
@@ -205,6 +238,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_kbar(Scenes sc, Nbr nb, const fl
 
 // vbar of the velocity the iteration started from: += (d delta / d v)^T deltabar, plus the
 // position terms of delta.
+template <bool CUBIC, bool EQ>
 __global__ void __launch_bounds__(kBlock) k_adj_dbar(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float4* __restrict__ bpos,
                                                    const float* __restrict__ bvol, const float4* __restrict__ bvel,
@@ -226,12 +260,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_dbar(Scenes sc, Nbr nb, const fl
     for (uint32_t k = 0; k < c; k++) {
         const size_t slot = size_t(k) * N + i;
         const uint32_t j = nb.ff[slot];
-        const f3 xij = xi - mk(pos[j]);
+        const float4 xj4 = pos[j];
+        const f3 xij = xi - mk(xj4);
         const float G = pG_ff[slot];
-        const float s = P.V * (di + dbar[j]);
-        accv = accv + (s * G) * xij;
         const float d = len3(xij);
-        accx = accx + s * Jmul(G, Gp_cubic(d, invH), d, xij, vi - mk(vel[j]));
+        const float Gp = Gpk<CUBIC>(d, invH);
+        const f3 vij = vi - mk(vel[j]);
+        if (EQ) {
+            const float s = P.V * (di + dbar[j]);
+            accv = accv + (s * G) * xij;
+            accx = accx + s * Jmul(G, Gp, d, xij, vij);
+        } else {
+            const float2 o = other_end<CUBIC, EQ>(G, Gp, d, xj4);
+            const float si = P.V * di, sj = P.V * dbar[j];
+            accv = accv + (si * G + sj * o.x) * xij;
+            accx = accx + (si * Jmul(G, Gp, d, xij, vij) + sj * Jmul(o.x, o.y, d, xij, vij));
+        }
     }
     const uint32_t cb = nb.cfb[i];
     for (uint32_t k = 0; k < cb; k++) {
@@ -242,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_dbar(Scenes sc, Nbr nb, const fl
         const float s = bvol[j] * di;
         accv = accv + (s * G) * xib;
         const float d = len3(xib);
-        accx = accx + s * Jmul(G, Gp_cubic(d, invH), d, xib, vi - mk(bvel[j]));
+        accx = accx + s * Jmul(G, Gpk<CUBIC>(d, invH), d, xib, vi - mk(bvel[j]));
     }
     const float4 v4 = av[i], a4 = ax[i];
     av[i] = make_float4(v4.x + accv.x, v4.y + accv.y, v4.z + accv.z, v4.w);
@@ -421,11 +465,12 @@ struct PairVjp {
     f3 gx, gu;
     float rs, ro, fs, fo, mu, sigma;
 };
+template <bool CUBIC>
 __device__ __forceinline__ PairVjp cg_pair_vjp(const StepParams& P, f3 x, f3 p, f3 u, float ri, float rj, float fi,
                                                float fj, float mi, float mj, float hi, f3 lam) {
     const float invH = 1.0f / hi;
     const float d = len3(x);
-    const float W = Wk<true>(d, invH), G = Gk<true>(d, invH), STW = STWk(d, invH), STGs = STGk(d, invH);
+    const float W = Wk<CUBIC>(d, invH), G = Gk<CUBIC>(d, invH), STW = STWk(d, invH), STGs = STGk(d, invH);
     const f3 stG = STGs * x;
     const float mij = 0.5f * (mi + mj);
     const float invD = 2.0f / (ri + rj);
@@ -474,7 +519,7 @@ __device__ __forceinline__ PairVjp cg_pair_vjp(const StepParams& P, f3 x, f3 p, 
     ro += -cbar * c / rj;
     float dbar = cbar * c * (-2.0f * d / dd) - cbar * c * c4;
     const float Gbar = cbar * c1 / dd * c4;
-    dbar += Wbar * (G * d) + Gbar * Gp_cubic(d, invH) + STWbar * (STGs * d) + STGsbar * STGp(d, invH);
+    dbar += Wbar * (WGk<CUBIC>(d, invH) * d) + Gbar * Gpk<CUBIC>(d, invH) + STWbar * (STGs * d) + STGsbar * STGp(d, invH);
     if (d > 0.0f) gx = gx + (dbar / d) * x;
     o.gx = gx;
     o.rs = rs;
@@ -486,6 +531,7 @@ __device__ __forceinline__ PairVjp cg_pair_vjp(const StepParams& P, f3 x, f3 p, 
 // densities (rbar +=), ST factors (fbar, out) and per-workgroup partials of mubar, sigmabar.
 // Every pair is evaluated at both ends: from i (lambda_i) for i's own terms and mu/sigma, and
 // from the neighbour (lambda_j) for the terms in which i is the neighbour.
+template <bool CUBIC>
 __global__ void __launch_bounds__(kBlock) k_adj_cg_grad(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                       const float4* __restrict__ vin, const float4* __restrict__ vstar,
                                                       const float* __restrict__ rho, const float* __restrict__ fst,
@@ -509,8 +555,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_cg_grad(Scenes sc, Nbr nb, const
             const float4 xj4 = pos[j], vj4 = vin[j];
             const f3 xij = xi - mk(xj4), uij = ui - mk(vj4), pij = si - mk(vstar[j]);
             const float rj = rho[j], fj = fst[j], mj = vj4.w;
-            const PairVjp A = cg_pair_vjp(P, xij, pij, uij, ri, rj, fi, fj, mi, mj, hi, li);
-            const PairVjp Bq = cg_pair_vjp(P, neg(xij), neg(pij), neg(uij), rj, ri, fj, fi, mj, mi, xj4.w * 4.0f, mk(lam[j]));
+            const PairVjp A = cg_pair_vjp<CUBIC>(P, xij, pij, uij, ri, rj, fi, fj, mi, mj, hi, li);
+            const PairVjp Bq =
+                cg_pair_vjp<CUBIC>(P, neg(xij), neg(pij), neg(uij), rj, ri, fj, fi, mj, mi, xj4.w * 4.0f, mk(lam[j]));
             gx = gx - A.gx + Bq.gx;
             gu = gu - A.gu + Bq.gu;
             gr += -A.rs - Bq.ro;
@@ -528,6 +575,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_cg_grad(Scenes sc, Nbr nb, const
     block_partial2(pmu, psig, partials, nblk);
 }
 // The ST factor fst_i = sum_j m_j (v_in_i - v_in_j) . (G_ij x_ij): vbar_in and positions.
+template <bool CUBIC, bool EQ>
 __global__ void __launch_bounds__(kBlock) k_adj_fst(Scenes sc, Nbr nb, const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const float* __restrict__ pG_ff, const float* __restrict__ fbar,
                                                   float4* __restrict__ avin, float4* __restrict__ ax) {
@@ -544,13 +592,22 @@ __global__ void __launch_bounds__(kBlock) k_adj_fst(Scenes sc, Nbr nb, const flo
     for (uint32_t k = 0; k < c; k++) {
         const size_t slot = size_t(k) * N + i;
         const uint32_t j = nb.ff[slot];
-        const float4 vj4 = vin[j];
-        const f3 xij = xi - mk(pos[j]);
+        const float4 vj4 = vin[j], xj4 = pos[j];
+        const f3 xij = xi - mk(xj4);
         const float G = pG_ff[slot];
-        const float s = fi * vj4.w + fbar[j] * mi;
-        accv = accv + (s * G) * xij;
         const float d = len3(xij);
-        accx = accx + s * Jmul(G, Gp_cubic(d, invH), d, xij, ui - mk(vj4));
+        const float Gp = Gpk<CUBIC>(d, invH);
+        const f3 uij = ui - mk(vj4);
+        if (EQ) {
+            const float s = fi * vj4.w + fbar[j] * mi;
+            accv = accv + (s * G) * xij;
+            accx = accx + s * Jmul(G, Gp, d, xij, uij);
+        } else {
+            const float2 o = other_end<CUBIC, EQ>(G, Gp, d, xj4);
+            const float si = fi * vj4.w, sj = fbar[j] * mi;
+            accv = accv + (si * G + sj * o.x) * xij;
+            accx = accx + (si * Jmul(G, Gp, d, xij, uij) + sj * Jmul(o.x, o.y, d, xij, uij));
+        }
     }
     const float4 v4 = avin[i], a4 = ax[i];
     avin[i] = make_float4(v4.x + accv.x, v4.y + accv.y, v4.z + accv.z, v4.w);
@@ -611,6 +668,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_gvec(Scenes sc, Nbr nb, const fl
 }
 // positions from rhobar (rho_i = rho0 (V W0 + sum_j V W_ij + sum_b V_b W_ib)) and from abar
 // (a_i = sum_j |V G_ij x_ij|^2 + |g_i|^2).
+template <bool CUBIC, bool EQ>
 __global__ void __launch_bounds__(kBlock) k_adj_density(Scenes sc, Nbr nb, const float4* __restrict__ pos,
                                                       const float4* __restrict__ bpos, const float* __restrict__ bvol,
                                                       const float* __restrict__ pG_ff, const float* __restrict__ pG_fb,
@@ -634,14 +692,25 @@ __global__ void __launch_bounds__(kBlock) k_adj_density(Scenes sc, Nbr nb, const
     for (uint32_t k = 0; k < c; k++) {
         const size_t slot = size_t(k) * N + i;
         const uint32_t j = nb.ff[slot];
-        const f3 xij = xi - mk(pos[j]);
+        const float4 xj4 = pos[j];
+        const f3 xij = xi - mk(xj4);
         const float G = pG_ff[slot];
         const float d = len3(xij);
-        const float Gp = Gp_cubic(d, invH);
+        const float Gp = Gpk<CUBIC>(d, invH);
         const float4 gj4 = gv[j];
-        acc = acc + (P.rho0 * V * (ri + rbar[j]) * G) * xij;
-        acc = acc + (2.0f * V * V * (ai + gj4.w) * G * (G + Gp * d)) * xij;
-        acc = acc + (2.0f * V) * Jmul(G, Gp, d, xij, ai * gi - gj4.w * mk(gj4));
+        if (EQ) {
+            const float WG = CUBIC ? G : WGk<CUBIC>(d, invH);
+            acc = acc + (P.rho0 * V * (ri + rbar[j]) * WG) * xij;
+            acc = acc + (2.0f * V * V * (ai + gj4.w) * G * (G + Gp * d)) * xij;
+            acc = acc + (2.0f * V) * Jmul(G, Gp, d, xij, ai * gi - gj4.w * mk(gj4));
+        } else {
+            const float invHj = 1.0f / (xj4.w * 4.0f);
+            const float2 o = other_end<CUBIC, EQ>(G, Gp, d, xj4);
+            const float WGi = CUBIC ? G : WGk<CUBIC>(d, invH), WGj = CUBIC ? o.x : WGk<CUBIC>(d, invHj);
+            acc = acc + (P.rho0 * V * (ri * WGi + rbar[j] * WGj)) * xij;
+            acc = acc + (2.0f * V * V * (ai * G * (G + Gp * d) + gj4.w * o.x * (o.x + o.y * d))) * xij;
+            acc = acc + (2.0f * V) * (Jmul(G, Gp, d, xij, ai * gi) - Jmul(o.x, o.y, d, xij, gj4.w * mk(gj4)));
+        }
     }
     const uint32_t cb = nb.cfb[i];
     for (uint32_t k = 0; k < cb; k++) {
@@ -651,8 +720,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_density(Scenes sc, Nbr nb, const
         const float G = pG_fb[slot];
         const float d = len3(xib);
         const float vb = bvol[j];
-        acc = acc + (P.rho0 * vb * ri * G) * xib;
-        acc = acc + (2.0f * ai * vb) * Jmul(G, Gp_cubic(d, invH), d, xib, gi);
+        acc = acc + (P.rho0 * vb * ri * (CUBIC ? G : WGk<CUBIC>(d, invH))) * xib;
+        acc = acc + (2.0f * ai * vb) * Jmul(G, Gpk<CUBIC>(d, invH), d, xib, gi);
     }
     const float4 a4 = ax[i];
     ax[i] = make_float4(a4.x + acc.x, a4.y + acc.y, a4.z + acc.z, a4.w);
@@ -712,21 +781,43 @@ void adj_peval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const
     if (df) launch_k(k_adj_peval<true>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
     else launch_k(k_adj_peval<false>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, rho, alpha, pG_ff, pG_fb, aux);
 }
-void adj_kbar(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
+// f(cubic, eq) with both flags as compile-time constants: the kernel instance for the kernel set
+// (cubic spline or poly6/spiky) and for equal radii or not
+template <class F>
+static void by_kind(AdjKind k, F&& f) {
+    using T = std::true_type;
+    using U = std::false_type;
+    if (k.cubic) {
+        if (k.eq) f(T{}, T{});
+        else f(T{}, U{});
+    } else {
+        if (k.eq) f(U{}, T{});
+        else f(U{}, U{});
+    }
+}
+
+void adj_kbar(AdjKind k, int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
               const float* pG_ff, const float* pG_fb, const float4* aux, const float* alpha, const float4* av,
               const uint32_t* L, uint32_t it, float* dbar, float4* ax, float* abar, float* rbar, hipStream_t s) {
     if (!sc.N) return;
-    if (df)
-        launch_k(k_adj_kbar<true>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
-                                                      abar, rbar);
-    else
-        launch_k(k_adj_kbar<false>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it, dbar, ax,
-                                                       abar, rbar);
+    by_kind(k, [&](auto C, auto E) {
+        constexpr bool c = decltype(C)::value, e = decltype(E)::value;
+        if (df)
+            launch_k(k_adj_kbar<true, c, e>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L, it,
+                     dbar, ax, abar, rbar);
+        else
+            launch_k(k_adj_kbar<false, c, e>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, aux, alpha, av, L,
+                     it, dbar, ax, abar, rbar);
+    });
 }
-void adj_dbar(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos, const float* bvol,
-              const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar, const uint32_t* L, uint32_t it,
-              float4* av, float4* ax, hipStream_t s) {
-    if (sc.N) launch_k(k_adj_dbar, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol, bvel, pG_ff, pG_fb, dbar, L, it, av, ax);
+void adj_dbar(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+              const float* bvol, const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar,
+              const uint32_t* L, uint32_t it, float4* av, float4* ax, hipStream_t s) {
+    if (!sc.N) return;
+    by_kind(k, [&](auto C, auto E) {
+        launch_k(k_adj_dbar<decltype(C)::value, decltype(E)::value>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vel, bpos, bvol,
+                 bvel, pG_ff, pG_fb, dbar, L, it, av, ax);
+    });
 }
 void adj_rev_index(const Scenes& sc, const Nbr& nb, uint32_t* rev, uint32_t* bad, hipStream_t s) {
     if (sc.N) launch_k(k_adj_rev_index, nbk(sc.N), kBlock, 0, s, sc, nb, rev, bad);
@@ -742,16 +833,23 @@ void adj_cg_applyT(const Scenes& sc, const Nbr& nb, const float2* cab, const flo
                    const SolverDev* st, double* partials, hipStream_t s) {
     if (sc.N) launch_k(k_adj_cg_applyT, nbk(sc.N), kBlock, 0, s, sc, nb, cab, cabT, rec, Ap, st, partials);
 }
-void adj_cg_grad(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar, const float* rho,
-                 const float* fst, const float4* lam, float4* avin, float4* ax, float* rbar, float* fbar, double* partials,
-                 hipStream_t s) {
+void adj_cg_grad(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar,
+                 const float* rho, const float* fst, const float4* lam, float4* avin, float4* ax, float* rbar, float* fbar,
+                 double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nbk(sc.N);
-    launch_k(k_adj_cg_grad, g, kBlock, 0, s, sc, nb, pos, vin, vstar, rho, fst, lam, avin, ax, rbar, fbar, partials, g);
+    if (k.cubic)
+        launch_k(k_adj_cg_grad<true>, g, kBlock, 0, s, sc, nb, pos, vin, vstar, rho, fst, lam, avin, ax, rbar, fbar, partials, g);
+    else
+        launch_k(k_adj_cg_grad<false>, g, kBlock, 0, s, sc, nb, pos, vin, vstar, rho, fst, lam, avin, ax, rbar, fbar, partials, g);
 }
-void adj_fst(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff, const float* fbar,
-             float4* avin, float4* ax, hipStream_t s) {
-    if (sc.N) launch_k(k_adj_fst, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vin, pG_ff, fbar, avin, ax);
+void adj_fst(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff,
+             const float* fbar, float4* avin, float4* ax, hipStream_t s) {
+    if (!sc.N) return;
+    by_kind(k, [&](auto C, auto E) {
+        launch_k(k_adj_fst<decltype(C)::value, decltype(E)::value>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, vin, pG_ff, fbar,
+                 avin, ax);
+    });
 }
 void adj_fold2(const Scenes& sc, const double* partials, double* acc0, double* acc1, hipStream_t s) {
     if (sc.N) launch_k(k_adj_fold2, sc.S, kBlock, 0, s, sc, partials, nbk(sc.N), acc0, acc1);
@@ -760,9 +858,13 @@ void adj_gvec(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* 
               const float* pG_fb, const float* abar, float4* gv, hipStream_t s) {
     if (sc.N) launch_k(k_adj_gvec, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, abar, gv);
 }
-void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
-                 const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s) {
-    if (sc.N) launch_k(k_adj_density, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol, pG_ff, pG_fb, rbar, gv, ax);
+void adj_density(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
+                 const float* pG_ff, const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s) {
+    if (!sc.N) return;
+    by_kind(k, [&](auto C, auto E) {
+        launch_k(k_adj_density<decltype(C)::value, decltype(E)::value>, nbk(sc.N), kBlock, 0, s, sc, nb, pos, bpos, bvol,
+                 pG_ff, pG_fb, rbar, gv, ax);
+    });
 }
 void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, const float4* al, float4* ax_out,
                    float4* av_out, float4* al_out, hipStream_t s) {

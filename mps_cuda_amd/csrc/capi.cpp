@@ -219,6 +219,7 @@ struct mps_ctx {
     uint32_t adj_n = 0, adj_every = 0;
     // layout the rollout was recorded on: a backward on any other layout is refused
     uint32_t adj_N = 0, adj_Nref = 0, adj_B = 0, adj_S = 0;
+    bool adj_eq = true;  // every scene's fluid radii equal (the reverse kernels' fast path)
     // Forget the recorded rollout: called by everything that replaces particles, walls, scenes
     // or the parameters the recomputation would read.
     void drop_rollout() {
@@ -1267,17 +1268,15 @@ struct mps_ctx {
         if (n_obs && !pos_out) throw ApiError{MPS_ERR_INVALID, "no output for the observed positions"};
         if (coupled()) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover coupled scenes"};
         if (nbobj > 1) throw ApiError{MPS_ERR_STATE, "the adjoint does not cover several wall objects"};
-        if (cfg.kernel != MPS_KERNEL_CUBIC) throw ApiError{MPS_ERR_STATE, "the adjoint needs the cubic kernel"};
         if (cfg.visc_solver != MPS_VISC_CG) throw ApiError{MPS_ERR_STATE, "the adjoint needs the CG viscosity solve"};
         if (N == 0 || n == 0) throw ApiError{MPS_ERR_INVALID, "nothing to roll out"};
         sync_params();
-        {  // equal fluid radii within a scene (symmetric lists, G_ij = G_ji)
+        {  // equal fluid radii within every scene: then G_ij = G_ji and the reverse kernels take the fast path
             std::vector<float4> hp = down(pos.p, N);
-            std::vector<uint32_t> hm = down(perm.p, N);
+            adj_eq = true;
             for (const SceneHost& h : scene_list()) {
                 const float r0 = hp[h.begin].w;
-                for (uint32_t q = h.begin; q < h.begin + h.nf; q++)
-                    if (hp[q].w != r0) throw ApiError{MPS_ERR_STATE, "the adjoint needs equal fluid radii within a scene"};
+                for (uint32_t q = h.begin; q < h.begin + h.nf; q++) adj_eq = adj_eq && hp[q].w == r0;
             }
         }
         every = std::max<uint32_t>(every, 1);
@@ -1368,8 +1367,10 @@ struct mps_ctx {
         pairG_fb.reserve(size_t(T.Kfb) * m);
         pairs_valid = false;
         halo_valid = false;
-        mps::launch::density_alpha(1, mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb, mps::Halo{}, T.pos.p,
-                                   T.snaps[0].p, T.bpos.p, T.bvol.p, rho.p, alpha.p, color.p, pairG_ff.p, pairG_fb.p, stream);
+        const mps::launch::AdjKind ak{cubic(), adj_eq ? 1 : 0};
+        mps::launch::density_alpha(cubic(), mps::launch::kDoDensity | mps::launch::kDoAlpha | mps::launch::kDoPairs, P, nb,
+                                   mps::Halo{}, T.pos.p, T.snaps[0].p, T.bpos.p, T.bvol.p, rho.p, alpha.p, color.p, pairG_ff.p,
+                                   pairG_fb.p, stream);
         abar.reserve(m); rbar.reserve(m); dbar.reserve(m); fbar.reserve(m); fstr.reserve(m);
         aux4.reserve(m); gvec.reserve(m); avin.reserve(m); adx2.reserve(m); adv2.reserve(m);
         mps::launch::zero_words(reinterpret_cast<uint32_t*>(abar.p), N, stream);
@@ -1385,9 +1386,9 @@ struct mps_ctx {
                 const float4* vk = T.snaps[base + it - 1].p;
                 mps::launch::adj_peval(df, P, nb, T.pos.p, vk, T.bpos.p, T.bvol.p, bvel.p, rho.p, alpha.p, pairG_ff.p,
                                        pairG_fb.p, aux4.p, stream);
-                mps::launch::adj_kbar(df, P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, aux4.p, alpha.p, adv.p,
+                mps::launch::adj_kbar(ak, df, P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, aux4.p, alpha.p, adv.p,
                                       d_L.p, it, dbar.p, adx.p, abar.p, rbar.p, stream);
-                mps::launch::adj_dbar(P, nb, T.pos.p, vk, T.bpos.p, T.bvol.p, bvel.p, pairG_ff.p, pairG_fb.p, dbar.p, d_L.p, it,
+                mps::launch::adj_dbar(ak, P, nb, T.pos.p, vk, T.bpos.p, T.bvol.p, bvel.p, pairG_ff.p, pairG_fb.p, dbar.p, d_L.p, it,
                                       adv.p, adx.p, stream);
             }
         };
@@ -1398,7 +1399,8 @@ struct mps_ctx {
             pairAB.reserve(size_t(T.Kff) * m); cabT.reserve(size_t(T.Kff) * m); rev.reserve(size_t(T.Kff) * m);
             cgrecB.reserve(2 * m);
             mps::launch::st_factor(P, nb, mps::Halo{}, T.pos.p, vin, pairG_ff.p, fstr.p, cgx.p, stream);
-            mps::launch::cg_setup(1, P, nb, mps::Halo{}, T.pos.p, vin, rho.p, fstr.p, nullptr, pairAB.p, cgb.p, nullptr, stream);
+            mps::launch::cg_setup(cubic(), P, nb, mps::Halo{}, T.pos.p, vin, rho.p, fstr.p, nullptr, pairAB.p, cgb.p, nullptr,
+                                  stream);
             mps::launch::zero_words(d_flags + 2, 1, stream);
             mps::launch::adj_rev_index(P, nb, rev.p, d_flags + 2, stream);
             mps::launch::adj_transpose_coeffs(P, nb, rev.p, pairAB.p, cabT.p, stream);
@@ -1406,15 +1408,15 @@ struct mps_ctx {
             lam.reserve(m);
             HIPCHK(hipMemcpyAsync(lam.p, cgx.p, sizeof(float4) * N, hipMemcpyDeviceToDevice, stream));
             lam_valid = true;
-            mps::launch::adj_cg_grad(P, nb, T.pos.p, vin, vst, rho.p, fstr.p, cgx.p, avin.p, adx.p, rbar.p, fbar.p, partials.p,
+            mps::launch::adj_cg_grad(ak, P, nb, T.pos.p, vin, vst, rho.p, fstr.p, cgx.p, avin.p, adx.p, rbar.p, fbar.p, partials.p,
                                      stream);
             mps::launch::adj_fold2(P, partials.p, acc_mu.p, acc_sig.p, stream);
-            mps::launch::adj_fst(P, nb, T.pos.p, vin, pairG_ff.p, fbar.p, avin.p, adx.p, stream);
+            mps::launch::adj_fst(ak, P, nb, T.pos.p, vin, pairG_ff.p, fbar.p, avin.p, adx.p, stream);
             std::swap(adv, avin);
         }
         loop_rev(1, T.ldf, 0);
         mps::launch::adj_gvec(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, abar.p, gvec.p, stream);
-        mps::launch::adj_density(P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, rbar.p, gvec.p, adx.p, stream);
+        mps::launch::adj_density(ak, P, nb, T.pos.p, T.bpos.p, T.bvol.p, pairG_ff.p, pairG_fb.p, rbar.p, gvec.p, adx.p, stream);
         lam2.reserve(m);
         mps::launch::adj_unpermute(N, T.src.p, adx.p, adv.p, lam_valid ? lam.p : nullptr, adx2.p, adv2.p, lam2.p, stream);
         std::swap(adx, adx2);

@@ -283,17 +283,22 @@ void pack_readback(uint32_t n, const float4* pos, const float4* color, const uin
 void pack_readback_f64(uint32_t n, const float4* pos, const float4* color, const uint32_t* perm, double* xyz, double* rad,
                        float* rgba, hipStream_t s);
 
-// ---- adjoint.hip (reverse mode of one step with frozen control flow; cubic kernel)
+// ---- adjoint.hip (reverse mode of one step with frozen control flow)
+// the kernel set (cubic spline, or poly6 density with the spiky gradient) and whether every
+// scene's fluid radii are equal (then a pair's G serves both ends)
+struct AdjKind {
+    int cubic, eq;
+};
 void adj_advect(const Scenes& sc, const float4* pos, const float4* vin, const float4* ax, float4* av, hipStream_t s);
 // aux = (kappa, e, mask, rho branch) recomputed from the velocity an iteration started from
 void adj_peval(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
                const float* bvol, const float4* bvel, const float* rho, const float* alpha, const float* pG_ff,
                const float* pG_fb, float4* aux, hipStream_t s);
-void adj_kbar(int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
+void adj_kbar(AdjKind k, int df, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol,
               const float* pG_ff, const float* pG_fb, const float4* aux, const float* alpha, const float4* av,
               const uint32_t* L, uint32_t it, float* dbar, float4* ax, float* abar, float* rbar, hipStream_t s);
-void adj_dbar(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos, const float* bvol,
-              const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar, const uint32_t* L, uint32_t it,
+void adj_dbar(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vel, const float4* bpos,
+              const float* bvol, const float4* bvel, const float* pG_ff, const float* pG_fb, const float* dbar, const uint32_t* L, uint32_t it,
               float4* av, float4* ax, hipStream_t s);
 void adj_rev_index(const Scenes& sc, const Nbr& nb, uint32_t* rev, uint32_t* bad, hipStream_t s);
 void adj_transpose_coeffs(const Scenes& sc, const Nbr& nb, const uint32_t* rev, const float2* cab, float2* cabT, hipStream_t s);
@@ -302,15 +307,15 @@ void adj_cg_init(const Scenes& sc, const float4* pos, const float4* b, float4* x
 void adj_cg_applyT(const Scenes& sc, const Nbr& nb, const float2* cab, const float2* cabT, const float4* rec, float4* Ap,
                    const SolverDev* st, double* partials, hipStream_t s);
 // partials: 2 * ceil(N / kBlock) doubles (mubar, sigmabar per workgroup)
-void adj_cg_grad(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar, const float* rho,
+void adj_cg_grad(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float4* vstar, const float* rho,
                  const float* fst, const float4* lam, float4* avin, float4* ax, float* rbar, float* fbar, double* partials,
                  hipStream_t s);
-void adj_fst(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff, const float* fbar,
+void adj_fst(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* vin, const float* pG_ff, const float* fbar,
              float4* avin, float4* ax, hipStream_t s);
 void adj_fold2(const Scenes& sc, const double* partials, double* acc0, double* acc1, hipStream_t s);
 void adj_gvec(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
               const float* pG_fb, const float* abar, float4* gv, hipStream_t s);
-void adj_density(const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
+void adj_density(AdjKind k, const Scenes& sc, const Nbr& nb, const float4* pos, const float4* bpos, const float* bvol, const float* pG_ff,
                  const float* pG_fb, const float* rbar, const float4* gv, float4* ax, hipStream_t s);
 // also carries the transposed solve's solution (al, may be null) back, as the next warm start
 void adj_unpermute(uint32_t n, const uint32_t* src, const float4* ax, const float4* av, const float4* al, float4* ax_out,

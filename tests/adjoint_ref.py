@@ -1,8 +1,10 @@
 """FP64 reference for the adjoint (test infrastructure).
 
 A torch restatement of the oracle's step (oracle/sph_oracle.cpp: density, alpha, gravity, the
-DF/CD loops, the viscosity/surface-tension solve, collision and advection) for a scene with the
-cubic kernel, equal fluid radii and no Z-sort, with its control flow frozen: the DF/CD iteration
+DF/CD loops, the viscosity/surface-tension solve, collision and advection) for one scene with
+either kernel set (cubic spline, or poly6 density with the spiky gradient), per-particle radii
+(support h_i = 4 r_i, neighbours within (h_i + h_j)/2, every kernel of particle i evaluated
+with h_i) and no Z-sort, with its control flow frozen: the DF/CD iteration
 counts are given per step (from the run being checked), the neighbour lists are recomputed from
 the positions (the kernels vanish smoothly at the support, so list changes do not matter), the
 viscosity solve is solved exactly (dense), which is the system the forward solve converges to:
@@ -14,6 +16,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+KW6 = 1.5666814710608447114749495456982
+KGS = 14.323944878270580219199538703526
 KCW = 2.5464790894703253723021402139602
 KCG = 15.278874536821952233812841283761
 KSTW = 7.2756545413437867780061148970292
@@ -27,16 +31,22 @@ def _safe(d):
     return torch.where(d > 0, d, torch.ones_like(d))
 
 
-def W(d, h):
+def W(d, h, cubic=True):
     q = d / h
+    if not cubic:  # poly6
+        return torch.where(q >= 1, torch.zeros_like(d), KW6 * ((1 - q * q) / h) ** 3)
     inner = KCW / h ** 3 * ((6 * q - 6) * q * q + 1)
     outer = KCW * 2 * ((1 - q) / h) ** 3
     return torch.where(q >= 1, torch.zeros_like(d), torch.where(q >= 0.5, outer, inner))
 
 
-def G(d, h):
+def G(d, h, cubic=True):
+    """grad W_ij = G x_ij: cubic spline or spiky."""
     q = d / h
     ds = _safe(d)
+    if not cubic:
+        val = -KGS / ds * ((1 - q) / h ** 2) ** 2
+        return torch.where((q < EPS) | (q >= 1), torch.zeros_like(d), val)
     inner = KCG / ds / h ** 4 * (3 * q - 2) * q
     outer = -KCG / ds * ((1 - q) / h ** 2) ** 2
     val = torch.where(q > 0.5, outer, inner)
@@ -55,6 +65,7 @@ def STG(d, h):
 
 
 def _pairs(a, b, cut, self_pairs):
+    """Index pairs with |a_i - b_j| < cut (a scalar or an (len(a), len(b)) matrix)."""
     with torch.no_grad():
         D = torch.cdist(a, b)
         m = D < cut
@@ -65,14 +76,16 @@ def _pairs(a, b, cut, self_pairs):
 
 
 class Replay:
-    """One scene: fluid (pos, radius r, mass m), walls (bpos, bradius), physics, material."""
+    """One scene: fluid (pos, radius r and mass m: scalars or per particle), walls (bpos,
+    bradius), physics, material; `cubic` picks the kernel set."""
 
-    def __init__(self, pos, r, m, bpos, br, physics, material, cg_factor=0.01):
+    def __init__(self, pos, r, m, bpos, br, physics, material, cg_factor=0.01, cubic=True):
         self.x0 = torch.as_tensor(np.asarray(pos, np.float64), dtype=T)
         self.N = self.x0.shape[0]
-        self.r = float(r)
-        self.h = 4.0 * self.r
-        self.m = torch.full((self.N,), float(m), dtype=T)
+        self.cubic = bool(cubic)
+        r32 = np.broadcast_to(np.asarray(r, np.float32), (self.N,))
+        self.h = torch.as_tensor(4.0 * r32.astype(np.float64), dtype=T)  # h_i = 4 r_i (exact in float)
+        self.m = torch.as_tensor(np.broadcast_to(np.asarray(m, np.float64), (self.N,)).copy(), dtype=T)
         self.xb = torch.as_tensor(np.asarray(bpos, np.float64), dtype=T)
         self.hb = 4.0 * float(br)
         self.dt = float(np.float32(physics["dt"]))
@@ -88,25 +101,26 @@ class Replay:
         with torch.no_grad():
             Ib, Jb = _pairs(self.xb, self.xb, self.hb, True)
             db = torch.linalg.norm(self.xb[Ib] - self.xb[Jb], dim=1)
-            S = torch.full((self.xb.shape[0],), float(W(torch.zeros(1, dtype=T), self.hb)[0]), dtype=T)
-            S.index_add_(0, Ib, W(db, self.hb))
+            S = torch.full((self.xb.shape[0],), float(W(torch.zeros(1, dtype=T), self.hb, cubic)[0]), dtype=T)
+            S.index_add_(0, Ib, W(db, self.hb, cubic))
         self.Sb = S
 
     def step(self, x, v, Vb, mu, sigma, ldf, lcd):
-        h, dt, V, rho0 = self.h, self.dt, self.V, self.rho0
+        h, dt, V, rho0, cub = self.h, self.dt, self.V, self.rho0, self.cubic
         Vb = 1.0 / (Vb + self.Sb)
-        I, J = _pairs(x, x, h, True)
-        Ib, Jb = _pairs(x, self.xb, 0.5 * (h + self.hb), False)
+        I, J = _pairs(x, x, 0.5 * (h[:, None] + h[None, :]), True)
+        Ib, Jb = _pairs(x, self.xb, 0.5 * (h[:, None] + self.hb), False)
+        hI, hB = h[I], h[Ib]  # each pair's kernels use the support of its own particle
         xij = x[I] - x[J]
         d = torch.linalg.norm(xij, dim=1)
         xib = x[Ib] - self.xb[Jb]
         db = torch.linalg.norm(xib, dim=1)
-        Gij, Gib = G(d, h), G(db, h)
+        Gij, Gib = G(d, hI, cub), G(db, hB, cub)
         N = self.N
         zero = torch.zeros(N, dtype=T)
         # density and alpha
-        rho = rho0 * (V * W(torch.zeros(1, dtype=T), h)[0] + zero.index_add(0, I, V * W(d, h))
-                      + zero.index_add(0, Ib, Vb[Jb] * W(db, h)))
+        rho = rho0 * (V * W(torch.zeros(N, dtype=T), h, cub) + zero.index_add(0, I, V * W(d, hI, cub))
+                      + zero.index_add(0, Ib, Vb[Jb] * W(db, hB, cub)))
         grad = (V * Gij)[:, None] * xij
         s = zero.index_add(0, I, (grad * grad).sum(1))
         gvec = torch.zeros(N, 3, dtype=T).index_add(0, I, grad).index_add(0, Ib, (Vb[Jb] * Gib)[:, None] * xib)
@@ -137,12 +151,12 @@ class Replay:
         uij = vin[I] - vin[J]
         fst = zero.index_add(0, I, m[J] * (uij * (Gij[:, None] * xij)).sum(1))
         invD = 2.0 / (rho[I] + rho[J])
-        stw = STW(d, h)
-        stg = STG(d, h)[:, None] * xij
+        stw = STW(d, hI)
+        stg = STG(d, hI)[:, None] * xij
         g = (stw + dt * ((uij * stg).sum(1) - (fst[I] + fst[J]) * invD * 0.5 * stw)) * invD
         sij = 0.5 * (m[I] + m[J]) * g
-        wij = m[J] / rho[J] * W(d, h)
-        cij = 0.5 * (m[I] + m[J]) / rho[J] / (d * d + 0.01 * h * h) * Gij / (d + FLT_EPS)
+        wij = m[J] / rho[J] * W(d, hI, cub)
+        cij = 0.5 * (m[I] + m[J]) / rho[J] / (d * d + 0.01 * hI * hI) * Gij / (d + FLT_EPS)
         cs = dt * sigma / m[I]
         aij = dt * (cs * sij + (self.xsph / dt) * wij)
         bij = -dt * (10.0 * rho0 * mu / rho[I]) * cij
@@ -175,3 +189,29 @@ class Replay:
         if observe is not None:
             return torch.stack([seen[int(t)] for t in observe])
         return x, v
+
+
+VARIANTS = ("cubic", "poly6", "radii")
+
+
+def droplet(variant="cubic", steps=4, n=300):
+    """The small droplet of the adjoint tests (radius 0.08 m, h 0.04 m, seed 3) in one of
+    VARIANTS: the cubic spline with equal radii, the poly6/spiky kernel set, or two fluid radii
+    (half of the particles, seeded, at 0.8 r)."""
+    from mps_cuda_amd import _capi, scenes
+    sc = scenes.diff_droplet(n=n, radius=0.08, seed=3, steps=steps, h=0.04)
+    if variant == "poly6":
+        sc.config["kernel"] = _capi.KERNEL_POLY6_SPIKY
+    elif variant == "radii":
+        small = np.random.default_rng(11).random(sc.n) < 0.5
+        sc.radius = np.where(small, np.float32(0.8) * sc.radius, sc.radius).astype(np.float32)
+    elif variant != "cubic":
+        raise ValueError(variant)
+    return sc
+
+
+def replay_for(sc, mass, bpos, bradius):
+    """Replay of a droplet() scene: fluid masses and wall particles as the run produced them."""
+    from mps_cuda_amd import _capi
+    return Replay(sc.pos, sc.radius, mass, bpos, bradius, sc.physics, sc.material,
+                  cubic=sc.config["kernel"] == _capi.KERNEL_CUBIC)

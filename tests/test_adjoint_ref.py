@@ -1,19 +1,21 @@
 """The FP64 reference of the adjoint (tests/adjoint_ref.py) pinned on the CPU: its replay of a
 rollout reproduces the FP64 oracle's trajectory (given the oracle's loop counts), and its
-autograd gradients match central finite differences of the same frozen map."""
+autograd gradients match central finite differences of the same frozen map. Three variants of
+the droplet: the cubic spline with equal radii, the poly6/spiky kernel set, and two fluid radii
+(r and 0.8 r, so the pair kernels differ between the two ends)."""
 import numpy as np
 import pytest
 import torch
 
 from mps_cuda_amd import scenes
 from oracle.binding import OracleSim
-from tests.adjoint_ref import Replay
+from tests.adjoint_ref import VARIANTS, droplet, replay_for
 
 STEPS = 4
 
 
-def _scene():
-    return scenes.diff_droplet(n=300, radius=0.08, seed=3, steps=STEPS, h=0.04)
+def _scene(variant="cubic"):
+    return droplet(variant, STEPS)
 
 
 def _oracle_run(sc, v0, mu, sigma):
@@ -35,12 +37,12 @@ def _oracle_run(sc, v0, mu, sigma):
 
 
 def _replay(sc, o):
-    m = o.get("FLUID_MASS")[0]
-    return Replay(sc.pos, sc.radius[0], m, o.get("BND_POS"), o.get("BND_RADIUS")[0], sc.physics, sc.material)
+    return replay_for(sc, o.get("FLUID_MASS"), o.get("BND_POS"), o.get("BND_RADIUS")[0])
 
 
-def test_replay_matches_oracle_trajectory():
-    sc = _scene()
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_replay_matches_oracle_trajectory(variant):
+    sc = _scene(variant)
     mu, sig = 0.02, 0.3
     o0, _ = _oracle_run(sc, None, mu, sig)  # for the wall particles
     o, counts = _oracle_run(sc, sc.extra["v0"], mu, sig)
@@ -54,8 +56,9 @@ def test_replay_matches_oracle_trajectory():
     assert ex < 1e-9 and ev < 1e-6, (ex, ev)
 
 
-def test_replay_gradients_match_finite_differences():
-    sc = _scene()
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_replay_gradients_match_finite_differences(variant):
+    sc = _scene(variant)
     o0, counts = _oracle_run(sc, sc.extra["v0"], 0.02, 0.3)
     rep = _replay(sc, o0)
     rng = np.random.default_rng(0)
