@@ -77,12 +77,15 @@ def _pairs(a, b, cut, self_pairs):
 
 class Replay:
     """One scene: fluid (pos, radius r and mass m: scalars or per particle), walls (bpos,
-    bradius), physics, material; `cubic` picks the kernel set."""
+    bradius), physics, material; `cubic` picks the kernel set; `accumulate_volumes` is the
+    reference's wall volume V <- 1/(V + S) carried across steps (faithful_quirks), else
+    V = 1/S every step."""
 
-    def __init__(self, pos, r, m, bpos, br, physics, material, cg_factor=0.01, cubic=True):
+    def __init__(self, pos, r, m, bpos, br, physics, material, cg_factor=0.01, cubic=True, accumulate_volumes=True):
         self.x0 = torch.as_tensor(np.asarray(pos, np.float64), dtype=T)
         self.N = self.x0.shape[0]
         self.cubic = bool(cubic)
+        self.accumulate = bool(accumulate_volumes)
         r32 = np.broadcast_to(np.asarray(r, np.float32), (self.N,))
         self.h = torch.as_tensor(4.0 * r32.astype(np.float64), dtype=T)  # h_i = 4 r_i (exact in float)
         self.m = torch.as_tensor(np.broadcast_to(np.asarray(m, np.float64), (self.N,)).copy(), dtype=T)
@@ -107,7 +110,7 @@ class Replay:
 
     def step(self, x, v, Vb, mu, sigma, ldf, lcd):
         h, dt, V, rho0, cub = self.h, self.dt, self.V, self.rho0, self.cubic
-        Vb = 1.0 / (Vb + self.Sb)
+        Vb = 1.0 / (Vb + self.Sb) if self.accumulate else 1.0 / self.Sb
         I, J = _pairs(x, x, 0.5 * (h[:, None] + h[None, :]), True)
         Ib, Jb = _pairs(x, self.xb, 0.5 * (h[:, None] + self.hb), False)
         hI, hB = h[I], h[Ib]  # each pair's kernels use the support of its own particle
@@ -214,4 +217,5 @@ def replay_for(sc, mass, bpos, bradius):
     """Replay of a droplet() scene: fluid masses and wall particles as the run produced them."""
     from mps_cuda_amd import _capi
     return Replay(sc.pos, sc.radius, mass, bpos, bradius, sc.physics, sc.material,
-                  cubic=sc.config["kernel"] == _capi.KERNEL_CUBIC)
+                  cubic=sc.config["kernel"] == _capi.KERNEL_CUBIC,
+                  accumulate_volumes=bool(sc.config.get("faithful_quirks", 1)))

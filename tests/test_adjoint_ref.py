@@ -82,3 +82,24 @@ def test_replay_gradients_match_finite_differences(variant):
     assert abs(fd_mu - gm) <= 1e-5 * abs(gm) + 1e-9, (float(fd_mu), float(gm))
     assert abs(fd_sig - gs) <= 1e-5 * abs(gs) + 1e-9, (float(fd_sig), float(gs))
     assert abs(fd_v - (gv * dv).sum()) <= 1e-5 * abs(fd_v), (float(fd_v), float((gv * dv).sum()))
+
+
+def test_replay_without_volume_accumulation_matches_oracle():
+    """faithful_quirks = 0 (wall volume 1/S every step, permuted with the walls by the Z-sort):
+    the replay without accumulation follows the oracle with the Z-sort every 2 frames. The
+    oracle relabels its fluid at each Z-sort, so positions are compared as point sets."""
+    sc = _scene()
+    sc.config.update(zsort_interval=2, faithful_quirks=0)
+    mu, sig = 0.02, 0.3
+    o, counts = _oracle_run(sc, sc.extra["v0"], mu, sig)
+    rep = _replay(sc, o)
+    assert not rep.accumulate
+    x, _ = rep.rollout(torch.as_tensor(sc.extra["v0"]), torch.tensor(mu, dtype=torch.float64),
+                       torch.tensor(sig, dtype=torch.float64), counts)
+    xo = o.get("FLUID_POS")
+    D = torch.cdist(x, torch.as_tensor(xo))
+    nn = D.argmin(dim=1).numpy()
+    assert len(set(nn.tolist())) == sc.n  # a one-to-one label map
+    assert (nn != np.arange(sc.n)).any()  # the Z-sort did relabel
+    ex = np.linalg.norm(x.numpy() - xo[nn]) / np.linalg.norm(xo)
+    assert ex < 1e-9, ex
