@@ -308,6 +308,58 @@ __device__ __forceinline__ bool within(const float4& xi, float ri, const float4&
     return __builtin_sqrtf(dd) < thr;
 }
 
+// within() for M candidates at once, as a bit mask (bit k: candidate k, the first n valid):
+// the squared distances of all M decide first, and the square root is taken only for those
+// within 1e-4 of the cut-off, so the common case runs without branches and the M LDS reads
+// are in flight together.
+template <int M>
+__device__ __forceinline__ uint32_t hit_mask(const float4& xi, float ri, const float4 (&q)[M], uint32_t n) {
+    uint32_t in = 0u, near = 0u;
+    float dd[M], thr[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const float ddx = xi.x - q[k].x, ddy = xi.y - q[k].y, ddz = xi.z - q[k].z;
+        dd[k] = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        const float rj = q[k].w * 4.0f;
+        thr[k] = (ri + rj) * 0.5f;
+        const float t2 = thr[k] * thr[k];
+        const bool valid = static_cast<uint32_t>(k) < n;
+        const bool lo = dd[k] < t2 * 0.9999f, hi = dd[k] > t2 * 1.0001f;
+        in |= static_cast<uint32_t>(valid && lo) << k;
+        near |= static_cast<uint32_t>(valid && !lo && !hi) << k;
+    }
+    if (near) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+            if ((near >> k) & 1u) in |= static_cast<uint32_t>(__builtin_sqrtf(dd[k]) < thr[k]) << k;
+    }
+    return in;
+}
+
+// hit_mask when every candidate has the particle's radius: thr, thr^2 (1 - 1e-4) and
+// thr^2 (1 + 1e-4) are the same for all pairs and come precomputed (the same float operations).
+template <int M>
+__device__ __forceinline__ uint32_t hit_mask_uniform(const float4& xi, float thr, float t2lo, float t2hi, const float4 (&q)[M],
+                                                     uint32_t n) {
+    uint32_t in = 0u, near = 0u;
+    float dd[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const float ddx = xi.x - q[k].x, ddy = xi.y - q[k].y, ddz = xi.z - q[k].z;
+        dd[k] = (ddx * ddx + ddy * ddy) + ddz * ddz;
+        const bool valid = static_cast<uint32_t>(k) < n;
+        const bool lo = dd[k] < t2lo, hi = dd[k] > t2hi;
+        in |= static_cast<uint32_t>(valid && lo) << k;
+        near |= static_cast<uint32_t>(valid && !lo && !hi) << k;
+    }
+    if (near) {
+#pragma unroll
+        for (int k = 0; k < M; k++)
+            if ((near >> k) & 1u) in |= static_cast<uint32_t>(__builtin_sqrtf(dd[k]) < thr) << k;
+    }
+    return in;
+}
+
 template <bool SELF, bool COLS>
 __device__ __forceinline__ uint32_t search_cells(float4 xi, uint32_t i, const HashGeom& g, const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ end, const uint32_t* __restrict__ ids,
@@ -450,9 +502,12 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     __shared__ float4 cpos[kSrchCand];
     __shared__ float4 bcand[kSrchBCand];
     __shared__ unsigned long long scan[kBlock];
-    __shared__ uint32_t ncell, tf, tb;
+    __shared__ uint32_t ncell, tf, tb, mixed;
     for (uint32_t t = tid; t < kSrchHash; t += kBlock) ckey[t] = kNoCell;
-    if (tid == 0) ncell = 0u;
+    if (tid == 0) {
+        ncell = 0u;
+        mixed = 0u;
+    }
     const float4 xi = live ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
     bool leader = live;
@@ -537,9 +592,13 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         __syncthreads();
         staged = tf <= kSrchCand && tb <= kSrchBCand;
     }
-    // 3. stage the candidates: copy t belongs to the last cell e with efo[e] <= t
+    // 3. stage the candidates: copy t belongs to the last cell e with efo[e] <= t. `mixed`
+    //    records whether any fluid (bit 0) or wall (bit 1) candidate has another radius than the
+    //    block's first particle, whose radius every staged candidate otherwise shares.
+    const float rref = pos[min(blk * kBlock, N - 1u)].w;
     if (staged) {
         const uint32_t Tf = tf, Tb = tb;
+        uint32_t mx = 0u;
         for (uint32_t t = tid; t < Tf; t += kBlock) {
             uint32_t lo = 0u, hi = nc;  // efo[lo] <= t < efo[hi]
             while (hi - lo > 1u) {
@@ -547,7 +606,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
                 if (efo[mid] <= t) lo = mid;
                 else hi = mid;
             }
-            cpos[t] = pos[efs[lo] + (t - efo[lo])];
+            const float4 v = pos[efs[lo] + (t - efo[lo])];
+            cpos[t] = v;
+            mx |= v.w != rref ? 1u : 0u;
         }
         for (uint32_t t = tid; t < Tb; t += kBlock) {
             uint32_t lo = 0u, hi = nc;
@@ -556,8 +617,11 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
                 if (ebo[mid] <= t) lo = mid;
                 else hi = mid;
             }
-            bcand[t] = bd.spos[ebs[lo] + (t - ebo[lo])];
+            const float4 v = bd.spos[ebs[lo] + (t - ebo[lo])];
+            bcand[t] = v;
+            mx |= v.w != rref ? 2u : 0u;
         }
+        if (mx) atomicOr(&mixed, mx);
         __syncthreads();
     }
     // 4. the search proper
@@ -565,6 +629,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     if (live) {
         if (staged) {
             const float ri = xi.w * 4.0f;
+            // every staged candidate with this particle's radius (the usual case): one threshold
+            const bool uf = (mixed & 1u) == 0u, ub = (mixed & 2u) == 0u;
+            const float thr = (ri + ri) * 0.5f, t2 = thr * thr, t2lo = t2 * 0.9999f, t2hi = t2 * 1.0001f;
             for (int t = 0; t < 27; t++) {
                 const i3 q = cell_offset(p, t);
                 const uint32_t c = grid_index(q.x, q.y, q.z, g);
@@ -573,29 +640,43 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
                 if (ckey[h] != c) continue;  // not reached: every cell of a run was registered by its first particle
                 const uint32_t e = cent[h];
                 const uint32_t f0 = efo[e], f1 = efo[e + 1], s0 = efs[e];
-                auto take = [&](uint32_t u) {
-                    const uint32_t j = s0 + (u - f0);
-                    if (j != i) {
-                        if (a < Kff) nff[size_t(a) * N + i] = j;
-                        a++;
+                // the cell's staged candidates, four at a time; hits take their ranks in slot
+                // order (the particle itself is not a neighbour)
+                for (uint32_t u = f0; u < f1; u += 4u) {
+                    const uint32_t n = min(f1 - u, 4u);
+                    float4 q[4];
+#pragma unroll
+                    for (uint32_t k = 0; k < 4u; k++) q[k] = cpos[k < n ? u + k : u];
+                    const uint32_t j0 = s0 + (u - f0);
+                    uint32_t in = uf ? hit_mask_uniform<4>(xi, thr, t2lo, t2hi, q, n) : hit_mask<4>(xi, ri, q, n);
+                    const uint32_t self = i - j0;
+                    if (self < 4u) in &= ~(1u << self);
+#pragma unroll
+                    for (uint32_t k = 0; k < 4u; k++) {
+                        if ((in >> k) & 1u) {
+                            const uint32_t rank = a + static_cast<uint32_t>(__builtin_popcount(in & ((1u << k) - 1u)));
+                            if (rank < Kff) nff[size_t(rank) * N + i] = j0 + k;
+                        }
                     }
-                };
-                uint32_t u = f0;
-                for (; u + 1u < f1; u += 2u) {
-                    const bool in0 = within(xi, ri, cpos[u]), in1 = within(xi, ri, cpos[u + 1u]);
-                    if (in0) take(u);
-                    if (in1) take(u + 1u);
+                    a += static_cast<uint32_t>(__builtin_popcount(in));
                 }
-                if (u < f1 && within(xi, ri, cpos[u])) take(u);
                 if (bd.n) {
                     const uint32_t w0 = ebo[e], w1 = ebo[e + 1], z0 = ebs[e];
-                    for (uint32_t u = w0; u < w1; u++) {
-                        if (within(xi, ri, bcand[u])) {
-                            const uint32_t s = z0 + (u - w0);
-                            const uint32_t j = bd.id ? bd.id[s] : s;
-                            if (b < Kfb) nfb[size_t(b) * N + i] = j;
-                            b++;
+                    for (uint32_t u = w0; u < w1; u += 4u) {
+                        const uint32_t n = min(w1 - u, 4u);
+                        float4 q[4];
+#pragma unroll
+                        for (uint32_t k = 0; k < 4u; k++) q[k] = bcand[k < n ? u + k : u];
+                        const uint32_t in = ub ? hit_mask_uniform<4>(xi, thr, t2lo, t2hi, q, n) : hit_mask<4>(xi, ri, q, n);
+                        const uint32_t sb = z0 + (u - w0);
+#pragma unroll
+                        for (uint32_t k = 0; k < 4u; k++) {
+                            if ((in >> k) & 1u) {
+                                const uint32_t rank = b + static_cast<uint32_t>(__builtin_popcount(in & ((1u << k) - 1u)));
+                                if (rank < Kfb) nfb[size_t(rank) * N + i] = bd.id ? bd.id[sb + k] : sb + k;
+                            }
                         }
+                        b += static_cast<uint32_t>(__builtin_popcount(in));
                     }
                 }
             }
