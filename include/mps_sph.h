@@ -200,6 +200,12 @@ typedef struct mps_config {
     uint32_t persistent_spin;    /* polls before a barrier wait of the persistent solver gives up
                                     (the solve then reruns with chunked launches); 0 = 4194304 */
     double adj_cg_tol;           /* adjoint: its r.r threshold, same convention as cg_tol (-1e-12) */
+    uint32_t cg_fold_in_kernel;  /* HIP library: 1 = the chunked viscosity CG folds its sums in the last
+                                    workgroup of the A.p and update launches (two launches per
+                                    iteration; arrivals counted with device-scope atomics); 0 =
+                                    separate one-workgroup fold launches (four; the default). Same
+                                    results; mps_autotune may change it. The allocation-free step
+                                    always uses the fold launches. */
 } mps_config;
 
 /* Per-step statistics (the reference prints these to the debugger only:
@@ -426,8 +432,9 @@ int mps_adj_backward_traj(mps_ctx* ctx, uint32_t n_obs, const uint32_t* obs_step
  * unchanged and keeps the fastest in the context's configuration, one dimension after the
  * other: graph replay of solver chunks on/off with host polling every 1, 4 or 16 iterations or
  * adaptive; then the viscosity operator's coefficient stream (both coefficients, or a_ij with
- * b_ij recomputed: 24- or 32-byte LDS records); then the persistent CG solver where the grid
- * fits the device. The state is restored after every candidate. */
+ * b_ij recomputed: 24- or 32-byte LDS records); then the CG's folds (separate launches, or in
+ * the last workgroup of each launch); then the persistent CG solver where the grid fits the
+ * device. The state is restored after every candidate. */
 typedef struct mps_tune_result {
     uint32_t use_graphs;        /* chosen */
     uint32_t host_sync_every;   /* chosen (0 = adaptive) */
@@ -438,6 +445,9 @@ typedef struct mps_tune_result {
     uint32_t persistent_solver; /* chosen last (1 only where the grid fits the device) */
     double ms_coeffs[2];        /* cg_stream_coeffs 1, 0 */
     double ms_persistent[2];    /* persistent_solver 0, 2 (2: 0.0 when not applicable) */
+    uint32_t cg_fold_in_kernel; /* chosen before the persistent solver */
+    uint32_t pad2;
+    double ms_fold[2];          /* cg_fold_in_kernel 0, 1 */
 } mps_tune_result;
 int mps_autotune(mps_ctx* ctx, uint32_t steps, mps_tune_result* out);
 

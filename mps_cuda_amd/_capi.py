@@ -54,7 +54,7 @@ class Config(C.Structure):
         ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
         ("halo_limit", C.c_uint32), ("couple_scenes", C.c_uint32), ("cg_stream_coeffs", C.c_uint32),
         ("persistent_solver", C.c_uint32), ("persistent_spin", C.c_uint32),
-        ("adj_cg_tol", C.c_double),
+        ("adj_cg_tol", C.c_double), ("cg_fold_in_kernel", C.c_uint32),
     ]
 
 
@@ -103,7 +103,8 @@ class Fence(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("use_graphs", C.c_uint32), ("host_sync_every", C.c_uint32), ("n_tried", C.c_uint32),
                 ("pad", C.c_uint32), ("ms_per_step", C.c_double * 8), ("cg_stream_coeffs", C.c_uint32),
-                ("persistent_solver", C.c_uint32), ("ms_coeffs", C.c_double * 2), ("ms_persistent", C.c_double * 2)]
+                ("persistent_solver", C.c_uint32), ("ms_coeffs", C.c_double * 2), ("ms_persistent", C.c_double * 2),
+                ("cg_fold_in_kernel", C.c_uint32), ("pad2", C.c_uint32), ("ms_fold", C.c_double * 2)]
 
 
 class SceneRange(C.Structure):
@@ -364,7 +365,7 @@ class HipSim:
         return dict(use_graphs=r.use_graphs, host_sync_every=r.host_sync_every,
                     ms_per_step=list(r.ms_per_step[:8]), cg_stream_coeffs=r.cg_stream_coeffs,
                     ms_coeffs=list(r.ms_coeffs), persistent_solver=r.persistent_solver,
-                    ms_persistent=list(r.ms_persistent))
+                    ms_persistent=list(r.ms_persistent), cg_fold_in_kernel=r.cg_fold_in_kernel, ms_fold=list(r.ms_fold))
 
     # ---- adjoint
     def adj_rollout(self, n_steps: int, checkpoint_every: int = 10) -> np.ndarray:

@@ -303,6 +303,7 @@ struct mps_ctx {
     double* d_err3 = nullptr;
     double* h_err3 = nullptr;
     DBuf<double> partials;
+    DBuf<uint32_t> arrive;  // per-workgroup arrival counters of the CG launches that fold in their last workgroup
     // ---- render feed: the state is packed on the compute stream into one of two staging slots
     // and copied to the host on a second stream behind an event, so the next step's kernels run
     // while the copy drains; a slot is packed again only after its previous copy finished.
@@ -601,6 +602,16 @@ struct mps_ctx {
     void ensure_partials() {
         const size_t nblk = (std::max<size_t>(std::max<size_t>(N, B), 1) + mps::dev::kBlock - 1) / mps::dev::kBlock;
         partials.reserve(3 * nblk + 8);
+    }
+    // The arrival counters of the fold-in-kernel CG launches: zero when (re)allocated; every
+    // launch leaves them at zero.
+    uint32_t* arrive_counters() {
+        const size_t nblk = (std::max<size_t>(N, 1) + mps::dev::kBlock - 1) / mps::dev::kBlock;
+        if (arrive.cap < nblk) {
+            arrive.reserve(nblk);
+            mps::launch::zero_words(arrive.p, arrive.cap, stream);
+        }
+        return arrive.p;
     }
     size_t table_cells() const { return size_t(S) * cells(); }
 
@@ -1108,9 +1119,25 @@ struct mps_ctx {
                                   (const void*)hl.nb16})
                 key.push_back(u(q));
             key.push_back(u(tol));
+            // the allocation-free step keeps the fold launches (their counters would be scratch)
+            const bool fold_in = cfg.cg_fold_in_kernel != 0 && !g_no_alloc;
+            uint32_t* arr = fold_in ? arrive_counters() : nullptr;
+            key.push_back(fold_in);
+            key.push_back(u(arr));
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
                     const uint32_t k = it + c;
+                    if (fold_in) {  // each launch's last workgroup folds (two launches per iteration)
+                        T_(K_CG_APPLY, [&] {
+                            mps::launch::cg_apply_fold(op, P, nbr(), hl, pos.p, v3r.p, pbuf(k - 1), pbuf(k), v3Ap.p, sd,
+                                                       partials.p, arr, stream);
+                        }, false);
+                        T_(K_CG_UPDATE, [&] {
+                            mps::launch::cg_update3_fold(P, v3x.p, v3r.p, pbuf(k), v3Ap.p, sd, partials.p, act, tol, arr,
+                                                         stream);
+                        }, false);
+                        continue;
+                    }
                     T_(K_CG_APPLY, [&] {
                         mps::launch::cg_apply(1, op, P, nbr(), hl, pos.p, nullptr, v3r.p, pbuf(k - 1), pbuf(k), v3Ap.p,
                                               nullptr, nullptr, nullptr, sd, partials.p, stream);
@@ -1131,7 +1158,7 @@ struct mps_ctx {
         if (prof_on) {  // A.p (with the direction) and the update ran for iterations 1..l-1
             kuse[K_CG_APPLY] += std::min(l - 1, cap);
             kuse[K_CG_UPDATE] += std::min(l - 1, cap);
-            kuse[K_FOLD] += 2 * std::min(l - 1, cap);
+            if (!cfg.cg_fold_in_kernel || g_no_alloc) kuse[K_FOLD] += 2 * std::min(l - 1, cap);
         }
         T_(K_COPY_OUT, [&] { mps::launch::copy_vel_out3(P, v3x.p, vel.p, stream); });
         for (uint32_t q = 0; q < S && q < sstats.size(); q++) {
@@ -1555,7 +1582,23 @@ struct mps_ctx {
             }
             cfg.cg_stream_coeffs = bc;
         }
-        // 3. the persistent CG solver, where every workgroup fits the device at once
+        // 3. the CG's folds: separate launches or in each launch's last workgroup
+        uint32_t bf = cfg.cg_fold_in_kernel;
+        if (cfg.visc_solver == MPS_VISC_CG) {
+            best = 1e300;
+            for (uint32_t v = 0; v < 2; v++) {
+                cfg.cg_fold_in_kernel = v;
+                const double ms = time_steps();
+                r.ms_fold[v] = ms;
+                n++;
+                if (ms < best) {
+                    best = ms;
+                    bf = v;
+                }
+            }
+            cfg.cg_fold_in_kernel = bf;
+        }
+        // 4. the persistent CG solver, where every workgroup fits the device at once
         uint32_t bp = 0;
         if (cfg.visc_solver == MPS_VISC_CG) {
             cfg.persistent_solver = 0;
@@ -1584,6 +1627,7 @@ struct mps_ctx {
             out->host_sync_every = bh;
             out->cg_stream_coeffs = bc;
             out->persistent_solver = bp;
+            out->cg_fold_in_kernel = bf;
             out->n_tried = n;
         }
     }
@@ -2390,6 +2434,7 @@ struct mps_ctx {
         nff.release(); cff.release(); nfb.release(); cfb.release(); nbb.release(); cbb.release(); nfr.release(); cfr.release();
         bobj.release(); bS4.release();
         partials.release();
+        arrive.release();
         pairG_ff.release(); pairG_fb.release(); pairAB.release(); pairA.release(); irho.release(); halo_idx.release(); halo_n.release(); nb16.release(); nstat.release(); cgrecA.release(); cgrecB.release();
         if (copy_stream) (void)hipStreamSynchronize(copy_stream);
         for (ReadbackSlot& sl : rb) {
@@ -2547,6 +2592,7 @@ int mps_config_init(mps_config* cfg) {
     cfg->host_sync_every = 0;
     cfg->adj_cg_max_iter = 300;
     cfg->adj_cg_tol = -1.0e-12;
+    cfg->cg_fold_in_kernel = 0;
     cfg->cg_stream_coeffs = 1;  // measured faster on the 1M dam break (recomputing b_ij is VALU-bound)
     return MPS_OK;
 }

@@ -47,7 +47,7 @@ struct GridSync {
 struct SolverDev {
     uint32_t done;     // loop finished (converged); later kernels of the chunk exit at once
     uint32_t l;        // reference loop variable when the loop ended (or next iteration)
-    uint32_t counter;  // unused (kept for layout)
+    uint32_t counter;  // workgroups of the current launch that arrived (folds in the last workgroup)
     uint32_t pad;
     double rr;         // CG: r.r of the current residual
     double pAp;        // CG: p.Ap
@@ -235,6 +235,15 @@ struct CgOperator {
 void cg_apply(int mode, const CgOperator& op, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos,
               const float4* x0, const dev::f3* rr, const dev::f3* pprev, dev::f3* pout, dev::f3* Ap, const float4* b,
               dev::f3* r, dev::f3* x, const SolverDev* st, double* partials, hipStream_t s);
+// A.p (mode 1) and the update, each with its fold (k_fold_cg 1 / 2) done by the scene's last
+// workgroup to finish (the solver record's counter counts arrivals): two launches per CG
+// iteration instead of four, with the same bits. arrive: one counter per workgroup (zero, and
+// zero again after every launch).
+void cg_apply_fold(const CgOperator& op, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const dev::f3* rr,
+                   const dev::f3* pprev, dev::f3* pout, dev::f3* Ap, SolverDev* st, double* partials, uint32_t* arrive,
+                   hipStream_t s);
+void cg_update3_fold(const Scenes& sc, dev::f3* x, dev::f3* r, const dev::f3* p, const dev::f3* Ap, SolverDev* st,
+                     double* partials, uint32_t* active, double tol, uint32_t* arrive, hipStream_t s);
 // Workgroups of the persistent CG kernel that fit the device at once (0 if unknown).
 // coherent: the barrier variant (0: agent-scope fences; 1: device-coherent accesses of the data
 // that crosses workgroups, no cache write-back or invalidation)

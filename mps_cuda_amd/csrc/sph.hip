@@ -962,7 +962,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 //         r = b - A x0, pout = r, x = x0, partial sums of r.r.
 // MODE 1: A p with f (r in rr), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
 // One workgroup's share of A.p: block `blk` of scene `sid`, search-direction factor bta.
-template <int MODE, bool RECOMP, bool CUBIC, bool COH = false>
+// PCOH: the partial sum is stored device-coherently (a workgroup of the same launch folds it).
+template <int MODE, bool RECOMP, bool CUBIC, bool COH = false, bool PCOH = COH>
 __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float bta, const Scenes& sc, const Nbr& nb,
                                                const Halo& hl, const float2* __restrict__ cab, const float* __restrict__ ca,
                                                const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -1115,7 +1116,7 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
             part = double(pi.x * o.x) + double(pi.y * o.y) + double(pi.z * o.z);
         }
     }
-    block_partial<COH>(part, partials, blk);
+    block_partial<PCOH>(part, partials, blk);
 }
 
 template <int MODE, bool RECOMP, bool CUBIC>
@@ -1202,7 +1203,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(Scenes sc, float4* __restr
 }
 
 // The update on packed triples (the forward solve): same expressions as k_cg_update.
-template <bool COH = false>
+template <bool COH = false, bool PCOH = COH>
 __device__ __forceinline__ void cg_update3_block(uint32_t blk, uint32_t end, float a, f3* __restrict__ x,
                                                  f3* __restrict__ r, const f3* __restrict__ p, const f3* __restrict__ Ap,
                                                  double* __restrict__ partials) {
@@ -1221,7 +1222,7 @@ __device__ __forceinline__ void cg_update3_block(uint32_t blk, uint32_t end, flo
         st3c<COH>(r + i, ri);
         part = double(ri.x * ri.x) + double(ri.y * ri.y) + double(ri.z * ri.z);
     }
-    block_partial<COH>(part, partials, blk);
+    block_partial<PCOH>(part, partials, blk);
 }
 __global__ void __launch_bounds__(kBlock) k_cg_update3(Scenes sc, f3* __restrict__ x, f3* __restrict__ r,
                                                      const f3* __restrict__ p, const f3* __restrict__ Ap,
@@ -1241,18 +1242,37 @@ template <bool COH>
 __device__ __forceinline__ double fold_as_fold_kernel(const double* __restrict__ partials, uint32_t n) {
     constexpr uint32_t V = kFoldThreads / kBlock;
     constexpr uint32_t kWaves = kFoldThreads / 64;
+    constexpr uint32_t R = 4;  // strides of kFoldThreads loaded per batch: R * V loads in flight
     __shared__ double wsum[kWaves];
     __shared__ double total;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    auto ld = [&](uint32_t k) -> double {
+        if constexpr (COH) return __hip_atomic_load(partials + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else return partials[k];
+    };
+    double s[V];
+#pragma unroll
+    for (uint32_t v = 0; v < V; v++) s[v] = 0.0;
+    for (uint32_t m0 = 0; m0 * kFoldThreads < n; m0 += R) {
+        double t[R][V];
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++)
+#pragma unroll
+            for (uint32_t v = 0; v < V; v++) {
+                const uint32_t k = threadIdx.x + v * kBlock + (m0 + r) * kFoldThreads;
+                t[r][v] = k < n ? ld(k) : 0.0;
+            }
+#pragma unroll
+        for (uint32_t r = 0; r < R; r++)
+#pragma unroll
+            for (uint32_t v = 0; v < V; v++)
+                if (threadIdx.x + v * kBlock + (m0 + r) * kFoldThreads < n) s[v] += t[r][v];
+    }
 #pragma unroll
     for (uint32_t v = 0; v < V; v++) {
-        double s = 0.0;
-        for (uint32_t k = threadIdx.x + v * kBlock; k < n; k += kFoldThreads) {
-            if constexpr (COH) s += __hip_atomic_load(partials + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            else s += partials[k];
-        }
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) wsum[wave + v * (kBlock / 64)] = s;
+        double x = s[v];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) wsum[wave + v * (kBlock / 64)] = x;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1387,6 +1407,84 @@ __global__ void __launch_bounds__(kBlock) k_cg_persistent(Scenes sc, Nbr nb, Hal
             }
         }
     }
+}
+
+// ----------------------------------------------------------------- folds in the last workgroup
+// The chunked CG without fold launches: each workgroup stores its partial device-coherently,
+// waits until the store is performed and counts itself; the scene's last workgroup to arrive
+// then sees every partial, folds them in the fold kernels' order (fold_as_fold_kernel, so the
+// scalars have the same bits) and updates the record as k_fold_cg<MODE> would, for the next
+// launch. Nothing waits for anything: the other workgroups simply end. Arrivals are counted in
+// two levels, as same-address atomics serialize: groups of kArriveGroup consecutive workgroups
+// of the scene count in arrive[blk0 + group], and the last of each group counts the group in
+// the solver record's counter. Every counter is back at zero when the launch ends.
+constexpr uint32_t kArriveGroup = 64;
+template <int MODE>
+__device__ __forceinline__ void fold_in_last_workgroup(const Scenes& sc, uint32_t sid, uint32_t blk,
+                                                       const double* __restrict__ partials, SolverDev* stv, uint32_t* active,
+                                                       double tol, uint32_t* arrive) {
+    __shared__ uint32_t last_s;
+    const StepParams P = sc.P[sid];
+    SolverDev* st = stv + sid;
+    __builtin_amdgcn_s_waitcnt(0);  // the partial (thread 0's coherent store) is performed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t grp = (blk - P.blk0) / kArriveGroup;
+        const uint32_t gsize = min(kArriveGroup, P.nblk - grp * kArriveGroup);
+        const uint32_t ngrp = (P.nblk + kArriveGroup - 1u) / kArriveGroup;
+        uint32_t* gc = arrive + P.blk0 + grp;
+        bool last = false;
+        if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == gsize) {
+            __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            last = __hip_atomic_fetch_add(&st->counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == ngrp;
+        }
+        last_s = last;
+    }
+    __syncthreads();
+    if (!last_s) return;
+    const double s = fold_as_fold_kernel<true>(partials + P.blk0, P.nblk);
+    if (threadIdx.x != 0) return;
+    __hip_atomic_store(&st->counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (MODE == 1) {
+        st->pAp = s;
+        st->alpha = float(st->rr / s);
+    } else {
+        const uint32_t it = st->l;
+        st->beta = float(s / st->rr);
+        st->rr = s;
+        if (it < MPS_MAX_ITER_LOG) st->log[it] = s;
+        st->l = it + 1;
+        if (s < st->thr || (tol < 0.0 && s == 0.0)) {
+            st->done = 1u;
+            atomicSub(active, 1u);
+        }
+    }
+}
+// k_cg_apply<1> + k_fold_cg<1> in one launch
+template <bool RECOMP, bool CUBIC>
+__global__ void __launch_bounds__(kBlock) k_cg_apply_fold(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
+                                                        const float* __restrict__ ca, const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel, const float* __restrict__ rho,
+                                                        const float* __restrict__ irho, const f3* __restrict__ rr,
+                                                        const f3* __restrict__ pprev, f3* __restrict__ pout,
+                                                        f3* __restrict__ Ap, SolverDev* st, double* __restrict__ partials,
+                                                        uint32_t* arrive) {
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t sid = sc.blk_scene[blk];
+    if (st[sid].done) return;
+    cg_apply_block<1, RECOMP, CUBIC, false, true>(blk, sid, st[sid].beta, sc, nb, hl, cab, ca, pos, vel, rho, irho, nullptr, rr,
+                                                  pprev, pout, Ap, nullptr, nullptr, nullptr, partials);
+    fold_in_last_workgroup<1>(sc, sid, blk, partials, st, nullptr, 0.0, arrive);
+}
+// k_cg_update3 + k_fold_cg<2> in one launch
+__global__ void __launch_bounds__(kBlock) k_cg_update3_fold(Scenes sc, f3* __restrict__ x, f3* __restrict__ r,
+                                                          const f3* __restrict__ p, const f3* __restrict__ Ap, SolverDev* st,
+                                                          double* __restrict__ partials, uint32_t* active, double tol,
+                                                          uint32_t* arrive) {
+    const uint32_t sid = sc.blk_scene[blockIdx.x];
+    if (st[sid].done) return;
+    cg_update3_block<false, true>(blockIdx.x, sc.P[sid].end, st[sid].alpha, x, r, p, Ap, partials);
+    fold_in_last_workgroup<2>(sc, sid, blockIdx.x, partials, st, active, tol, arrive);
 }
 
 // UpdateCGDirection_kernel (MPSCGKernel.cuh:123-133), p kept in the position records
@@ -1773,6 +1871,22 @@ void cg_apply(int mode, const CgOperator& op, const Scenes& sc, const Nbr& nb, c
         else MPS_CG_APPLY(1, true, false);
     }
 #undef MPS_CG_APPLY
+}
+void cg_apply_fold(const CgOperator& op, const Scenes& sc, const Nbr& nb, const Halo& hl, const float4* pos, const f3* rr,
+                   const f3* pprev, f3* pout, f3* Ap, SolverDev* st, double* partials, uint32_t* arrive, hipStream_t s) {
+    if (!sc.N) return;
+    const unsigned g = nb_(sc.N);
+#define MPS_CG_APPLY_FOLD(R, C)                                                                                    \
+    launch_k(k_cg_apply_fold<R, C>, g, kBlock, 0, s, sc, nb, hl, op.cab, op.ca, pos, op.vel, op.rho, op.irho, rr, pprev, \
+             pout, Ap, st, partials, arrive)
+    if (!op.recompute) MPS_CG_APPLY_FOLD(false, true);
+    else if (op.cubic) MPS_CG_APPLY_FOLD(true, true);
+    else MPS_CG_APPLY_FOLD(true, false);
+#undef MPS_CG_APPLY_FOLD
+}
+void cg_update3_fold(const Scenes& sc, f3* x, f3* r, const f3* p, const f3* Ap, SolverDev* st, double* partials,
+                     uint32_t* active, double tol, uint32_t* arrive, hipStream_t s) {
+    if (sc.N) launch_k(k_cg_update3_fold, nb_(sc.N), kBlock, 0, s, sc, x, r, p, Ap, st, partials, active, tol, arrive);
 }
 static const void* persistent_fn(const CgOperator& op, int coherent) {
     if (coherent) {
