@@ -76,13 +76,18 @@ def say(msg: str) -> None:
 
 def measured_traffic(kernel: str) -> float | None:
     """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
-    rocprofv3 counter passes over the same 1000 steps, each scaled by what the counter reads for
-    a plain copy of known size (the summary's last column: MB per working dispatch, mean)."""
+    rocprofv3 counter passes over the same timed window, each scaled by what the counter reads
+    for a plain copy of known size (the summary's last column: MB per working dispatch, mean).
+    The profiler may spell out more template arguments than the library's timing table
+    (`k_cg_apply<1, false, true>` for `k_cg_apply<1>`)."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) >= 5 and cells[0] == kernel:
+                if len(cells) < 5:
+                    continue
+                name = cells[0]
+                if name == kernel or (kernel.endswith(">") and name.startswith(kernel[:-1] + ",")):
                     return float(cells[-1]) * 1.0e6
     except (OSError, ValueError):
         return None
