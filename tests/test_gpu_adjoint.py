@@ -245,15 +245,20 @@ def test_autograd_trajectory_matches_the_c_abi():
     assert _rel(v0.grad.numpy(), gv0) < 1e-6 and _rel(mu.grad.numpy(), gmu) < 1e-6 and _rel(sig.grad.numpy(), gsig) < 1e-6
 
 
-def test_adjoint_matches_fp64_finite_differences():
-    """A 4096-particle droplet over 30 steps at (viscosity 0.025, surface tension 0.1): the GPU
-    adjoint's gradients of L = sum w . x_T (fixed random w, so both sides use the same cotangent)
-    against central differences (eps 1e-3, relative for the coefficients, along random unit
-    vectors per particle for the initial velocities) of the FP64 oracle's own forward (its CG run
-    to 1e-16 of the initial r.r), within 1e-3."""
+@pytest.mark.parametrize("kernel", ["cubic", "poly6"])
+def test_adjoint_matches_fp64_finite_differences(kernel):
+    """A 4096-particle droplet over 30 steps at (viscosity 0.025, surface tension 0.1), with the
+    cubic spline or the poly6/spiky kernels: the GPU adjoint's gradients of L = sum w . x_T (fixed
+    random w, so both sides use the same cotangent) against central differences (eps 1e-3,
+    relative for the coefficients, along random unit vectors per particle for the initial
+    velocities) of the FP64 oracle's own forward (its CG run to 1e-16 of the initial r.r), within
+    1e-3."""
+    from mps_cuda_amd import _capi
     from oracle.binding import OracleSim, set_threads
     steps = 30
     sc = scenes.diff_droplet(n=4096, radius=0.1, seed=5, steps=steps, h=0.02)
+    if kernel == "poly6":
+        sc.config["kernel"] = _capi.KERNEL_POLY6_SPIKY
     g = _load(sc)
     mu0, sig0 = 0.025, 0.1
     w = np.random.default_rng(8).normal(size=(sc.n, 3))
@@ -282,7 +287,7 @@ def test_adjoint_matches_fp64_finite_differences():
     fd_v = (o_loss(mu0, sig0, v0 + e * dv) - o_loss(mu0, sig0, v0 - e * dv)) / (2 * e)
     ad_v = float(np.sum(gv0.astype(np.float64) * dv))
     errs = (abs(gmu[0] - fd_mu) / abs(fd_mu), abs(gsig[0] - fd_sig) / abs(fd_sig), abs(ad_v - fd_v) / abs(fd_v))
-    say(f"[adjoint] 4096 x {steps} steps vs FP64 FD: mu {gmu[0]:.6e}/{fd_mu:.6e}  sigma {gsig[0]:.6e}/{fd_sig:.6e}  "
+    say(f"[adjoint] {kernel}, 4096 x {steps} steps vs FP64 FD: mu {gmu[0]:.6e}/{fd_mu:.6e}  sigma {gsig[0]:.6e}/{fd_sig:.6e}  "
         f"v0 {ad_v:.6e}/{fd_v:.6e}  rel errs {errs}")
     assert max(errs) < 1e-3, errs
 
