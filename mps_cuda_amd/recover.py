@@ -22,11 +22,14 @@ to the initial velocities' response. With the initial velocities known the probl
 scene and is solved directly. Each scene then takes the largest of 1, 1/4, 1/16, 1/64 of its
 step (after clipping the log steps to `max_log_step`) that lowers its loss, or keeps its values.
 
-The initial velocity is only partly observable: the first step's divergence-free solve removes
-its compressive part and viscosity damps its fine scales. Besides the raw error, the summary
-reports the error of its response: the velocity after the first step (at the hidden
-coefficients) from the recovered initial velocity against that from the hidden one, relative to
-the part of it that the hidden initial velocity contributes.
+The initial velocities are sought as trilinear interpolation of nodal values on a lattice of
+spacing `v0_grid` per scene (0: one unknown per particle). The initial velocity is only partly
+observable: the first step's divergence-free solve removes its compressive part and viscosity
+damps its fine scales. Besides the raw error, the summary reports the error of its response:
+the velocity after the first step, and at the first observation, at the hidden coefficients,
+from the recovered initial velocity against that from the hidden one, relative to the part of
+it that the hidden initial velocity contributes; and the trajectory fit (rms distance to the
+observed positions against their rms displacement from the start).
 
     python -m mps_cuda_amd.recover [--scenes 32] [--steps 80] [--iters 10] [--inner 6] [--out result.json]
     python -m mps_cuda_amd.recover --known-v0        # coefficients only, hidden v0 given
@@ -121,29 +124,83 @@ def _jv(b: _Batch, x, v0, mu, sig, pv, fd_v):
     return (xp - x) / b.spread(eps)[None]
 
 
-def _cgls_v(b: _Batch, x, r, v0, mu, sig, inner, fd_v):
-    """min |r + J_v d_v|^2 per scene by `inner` CGLS iterations (the coefficients held).
-    Returns (d_v, relative residual per scene)."""
+class _Grid:
+    """Initial velocities as trilinear interpolation of nodal values on a per-scene lattice of
+    spacing h (v0 = B c; the adjoint's gradient maps back as B^T g). Keeps the unknowns to the
+    smooth fields the observations can determine: the first step's divergence-free solve and the
+    viscosity remove most of what a finer field would add. h = 0: one unknown per particle."""
+
+    def __init__(self, scene_list, h):
+        self.h = float(h)
+        if self.h <= 0.0:
+            self.B = self.T = lambda a: a
+            return
+        idx, wts, nsid, base = [], [], [], 0
+        for q, sc in enumerate(scene_list):
+            p = np.asarray(sc.pos, np.float64)
+            lo = p.min(axis=0) - 0.5 * self.h
+            dims = np.floor((p.max(axis=0) - lo) / self.h).astype(np.int64) + 2
+            f = (p - lo) / self.h
+            i0 = np.floor(f).astype(np.int64)
+            t = f - i0
+            for c in range(8):
+                o = np.array([(c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1])
+                k = i0 + o
+                node = base + (k[:, 0] * dims[1] + k[:, 1]) * dims[2] + k[:, 2]
+                w = np.prod(np.where(o[None, :] == 1, t, 1.0 - t), axis=1)
+                idx.append(node)
+                wts.append(w)
+            nsid.append(np.full(int(np.prod(dims)), q))
+            base += int(np.prod(dims))
+        n = sum(s.n for s in scene_list)
+        self.idx = torch.as_tensor(np.stack([np.concatenate(idx[c::8]) for c in range(8)], axis=1))  # (n, 8)
+        self.w = torch.as_tensor(np.stack([np.concatenate(wts[c::8]) for c in range(8)], axis=1))    # (n, 8)
+        assert self.idx.shape == (n, 8)
+        self.M = base
+        self.node_sid = torch.as_tensor(np.concatenate(nsid))
+
+        def B(c):  # nodes (M, 3) -> particles (n, 3)
+            return (self.w[:, :, None] * c[self.idx]).sum(dim=1)
+
+        def T(g):  # particles (n, 3) -> nodes (M, 3)
+            return torch.zeros(self.M, 3, dtype=torch.float64).index_add_(
+                0, self.idx.reshape(-1), (self.w[:, :, None] * g[:, None, :]).reshape(-1, 3))
+
+        self.B, self.T = B, T
+
+    def ssum(self, b: "_Batch", a):
+        """Per-scene sums of an array in the unknowns' space."""
+        if self.h <= 0.0:
+            return b.ssum(a)
+        return torch.zeros(b.S, dtype=torch.float64).index_add_(0, self.node_sid, a.sum(dim=1))
+
+    def spread(self, b: "_Batch", s):
+        return b.spread(s) if self.h <= 0.0 else s[self.node_sid][:, None]
+
+
+def _cgls_v(b: _Batch, grid: _Grid, x, r, v0, mu, sig, inner, fd_v):
+    """min |r + J_v B d|^2 per scene by `inner` CGLS iterations in the grid's unknowns (the
+    coefficients held). Returns (B d, relative residual per scene)."""
     rho = -r
     r0 = b.ssum(rho * rho)
-    s = b.vjp(rho)
+    s = grid.T(b.vjp(rho))
     p = s
-    gamma = b.ssum(s * s)
-    d = torch.zeros_like(v0)
+    gamma = grid.ssum(b, s * s)
+    d = torch.zeros_like(s)
     for i in range(inner):
-        q = _jv(b, x, v0, mu, sig, p, fd_v)
+        q = _jv(b, x, v0, mu, sig, grid.B(p), fd_v)
         delta = b.ssum(q * q)
         alpha = torch.where(delta > 0, gamma / delta.clamp_min(1e-300), torch.zeros_like(delta))
-        d = d + b.spread(alpha) * p
+        d = d + grid.spread(b, alpha) * p
         rho = rho - b.spread(alpha)[None] * q
         if i == inner - 1:
             break
-        s = b.vjp(rho)
-        gnew = b.ssum(s * s)
+        s = grid.T(b.vjp(rho))
+        gnew = grid.ssum(b, s * s)
         beta = torch.where(gamma > 0, gnew / gamma.clamp_min(1e-300), torch.zeros_like(gnew))
-        p = s + b.spread(beta) * p
+        p = s + grid.spread(b, beta) * p
         gamma = gnew
-    return d, (b.ssum(rho * rho) / r0.clamp_min(1e-300)).sqrt()
+    return grid.B(d), (b.ssum(rho * rho) / r0.clamp_min(1e-300)).sqrt()
 
 
 def _gn_small(b: _Batch, r, cols):
@@ -184,10 +241,11 @@ def _backtrack(b: _Batch, target, per, v0, lm, ls, dv, dm, ds, tries=4):
 
 
 def run(n_scenes=32, steps=80, iters=10, inner=6, seed=4, n_min=16384, n_max=131072, every=10, obs_every=10,
-        recover_v0=True, fd_log=0.05, fd_v=0.01, max_log_step=1.0, log=sys.stderr):
+        recover_v0=True, fd_log=0.05, fd_v=0.01, max_log_step=1.0, v0_grid=0.05, log=sys.stderr):
     sc, mu_true, sig_true = make_scenes(n_scenes, seed, n_min, n_max, steps)
     obs = list(range(obs_every, steps + 1, obs_every))
     b = _Batch(sc, steps, every, obs)
+    grid = _Grid(sc, v0_grid if recover_v0 else 0.0)
     v0_true = torch.as_tensor(np.concatenate([s.vel for s in sc]), dtype=torch.float64)
     t0 = time.perf_counter()
     target = b.forward(v0_true, mu_true, sig_true)
@@ -213,7 +271,7 @@ def run(n_scenes=32, steps=80, iters=10, inner=6, seed=4, n_min=16384, n_max=131
         res = zero
         kept = 0
         if recover_v0:  # initial velocities, coefficients held (the recorded rollout is at this point)
-            dv, res = _cgls_v(b, x, r, v0, lm.exp(), ls.exp(), inner, fd_v)
+            dv, res = _cgls_v(b, grid, x, r, v0, lm.exp(), ls.exp(), inner, fd_v)
             v0, lm, ls, ok = _backtrack(b, target, per, v0, lm, ls, dv, zero, zero)
             kept += int((~ok).sum())
             x = b.forward(v0, lm.exp(), ls.exp())
@@ -248,9 +306,18 @@ def run(n_scenes=32, steps=80, iters=10, inner=6, seed=4, n_min=16384, n_max=131
               f"kept {kept}  ({time.perf_counter() - t_start:.0f} s)", file=log, flush=True)
     t_solve = time.perf_counter() - t_start
     em, es, ev = errors()
-    # the initial velocity's response: the velocity after the first step at the hidden coefficients
-    x1 = [b.forward(v, mu_true, sig_true, obs=(1,))[0] for v in (v0, v0_true, torch.zeros_like(v0))]
-    resp = (b.ssum((x1[0] - x1[1]) ** 2) / b.ssum((x1[1] - x1[2]) ** 2)).sqrt().numpy()
+    # the initial velocity's response: the velocity after the first step, and the velocity at
+    # the first observation (positions after steps k and k + 1), at the hidden coefficients
+    k1 = obs[0]
+    xr = [b.forward(v, mu_true, sig_true, obs=(1, k1, k1 + 1)) for v in (v0, v0_true, torch.zeros_like(v0))]
+    resp = (b.ssum((xr[0][0] - xr[1][0]) ** 2) / b.ssum((xr[1][0] - xr[2][0]) ** 2)).sqrt().numpy()
+    vk = [x[2] - x[1] for x in xr]
+    resp_k = (b.ssum((vk[0] - vk[1]) ** 2) / b.ssum((vk[1] - vk[2]) ** 2)).sqrt().numpy()
+    # the fit itself: rms distance to the observed positions against the rms displacement of the
+    # observed positions from the start
+    x0 = torch.as_tensor(np.concatenate([s_.pos for s_ in sc]), dtype=torch.float64)
+    rms_fit = (per / (3.0 * len(obs) * b.sizes.to(torch.float64))).sqrt().numpy()
+    rms_disp = (b.ssum((target - x0[None]) ** 2) / (3.0 * len(obs) * b.sizes.to(torch.float64))).sqrt().numpy()
     mu, sig = lm.exp().numpy(), ls.exp().numpy()
     return {
         "method": "Gauss-Newton by blocks with per-scene backtracking: " + (
@@ -260,6 +327,7 @@ def run(n_scenes=32, steps=80, iters=10, inner=6, seed=4, n_min=16384, n_max=131
         "inner": inner if recover_v0 else 0,
         "scenes": n_scenes, "particles": b.n, "steps": steps, "observed_steps": obs, "iterations": iters,
         "fd_log": fd_log, "fd_v": fd_v, "max_log_step": max_log_step, "checkpoint_every": every,
+        "v0_grid": v0_grid if recover_v0 else None, "v0_unknowns": (3 * grid.M if grid.h > 0 else 3 * b.n) if recover_v0 else 0,
         "sizes": f"log-uniform {n_min}..{n_max}, seed {seed}", "start": {"viscosity": MU0, "surface_tension": SIGMA0},
         "target_rollout_s": t_target, "solve_s": t_solve,
         "forward_rollouts": b.fwd[0], "forward_s_each": b.fwd[1] / max(b.fwd[0], 1),
@@ -274,7 +342,13 @@ def run(n_scenes=32, steps=80, iters=10, inner=6, seed=4, n_min=16384, n_max=131
                              "rel_err_all": float(torch.linalg.norm(v0 - v0_true) / torch.linalg.norm(v0_true)),
                              "first_step_response_err_per_scene": resp.tolist(),
                              "first_step_response_err_median": float(np.median(resp)),
-                             "first_step_response_err_max": float(resp.max())},
+                             "first_step_response_err_max": float(resp.max()),
+                             "first_observation_velocity_err_per_scene": resp_k.tolist(),
+                             "first_observation_velocity_err_median": float(np.median(resp_k)),
+                             "first_observation_step": k1},
+        "trajectory_fit": {"rms_distance_per_scene": rms_fit.tolist(), "rms_displacement_per_scene": rms_disp.tolist(),
+                           "rms_distance_median": float(np.median(rms_fit)),
+                           "rms_distance_over_displacement_median": float(np.median(rms_fit / rms_disp))},
         "history": history,
     }
 
@@ -323,6 +397,8 @@ def main() -> int:
     ap.add_argument("--obs-every", type=int, default=10)
     ap.add_argument("--fd-log", type=float, default=0.05)
     ap.add_argument("--fd-v", type=float, default=0.01)
+    ap.add_argument("--v0-grid", type=float, default=0.05,
+                    help="spacing (m) of the trilinear lattice the initial velocities are sought on (0: per particle)")
     ap.add_argument("--known-v0", action="store_true", help="keep the hidden initial velocities (do not recover them)")
     ap.add_argument("--scan", type=int, default=0, help="instead: loss scan over this many surface-tension factors")
     ap.add_argument("--out", default=None)
@@ -332,7 +408,7 @@ def main() -> int:
         print(json.dumps({"scan_points": a.scan, "scenes": a.scenes}), flush=True)
     else:
         res = run(a.scenes, a.steps, a.iters, a.inner, a.seed, a.n_min, a.n_max, obs_every=a.obs_every,
-                  recover_v0=not a.known_v0, fd_log=a.fd_log, fd_v=a.fd_v)
+                  recover_v0=not a.known_v0, fd_log=a.fd_log, fd_v=a.fd_v, v0_grid=a.v0_grid)
         print(json.dumps({k: v for k, v in res.items() if k != "history" and not isinstance(v, dict)}), flush=True)
     if a.out:
         with open(a.out, "w") as f:
