@@ -466,18 +466,51 @@ __global__ void __launch_bounds__(kCountFoldThreads) k_count_fold(const uint2* _
 //
 // A workgroup owns 256 consecutive fluid slots (sorted by cell, so each cell's particles are
 // one run). The first particle of each run registers the 27 cells around it in an LDS hash
-// of the block's distinct neighbour cells; each distinct cell's bounds are read once, and
-// all fluid (and wall) particles of those cells are staged in LDS. Every particle then walks
-// its 27 cells in the reference order over the staged copies, so the lists are the same as
-// from the global tables. A block with more distinct cells or candidates than the LDS holds
-// searches the global tables.
+// of the block's distinct neighbour cells. The distinct cells are sorted by their linear key,
+// each one's bounds are read once, and all fluid (and wall) particles of those cells are
+// staged in LDS in that order, each with its slot (a wall particle: its id) in place of the
+// radius. The cells x-1, x, x+1 of one (y, z) row have consecutive keys, so their staged
+// copies form one run: every particle walks its 27 cells in the reference order as 9 row
+// runs. A block with more distinct cells or candidates than the LDS holds, or whose staged
+// fluid (or wall) particles do not all share one radius, searches the global tables for that
+// set; the lists are the same either way.
 constexpr uint32_t kSrchHash = 1024;   // LDS hash slots for cell indices
 constexpr uint32_t kSrchCells = 384;   // distinct neighbour cells per block
+constexpr uint32_t kSrchSort = 512;    // width of the key sort (a power of two >= kSrchCells)
 constexpr uint32_t kSrchCand = 1792;   // staged fluid candidates
 constexpr uint32_t kSrchBCand = 512;   // staged wall candidates
+constexpr uint32_t kSrchPad = 4;       // a group of four read at the end of a run stays in the array
 constexpr uint32_t kNoCell = 0xFFFFFFFFu;
+static_assert(kSrchSort >= kSrchCells && (kSrchSort & (kSrchSort - 1u)) == 0u, "sort width");
+static_assert(kSrchSort * sizeof(uint32_t) <= kBlock * sizeof(unsigned long long), "the sort reuses the scan buffer");
 __device__ __forceinline__ uint32_t cell_hash(uint32_t c) { return (c * 2654435761u) >> 22; }
-__device__ __forceinline__ i3 cell_offset(i3 p, int t) { return {p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1}; }
+
+// The hits among staged candidates [lo, hi) in order: four candidates per step (hit_mask_uniform,
+// the threshold of equal radii), a hit mask per 32 candidates, then the hits' ids (held in w) as
+// list entries cnt, cnt + 1, ... (column-major, stride N; entries past K are counted only).
+// `self` is the particle's own staged index, which is not a neighbour (any value >= hi: none).
+__device__ __forceinline__ void walk_run(const float4* cand, uint32_t lo, uint32_t hi, const float4& xi, float thr, float t2lo,
+                                         float t2hi, uint32_t self, uint32_t* __restrict__ out, uint32_t K, uint32_t N,
+                                         uint32_t i, uint32_t& cnt) {
+    for (uint32_t c0 = lo; c0 < hi; c0 += 32u) {
+        const uint32_t n = min(hi - c0, 32u);
+        uint32_t m = 0u;
+        for (uint32_t q = 0; q < n; q += 4u) {
+            float4 v[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; k++) v[k] = cand[c0 + q + k];  // <= hi + 2 < array size
+            m |= hit_mask_uniform<4>(xi, thr, t2lo, t2hi, v, n - q) << q;
+        }
+        const uint32_t d = self - c0;
+        if (d < 32u) m &= ~(1u << d);
+        while (m) {
+            const uint32_t b = static_cast<uint32_t>(__builtin_ctz(m));
+            m &= m - 1u;
+            if (cnt < K) out[size_t(cnt) * N + i] = __float_as_uint(cand[c0 + b].w);
+            cnt++;
+        }
+    }
+}
 
 __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __restrict__ pos, Scenes sc, uint32_t C, HashGeom g,
                                                           HashView fl, HashView bd, uint32_t* __restrict__ nff, uint32_t Kff,
@@ -499,8 +532,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     __shared__ uint16_t cent[kSrchHash];
     __shared__ uint32_t ecell[kSrchCells], efs[kSrchCells], ebs[kSrchCells];
     __shared__ uint16_t efo[kSrchCells + 1], ebo[kSrchCells + 1];
-    __shared__ float4 cpos[kSrchCand];
-    __shared__ float4 bcand[kSrchBCand];
+    __shared__ float4 cpos[kSrchCand + kSrchPad];
+    __shared__ float4 bcand[kSrchBCand + kSrchPad];
     __shared__ unsigned long long scan[kBlock];
     __shared__ uint32_t ncell, tf, tb, mixed;
     for (uint32_t t = tid; t < kSrchHash; t += kBlock) ckey[t] = kNoCell;
@@ -522,17 +555,13 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     if (leader) {
         for (int t = 0; t < 27; t++) {
             if (*(volatile uint32_t*)&ncell > kSrchCells) break;
-            const i3 q = cell_offset(p, t);
-            const uint32_t c = grid_index(q.x, q.y, q.z, g);
+            const uint32_t c = grid_index(p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1, g);
             uint32_t h = cell_hash(c);
             for (uint32_t probe = 0; probe < kSrchHash; probe++) {
                 const uint32_t prev = atomicCAS(&ckey[h], kNoCell, c);
                 if (prev == kNoCell) {
                     const uint32_t e = atomicAdd(&ncell, 1u);
-                    if (e < kSrchCells) {
-                        cent[h] = static_cast<uint16_t>(e);
-                        ecell[e] = c;
-                    }
+                    if (e < kSrchCells) ecell[e] = c;
                     break;
                 }
                 if (prev == c) break;
@@ -543,7 +572,45 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     __syncthreads();
     const uint32_t nc = ncell;
     bool staged = nc <= kSrchCells;
-    // 2. bounds of each distinct cell (read once) and offsets of its staged copies
+    // the position of key c in the sorted cell list, or kNoCell (not reached for a registered cell)
+    auto lookup = [&](uint32_t c) -> uint32_t {
+        uint32_t h = cell_hash(c);
+        for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
+        return ckey[h] == c ? static_cast<uint32_t>(cent[h]) : kNoCell;
+    };
+    // 2. the distinct cells in ascending key order (bitonic sort in the scan buffer); the hash
+    //    then maps each key to its sorted position
+    if (staged) {
+        uint32_t* sk = reinterpret_cast<uint32_t*>(scan);
+        uint32_t L = 1u;
+        while (L < nc) L <<= 1;
+        for (uint32_t t = tid; t < L; t += kBlock) sk[t] = t < nc ? ecell[t] : kNoCell;
+        __syncthreads();
+        for (uint32_t k = 2u; k <= L; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+                for (uint32_t t = tid; t < L; t += kBlock) {
+                    const uint32_t o = t ^ j;
+                    if (o > t) {
+                        const uint32_t x = sk[t], y = sk[o];
+                        if ((x > y) == ((t & k) == 0u)) {
+                            sk[t] = y;
+                            sk[o] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t e = tid; e < nc; e += kBlock) {
+            const uint32_t c = sk[e];
+            ecell[e] = c;
+            uint32_t h = cell_hash(c);
+            for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
+            cent[h] = static_cast<uint16_t>(e);
+        }
+        __syncthreads();
+    }
+    // 3. bounds of each distinct cell (read once) and offsets of its staged copies
     if (staged) {
         uint32_t fc[2] = {0u, 0u}, bc[2] = {0u, 0u};
 #pragma unroll
@@ -592,97 +659,80 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         __syncthreads();
         staged = tf <= kSrchCand && tb <= kSrchBCand;
     }
-    // 3. stage the candidates: copy t belongs to the last cell e with efo[e] <= t. `mixed`
-    //    records whether any fluid (bit 0) or wall (bit 1) candidate has another radius than the
-    //    block's first particle, whose radius every staged candidate otherwise shares.
-    const float rref = pos[min(blk * kBlock, N - 1u)].w;
+    // 4. stage the candidates with their ids: copy t belongs to the last cell e with
+    //    eo[e] <= t. `mixed` records whether some fluid (bit 0) or wall (bit 1) candidate has
+    //    another radius than the set's first staged candidate.
+    float rw = 0.0f;  // the walls' common radius
     if (staged) {
         const uint32_t Tf = tf, Tb = tb;
+        auto owner = [&](const uint16_t* eo, uint32_t t) {
+            uint32_t lo = 0u, hi = nc;  // eo[lo] <= t < eo[hi]
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (eo[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        };
         uint32_t mx = 0u;
-        for (uint32_t t = tid; t < Tf; t += kBlock) {
-            uint32_t lo = 0u, hi = nc;  // efo[lo] <= t < efo[hi]
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (efo[mid] <= t) lo = mid;
-                else hi = mid;
+        if (Tf) {
+            const float rf = pos[efs[owner(efo, 0u)]].w;
+            for (uint32_t t = tid; t < Tf; t += kBlock) {
+                const uint32_t e = owner(efo, t);
+                const uint32_t s = efs[e] + (t - efo[e]);
+                const float4 v = pos[s];
+                cpos[t] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
+                mx |= v.w != rf ? 1u : 0u;
             }
-            const float4 v = pos[efs[lo] + (t - efo[lo])];
-            cpos[t] = v;
-            mx |= v.w != rref ? 1u : 0u;
         }
-        for (uint32_t t = tid; t < Tb; t += kBlock) {
-            uint32_t lo = 0u, hi = nc;
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (ebo[mid] <= t) lo = mid;
-                else hi = mid;
+        if (Tb) {
+            rw = bd.spos[ebs[owner(ebo, 0u)]].w;
+            for (uint32_t t = tid; t < Tb; t += kBlock) {
+                const uint32_t e = owner(ebo, t);
+                const uint32_t sb = ebs[e] + (t - ebo[e]);
+                const float4 v = bd.spos[sb];
+                bcand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(bd.id ? bd.id[sb] : sb));
+                mx |= v.w != rw ? 2u : 0u;
             }
-            const float4 v = bd.spos[ebs[lo] + (t - ebo[lo])];
-            bcand[t] = v;
-            mx |= v.w != rref ? 2u : 0u;
         }
         if (mx) atomicOr(&mixed, mx);
         __syncthreads();
     }
-    // 4. the search proper
+    // 5. the search proper: 9 rows of three cells (z, then y offsets), each row's cells in x order
     uint32_t a = 0u, b = 0u;
     if (live) {
-        if (staged) {
-            const float ri = xi.w * 4.0f;
-            // every staged candidate with this particle's radius (the usual case): one threshold
-            const bool uf = (mixed & 1u) == 0u, ub = (mixed & 2u) == 0u;
-            const float thr = (ri + ri) * 0.5f, t2 = thr * thr, t2lo = t2 * 0.9999f, t2hi = t2 * 1.0001f;
-            for (int t = 0; t < 27; t++) {
-                const i3 q = cell_offset(p, t);
-                const uint32_t c = grid_index(q.x, q.y, q.z, g);
-                uint32_t h = cell_hash(c);
-                for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
-                if (ckey[h] != c) continue;  // not reached: every cell of a run was registered by its first particle
-                const uint32_t e = cent[h];
-                const uint32_t f0 = efo[e], f1 = efo[e + 1], s0 = efs[e];
-                // the cell's staged candidates, four at a time; hits take their ranks in slot
-                // order (the particle itself is not a neighbour)
-                for (uint32_t u = f0; u < f1; u += 4u) {
-                    const uint32_t n = min(f1 - u, 4u);
-                    float4 q[4];
-#pragma unroll
-                    for (uint32_t k = 0; k < 4u; k++) q[k] = cpos[k < n ? u + k : u];
-                    const uint32_t j0 = s0 + (u - f0);
-                    uint32_t in = uf ? hit_mask_uniform<4>(xi, thr, t2lo, t2hi, q, n) : hit_mask<4>(xi, ri, q, n);
-                    const uint32_t self = i - j0;
-                    if (self < 4u) in &= ~(1u << self);
-#pragma unroll
-                    for (uint32_t k = 0; k < 4u; k++) {
-                        if ((in >> k) & 1u) {
-                            const uint32_t rank = a + static_cast<uint32_t>(__builtin_popcount(in & ((1u << k) - 1u)));
-                            if (rank < Kff) nff[size_t(rank) * N + i] = j0 + k;
-                        }
-                    }
-                    a += static_cast<uint32_t>(__builtin_popcount(in));
-                }
-                if (bd.n) {
-                    const uint32_t w0 = ebo[e], w1 = ebo[e + 1], z0 = ebs[e];
-                    for (uint32_t u = w0; u < w1; u += 4u) {
-                        const uint32_t n = min(w1 - u, 4u);
-                        float4 q[4];
-#pragma unroll
-                        for (uint32_t k = 0; k < 4u; k++) q[k] = bcand[k < n ? u + k : u];
-                        const uint32_t in = ub ? hit_mask_uniform<4>(xi, thr, t2lo, t2hi, q, n) : hit_mask<4>(xi, ri, q, n);
-                        const uint32_t sb = z0 + (u - w0);
-#pragma unroll
-                        for (uint32_t k = 0; k < 4u; k++) {
-                            if ((in >> k) & 1u) {
-                                const uint32_t rank = b + static_cast<uint32_t>(__builtin_popcount(in & ((1u << k) - 1u)));
-                                if (rank < Kfb) nfb[size_t(rank) * N + i] = bd.id ? bd.id[sb + k] : sb + k;
-                            }
-                        }
-                        b += static_cast<uint32_t>(__builtin_popcount(in));
-                    }
+        const uint32_t mix = staged ? mixed : 3u;
+        const float ri = xi.w * 4.0f;
+        const uint32_t xm = static_cast<uint32_t>(p.x) & (g.hx - 1u);
+        const bool contig = xm != 0u && xm != g.hx - 1u;  // x-1, x, x+1 have consecutive keys
+        const uint32_t nseg = contig ? 1u : 3u, span = contig ? 3u : 1u;
+        auto rows = [&](const float4* cand, const uint16_t* eo, float thr, uint32_t self, uint32_t* out, uint32_t K,
+                        uint32_t& cnt) {
+            const float t2 = thr * thr, t2lo = t2 * 0.9999f, t2hi = t2 * 1.0001f;
+            for (int r = 0; r < 9; r++) {
+                const int yy = p.y + r % 3 - 1, zz = p.z + r / 3 - 1;
+                for (uint32_t s = 0; s < nseg; s++) {
+                    const uint32_t e = lookup(grid_index(p.x - 1 + static_cast<int>(s), yy, zz, g));
+                    if (e == kNoCell) continue;
+                    walk_run(cand, eo[e], eo[e + span], xi, thr, t2lo, t2hi, self, out, K, N, i, cnt);
                 }
             }
+        };
+        if (!(mix & 1u)) {
+            const uint32_t eh = lookup(grid_index(p.x, p.y, p.z, g));
+            const uint32_t self = eh == kNoCell ? kNoCell : efo[eh] + (i - efs[eh]);
+            rows(cpos, efo, (ri + ri) * 0.5f, self, nff, Kff, a);
         } else {
             a = search_cells<true, true>(xi, i, g, fstart, fend, fl.id, fl.spos, nff, Kff, N);
-            if (bd.n) b = search_cells<false, true>(xi, i, g, bstart, bend, bd.id, bd.spos, nfb, Kfb, N);
+        }
+        if (bd.n) {
+            if (staged && tb == 0u) {
+                b = 0u;  // no wall particle in any cell of the block
+            } else if (!(mix & 2u)) {
+                rows(bcand, ebo, (ri + rw * 4.0f) * 0.5f, kNoCell, nfb, Kfb, b);
+            } else {
+                b = search_cells<false, true>(xi, i, g, bstart, bend, bd.id, bd.spos, nfb, Kfb, N);
+            }
         }
     }
     if (i < N) {
