@@ -465,25 +465,45 @@ __global__ void __launch_bounds__(kCountFoldThreads) k_count_fold(const uint2* _
 // Scene s's cells are entries [s*C, (s+1)*C) of both tables.
 //
 // A workgroup owns 256 consecutive fluid slots (sorted by cell, so each cell's particles are
-// one run). The first particle of each run registers the 27 cells around it in an LDS hash
-// of the block's distinct neighbour cells. The distinct cells are sorted by their linear key,
-// each one's bounds are read once, and all fluid (and wall) particles of those cells are
-// staged in LDS in that order, each with its slot (a wall particle: its id) in place of the
-// radius. The cells x-1, x, x+1 of one (y, z) row have consecutive keys, so their staged
-// copies form one run: every particle walks its 27 cells in the reference order as 9 row
-// runs. A block with more distinct cells or candidates than the LDS holds, or whose staged
-// fluid (or wall) particles do not all share one radius, searches the global tables for that
-// set; the lists are the same either way.
-constexpr uint32_t kSrchHash = 1024;   // LDS hash slots for cell indices
-constexpr uint32_t kSrchCells = 384;   // distinct neighbour cells per block
-constexpr uint32_t kSrchSort = 512;    // width of the key sort (a power of two >= kSrchCells)
-constexpr uint32_t kSrchCand = 1792;   // staged fluid candidates
-constexpr uint32_t kSrchBCand = 512;   // staged wall candidates
-constexpr uint32_t kSrchPad = 4;       // a group of four read at the end of a run stays in the array
-constexpr uint32_t kNoCell = 0xFFFFFFFFu;
-static_assert(kSrchSort >= kSrchCells && (kSrchSort & (kSrchSort - 1u)) == 0u, "sort width");
-static_assert(kSrchSort * sizeof(uint32_t) <= kBlock * sizeof(unsigned long long), "the sort reuses the scan buffer");
-__device__ __forceinline__ uint32_t cell_hash(uint32_t c) { return (c * 2654435761u) >> 22; }
+// one run). The first particle of each run registers the 9 (y, z) rows of its 27 cells in an
+// LDS hash of the block's rows (a row keyed by its wrapped y, z), widening the row's x range to
+// cover x-1..x+1. Each row's cells are then numbered in x order over that range, each cell's
+// bounds are read once, and all fluid (then wall) particles of those cells are staged in LDS
+// in that order, each with its slot (a wall particle: its id) in place of the radius. So the
+// three cells x-1, x, x+1 of any row of a particle are one contiguous run of staged copies, and
+// every particle walks its 27 cells in the reference order as 9 row runs. A block with more
+// rows, cells or candidates than the LDS holds, or whose staged fluid (or wall) particles do
+// not all share one radius, searches the global tables for that set; the lists are the same
+// either way.
+constexpr uint32_t kSrchRowHash = 128;  // LDS hash slots for rows
+constexpr uint32_t kSrchRows = 96;      // rows per block
+constexpr uint32_t kSrchCells = 384;    // cells per block (the rows' x ranges)
+constexpr uint32_t kSrchCand = 1920;    // staged candidates (fluid, then walls)
+constexpr uint32_t kSrchPad = 4;        // a group of four read at the end of a run stays in the array
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t row_hash(uint32_t c) { return (c * 2654435761u) >> 25; }
+static_assert(kSrchRowHash == (1u << 7), "row_hash gives 7 bits");
+static_assert(kSrchRows <= kBlock && kSrchCells <= 2u * kBlock, "one row per thread, two cells per thread");
+
+// Inclusive scan of a 64-bit value over the kBlock threads; `ws` holds kBlock / 64 wave totals
+// (synchronise before reusing it). `total` receives the block's sum.
+__device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long v, unsigned long long* ws,
+                                                             unsigned long long& total) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    for (uint32_t off = 1; off < 64u; off <<= 1) {
+        const uint32_t lo = __shfl_up(static_cast<uint32_t>(v), off, 64);
+        const uint32_t hi = __shfl_up(static_cast<uint32_t>(v >> 32), off, 64);
+        if (lane >= off) v += (static_cast<unsigned long long>(hi) << 32) | lo;
+    }
+    if (lane == 63u) ws[w] = v;
+    __syncthreads();
+    total = 0ull;
+    for (uint32_t q = 0; q < kBlock / 64u; q++) {
+        if (q < w) v += ws[q];
+        total += ws[q];
+    }
+    return v;
+}
 
 // The hits among staged candidates [lo, hi) in order: four candidates per step (hit_mask_uniform,
 // the threshold of equal radii), a hit mask per 32 candidates, then the hits' ids (held in w) as
@@ -528,17 +548,22 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     const uint32_t* fend = fl.end + base;
     const uint32_t* bstart = bd.start + base;
     const uint32_t* bend = bd.end + base;
-    __shared__ uint32_t ckey[kSrchHash];
-    __shared__ uint16_t cent[kSrchHash];
+    __shared__ uint32_t rkey[kSrchRowHash];      // row hash: wrapped key of the row's cell x = 0
+    __shared__ int rxlo[kSrchRowHash], rxhi[kSrchRowHash];  // the row's x range (unwrapped)
+    __shared__ uint16_t rbase[kSrchRowHash];     // the row's first cell
+    __shared__ uint8_t rslot[kSrchRows];         // hash slot of each row, in registration order
     __shared__ uint32_t ecell[kSrchCells], efs[kSrchCells], ebs[kSrchCells];
     __shared__ uint16_t efo[kSrchCells + 1], ebo[kSrchCells + 1];
-    __shared__ float4 cpos[kSrchCand + kSrchPad];
-    __shared__ float4 bcand[kSrchBCand + kSrchPad];
-    __shared__ unsigned long long scan[kBlock];
-    __shared__ uint32_t ncell, tf, tb, mixed;
-    for (uint32_t t = tid; t < kSrchHash; t += kBlock) ckey[t] = kNoCell;
+    __shared__ float4 cand[kSrchCand + kSrchPad];
+    __shared__ unsigned long long ws[kBlock / 64];
+    __shared__ uint32_t nrow, mixed;
+    if (tid < kSrchRowHash) {
+        rkey[tid] = kNoRow;
+        rxlo[tid] = 0x7FFFFFFF;
+        rxhi[tid] = -0x7FFFFFFF - 1;
+    }
     if (tid == 0) {
-        ncell = 0u;
+        nrow = 0u;
         mixed = 0u;
     }
     const float4 xi = live ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -550,67 +575,51 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         leader = pp.x != p.x || pp.y != p.y || pp.z != p.z;
     }
     __syncthreads();
-    // 1. distinct neighbour cells (inserts stop once the block is known to be over the cap,
-    //    so the table never fills; every probe loop is bounded anyway)
+    // 1. rows (inserts stop once the block is known to be over the cap, so the table never
+    //    fills; every probe loop is bounded anyway)
     if (leader) {
-        for (int t = 0; t < 27; t++) {
-            if (*(volatile uint32_t*)&ncell > kSrchCells) break;
-            const uint32_t c = grid_index(p.x + (t % 3) - 1, p.y + (t / 3) % 3 - 1, p.z + t / 9 - 1, g);
-            uint32_t h = cell_hash(c);
-            for (uint32_t probe = 0; probe < kSrchHash; probe++) {
-                const uint32_t prev = atomicCAS(&ckey[h], kNoCell, c);
-                if (prev == kNoCell) {
-                    const uint32_t e = atomicAdd(&ncell, 1u);
-                    if (e < kSrchCells) ecell[e] = c;
+        for (int r = 0; r < 9; r++) {
+            if (*(volatile uint32_t*)&nrow > kSrchRows) break;
+            const uint32_t k = grid_index(0, p.y + r % 3 - 1, p.z + r / 3 - 1, g);
+            uint32_t h = row_hash(k);
+            for (uint32_t probe = 0; probe < kSrchRowHash; probe++) {
+                const uint32_t prev = atomicCAS(&rkey[h], kNoRow, k);
+                if (prev == kNoRow) {
+                    const uint32_t e = atomicAdd(&nrow, 1u);
+                    if (e < kSrchRows) rslot[e] = static_cast<uint8_t>(h);
+                }
+                if (prev == kNoRow || prev == k) {
+                    atomicMin(&rxlo[h], p.x - 1);
+                    atomicMax(&rxhi[h], p.x + 1);
                     break;
                 }
-                if (prev == c) break;
-                h = (h + 1u) & (kSrchHash - 1u);
+                h = (h + 1u) & (kSrchRowHash - 1u);
             }
         }
     }
     __syncthreads();
-    const uint32_t nc = ncell;
-    bool staged = nc <= kSrchCells;
-    // the position of key c in the sorted cell list, or kNoCell (not reached for a registered cell)
-    auto lookup = [&](uint32_t c) -> uint32_t {
-        uint32_t h = cell_hash(c);
-        for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
-        return ckey[h] == c ? static_cast<uint32_t>(cent[h]) : kNoCell;
-    };
-    // 2. the distinct cells in ascending key order (bitonic sort in the scan buffer); the hash
-    //    then maps each key to its sorted position
+    const uint32_t nr = nrow;
+    bool staged = nr <= kSrchRows;
+    uint32_t nc = 0u;
+    // 2. number each row's cells over its x range (rows in registration order)
     if (staged) {
-        uint32_t* sk = reinterpret_cast<uint32_t*>(scan);
-        uint32_t L = 1u;
-        while (L < nc) L <<= 1;
-        for (uint32_t t = tid; t < L; t += kBlock) sk[t] = t < nc ? ecell[t] : kNoCell;
-        __syncthreads();
-        for (uint32_t k = 2u; k <= L; k <<= 1) {
-            for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
-                for (uint32_t t = tid; t < L; t += kBlock) {
-                    const uint32_t o = t ^ j;
-                    if (o > t) {
-                        const uint32_t x = sk[t], y = sk[o];
-                        if ((x > y) == ((t & k) == 0u)) {
-                            sk[t] = y;
-                            sk[o] = x;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (uint32_t e = tid; e < nc; e += kBlock) {
-            const uint32_t c = sk[e];
-            ecell[e] = c;
-            uint32_t h = cell_hash(c);
-            for (uint32_t probe = 0; probe < kSrchHash && ckey[h] != c; probe++) h = (h + 1u) & (kSrchHash - 1u);
-            cent[h] = static_cast<uint16_t>(e);
+        const uint32_t h = tid < nr ? rslot[tid] : 0u;
+        const uint32_t len = tid < nr ? static_cast<uint32_t>(min(static_cast<long long>(rxhi[h]) - rxlo[h] + 1, 
+                                                                 static_cast<long long>(kSrchCells) + 1)) : 0u;
+        unsigned long long tot;
+        const uint32_t b0 = static_cast<uint32_t>(block_scan_u64(len, ws, tot)) - len;
+        nc = static_cast<uint32_t>(tot);
+        staged = nc <= kSrchCells;
+        if (staged && tid < nr) {
+            rbase[h] = static_cast<uint16_t>(b0);
+            const uint32_t k0 = rkey[h];
+            for (uint32_t q = 0; q < len; q++) ecell[b0 + q] = k0 + (static_cast<uint32_t>(rxlo[h] + static_cast<int>(q)) & (g.hx - 1u));
         }
         __syncthreads();
     }
-    // 3. bounds of each distinct cell (read once) and offsets of its staged copies
+    // 3. bounds of each cell (read once) and offsets of its staged copies: fluid from 0, walls
+    //    after all fluid
+    uint32_t tf = 0u, tb = 0u;
     if (staged) {
         uint32_t fc[2] = {0u, 0u}, bc[2] = {0u, 0u};
 #pragma unroll
@@ -629,42 +638,35 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
             }
         }
         const unsigned long long mine = (static_cast<unsigned long long>(bc[0] + bc[1]) << 32) | (fc[0] + fc[1]);
-        scan[tid] = mine;
-        __syncthreads();
-        for (uint32_t off = 1; off < kBlock; off <<= 1) {  // inclusive Hillis-Steele scan of (wall, fluid) counts
-            const unsigned long long v = tid >= off ? scan[tid - off] : 0ull;
-            __syncthreads();
-            scan[tid] += v;
-            __syncthreads();
-        }
-        const unsigned long long ex = scan[tid] - mine;
+        unsigned long long tot;
+        const unsigned long long ex = block_scan_u64(mine, ws, tot) - mine;
+        tf = static_cast<uint32_t>(tot & 0xFFFFFFFFull);
+        tb = static_cast<uint32_t>(tot >> 32);
+        staged = static_cast<unsigned long long>(tf) + tb <= kSrchCand;
         uint32_t fo = static_cast<uint32_t>(ex & 0xFFFFFFFFull), bo = static_cast<uint32_t>(ex >> 32);
+        if (staged) {
 #pragma unroll
-        for (uint32_t q = 0; q < 2; q++) {
-            const uint32_t e = 2u * tid + q;
-            if (e < nc) {
-                efo[e] = static_cast<uint16_t>(fo);
-                ebo[e] = static_cast<uint16_t>(bo);
+            for (uint32_t q = 0; q < 2; q++) {
+                const uint32_t e = 2u * tid + q;
+                if (e < nc) {
+                    efo[e] = static_cast<uint16_t>(fo);
+                    ebo[e] = static_cast<uint16_t>(tf + bo);
+                }
+                fo += fc[q];
+                bo += bc[q];
             }
-            fo += fc[q];
-            bo += bc[q];
-        }
-        if (tid == kBlock - 1) {
-            const unsigned long long tot = scan[kBlock - 1];
-            tf = static_cast<uint32_t>(tot & 0xFFFFFFFFull);
-            tb = static_cast<uint32_t>(tot >> 32);
-            efo[nc] = static_cast<uint16_t>(tf);
-            ebo[nc] = static_cast<uint16_t>(tb);
+            if (tid == 0) {
+                efo[nc] = static_cast<uint16_t>(tf);
+                ebo[nc] = static_cast<uint16_t>(tf + tb);
+            }
         }
         __syncthreads();
-        staged = tf <= kSrchCand && tb <= kSrchBCand;
     }
     // 4. stage the candidates with their ids: copy t belongs to the last cell e with
     //    eo[e] <= t. `mixed` records whether some fluid (bit 0) or wall (bit 1) candidate has
     //    another radius than the set's first staged candidate.
     float rw = 0.0f;  // the walls' common radius
     if (staged) {
-        const uint32_t Tf = tf, Tb = tb;
         auto owner = [&](const uint16_t* eo, uint32_t t) {
             uint32_t lo = 0u, hi = nc;  // eo[lo] <= t < eo[hi]
             while (hi - lo > 1u) {
@@ -675,53 +677,53 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
             return lo;
         };
         uint32_t mx = 0u;
-        if (Tf) {
+        if (tf) {
             const float rf = pos[efs[owner(efo, 0u)]].w;
-            for (uint32_t t = tid; t < Tf; t += kBlock) {
+            for (uint32_t t = tid; t < tf; t += kBlock) {
                 const uint32_t e = owner(efo, t);
                 const uint32_t s = efs[e] + (t - efo[e]);
                 const float4 v = pos[s];
-                cpos[t] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
+                cand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
                 mx |= v.w != rf ? 1u : 0u;
             }
         }
-        if (Tb) {
-            rw = bd.spos[ebs[owner(ebo, 0u)]].w;
-            for (uint32_t t = tid; t < Tb; t += kBlock) {
+        if (tb) {
+            rw = bd.spos[ebs[owner(ebo, tf)]].w;
+            for (uint32_t t = tf + tid; t < tf + tb; t += kBlock) {
                 const uint32_t e = owner(ebo, t);
                 const uint32_t sb = ebs[e] + (t - ebo[e]);
                 const float4 v = bd.spos[sb];
-                bcand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(bd.id ? bd.id[sb] : sb));
+                cand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(bd.id ? bd.id[sb] : sb));
                 mx |= v.w != rw ? 2u : 0u;
             }
         }
         if (mx) atomicOr(&mixed, mx);
         __syncthreads();
     }
-    // 5. the search proper: 9 rows of three cells (z, then y offsets), each row's cells in x order
+    // 5. the search proper: 9 rows (z, then y offsets), each the run of cells x-1, x, x+1
     uint32_t a = 0u, b = 0u;
     if (live) {
         const uint32_t mix = staged ? mixed : 3u;
         const float ri = xi.w * 4.0f;
-        const uint32_t xm = static_cast<uint32_t>(p.x) & (g.hx - 1u);
-        const bool contig = xm != 0u && xm != g.hx - 1u;  // x-1, x, x+1 have consecutive keys
-        const uint32_t nseg = contig ? 1u : 3u, span = contig ? 3u : 1u;
-        auto rows = [&](const float4* cand, const uint16_t* eo, float thr, uint32_t self, uint32_t* out, uint32_t K,
-                        uint32_t& cnt) {
+        // the particle's cell x + dx in row (y, z)
+        auto cell_of = [&](int dx, int y, int z) -> uint32_t {
+            const uint32_t k = grid_index(0, y, z, g);
+            uint32_t h = row_hash(k);
+            for (uint32_t probe = 0; probe < kSrchRowHash && rkey[h] != k; probe++) h = (h + 1u) & (kSrchRowHash - 1u);
+            return rkey[h] == k ? rbase[h] + static_cast<uint32_t>(p.x + dx - rxlo[h]) : kNoRow;
+        };
+        auto rows = [&](const uint16_t* eo, float thr, uint32_t self, uint32_t* out, uint32_t K, uint32_t& cnt) {
             const float t2 = thr * thr, t2lo = t2 * 0.9999f, t2hi = t2 * 1.0001f;
             for (int r = 0; r < 9; r++) {
-                const int yy = p.y + r % 3 - 1, zz = p.z + r / 3 - 1;
-                for (uint32_t s = 0; s < nseg; s++) {
-                    const uint32_t e = lookup(grid_index(p.x - 1 + static_cast<int>(s), yy, zz, g));
-                    if (e == kNoCell) continue;
-                    walk_run(cand, eo[e], eo[e + span], xi, thr, t2lo, t2hi, self, out, K, N, i, cnt);
-                }
+                const uint32_t e = cell_of(-1, p.y + r % 3 - 1, p.z + r / 3 - 1);
+                if (e == kNoRow) continue;  // not reached: every row of a run was registered
+                walk_run(cand, eo[e], eo[e + 3u], xi, thr, t2lo, t2hi, self, out, K, N, i, cnt);
             }
         };
         if (!(mix & 1u)) {
-            const uint32_t eh = lookup(grid_index(p.x, p.y, p.z, g));
-            const uint32_t self = eh == kNoCell ? kNoCell : efo[eh] + (i - efs[eh]);
-            rows(cpos, efo, (ri + ri) * 0.5f, self, nff, Kff, a);
+            const uint32_t eh = cell_of(0, p.y, p.z);
+            const uint32_t self = eh == kNoRow ? kNoRow : efo[eh] + (i - efs[eh]);
+            rows(efo, (ri + ri) * 0.5f, self, nff, Kff, a);
         } else {
             a = search_cells<true, true>(xi, i, g, fstart, fend, fl.id, fl.spos, nff, Kff, N);
         }
@@ -729,7 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
             if (staged && tb == 0u) {
                 b = 0u;  // no wall particle in any cell of the block
             } else if (!(mix & 2u)) {
-                rows(bcand, ebo, (ri + rw * 4.0f) * 0.5f, kNoCell, nfb, Kfb, b);
+                rows(ebo, (ri + rw * 4.0f) * 0.5f, kNoRow, nfb, Kfb, b);
             } else {
                 b = search_cells<false, true>(xi, i, g, bstart, bend, bd.id, bd.spos, nfb, Kfb, N);
             }
