@@ -308,53 +308,28 @@ __device__ __forceinline__ bool within(const float4& xi, float ri, const float4&
     return __builtin_sqrtf(dd) < thr;
 }
 
-// within() for M candidates at once, as a bit mask (bit k: candidate k, the first n valid):
-// the squared distances of all M decide first, and the square root is taken only for those
-// within 1e-4 of the cut-off, so the common case runs without branches and the M LDS reads
-// are in flight together.
-template <int M>
-__device__ __forceinline__ uint32_t hit_mask(const float4& xi, float ri, const float4 (&q)[M], uint32_t n) {
-    uint32_t in = 0u, near = 0u;
-    float dd[M], thr[M];
+// within() for four candidates of one radius, as a bit mask (bit k: candidate k). The cut-off
+// thr = (h_i + h_j) * 0.5 (h = 4 r) is then the same for every pair, and it, t2lo = thr^2 * 0.9999
+// and t2hi = thr^2 * 1.0001 come precomputed with the same float operations. The squared distances decide first, and the square
+// root is taken only within 1e-4 of the cut-off. The two range tests are sign bits rather than
+// comparisons: for finite values dd - t2lo is negative exactly when dd < t2lo, and t2hi - dd
+// exactly when dd > t2hi (two unequal floats differ by a nonzero float; equal ones give +0).
+// A NaN distance sets both bits or neither, and is then no hit.
+__device__ __forceinline__ uint32_t hit_bits4(const float4& xi, float thr, float t2lo, float t2hi, const float4 (&q)[4]) {
+    uint32_t lo = 0u, hi = 0u;
+    float dd[4];
 #pragma unroll
-    for (int k = 0; k < M; k++) {
+    for (int k = 0; k < 4; k++) {
         const float ddx = xi.x - q[k].x, ddy = xi.y - q[k].y, ddz = xi.z - q[k].z;
         dd[k] = (ddx * ddx + ddy * ddy) + ddz * ddz;
-        const float rj = q[k].w * 4.0f;
-        thr[k] = (ri + rj) * 0.5f;
-        const float t2 = thr[k] * thr[k];
-        const bool valid = static_cast<uint32_t>(k) < n;
-        const bool lo = dd[k] < t2 * 0.9999f, hi = dd[k] > t2 * 1.0001f;
-        in |= static_cast<uint32_t>(valid && lo) << k;
-        near |= static_cast<uint32_t>(valid && !lo && !hi) << k;
+        lo |= (__float_as_uint(dd[k] - t2lo) >> 31) << k;
+        hi |= (__float_as_uint(t2hi - dd[k]) >> 31) << k;
     }
+    uint32_t in = lo & ~hi;
+    const uint32_t near = ~(lo | hi) & 15u;
     if (near) {
 #pragma unroll
-        for (int k = 0; k < M; k++)
-            if ((near >> k) & 1u) in |= static_cast<uint32_t>(__builtin_sqrtf(dd[k]) < thr[k]) << k;
-    }
-    return in;
-}
-
-// hit_mask when every candidate has the particle's radius: thr, thr^2 (1 - 1e-4) and
-// thr^2 (1 + 1e-4) are the same for all pairs and come precomputed (the same float operations).
-template <int M>
-__device__ __forceinline__ uint32_t hit_mask_uniform(const float4& xi, float thr, float t2lo, float t2hi, const float4 (&q)[M],
-                                                     uint32_t n) {
-    uint32_t in = 0u, near = 0u;
-    float dd[M];
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-        const float ddx = xi.x - q[k].x, ddy = xi.y - q[k].y, ddz = xi.z - q[k].z;
-        dd[k] = (ddx * ddx + ddy * ddy) + ddz * ddz;
-        const bool valid = static_cast<uint32_t>(k) < n;
-        const bool lo = dd[k] < t2lo, hi = dd[k] > t2hi;
-        in |= static_cast<uint32_t>(valid && lo) << k;
-        near |= static_cast<uint32_t>(valid && !lo && !hi) << k;
-    }
-    if (near) {
-#pragma unroll
-        for (int k = 0; k < M; k++)
+        for (int k = 0; k < 4; k++)
             if ((near >> k) & 1u) in |= static_cast<uint32_t>(__builtin_sqrtf(dd[k]) < thr) << k;
     }
     return in;
@@ -505,7 +480,7 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
     return v;
 }
 
-// The hits among staged candidates [lo, hi) in order: four candidates per step (hit_mask_uniform,
+// The hits among staged candidates [lo, hi) in order: four candidates per step (hit_bits4,
 // the threshold of equal radii), a hit mask per 32 candidates, then the hits' ids (held in w) as
 // list entries cnt, cnt + 1, ... (column-major, stride N; entries past K are counted only).
 // `self` is the particle's own staged index, which is not a neighbour (any value >= hi: none).
@@ -519,8 +494,9 @@ __device__ __forceinline__ void walk_run(const float4* cand, uint32_t lo, uint32
             float4 v[4];
 #pragma unroll
             for (uint32_t k = 0; k < 4u; k++) v[k] = cand[c0 + q + k];  // <= hi + 2 < array size
-            m |= hit_mask_uniform<4>(xi, thr, t2lo, t2hi, v, n - q) << q;
+            m |= hit_bits4(xi, thr, t2lo, t2hi, v) << q;
         }
+        m &= n < 32u ? (1u << n) - 1u : ~0u;  // the group of four read past hi
         const uint32_t d = self - c0;
         if (d < 32u) m &= ~(1u << d);
         while (m) {
@@ -556,7 +532,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     __shared__ uint16_t efo[kSrchCells + 1], ebo[kSrchCells + 1];
     __shared__ float4 cand[kSrchCand + kSrchPad];
     __shared__ unsigned long long ws[kBlock / 64];
-    __shared__ uint32_t nrow, mixed;
+    __shared__ uint32_t nrow, mixed, first[2];  // first: first cell with fluid, with walls
     if (tid < kSrchRowHash) {
         rkey[tid] = kNoRow;
         rxlo[tid] = 0x7FFFFFFF;
@@ -565,6 +541,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
     if (tid == 0) {
         nrow = 0u;
         mixed = 0u;
+        first[0] = first[1] = kNoRow;
     }
     const float4 xi = live ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const i3 p = grid_pos(xi.x, xi.y, xi.z, g);
@@ -645,12 +622,17 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         staged = static_cast<unsigned long long>(tf) + tb <= kSrchCand;
         uint32_t fo = static_cast<uint32_t>(ex & 0xFFFFFFFFull), bo = static_cast<uint32_t>(ex >> 32);
         if (staged) {
+            // each copy's w first holds its cell (read back by the staging pass)
 #pragma unroll
             for (uint32_t q = 0; q < 2; q++) {
                 const uint32_t e = 2u * tid + q;
                 if (e < nc) {
                     efo[e] = static_cast<uint16_t>(fo);
                     ebo[e] = static_cast<uint16_t>(tf + bo);
+                    for (uint32_t k = 0; k < fc[q]; k++) cand[fo + k].w = __uint_as_float(e);
+                    for (uint32_t k = 0; k < bc[q]; k++) cand[tf + bo + k].w = __uint_as_float(e);
+                    if (fc[q]) atomicMin(&first[0], e);
+                    if (bc[q]) atomicMin(&first[1], e);
                 }
                 fo += fc[q];
                 bo += bc[q];
@@ -662,25 +644,16 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         }
         __syncthreads();
     }
-    // 4. stage the candidates with their ids: copy t belongs to the last cell e with
-    //    eo[e] <= t. `mixed` records whether some fluid (bit 0) or wall (bit 1) candidate has
-    //    another radius than the set's first staged candidate.
+    // 4. stage the candidates with their ids (copy t of cell e is particle efs[e] + t - efo[e]).
+    //    `mixed` records whether some fluid (bit 0) or wall (bit 1) candidate has another radius
+    //    than the first staged candidate of its set.
     float rw = 0.0f;  // the walls' common radius
     if (staged) {
-        auto owner = [&](const uint16_t* eo, uint32_t t) {
-            uint32_t lo = 0u, hi = nc;  // eo[lo] <= t < eo[hi]
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (eo[mid] <= t) lo = mid;
-                else hi = mid;
-            }
-            return lo;
-        };
         uint32_t mx = 0u;
         if (tf) {
-            const float rf = pos[efs[owner(efo, 0u)]].w;
+            const float rf = pos[efs[first[0]]].w;
             for (uint32_t t = tid; t < tf; t += kBlock) {
-                const uint32_t e = owner(efo, t);
+                const uint32_t e = __float_as_uint(cand[t].w);
                 const uint32_t s = efs[e] + (t - efo[e]);
                 const float4 v = pos[s];
                 cand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
@@ -688,9 +661,9 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
             }
         }
         if (tb) {
-            rw = bd.spos[ebs[owner(ebo, tf)]].w;
+            rw = bd.spos[ebs[first[1]]].w;
             for (uint32_t t = tf + tid; t < tf + tb; t += kBlock) {
-                const uint32_t e = owner(ebo, t);
+                const uint32_t e = __float_as_uint(cand[t].w);
                 const uint32_t sb = ebs[e] + (t - ebo[e]);
                 const float4 v = bd.spos[sb];
                 cand[t] = make_float4(v.x, v.y, v.z, __uint_as_float(bd.id ? bd.id[sb] : sb));
@@ -709,6 +682,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors_fluid(const float4* __rest
         auto cell_of = [&](int dx, int y, int z) -> uint32_t {
             const uint32_t k = grid_index(0, y, z, g);
             uint32_t h = row_hash(k);
+#pragma unroll 1
             for (uint32_t probe = 0; probe < kSrchRowHash && rkey[h] != k; probe++) h = (h + 1u) & (kSrchRowHash - 1u);
             return rkey[h] == k ? rbase[h] + static_cast<uint32_t>(p.x + dx - rxlo[h]) : kNoRow;
         };
