@@ -121,9 +121,9 @@ __device__ __forceinline__ double fold(const double* __restrict__ partials, uint
 // memory-side cache, while the larger per-pair coefficient streams are read non-temporally.
 constexpr uint32_t kListBatch = 4;       // once-per-step walks (density/alpha, ST factor, CG setup)
 constexpr uint32_t kListBatchSolve = 2;  // solver iterations (DF/CD evaluate and apply, A.p): measured best
-// A.p loads its first entries before staging the halo (a multiple of kListBatchSolve; 16 keeps
-// the kernel at five workgroups per CU: VGPRs stay under what the LDS already allows)
-constexpr uint32_t kListPrefetchApply = 16;
+// A.p loads its first entries before staging the halo (a multiple of kListBatchSolve). Measured
+// on the 1M dam break, one run: 4 / 8 / 12 / 16 entries give 89.3 / 86.5 / 89.3 / 94.1 us per A.p.
+constexpr uint32_t kListPrefetchApply = 8;
 static_assert(kListPrefetchApply % kListBatchSolve == 0, "the batches after the prefetch stay aligned");
 
 // Calls body(t, s) for every halo record t < H (s = its slot), with R slots per thread loaded
@@ -148,6 +148,18 @@ __device__ __forceinline__ void for_halo(const uint32_t* __restrict__ hs, uint32
 // Halo count of workgroup blk: > kHaloCap (not staged) when there are no halos (hl.n null: the
 // adjoint's kernels run on its taped lists).
 __device__ __forceinline__ uint32_t halo_count(const Halo& hl, uint32_t blk) { return hl.n ? hl.n[blk] : kHaloCap + 1u; }
+
+// LDS staging depth (records per workgroup, <= kHaloCap) of the kernels whose records are wide
+// enough for the depth to decide how many workgroups share a CU's 160 KiB. A block with a deeper
+// halo gathers from global memory instead, with the same results; at these depths that is a few
+// blocks in a thousand, and the added workgroup per CU pays for them:
+// DF/CD evaluation, 24-byte records: 1104 -> 26.5 KB, six per CU (1264: five);
+// CG setup, 36-byte records: 1104 -> 39.7 KB, four per CU (1264: three).
+// A.p keeps the full depth: its blocks over 1104 cost more than a sixth workgroup gains.
+constexpr uint32_t kStageEval = 1104;
+constexpr uint32_t kStageSetup = 1104;
+constexpr uint32_t kStageAp = kHaloCap;
+static_assert(kStageEval <= kHaloCap && kStageSetup <= kHaloCap && kStageAp <= kHaloCap, "staging depths");
 
 // ----------------------------------------------------------------- boundary volume
 // ComputeBoundaryParticleVolume_kernel (MPSSPHBaseUtil.cuh:10-41): S_b = W(0) + sum W_bk.
@@ -444,10 +456,10 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
     const uint32_t N = P.N;
     const bool live = i < P.end;
     const uint32_t c = live ? nb.cff[i] : 0u;
-    __shared__ float4 la[kHaloCap];
-    __shared__ float2 lb[kHaloCap];
+    __shared__ float4 la[kStageEval];
+    __shared__ float2 lb[kStageEval];
     const uint32_t H = hl.n[blk];
-    const bool staged = H <= kHaloCap;
+    const bool staged = H <= kStageEval;
     constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
     float gn[U];
@@ -848,10 +860,10 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
     const bool live = i < P.end;
     const uint32_t c = live ? nb.cff[i] : 0u;
     // the block's halo (x_j, v_j, m_j, rho_j, f_j) in LDS (see k_cg_apply)
-    __shared__ float4 la[kHaloCap], lb[kHaloCap];
-    __shared__ float lc[kHaloCap];
+    __shared__ float4 la[kStageSetup], lb[kStageSetup];
+    __shared__ float lc[kStageSetup];
     const uint32_t H = halo_count(hl, blk);
-    const bool staged = H <= kHaloCap;
+    const bool staged = H <= kStageSetup;
     constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
     auto fetch = [&](uint32_t k0) {
@@ -984,10 +996,10 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
         return {a.x + bta * q.x, a.y + bta * q.y, a.z + bta * q.z};
     };
     using LB = typename std::conditional<RECOMP, float4, float2>::type;
-    __shared__ float4 la[kHaloCap];
-    __shared__ LB lb[kHaloCap];
+    __shared__ float4 la[kStageAp];
+    __shared__ LB lb[kStageAp];
     const uint32_t H = hl.n[blk];
-    const bool staged = H <= kHaloCap;  // same for the whole block
+    const bool staged = H <= kStageAp;  // same for the whole block
     using Coef = typename std::conditional<RECOMP, float, float2>::type;
     auto coef = [&](size_t slot) -> Coef {
         if constexpr (RECOMP) {
