@@ -124,7 +124,15 @@ constexpr uint32_t kListBatchSolve = 2;  // solver iterations (DF/CD evaluate an
 // A.p loads its first entries before staging the halo (a multiple of kListBatchSolve). Measured
 // on the 1M dam break, one run: 4 / 8 / 12 / 16 entries give 89.3 / 86.5 / 89.3 / 94.1 us per A.p.
 constexpr uint32_t kListPrefetchApply = 8;
-static_assert(kListPrefetchApply % kListBatchSolve == 0, "the batches after the prefetch stay aligned");
+constexpr uint32_t kListBatchAp = kListBatchSolve;  // A.p's entries in flight after the prefetch
+// The entries the DF/CD evaluation and the pressure apply load before their halo staging (a
+// multiple of kListBatchSolve). Measured on the 1M dam break, one run: the evaluation takes
+// 91 / 75 / 88 us with 2 / 4 / 8.
+constexpr uint32_t kListPrefetchEval = 4;
+constexpr uint32_t kListPrefetchPressure = 4;
+static_assert(kListPrefetchEval % kListBatchSolve == 0 && kListPrefetchPressure % kListBatchSolve == 0,
+              "the batches after the prefetch stay aligned");
+static_assert(kListPrefetchApply % kListBatchAp == 0, "the batches after the prefetch stay aligned");
 
 // Calls body(t, s) for every halo record t < H (s = its slot), with R slots per thread loaded
 // ahead so the bodies' loads can be in flight together.
@@ -473,7 +481,18 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
             }
         }
     };
-    fetch(0);
+    // the first kPre entries are loaded before the halo staging, as in A.p
+    constexpr uint32_t kPre = kListPrefetchEval;
+    uint32_t jp[kPre];
+    float gp[kPre];
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; q++) {
+        if (q < c) {
+            const size_t slot = size_t(q) * N + i;
+            jp[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
+            gp[q] = __builtin_nontemporal_load(pG_ff + slot);
+        }
+    }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
         for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
@@ -487,7 +506,20 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
     if (live) {
         const f3 xi = mk(pos[i]), vi = mk(vel[i]);
         float df = 0.0f;
-        for (uint32_t k = 0; k < c; k += U) {
+        if (kPre < c) fetch(kPre);
+#pragma unroll
+        for (uint32_t q = 0; q < kPre; q++) {
+            if (q < c) {
+                if (staged) {
+                    const float4 a = la[jp[q]];
+                    const float2 b = lb[jp[q]];
+                    df += P.V * gp[q] * dot3(xi - mk(a), vi - f3{a.w, b.x, b.y});
+                } else {
+                    df += P.V * gp[q] * dot3(xi - mk(pos[jp[q]]), vi - mk(vel[jp[q]]));
+                }
+            }
+        }
+        for (uint32_t k = kPre; k < c; k += U) {
             uint32_t j[U];
             float G[U];
 #pragma unroll
@@ -667,7 +699,18 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
             }
         }
     };
-    fetch(0);
+    // the first kPre entries are loaded before the halo staging, as in A.p
+    constexpr uint32_t kPre = kListPrefetchPressure;
+    uint32_t jp[kPre];
+    float gp[kPre];
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; q++) {
+        if (q < c) {
+            const size_t slot = size_t(q) * N + i;
+            jp[q] = staged ? uint32_t(hl.nb16[slot]) : __builtin_nontemporal_load(nb.ff + slot);
+            gp[q] = __builtin_nontemporal_load(pG_ff + slot);
+        }
+    }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
         for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
@@ -680,7 +723,19 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     const f3 xi = mk(pos[i]);
     const float pi = kappa[i];
     f3 ff{0.f, 0.f, 0.f};
-    for (uint32_t k = 0; k < c; k += U) {
+    if (kPre < c) fetch(kPre);
+#pragma unroll
+    for (uint32_t q = 0; q < kPre; q++) {
+        if (q < c) {
+            if (staged) {
+                const float4 a = la[jp[q]];
+                ff = ff - 0.5f * P.V * (pi + a.w) * gp[q] * (xi - mk(a));
+            } else {
+                ff = ff - 0.5f * P.V * (pi + kappa[jp[q]]) * gp[q] * (xi - mk(pos[jp[q]]));
+            }
+        }
+    }
+    for (uint32_t k = kPre; k < c; k += U) {
         uint32_t j[U];
         float G[U];
 #pragma unroll
@@ -1009,7 +1064,7 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
             return __builtin_bit_cast(float2, d);
         }
     };
-    constexpr uint32_t U = kListBatchSolve;
+    constexpr uint32_t U = kListBatchAp;
     uint32_t jn[U];
     Coef abn[U];
     auto fetch = [&](uint32_t k0) {
