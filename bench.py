@@ -20,7 +20,8 @@ per-kernel times. The CPU baseline steps from the same kept state.
 
 Prints ONE JSON line. Extra objects: `roofline` (dominant kernel: algorithmic bytes from the
 measured neighbour counts and iteration counts / its HIP-event time; `traffic` = its HBM bytes
-per launch from the committed rocprofv3 counter summary), `cpu_baseline` (the FP32 CPU
+per launch from the committed rocprofv3 counter summary, null unless that summary covers the
+same steps), `cpu_baseline` (the FP32 CPU
 oracle on a bounded sample of the same workload), per-kernel table.
 """
 from __future__ import annotations
@@ -28,6 +29,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -74,12 +76,26 @@ def say(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel: str) -> float | None:
+def traffic_window() -> list | None:
+    """The steps the counter passes of the traffic summary covered (its title: "steps A..B")."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"steps (\d+)\.\.(\d+)", f.readline())
+    except OSError:
+        return None
+    return [int(m.group(1)), int(m.group(2))] if m else None
+
+
+def measured_traffic(kernel: str, window: list) -> float | None:
     """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
     rocprofv3 counter passes over the same timed window, each scaled by what the counter reads
     for a plain copy of known size (the summary's last column: MB per working dispatch, mean).
-    The profiler may spell out more template arguments than the library's timing table
-    (`k_cg_apply<1, false, true>` for `k_cg_apply<1>`)."""
+    None when the summary covers other steps than `window` (the lists and halos grow as the
+    fluid settles, so bytes per launch depend on the window). The profiler may spell out more
+    template arguments than the library's timing table (`k_cg_apply<1, false, true>` for
+    `k_cg_apply<1>`)."""
+    if traffic_window() != list(window):
+        return None
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -458,8 +474,9 @@ def main() -> int:
         achieved = kbytes[dom] / (p["total_ms"] * 1e-3) / 1e9
         roofline = {
             "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(dom, [args.settle, args.settle + args.steps - 1]),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT), "traffic_window": traffic_window(),
             "algorithmic_bytes_per_launch": kbytes[dom] / max(p["useful"], 1),
             "avg_launch_us": 1000.0 * p["total_ms"] / max(p["launches"], 1),
             "launches": p["launches"], "useful_launches": p["useful"],

@@ -167,7 +167,11 @@ __device__ __forceinline__ uint32_t halo_count(const Halo& hl, uint32_t blk) { r
 constexpr uint32_t kStageEval = 1104;
 constexpr uint32_t kStageSetup = 1104;
 constexpr uint32_t kStageAp = kHaloCap;
-static_assert(kStageEval <= kHaloCap && kStageSetup <= kHaloCap && kStageAp <= kHaloCap, "staging depths");
+// the others stage every halo the build keeps
+constexpr uint32_t kStageDensity = kHaloCap, kStageST = kHaloCap, kStagePApply = kHaloCap;
+static_assert(kStageEval <= kHaloCap && kStageSetup <= kHaloCap && kStageAp <= kHaloCap && kStageDensity <= kHaloCap &&
+                  kStageST <= kHaloCap && kStagePApply <= kHaloCap,
+              "staging depths");
 
 // ----------------------------------------------------------------- boundary volume
 // ComputeBoundaryParticleVolume_kernel (MPSSPHBaseUtil.cuh:10-41): S_b = W(0) + sum W_bk.
@@ -266,9 +270,9 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
     const bool walk = dens || alph || pairs;
     const uint32_t c = live && walk ? nb.cff[i] : 0u;
     // the block's halo positions in LDS (see k_cg_apply)
-    __shared__ float4 la[kHaloCap];
+    __shared__ float4 la[kStageDensity];
     const uint32_t H = halo_count(hl, blk);
-    const bool staged = walk && H <= kHaloCap;
+    const bool staged = walk && H <= kStageDensity;
     constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
     auto fetch = [&](uint32_t k0) {
@@ -683,9 +687,9 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     const uint32_t N = P.N;
     const bool live = i < P.end;
     const uint32_t c = live ? nb.cff[i] : 0u;
-    __shared__ float4 la[kHaloCap];
+    __shared__ float4 la[kStagePApply];
     const uint32_t H = hl.n[blk];
-    const bool staged = H <= kHaloCap;
+    const bool staged = H <= kStagePApply;
     constexpr uint32_t U = kListBatchSolve;
     uint32_t jn[U];
     float gn[U];
@@ -826,9 +830,9 @@ __global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, Halo hl
     const bool live = i < P.end;
     const uint32_t c = live ? nb.cff[i] : 0u;
     // the block's halo (x_j, v_j, m_j) in LDS (see k_cg_apply)
-    __shared__ float4 la[kHaloCap], lb[kHaloCap];
+    __shared__ float4 la[kStageST], lb[kStageST];
     const uint32_t H = halo_count(hl, blk);
-    const bool staged = H <= kHaloCap;
+    const bool staged = H <= kStageST;
     constexpr uint32_t U = kListBatch;
     uint32_t jn[U];
     float gn[U];
