@@ -206,6 +206,12 @@ typedef struct mps_config {
                                     separate one-workgroup fold launches (four; the default). Same
                                     results; mps_autotune may change it. The allocation-free step
                                     always uses the fold launches. */
+    uint32_t cg_halo_depth;      /* HIP library: LDS depth of the streamed viscosity operator's block
+                                    halos. 0 = chosen each step from the previous list build (the
+                                    deep form once more than 1 in 1000 workgroups have halos over
+                                    the shallow depth); 1 = shallow (1264 records, five workgroups
+                                    per CU); 2 = deep (1536, four per CU). A deeper halo is gathered
+                                    from global memory. Same results. */
 } mps_config;
 
 /* Per-step statistics (the reference prints these to the debugger only:

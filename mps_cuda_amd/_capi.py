@@ -54,7 +54,7 @@ class Config(C.Structure):
         ("host_sync_every", C.c_uint32), ("adj_cg_max_iter", C.c_uint32), ("use_graphs", C.c_uint32),
         ("halo_limit", C.c_uint32), ("couple_scenes", C.c_uint32), ("cg_stream_coeffs", C.c_uint32),
         ("persistent_solver", C.c_uint32), ("persistent_spin", C.c_uint32),
-        ("adj_cg_tol", C.c_double), ("cg_fold_in_kernel", C.c_uint32),
+        ("adj_cg_tol", C.c_double), ("cg_fold_in_kernel", C.c_uint32), ("cg_halo_depth", C.c_uint32),
     ]
 
 

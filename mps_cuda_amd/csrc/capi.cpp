@@ -298,6 +298,7 @@ struct mps_ctx {
     mps::GridSync* d_grid = nullptr;  // barrier words of the persistent solver (allocated at creation)
     uint32_t persist_cap[6] = {0, 0, 0, 0, 0, 0};  // co-resident workgroups per kernel variant (0 = not asked yet)
     uint32_t persist_fallbacks = 0;       // persistent solves that gave up and reran chunked
+    uint32_t deep_halos = 0;              // workgroups of the last list build with halos over kHaloShallow
     uint32_t* d_flags = nullptr;  // [0] nonfinite, [1] sampling mismatch, [4..6] halo statistics of the step
     uint32_t* h_flags = nullptr;
     double* d_err3 = nullptr;
@@ -1049,6 +1050,10 @@ struct mps_ctx {
         mps::launch::CgOperator op;
         op.recompute = cfg.cg_stream_coeffs ? 0 : 1;
         op.cubic = cubic();
+        {  // halo depth of the streamed operator (the last list build's halos decide when asked to)
+            const uint64_t nb = (uint64_t(N) + mps::dev::kBlock - 1) / mps::dev::kBlock;
+            op.deep = !op.recompute && (cfg.cg_halo_depth == 2 || (cfg.cg_halo_depth == 0 && uint64_t(deep_halos) * 1000u > nb));
+        }
         if (op.recompute) {
             pairA.reserve(size_t(Kff) * m);
             irho.reserve(m);
@@ -1124,6 +1129,7 @@ struct mps_ctx {
             uint32_t* arr = fold_in ? arrive_counters() : nullptr;
             key.push_back(fold_in);
             key.push_back(u(arr));
+            key.push_back(uint64_t(op.deep));
             launch_chunk(std::move(key), [&] {
                 for (uint32_t c = 0; c < chunk; c++) {
                     const uint32_t k = it + c;
@@ -1706,6 +1712,7 @@ struct mps_ctx {
         st->halo_max = h_flags[4];
         st->halo_spill = h_flags[5];
         st->halo_blocks = h_flags[6];
+        deep_halos = h_flags[7];
         st->nnz_ff = nnz_ff;
         st->nnz_fb = nnz_fb;
         st->nnz_bb = nnz_bb;
@@ -2593,6 +2600,7 @@ int mps_config_init(mps_config* cfg) {
     cfg->adj_cg_max_iter = 300;
     cfg->adj_cg_tol = -1.0e-12;
     cfg->cg_fold_in_kernel = 0;
+    cfg->cg_halo_depth = 0;
     cfg->cg_stream_coeffs = 1;  // measured faster on the 1M dam break (recomputing b_ij is VALU-bound)
     return MPS_OK;
 }
@@ -2602,6 +2610,7 @@ int mps_create(const mps_config* cfg, mps_ctx** out) {
     for (int k = 0; k < 3; k++)
         if (cfg->hash_size[k] == 0 || (cfg->hash_size[k] & (cfg->hash_size[k] - 1))) return MPS_ERR_INVALID;
     if (size_t(cfg->hash_size[0]) * cfg->hash_size[1] * cfg->hash_size[2] > (size_t(1) << 31)) return MPS_ERR_INVALID;
+    if (cfg->cg_halo_depth > 2) return MPS_ERR_INVALID;
     *out = nullptr;
     mps_ctx* c = new (std::nothrow) mps_ctx;
     if (!c) return MPS_ERR_NOMEM;

@@ -146,29 +146,34 @@ __global__ void __launch_bounds__(kBlock) k_halo_build(Scenes sc, Nbr nb, uint32
 }
 
 // stats[0] = largest halo staged, stats[1] = blocks not staged, stats[2] = blocks with a halo
-// or not staged: one workgroup folds the per-block counts (no same-address atomics from every
-// workgroup of the build).
+// or not staged, stats[3] = blocks whose halo is staged but deeper than kHaloShallow: one
+// workgroup folds the per-block counts (no same-address atomics from every workgroup of the
+// build).
 constexpr int kHaloStatThreads = 1024;
 __global__ void __launch_bounds__(kHaloStatThreads) k_halo_stats(const uint32_t* __restrict__ hn, uint32_t nblk,
                                                                 uint32_t* __restrict__ stats) {
-    __shared__ uint32_t wm[kHaloStatThreads / 64], ws[kHaloStatThreads / 64], wb[kHaloStatThreads / 64];
-    uint32_t mx = 0u, sp = 0u, bl = 0u;
+    __shared__ uint32_t wm[kHaloStatThreads / 64], ws[kHaloStatThreads / 64], wb[kHaloStatThreads / 64],
+        wd[kHaloStatThreads / 64];
+    uint32_t mx = 0u, sp = 0u, bl = 0u, dp = 0u;
     for (uint32_t b = threadIdx.x; b < nblk; b += blockDim.x) {
         const uint32_t h = hn[b];
         if (h > kHaloCap) sp++;
         else mx = max(mx, h);
         if (h) bl++;
+        if (h > kHaloShallow && h <= kHaloCap) dp++;
     }
     for (int off = 32; off > 0; off >>= 1) {
         mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
         sp += (uint32_t)__shfl_xor((int)sp, off, 64);
         bl += (uint32_t)__shfl_xor((int)bl, off, 64);
+        dp += (uint32_t)__shfl_xor((int)dp, off, 64);
     }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     if (lane == 0) {
         wm[wave] = mx;
         ws[wave] = sp;
         wb[wave] = bl;
+        wd[wave] = dp;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -176,10 +181,12 @@ __global__ void __launch_bounds__(kHaloStatThreads) k_halo_stats(const uint32_t*
             mx = max(mx, wm[w]);
             sp += ws[w];
             bl += wb[w];
+            dp += wd[w];
         }
         stats[0] = mx;
         stats[1] = sp;
         stats[2] = bl;
+        stats[3] = dp;
     }
 }
 

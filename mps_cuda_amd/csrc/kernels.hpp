@@ -98,7 +98,11 @@ constexpr uint32_t kMaxWallObjects = 4;
 // Block halos of the fluid-fluid lists (halo.hip): for workgroup b of slots, idx[b*kHaloCap + t]
 // (t < n[b]) are the distinct neighbour slots in ascending order and nb16[k*N + i] is the
 // position of nff[k*N + i] in its block's list. n[b] > kHaloCap: not staged (use nff).
-constexpr uint32_t kHaloCap = 1264;
+// Each staged kernel stages the halos up to its own LDS depth and gathers deeper ones from
+// global memory. A.p comes in two depths: kHaloShallow (five workgroups per CU) and kHaloCap
+// (four); the deep one pays once a few blocks in a thousand have halos over kHaloShallow.
+constexpr uint32_t kHaloCap = 1536;
+constexpr uint32_t kHaloShallow = 1264;
 struct Halo {
     const uint32_t* idx;
     const uint32_t* n;
@@ -226,6 +230,7 @@ void cg_pack(uint32_t N, const float4* pos, const float4* x0, float4* rec, hipSt
 struct CgOperator {
     int recompute = 0;
     int cubic = 1;
+    int deep = 0;  // the streamed operator stages halos up to kHaloCap (else kHaloShallow)
     const float2* cab = nullptr;
     const float* ca = nullptr;
     const float4* vel = nullptr;

@@ -157,18 +157,18 @@ __device__ __forceinline__ void for_halo(const uint32_t* __restrict__ hs, uint32
 // adjoint's kernels run on its taped lists).
 __device__ __forceinline__ uint32_t halo_count(const Halo& hl, uint32_t blk) { return hl.n ? hl.n[blk] : kHaloCap + 1u; }
 
-// LDS staging depth (records per workgroup, <= kHaloCap) of the kernels whose records are wide
-// enough for the depth to decide how many workgroups share a CU's 160 KiB. A block with a deeper
-// halo gathers from global memory instead, with the same results; at these depths that is a few
-// blocks in a thousand, and the added workgroup per CU pays for them:
-// DF/CD evaluation, 24-byte records: 1104 -> 26.5 KB, six per CU (1264: five);
-// CG setup, 36-byte records: 1104 -> 39.7 KB, four per CU (1264: three).
-// A.p keeps the full depth: its blocks over 1104 cost more than a sixth workgroup gains.
+// LDS staging depth (records per workgroup, <= kHaloCap) of each halo-staged kernel: with wide
+// records the depth decides how many workgroups share a CU's 160 KiB. A block with a deeper halo
+// gathers from global memory instead, with the same results. On the 1M dam break:
+// DF/CD evaluation, 24-byte records: 1104 -> 26.5 KB, six per CU (1264: five), 90 -> 82 us;
+// CG setup, 36-byte records: 1104 -> 39.7 KB, four per CU (1264: three), 213 -> 191 us;
+// A.p, 24-byte records: kHaloShallow = 1264 -> 30.3 KB, five per CU (1136, six per CU, was
+// slower: its blocks over the depth cost more than a sixth workgroup gains), or kHaloCap when the
+// host sees enough deeper halos (kernels.hpp).
 constexpr uint32_t kStageEval = 1104;
 constexpr uint32_t kStageSetup = 1104;
-constexpr uint32_t kStageAp = kHaloCap;
-// the others stage every halo the build keeps
-constexpr uint32_t kStageDensity = kHaloCap, kStageST = kHaloCap, kStagePApply = kHaloCap;
+constexpr uint32_t kStageAp = kHaloShallow;
+constexpr uint32_t kStageDensity = kHaloShallow, kStageST = kHaloShallow, kStagePApply = kHaloShallow;
 static_assert(kStageEval <= kHaloCap && kStageSetup <= kHaloCap && kStageAp <= kHaloCap && kStageDensity <= kHaloCap &&
                   kStageST <= kHaloCap && kStagePApply <= kHaloCap,
               "staging depths");
@@ -1034,7 +1034,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_pack(uint32_t N, const float4* __
 // MODE 1: A p with f (r in rr), partial sums of p.Ap (MPSCGKernel.cuh:21-67).
 // One workgroup's share of A.p: block `blk` of scene `sid`, search-direction factor bta.
 // PCOH: the partial sum is stored device-coherently (a workgroup of the same launch folds it).
-template <int MODE, bool RECOMP, bool CUBIC, bool COH = false, bool PCOH = COH>
+template <int MODE, bool RECOMP, bool CUBIC, bool COH = false, bool PCOH = COH, uint32_t DEPTH = kStageAp>
 __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float bta, const Scenes& sc, const Nbr& nb,
                                                const Halo& hl, const float2* __restrict__ cab, const float* __restrict__ ca,
                                                const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -1055,10 +1055,10 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
         return {a.x + bta * q.x, a.y + bta * q.y, a.z + bta * q.z};
     };
     using LB = typename std::conditional<RECOMP, float4, float2>::type;
-    __shared__ float4 la[kStageAp];
-    __shared__ LB lb[kStageAp];
+    __shared__ float4 la[DEPTH];
+    __shared__ LB lb[DEPTH];
     const uint32_t H = hl.n[blk];
-    const bool staged = H <= kStageAp;  // same for the whole block
+    const bool staged = H <= DEPTH;  // same for the whole block
     using Coef = typename std::conditional<RECOMP, float, float2>::type;
     auto coef = [&](size_t slot) -> Coef {
         if constexpr (RECOMP) {
@@ -1190,7 +1190,7 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
     block_partial<PCOH>(part, partials, blk);
 }
 
-template <int MODE, bool RECOMP, bool CUBIC>
+template <int MODE, bool RECOMP, bool CUBIC, bool DEEP = false>
 __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl, const float2* __restrict__ cab,
                                                    const float* __restrict__ ca, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const float* __restrict__ rho,
@@ -1203,8 +1203,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Scenes sc, Nbr nb, Halo hl,
     const uint32_t sid = sc.blk_scene[blk];
     if (MODE == 1 && st[sid].done) return;
     const float bta = MODE == 1 ? st[sid].beta : 0.0f;
-    cg_apply_block<MODE, RECOMP, CUBIC>(blk, sid, bta, sc, nb, hl, cab, ca, pos, vel, rho, irho, x0, rr, pprev, pout, Ap, b,
-                                        r, x, partials);
+    cg_apply_block<MODE, RECOMP, CUBIC, false, false, DEEP ? kHaloCap : kStageAp>(
+        blk, sid, bta, sc, nb, hl, cab, ca, pos, vel, rho, irho, x0, rr, pprev, pout, Ap, b, r, x, partials);
 }
 
 // Scalars of CGSolver (MPSSolver.cuh:31-65). MODE 0: r.r of the initial residual and the
@@ -1928,10 +1928,13 @@ void cg_apply(int mode, const CgOperator& op, const Scenes& sc, const Nbr& nb, c
               const SolverDev* st, double* partials, hipStream_t s) {
     if (!sc.N) return;
     const unsigned g = nb_(sc.N);
-#define MPS_CG_APPLY(M, R, C)                                                                                             \
-    launch_k(k_cg_apply<M, R, C>, g, kBlock, 0, s, sc, nb, hl, op.cab, op.ca, pos, op.vel, op.rho, op.irho, x0, rr, pprev, \
-             pout, Ap, b, r, x, st, partials)
-    if (!op.recompute) {
+#define MPS_CG_APPLY(M, R, C, ...)                                                                                   \
+    launch_k(k_cg_apply<M, R, C, ##__VA_ARGS__>, g, kBlock, 0, s, sc, nb, hl, op.cab, op.ca, pos, op.vel, op.rho, op.irho, \
+             x0, rr, pprev, pout, Ap, b, r, x, st, partials)
+    if (!op.recompute && op.deep) {
+        if (mode == 0) MPS_CG_APPLY(0, false, true, true);
+        else MPS_CG_APPLY(1, false, true, true);
+    } else if (!op.recompute) {
         if (mode == 0) MPS_CG_APPLY(0, false, true);
         else MPS_CG_APPLY(1, false, true);
     } else if (op.cubic) {
