@@ -218,7 +218,7 @@ def cpu_baseline(scene, snap, budget_s: float, max_steps: int) -> dict:
     }
 
 
-def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
+def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2, graphs: str = "auto") -> dict:
     """The batched form against the plain path on the same GPU: `n_scenes` independent scenes
     (sizes log-uniform 8k-256k, blocks or falling spheres, per-scene viscosity and surface
     tension) stepped together in one context, then each scene stepped alone in a context of
@@ -230,6 +230,8 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
     walls = [scenes.sample_walls(HipSim, s) for s in sc]
     total = sum(s.n for s in sc)
     cfg = scenes.make_config(sc[0])
+    if graphs != "auto":
+        cfg.use_graphs = int(graphs)
     print(f"batched leg: {n_scenes} scenes, {total} particles", file=sys.stderr, flush=True)
 
     bat = HipSim(cfg)
@@ -237,8 +239,8 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
     for _ in range(warmup):
         bat.step()
     # launch mode (graph replay of solver chunks or direct launches, host-poll interval) picked
-    # by the library's autotuner on this batch; the timed steps use it
-    tuning = bat.autotune(2)
+    # by the library's autotuner on this batch, unless given; the timed steps use it
+    tuning = bat.autotune(2) if graphs == "auto" else None
     bat.synchronize()
     l_cg = []
     t0 = time.perf_counter()
@@ -271,12 +273,13 @@ def batched_leg(n_scenes: int, steps: int, warmup: int, seed: int = 2) -> dict:
         "value": v_bat, "ms_per_step": 1000.0 * t_bat / steps,
         "plain_value": v_plain, "plain_ms_per_step": 1000.0 * t_plain / steps,
         "speedup": v_bat / v_plain, "l_cg_max_mean": sum(l_cg) / len(l_cg),
-        "timed_launch_mode": {"use_graphs": tuning["use_graphs"], "host_sync_every": tuning["host_sync_every"]},
+        "timed_launch_mode": ({"use_graphs": tuning["use_graphs"], "host_sync_every": tuning["host_sync_every"]} if tuning
+                              else {"use_graphs": int(cfg.use_graphs), "host_sync_every": int(cfg.host_sync_every)}),
         "solver_tuning": tuning,
     }
 
 
-def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
+def adjoint_leg(n: int, steps: int, every: int, seed: int = 0, graphs: str = "auto") -> dict:
     """The differentiable droplet: a sphere of n particles falling for `steps` steps; target
     from viscosity 0.02, surface tension 0.3 and a smooth seeded initial velocity; loss = MSE of
     the final positions. At the guess (0.025, 0.1, same initial velocity) time one plain forward,
@@ -286,7 +289,10 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
     from mps_cuda_amd import HipSim, scenes
 
     sc = scenes.diff_droplet(n=n, steps=steps, seed=seed)
-    g = HipSim(scenes.make_config(sc))
+    cfg = scenes.make_config(sc)
+    if graphs != "auto":
+        cfg.use_graphs = int(graphs)
+    g = HipSim(cfg)
     v0 = sc.extra["v0"]
 
     def reset(mu, sig, v):
@@ -304,8 +310,9 @@ def adjoint_leg(n: int, steps: int, every: int, seed: int = 0) -> dict:
         return g.get("FLUID_POS").astype(np.float64), time.perf_counter() - t0
 
     target, _ = forward(0.02, 0.3, v0)
-    # graph replay and the poll interval for this small, many-iteration scene (state kept)
-    tuning = g.autotune(2)
+    # graph replay and the poll interval for this small, many-iteration scene (state kept),
+    # unless given
+    tuning = g.autotune(2) if graphs == "auto" else None
     mu0, sig0 = 0.025, 0.1
     x, t_fwd = forward(mu0, sig0, v0)
 
@@ -358,7 +365,8 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--graphs", default="auto", choices=("auto", "0", "1"),
                     help="replay solver iteration chunks from HIP graphs: auto = the library's autotuner "
-                         "picks graphs on/off and the host-poll interval after the warmup")
+                         "picks graphs on/off and the host-poll interval after the warmup, in every leg; "
+                         "0 or 1 fixes it and runs no autotuner")
     ap.add_argument("--cg-stream-coeffs", type=int, default=None, choices=(0, 1),
                     help="viscosity operator: 1 streams both pair coefficients, 0 streams a_ij and recomputes "
                          "b_ij (default: the library's configuration default)")
@@ -490,10 +498,10 @@ def main() -> int:
 
     sim.profile(False)
     sim.close()
-    batched = batched_leg(args.batch_scenes, args.batch_steps, args.warmup) if args.batch_scenes > 0 else None
+    batched = batched_leg(args.batch_scenes, args.batch_steps, args.warmup, graphs=args.graphs) if args.batch_scenes > 0 else None
     if batched:
         say(f"batched leg: {batched.get('speedup', batched)}")
-    adjoint = adjoint_leg(args.adjoint_particles, args.adjoint_steps, 10) if args.adjoint_particles > 0 else None
+    adjoint = adjoint_leg(args.adjoint_particles, args.adjoint_steps, 10, graphs=args.graphs) if args.adjoint_particles > 0 else None
     if adjoint:
         say("adjoint leg done")
 
