@@ -9,11 +9,12 @@ viscosity 0.01, surface tension 0.05, FP32.
 A "step" is one System::OnUpdate: hash, neighbour lists, boundary volume, density/alpha,
 divergence-free solve, viscosity + surface-tension CG, constant-density solve, advection.
 `value` = fluid particles x timed steps / wall time of the timed steps (inputs resident in
-HBM). The W warm-up steps run first (then the library's autotuner picks graph replay and the
-host-poll interval); the scene is then reloaded and advanced --settle steps untimed (default
-500: past the start-up transient, where the lattice collapses from 0.52 rho0 towards rest
-density, the neighbour counts rise from ~28 to ~40 per particle and the CG iteration count from
-~13 to ~50), and that state is kept. The timed steps are steps settle..settle+K-1 from it (no
+HBM). The W warm-up steps run first; the scene is then reloaded and advanced --settle steps
+untimed (default 500: past the start-up transient, where the lattice collapses from 0.52 rho0
+towards rest density, the neighbour counts rise from ~28 to ~40 per particle and the CG iteration
+count from ~13 to ~50). The library's autotuner then picks its launch settings by timing trial
+steps from that state, which it restores, and the state is kept. The timed steps are steps
+settle..settle+K-1 from it (no
 device allocation inside the region, no per-kernel events); the same steps then run again from
 the kept state with HIP events around every kernel on the context's stream, which gives the
 per-kernel times. The CPU baseline steps from the same kept state.
@@ -86,16 +87,17 @@ def traffic_window() -> list | None:
     return [int(m.group(1)), int(m.group(2))] if m else None
 
 
-def measured_traffic(kernel: str, window: list) -> float | None:
+def measured_traffic(kernel: str, window: list, variant: str = "") -> float | None:
     """HBM bytes per launch of `kernel` on this workload: FETCH_SIZE + WRITE_SIZE from two
     rocprofv3 counter passes over the same timed window, each scaled by what the counter reads
     for a plain copy of known size (the summary's last column: MB per working dispatch, mean).
     None when the summary covers other steps than `window` (the lists and halos grow as the
-    fluid settles, so bytes per launch depend on the window). The profiler may spell out more
-    template arguments than the library's timing table (`k_cg_apply<1, false, true>` for
-    `k_cg_apply<1>`)."""
+    fluid settles, so bytes per launch depend on the window). The profiler spells out the
+    template arguments that the library's timing table leaves out: `variant` selects among them
+    (`k_cg_apply<1>` with variant "false" is the summary's `k_cg_apply<1, false, ...>` row)."""
     if traffic_window() != list(window):
         return None
+    want = kernel[:-1] + ", " + variant if variant else kernel[:-1] + ","
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -103,7 +105,7 @@ def measured_traffic(kernel: str, window: list) -> float | None:
                 if len(cells) < 5:
                     continue
                 name = cells[0]
-                if name == kernel or (kernel.endswith(">") and name.startswith(kernel[:-1] + ",")):
+                if name == kernel or (kernel.endswith(">") and name.startswith(want)):
                     return float(cells[-1]) * 1.0e6
     except (OSError, ValueError):
         return None
@@ -393,10 +395,11 @@ def main() -> int:
     scenes.load_into(sim, scene)
     for _ in range(args.warmup):
         sim.step()
-    tuned = sim.autotune(2) if args.graphs == "auto" else None
     scenes.load_into(sim, scene)
     for _ in range(args.settle):
         sim.step()
+    # the launch settings timed on the state the window starts from (restored after every trial)
+    tuned = sim.autotune(2) if args.graphs == "auto" else None
     kept = snapshot(sim)
     say(f"settled: {args.settle} untimed steps, frame {kept['frame']}")
 
@@ -466,6 +469,9 @@ def main() -> int:
             kb = kernel_bytes(name, N, eff, efb)
             kbytes[name] = kbytes.get(name, 0.0) + u * kb
 
+    # the viscosity operator that ran: both pair coefficients streamed, or b_ij recomputed
+    stream_coeffs = int(tuned["cg_stream_coeffs"]) if tuned else int(cfg.cg_stream_coeffs)
+    recomputed = stream_coeffs == 0
     roofline = None
     kernels = {}
     for name, p in prof.items():
@@ -483,7 +489,8 @@ def main() -> int:
         roofline = {
             "kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom, [args.settle, args.settle + args.steps - 1]),
+            "traffic": measured_traffic(dom, [args.settle, args.settle + args.steps - 1],
+                                        ("true" if recomputed else "false") if dom.startswith("k_cg_apply<") else ""),
             "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT), "traffic_window": traffic_window(),
             "algorithmic_bytes_per_launch": kbytes[dom] / max(p["useful"], 1),
             "avg_launch_us": 1000.0 * p["total_ms"] / max(p["launches"], 1),
@@ -532,7 +539,7 @@ def main() -> int:
                         f"(timed after {args.settle} untimed steps from the initial lattice)",
             "window": [args.settle, args.settle + args.steps - 1],
             "n_fluid": N, "n_boundary": B, "hash_size": list(cfg.hash_size),
-            "cg_stream_coeffs": int(cfg.cg_stream_coeffs),
+            "cg_stream_coeffs": stream_coeffs,
             "solver_tuning": tuned,
         },
         "roofline": roofline, "cpu_baseline": cpu, "iterations": iters, "batched": batched,
