@@ -367,8 +367,9 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--graphs", default="auto", choices=("auto", "0", "1"),
                     help="replay solver iteration chunks from HIP graphs: auto = the library's autotuner "
-                         "picks graphs on/off and the host-poll interval after the warmup, in every leg; "
-                         "0 or 1 fixes it and runs no autotuner")
+                         "picks its launch settings (graphs, host-poll interval, operator form, folds, "
+                         "persistent solver) in every leg, for the dam break from the settled state; "
+                         "0 or 1 fixes graph replay and runs no autotuner")
     ap.add_argument("--cg-stream-coeffs", type=int, default=None, choices=(0, 1),
                     help="viscosity operator: 1 streams both pair coefficients, 0 streams a_ij and recomputes "
                          "b_ij (default: the library's configuration default)")
