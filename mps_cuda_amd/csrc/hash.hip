@@ -11,7 +11,9 @@
 //     to column-major ELL storage nbr[k*N + i] (a wave reads 256 contiguous bytes per k),
 //     the boundary-boundary list to rows of K slots (it is read once, when the walls move),
 //     the per-particle count to cnt[i]; overflow past the capacity is counted and reported
-//     so the host can grow the capacity and search again.
+//     so the host can grow the capacity and search again;
+//   * the fluid search stages a workgroup's candidates in LDS, row by row, so that each
+//     particle walks its 27 cells as 9 contiguous runs (k_neighbors_fluid).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
