@@ -172,9 +172,10 @@ typedef struct mps_config {
     uint32_t adj_cg_max_iter;    /* adjoint: cap of the transposed viscosity solve (300) */
     uint32_t use_graphs;         /* HIP library: replay solver iteration chunks from cached HIP
                                     graphs (same results; not while timing kernels) */
-    uint32_t halo_limit;         /* HIP library: largest block halo of the viscosity operator staged in
-                                    LDS; blocks with more gather from global memory. 0 = the LDS
-                                    capacity (1264); smaller values exist to test the global path */
+    uint32_t halo_limit;         /* HIP library: largest block halo listed for LDS staging; blocks
+                                    with more gather from global memory in every halo-staged kernel.
+                                    0 = the largest listed (1536; each kernel then stages up to its
+                                    own LDS depth); smaller values exist to test the global path */
     uint32_t couple_scenes;      /* 1: the scenes of a batch are fluid objects of one world (same dt,
                                     gravity, box; walls given per scene): density, factor, density
                                     change and pressure force include the other scenes' particles
@@ -235,9 +236,9 @@ typedef struct mps_step_stats {
     double err_df[MPS_MAX_ITER_LOG];  /* mean positive error of each DF evaluation */
     double err_cd[MPS_MAX_ITER_LOG];
     double rr_cg[MPS_MAX_ITER_LOG];   /* r.r of each CG iteration (or Jacobi error) */
-    /* block halos of the viscosity operator (no reference counterpart): largest halo staged in
-     * LDS, 256-slot blocks over the LDS cap (these gather from global memory), blocks with
-     * particles; zero when the CG did not run */
+    /* block halos of the fluid lists (no reference counterpart): largest halo listed for LDS
+     * staging (at most 1536), 256-slot blocks over that limit (these gather from global memory),
+     * blocks with particles; zero when the CG did not run */
     uint32_t halo_max;
     uint32_t halo_spill;
     uint32_t halo_blocks;
