@@ -287,7 +287,7 @@ __global__ void __launch_bounds__(kBlock) k_density_alpha(int flags, Scenes sc, 
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) { la[t] = pos[s]; });
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) { la[t] = pos[s]; });
         __syncthreads();
     }
     if (!live) return;
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_eval(Scenes sc, Nbr nb, Hal
     }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float2(v.y, v.z);
@@ -717,7 +717,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_apply(Scenes sc, Nbr nb, Ha
     }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s];
             la[t] = make_float4(x.x, x.y, x.z, kappa[s]);
         });
@@ -849,7 +849,7 @@ __global__ void __launch_bounds__(kBlock) k_st_factor(Scenes sc, Nbr nb, Halo hl
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float4(v.y, v.z, v.w, 0.0f);
@@ -937,7 +937,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_setup(Scenes sc, Nbr nb, Halo hl,
     fetch(0);
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 x = pos[s], v = vel[s];
             la[t] = make_float4(x.x, x.y, x.z, v.x);
             lb[t] = make_float4(v.y, v.z, v.w, rho[s]);
@@ -1096,7 +1096,7 @@ __device__ __forceinline__ void cg_apply_block(uint32_t blk, uint32_t sid, float
     }
     if (staged) {
         const uint32_t* hs = hl.idx + size_t(blk) * kHaloCap;
-        for_halo<2>(hs, H, [&](uint32_t t, uint32_t s) {
+        for_halo<4>(hs, H, [&](uint32_t t, uint32_t s) {
             const float4 xs = pos[s];
             const f3 v = vec(s);
             la[t] = make_float4(xs.x, xs.y, xs.z, v.x);
